@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Reconstruction throughput on one MI355X at the 256^3 configuration:
+a 256^3 map of 400 Gaussian blobs, 100k particles of 256^2 FP32 at known
+uniform views, per-particle defocus U[0.8,3.5] um and astigmatism U[0,0.2] um,
+SNR 0.05, NN insertion, pad 2 (ft 512), CTF flip.
+
+A step = one complete reconstruction of all particles already resident in HBM:
+reset the accumulators, background + pad + 2D FFT + phase shift + CTF +
+central-section insertion of every particle, then sum/weigh/centre/3D inverse
+FFT to the real-space map. Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--particles P] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=100000)
+    ap.add_argument("--box", type=int, default=256)
+    ap.add_argument("--blobs", type=int, default=400)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    from bsoft_amd import abi, synth
+    from bsoft_amd.recon import Reconstruction, part_ft_size
+
+    dev = torch.device("cuda:0")
+    box, N = args.box, args.box
+    ft = part_ft_size(box, 1, 2)
+    P = args.particles
+    log(f"[bench] generating {P} particles {box}^2, map {N}^3 ({args.blobs} blobs), ft {ft}")
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(0)
+    bm = synth.make_blob_map(N, args.blobs, rng)
+    ps = synth.make_particles(bm, P, box, rng, pix=1.5, defocus=(8e3, 3.5e4), astig=(0.0, 2e3),
+                              snr=0.05, device=dev, seed=0, chunk=1024)
+    torch.cuda.synchronize()
+    log(f"[bench] data ready in {time.perf_counter() - t0:.1f} s")
+
+    flags = abi.BSX_FLAG_TIMING
+    rec = Reconstruction(N, box, ft, 1.5, ctf_action=abi.BSX_CTF_FLIP, flags=flags,
+                         batch=args.batch)
+    out_map = None
+
+    def step():
+        nonlocal out_map
+        rec.reset()
+        rec.insert(ps.images, ps.parts)
+        r = rec.finish()
+        out_map = r.map
+        return r
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    ins_ms, launches = 0.0, 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = step()
+        ms, nl, _ = rec.last_timing()
+        ins_ms += ms
+        launches += nl
+    torch.cuda.synchronize()
+    T = time.perf_counter() - t0
+    log(f"[bench] {args.steps} steps in {T:.3f} s; coverage {r.coverage}, friedel {r.friedel:.2e}")
+
+    # roofline of the dominant kernel (column FFT + phase/CTF + insertion):
+    # algorithmic bytes per particle = half-spectrum column input
+    # (ft/2+1)*box*8 + per inserted sample the 12-byte (re, im, w) record read
+    # and written (24 B); NN inserts all ft^2 samples at the default resolution.
+    n_s = ft * ft
+    bytes_pp = (ft // 2 + 1) * box * 8 + 24 * n_s
+    bpl = rec.params.batch if rec.params.batch else None
+    avg_launch_ms = ins_ms / max(1, launches)
+    per_launch_parts = P * args.steps / max(1, launches)
+    achieved = bytes_pp * per_launch_parts / (avg_launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": f"k_cols_insert<{ft}>", "avg_launch_ms": round(avg_launch_ms, 4),
+                "particles_per_launch": round(per_launch_parts, 1),
+                "algorithmic_bytes_per_particle": bytes_pp}
+
+    result = {
+        "metric": "particles/s (search+recon, 256^3) learned-proposal vs plain GPU; FSC vs CPU",
+        "value": round(P * args.steps / T, 1),
+        "unit": "particles/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * T / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"reconstruction {N}^3 map from {P} particles {box}^2, "
+                               f"ft {ft}, NN, CTF flip, defocus U[0.8,3.5]um, astig U[0,0.2]um, "
+                               f"SNR 0.05, {args.blobs}-blob map",
+                   "particles": P, "box": box, "map": N, "ft": ft},
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu:
+        result["cpu_baseline"], result["fsc_vs_cpu"] = cpu_baseline(args, ps, rec.params, ft)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def cpu_baseline(args, ps, params, ft):
+    """The FP32 CPU restatement (OpenMP over partial maps, like breconstruct
+    -threads T) timed on a bounded sample of the same workload, plus the GPU
+    result on the same sample for the FSC comparison."""
+    import torch
+    from bsoft_amd import synth
+    from bsoft_amd.recon import Reconstruction
+    from oracle.oracle import Oracle, OracleRecon
+
+    threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+    rp = params
+    rp_cpu = type(rp)()
+    for f, _ in rp._fields_:
+        setattr(rp_cpu, f, getattr(rp, f))
+    rp_cpu.flags = 0
+    # calibrate on one particle per thread, then size the sample to the budget
+    ncal = threads
+    o = OracleRecon(rp_cpu, threads, 32)
+    imgs = ps.images[:ncal].cpu().numpy()
+    t0 = time.perf_counter()
+    o.insert(synth.particles_list(ps.parts[:ncal]), imgs, threads)
+    tcal = time.perf_counter() - t0
+    rate = ncal / tcal
+    nsamp = int(max(threads, min(len(ps.parts) - ncal, rate * args.cpu_seconds)))
+    nsamp = (nsamp // threads) * threads
+    log(f"[bench] cpu: {threads} threads, {rate:.1f} particles/s calibration, sample {nsamp}")
+    imgs = ps.images[ncal:ncal + nsamp].cpu().numpy()
+    t0 = time.perf_counter()
+    o.insert(synth.particles_list(ps.parts[ncal:ncal + nsamp]), imgs, threads)
+    tins = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    m, _, ftc, st = o.finish(want_ft=True)
+    tfin = time.perf_counter() - t0
+    P = len(ps.parts)
+    full = tins / nsamp * P + tfin
+    log(f"[bench] cpu: insert {nsamp} in {tins:.1f} s, finish {tfin:.1f} s -> {P / full:.2f} particles/s")
+    # GPU on the same ncal + nsamp particles, FSC against the CPU map
+    n = ncal + nsamp
+    g = Reconstruction(rp.size, rp.box, rp.ft_size, rp.sampling, hi_res=rp.hi_res,
+                       ctf_action=rp.ctf_action, wiener=rp.wiener)
+    g.insert(ps.images[:n].contiguous(), ps.parts[:n])
+    gr = g.finish(want_ft=True)
+    torch.cuda.synchronize()
+    fsc = Oracle(64).fsc(gr.ft.cpu().numpy(), ftc, rp.sampling, rp.hi_res)
+    err = float(np.abs(gr.map.cpu().numpy().astype(np.float64) - m).max() / np.abs(m).max())
+    cpu = {"value": round(P / full, 3), "unit": "particles/s", "cores": threads, "kind": "port",
+           "sample": f"FP32 CPU restatement, OpenMP over {threads} partial maps: {n} of the "
+                     f"{P} particles inserted in {tins:.1f} s (+{tcal:.1f} s calibration), "
+                     f"finish {tfin:.1f} s; value = {P} / (per-particle time x {P} + finish)"}
+    fsc_info = {"particles": n, "min_fsc": round(float(fsc.min()), 7),
+                "max_rel_voxel_err": err, "shells": len(fsc)}
+    return cpu, fsc_info
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,389 @@
+// bsx_extract.hip — the mirror of insertion and the quality measures:
+//   k_real_to_complex + 3 axis FFTs + k_phase_origin : map transform
+//       (Bimage::project prologue, src/img/Bimage_project.cpp:373-375)
+//   k_extract        : central sections by FSI-kernel gather interpolation
+//       (Bimage::central_section, Bimage_project.cpp:288-338;
+//        Bimage::fspace_interpolate, Bimage_fspace.cpp:85-147;
+//        FSI_Kernel, include/FSI_Kernel.h:28-68), centre shift as project()
+//   k_fsc            : FP64 shell sums of Bimage::fsc (Bimage_resolution.cpp:330-406)
+//   k_frc_pairs      : img_recip_space_fom, fom_type 0 (src/mg/mg_refine.cpp:660-781)
+#include <cmath>
+#include <vector>
+
+#include "bsx_internal.h"
+
+namespace bsx {
+int fft_axis_ex(float2* data, float* real_out, int nx, int ny, int nz, int64_t batch, int axis,
+                int sign, float scale, hipStream_t s);
+}
+using namespace bsx;
+
+// ------------------------------------------------------------ map transform
+__global__ void k_real_to_complex(const float* __restrict__ in, float2* __restrict__ out,
+                                  long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = make_float2(in[i], 0.f);
+}
+
+// phase_shift_to_origin (Bimage_complex.cpp:490-543): x e^{i phi},
+// phi = h*sx + (l*sz + k*sy), s = origin/size * 2 pi.
+__global__ void k_phase_origin(float2* __restrict__ v, int N, double sx, double sy, double sz) {
+    const long long n3 = (long long)N * N * N;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long xx = i % N, yy = (i / N) % N, zz = i / ((long long)N * N);
+        const long h = xx > (N - 1) / 2 ? xx - N : xx;
+        const long k = yy > (N - 1) / 2 ? yy - N : yy;
+        const long l = zz > (N - 1) / 2 ? zz - N : zz;
+        const double phi = h * sx + (l * sz + k * sy);
+        double sn, cs;
+        sincos(phi, &sn, &cs);
+        const float2 a = v[i];
+        v[i] = make_float2((float)(a.x * cs - a.y * sn), (float)(a.x * sn + a.y * cs));
+    }
+}
+
+// ------------------------------------------------------------- extraction
+// One thread per section pixel; blockIdx.y = view. Kernel table in LDS.
+template <int WD>
+__global__ __launch_bounds__(256) void k_extract(const float2* __restrict__ ft, int N,
+                                                 const double* __restrict__ mats, double inv,
+                                                 double rs, double maxs2, int center,
+                                                 const float* __restrict__ ktab,
+                                                 float2* __restrict__ out) {
+    constexpr int NDIV = 512;
+    __shared__ float K[(NDIV + 1) * WD];
+    for (int i = threadIdx.x; i < (NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
+    __syncthreads();
+    const int v = blockIdx.y;
+    const long long npix = (long long)N * N;
+    const long long pix = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= npix) return;
+    const double* M = mats + 9 * v;
+    const int xx = (int)(pix % N), yy = (int)(pix / N);
+    const int hx = (N - 1) / 2;
+    const double ivy = (double)(yy > hx ? yy - N : yy) * inv;
+    const double ivx = (double)(xx > hx ? xx - N : xx) * inv;
+    const double s2 = ivx * ivx + ivy * ivy;
+    double vr = 0, vi = 0;
+    if (s2 <= maxs2) {
+        double m[3];
+        for (int k = 0; k < 3; ++k) m[k] = (ivx * M[3 * k] + ivy * M[3 * k + 1] + 0.0 * M[3 * k + 2]) * rs;
+        constexpr int HW = (WD - 1) / 2;
+        long ix = (long)floor(m[0]), iy = (long)floor(m[1]), iz = (long)floor(m[2]);
+        const int ikx = ((int)((m[0] - ix) * NDIV + 0.5)) * WD;
+        const int iky = ((int)((m[1] - iy) * NDIV + 0.5)) * WD;
+        const int ikz = ((int)((m[2] - iz) * NDIV + 0.5)) * WD;
+        ix -= HW;
+        iy -= HW;
+        iz -= HW;
+        long jxs[WD];
+#pragma unroll
+        for (int q = 0; q < WD; ++q) {
+            // single-step wrap as the reference, made total so that no
+            // index can leave the volume for any coordinate
+            long jx = (ix + q) % N;
+            if (jx < 0) jx += N;
+            jxs[q] = jx;
+        }
+#pragma unroll 1
+        for (int zz = 0; zz < WD; ++zz) {
+            long jz = (iz + zz) % N;
+            if (jz < 0) jz += N;
+            float pr = 0.f, pi = 0.f;
+#pragma unroll 1
+            for (int yy2 = 0; yy2 < WD; ++yy2) {
+                long jy = (iy + yy2) % N;
+                if (jy < 0) jy += N;
+                const float w = K[iky + yy2] * K[ikz + zz];
+                const float2* row = ft + (jz * N + jy) * (long)N;
+#pragma unroll
+                for (int q = 0; q < WD; ++q) {
+                    const float kw = K[ikx + q] * w;
+                    const float2 a = row[jxs[q]];
+                    pr = fmaf(a.x, kw, pr);
+                    pi = fmaf(a.y, kw, pi);
+                }
+            }
+            vr += pr;
+            vi += pi;
+        }
+    }
+    if (center) {
+        // phase_shift_to_center of a 2D section: x (-1)^(h+k) for even N
+        const long h = xx > hx ? xx - N : xx, k = yy > hx ? yy - N : yy;
+        if ((N & 1) == 0) {
+            if ((h + k) & 1) { vr = -vr; vi = -vi; }
+        } else {
+            const double t = -2 * M_PI * (double)(N / 2) / N;
+            double sn, cs;
+            sincos(h * t + k * t, &sn, &cs);
+            const double a = vr, b = vi;
+            vr = a * cs - b * sn;
+            vi = a * sn + b * cs;
+        }
+    }
+    out[(long long)v * npix + pix] = make_float2((float)vr, (float)vi);
+}
+
+// -------------------------------------------------------------------- FSC
+__global__ __launch_bounds__(256) void k_fsc(const float2* __restrict__ a,
+                                             const float2* __restrict__ b, int N, long maxrad,
+                                             double* __restrict__ sums) {
+    extern __shared__ double sh[];   // 3 * maxrad
+    for (long i = threadIdx.x; i < 3 * maxrad; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    const long long n3 = (long long)N * N * N;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long xx = i % N, yy = (i / N) % N, zz = i / ((long long)N * N);
+        const double h = xx > (N - 1) / 2 ? (double)(xx - N) : (double)xx;
+        const double k = yy > (N - 1) / 2 ? (double)(yy - N) : (double)yy;
+        const double l = zz > (N - 1) / 2 ? (double)(zz - N) : (double)zz;
+        // radius = rad_scale*sqrt(rx^2+ry^2+rz^2), rx = h/(N pix): voxel units
+        const double radius = sqrt(h * h + k * k + l * l);
+        const long ir = (long)radius;
+        if (ir >= maxrad) continue;
+        const double f = radius - ir, f1 = 1 - f;
+        const float2 u = a[i], v = b[i];
+        const double p1 = (double)u.x * u.x + (double)u.y * u.y;
+        const double p2 = (double)v.x * v.x + (double)v.y * v.y;
+        const double re = (double)u.x * v.x + (double)u.y * v.y;
+        atomicAdd(&sh[ir], f1 * p1);
+        atomicAdd(&sh[maxrad + ir], f1 * p2);
+        atomicAdd(&sh[2 * maxrad + ir], f1 * re);
+        if (ir + 1 < maxrad) {
+            atomicAdd(&sh[ir + 1], f * p1);
+            atomicAdd(&sh[maxrad + ir + 1], f * p2);
+            atomicAdd(&sh[2 * maxrad + ir + 1], f * re);
+        }
+    }
+    __syncthreads();
+    for (long i = threadIdx.x; i < 3 * maxrad; i += blockDim.x)
+        if (sh[i] != 0) atomicAdd(&sums[i], sh[i]);
+}
+
+// ------------------------------------------------------------ FRC scoring
+// One workgroup per (particle, reference) pair. Ring sums in LDS (FP64).
+__global__ __launch_bounds__(256) void k_frc_pairs(const float2* __restrict__ part,
+                                                   const float2* __restrict__ ref, int N,
+                                                   const int* __restrict__ pairs,
+                                                   double min_rad_sq, double max_rad_cv_sq,
+                                                   double max_rad, double max_rad_cv,
+                                                   double* __restrict__ out,
+                                                   double* __restrict__ out_cv) {
+    extern __shared__ double rs[];   // 3 * N
+    double* fom = rs;
+    double* dsum = rs + N;
+    double* msum = rs + 2 * N;
+    for (int i = threadIdx.x; i < 3 * N; i += blockDim.x) rs[i] = 0;
+    __syncthreads();
+    const int ip = pairs[2 * blockIdx.x], ir = pairs[2 * blockIdx.x + 1];
+    const long long np = (long long)N * N;
+    const float2* P = part + (long long)ip * np;
+    const float2* R = ref + (long long)ir * np;
+    const int hx = (N - 1) / 2;
+    for (long long i = threadIdx.x; i < np; i += blockDim.x) {
+        const int xx = (int)(i % N), yy = (int)(i / N);
+        const double ix = xx > hx ? (double)xx - N : (double)xx;
+        const double iy = yy > hx ? (double)yy - N : (double)yy;
+        const double d2 = ix * ix + iy * iy;
+        if (d2 >= min_rad_sq && d2 <= max_rad_cv_sq) {
+            const int j = (int)sqrt(d2);
+            if (j >= N) continue;
+            const float2 d = P[i], r = R[i];
+            atomicAdd(&fom[j], (double)d.x * r.x + (double)d.y * r.y);
+            atomicAdd(&msum[j], (double)r.x * r.x + (double)r.y * r.y);
+            atomicAdd(&dsum[j], (double)d.x * d.x + (double)d.y * d.y);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double f = 0, wsum = 0, cv = 0;
+        for (int j = 0; j < max_rad && j < N; ++j) {
+            wsum += 1;
+            const double den = dsum[j] * msum[j];
+            if (den > 0) f += fom[j] / sqrt(den);
+        }
+        f = wsum > 0 ? f / wsum : 0;
+        wsum = 0;
+        for (int j = (int)max_rad; j < max_rad_cv && j < N; ++j) {
+            wsum += 1;
+            const double den = dsum[j] * msum[j];
+            if (den > 0) cv += fom[j] / sqrt(den);
+        }
+        out[blockIdx.x] = f;
+        if (out_cv) out_cv[blockIdx.x] = wsum > 0 ? cv / wsum : 0;
+    }
+}
+
+// ================================================================ C-ABI
+static double check_res(double res, int n, double sampling, int ndim) {
+    if (res > n * sampling) return n * sampling;
+    double rm = 2 / sqrt(ndim / (sampling * sampling));
+    return res < rm ? rm : res;
+}
+
+// FSI_Kernel table (include/FSI_Kernel.h:28-68), double precision, stored FP32.
+static std::vector<float> fsi_table(int wd, int power) {
+    const int ndiv = 512;
+    if (power < 0) wd = 1;
+    std::vector<double> data((size_t)(ndiv + 1) * wd, 0.0);
+    if (power < 0) {
+        for (auto& d : data) d = 1;
+    } else {
+        int wd_2 = (wd - 1) / 2;
+        double iwd = 1.0 / wd, piwd = M_PI * iwd, incr = 1.0 / ndiv, arg = incr;
+        data[wd_2] = 1;
+        long ik = wd;
+        for (long i = 1; i < ndiv; i++) {
+            double arg1 = M_PI * (arg + wd_2);
+            double s1 = iwd * sin(arg1);
+            double arg2 = arg1 * iwd;
+            for (int j = 0; j < wd; j++, ik++) {
+                double s2 = sin(arg2), c = cos(arg2);
+                data[ik] = (s1 / s2) * pow(c, power);
+                arg2 -= piwd;
+                s1 = -s1;
+            }
+            arg += incr;
+        }
+        ik += wd_2 + 1;
+        data[ik] = 1;
+    }
+    return std::vector<float>(data.begin(), data.end());
+}
+
+extern "C" {
+
+int bsx_map_transform(const float* map_dev, int n, const double origin[3], float* ft_dev,
+                      void* stream) {
+    BSX_REQUIRE(map_dev && ft_dev && origin, BSX_ERR_ARG, "bsx_map_transform: null argument");
+    BSX_REQUIRE(fft_length_supported(n), BSX_ERR_ARG, "bsx_map_transform: size %d unsupported", n);
+    hipStream_t s = (hipStream_t)stream;
+    const long long n3 = (long long)n * n * n;
+    float2* v = (float2*)ft_dev;
+    hipLaunchKernelGGL(k_real_to_complex, dim3(2048), dim3(256), 0, s, map_dev, v, n3);
+    BSX_HIP(hipGetLastError());
+    const float sc = (float)(1.0 / sqrt((double)n3));
+    int r = fft_axis_ex(v, nullptr, n, n, n, 1, 0, -1, 1.0f, s);
+    if (r == BSX_OK) r = fft_axis_ex(v, nullptr, n, n, n, 1, 1, -1, 1.0f, s);
+    if (r == BSX_OK) r = fft_axis_ex(v, nullptr, n, n, n, 1, 2, -1, sc, s);
+    if (r != BSX_OK) return r;
+    const double sx = origin[0] / n * 2 * M_PI, sy = origin[1] / n * 2 * M_PI,
+                 sz = origin[2] / n * 2 * M_PI;
+    hipLaunchKernelGGL(k_phase_origin, dim3(2048), dim3(256), 0, s, v, n, sx, sy, sz);
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_extract(const float* ft_dev, int n, double sampling, const double* mats, int64_t nviews,
+                double resolution, int kernel_width, int kernel_power, int center, float* out_dev,
+                void* stream) {
+    BSX_REQUIRE(ft_dev && mats && out_dev, BSX_ERR_ARG, "bsx_extract: null argument");
+    BSX_REQUIRE(n >= 2 && nviews >= 0 && nviews < 65536, BSX_ERR_ARG,
+                "bsx_extract: n %d / nviews %lld out of range", n, (long long)nviews);
+    if (nviews == 0) return BSX_OK;
+    if (kernel_power < 0) kernel_width = 1;
+    BSX_REQUIRE(kernel_width >= 1 && kernel_width <= 10, BSX_ERR_ARG,
+                "bsx_extract: kernel width %d out of range 1..10", kernel_width);
+    if (sampling < 1e-10) sampling = 1;
+    if (resolution < 1e-10) resolution = sampling;
+    for (int64_t i = 0; i < 9 * nviews; ++i)
+        BSX_REQUIRE(std::isfinite(mats[i]) && fabs(mats[i]) <= 1.0 + 1e-9, BSX_ERR_ARG,
+                    "bsx_extract: matrix %lld is not a rotation", (long long)(i / 9));
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<float> tab = fsi_table(kernel_width, kernel_power);
+    float* dtab = nullptr;
+    double* dmats = nullptr;
+    BSX_HIP(hipMalloc(&dtab, sizeof(float) * tab.size()));
+    BSX_HIP(hipMalloc(&dmats, sizeof(double) * 9 * nviews));
+    BSX_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
+    BSX_HIP(hipMemcpyAsync(dmats, mats, sizeof(double) * 9 * nviews, hipMemcpyHostToDevice, s));
+    const double inv = 1.0 / (n * sampling), rs = n * sampling;
+    const double maxs2 = 1.0 / (resolution * resolution);
+    dim3 grid((unsigned)(((long long)n * n + 255) / 256), (unsigned)nviews);
+    const float2* ft = (const float2*)ft_dev;
+    float2* out = (float2*)out_dev;
+    switch (kernel_width) {
+#define BSX_EXT(W) case W: hipLaunchKernelGGL((k_extract<W>), grid, dim3(256), 0, s, ft, n, dmats, inv, rs, maxs2, center, dtab, out); break;
+        BSX_EXT(1) BSX_EXT(2) BSX_EXT(3) BSX_EXT(4) BSX_EXT(5)
+        BSX_EXT(6) BSX_EXT(7) BSX_EXT(8) BSX_EXT(9) BSX_EXT(10)
+#undef BSX_EXT
+    }
+    hipError_t le = hipGetLastError();
+    BSX_HIP(hipStreamSynchronize(s));
+    (void)hipFree(dtab);
+    (void)hipFree(dmats);
+    BSX_HIP(le);
+    return BSX_OK;
+}
+
+int bsx_fsc(const float* a_dev, const float* b_dev, int n, double sampling, double hi_res,
+            double* fsc, int* nshell, void* stream) {
+    BSX_REQUIRE(a_dev && b_dev && fsc && nshell, BSX_ERR_ARG, "bsx_fsc: null argument");
+    BSX_REQUIRE(n >= 2 && sampling > 0, BSX_ERR_ARG, "bsx_fsc: bad size/sampling");
+    hipStream_t s = (hipStream_t)stream;
+    hi_res = check_res(hi_res, n, sampling, 3);
+    long maxrad = (long)(n * sampling / hi_res);
+    BSX_REQUIRE(maxrad <= *nshell, BSX_ERR_ARG, "bsx_fsc: %ld shells exceed capacity %d", maxrad,
+                *nshell);
+    double* d = nullptr;
+    BSX_HIP(hipMalloc(&d, sizeof(double) * 3 * (maxrad + 1)));
+    BSX_HIP(hipMemsetAsync(d, 0, sizeof(double) * 3 * (maxrad + 1), s));
+    hipLaunchKernelGGL(k_fsc, dim3(1024), dim3(256), sizeof(double) * 3 * maxrad, s,
+                       (const float2*)a_dev, (const float2*)b_dev, n, maxrad, d);
+    hipError_t le = hipGetLastError();
+    std::vector<double> h(3 * maxrad);
+    if (maxrad > 0)
+        BSX_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * 3 * maxrad, hipMemcpyDeviceToHost, s));
+    BSX_HIP(hipStreamSynchronize(s));
+    (void)hipFree(d);
+    BSX_HIP(le);
+    for (long i = 0; i < maxrad; ++i) {
+        double den = h[i] * h[maxrad + i];
+        fsc[i] = den > 0 ? h[2 * maxrad + i] / sqrt(den) : 0;
+    }
+    if (maxrad > 0) fsc[0] = 1;
+    *nshell = (int)maxrad;
+    return BSX_OK;
+}
+
+int bsx_score_frc(const float* part_dev, int64_t nparts, const float* ref_dev, int64_t nrefs,
+                  int n, double sampling, double hi_res, double lo_res, const int32_t* pairs,
+                  int64_t npairs, double* out, void* stream) {
+    BSX_REQUIRE(part_dev && ref_dev && pairs && out, BSX_ERR_ARG, "bsx_score_frc: null argument");
+    BSX_REQUIRE(n >= 2 && npairs >= 0, BSX_ERR_ARG, "bsx_score_frc: bad sizes");
+    if (npairs == 0) return BSX_OK;
+    for (int64_t i = 0; i < npairs; ++i)
+        BSX_REQUIRE(pairs[2 * i] >= 0 && pairs[2 * i] < nparts && pairs[2 * i + 1] >= 0 &&
+                        pairs[2 * i + 1] < nrefs,
+                    BSX_ERR_ARG, "bsx_score_frc: pair %lld (%d, %d) out of range", (long long)i,
+                    pairs[2 * i], pairs[2 * i + 1]);
+    hipStream_t s = (hipStream_t)stream;
+    if (lo_res > 0 && hi_res > lo_res) std::swap(hi_res, lo_res);
+    hi_res = check_res(hi_res, n, sampling, 2);
+    double max_rad = n * sampling / hi_res;
+    double max_rad_cv = 1.1 * max_rad;
+    if (max_rad_cv <= max_rad) max_rad_cv = max_rad + 1;
+    double min_rad_sq = 0;
+    if (lo_res > 0) { min_rad_sq = n * sampling / lo_res; min_rad_sq *= min_rad_sq; }
+    int* dp = nullptr;
+    double* dout = nullptr;
+    BSX_HIP(hipMalloc(&dp, sizeof(int) * 2 * npairs));
+    BSX_HIP(hipMalloc(&dout, sizeof(double) * npairs));
+    BSX_HIP(hipMemcpyAsync(dp, pairs, sizeof(int) * 2 * npairs, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_frc_pairs, dim3((unsigned)npairs), dim3(256), sizeof(double) * 3 * n, s,
+                       (const float2*)part_dev, (const float2*)ref_dev, n, dp, min_rad_sq,
+                       max_rad_cv * max_rad_cv, max_rad, max_rad_cv, dout, (double*)nullptr);
+    hipError_t le = hipGetLastError();
+    BSX_HIP(hipMemcpyAsync(out, dout, sizeof(double) * npairs, hipMemcpyDeviceToHost, s));
+    BSX_HIP(hipStreamSynchronize(s));
+    (void)hipFree(dp);
+    (void)hipFree(dout);
+    BSX_HIP(le);
+    return BSX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+"""Host mirror of Bsoft's reconstruction / projection entry points, calling
+libbsx.so (include/bsx.h) through ctypes.
+
+Names and argument meanings follow the reference:
+  part_ft_size                  src/mg/mg_reconstruct.cpp:38
+  particle_reconstruct          src/mg/mg_reconstruct.cpp:100
+  img_reconstruction_sum_weigh  src/mg/mg_reconstruct.cpp:266
+  fft (fft_setup_plan + fftw)   src/util/fft.cpp:31-114, Bimage_fft.cpp:176
+  project / central_section     src/img/Bimage_project.cpp:288-443
+  fsc                           src/img/Bimage_resolution.cpp:330
+  img_recip_space_fom           src/mg/mg_refine.cpp:660
+
+Device arrays are torch tensors on a ROCm device; there is no CPU path: the
+library raises when it is missing and every computation runs in HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import abi
+from .abi import ReconParams, Stats, call
+from .synth import PARTICLE_DTYPE, particles_ctypes, view_to_matrix
+
+
+def _ptr(t: torch.Tensor | None):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("device tensor required (libbsx works on HBM-resident data)")
+    if not t.is_contiguous():
+        raise ValueError("contiguous tensor required")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def part_ft_size(xsize: int, scale: float = 1.0, pad_factor: int = 2) -> int:
+    """part_ft_size (mg_reconstruct.cpp:38-64): transform size of padded particles."""
+    return abi.lib().bsx_part_ft_size(int(xsize), float(scale), int(pad_factor))
+
+
+def check_resolution(res: float, n: int, sampling: float, ndim: int = 3) -> float:
+    """Bimage::check_resolution (Bimage.cpp:633-648)."""
+    if res > n * sampling:
+        return n * sampling
+    rm = 2 / np.sqrt(ndim / sampling ** 2)
+    return max(res, rm)
+
+
+@dataclass
+class FinishResult:
+    map: torch.Tensor
+    fom: torch.Tensor | None
+    ft: torch.Tensor | None
+    coverage: int
+    friedel: float
+    particles: int
+    weight_sum: float
+
+
+class Reconstruction:
+    """Device accumulators of one reconstruction (the `prec` Bimage chain of
+    particle_reconstruct: complex sum, power, weight, weight^2)."""
+
+    def __init__(self, size: int, box: int, ft_size: int, sampling: float, hi_res: float = 0.0,
+                 lo_res: float = 0.0, interp: int = 0, ctf_action: int = 0, wiener: float = 0.0,
+                 flags: int = 0, nhalves: int = 1, batch: int = 0, mem_cap: int = 0):
+        if ctf_action > 2 and wiener < 0.01:            # breconstruct.cpp:187
+            wiener = 0.2
+        if hi_res < 0.1:                                  # breconstruct.cpp:236
+            hi_res = sampling
+        self.params = ReconParams(size=size, box=box, ft_size=ft_size, interp=interp,
+                                  sampling=sampling, hi_res=hi_res, lo_res=lo_res,
+                                  ctf_action=ctf_action, flags=flags, wiener=wiener,
+                                  nhalves=nhalves, batch=batch, mem_cap=mem_cap)
+        self.h = C.c_void_p()
+        call("bsx_recon_create", C.byref(self.h), C.byref(self.params))
+        self.size = size
+
+    def close(self):
+        if self.h:
+            abi.lib().bsx_recon_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, stream=None):
+        call("bsx_recon_reset", self.h, _stream(stream))
+
+    def insert(self, imgs: torch.Tensor, parts: np.ndarray, stream=None):
+        """Insert particles: imgs (n, box, box) float32 on the device."""
+        parts = np.ascontiguousarray(parts, dtype=PARTICLE_DTYPE)
+        n = len(parts)
+        if n and (imgs.dtype != torch.float32 or tuple(imgs.shape[1:]) != (self.params.box,) * 2
+                  or imgs.shape[0] < n):
+            raise ValueError(f"images must be float32 (n, {self.params.box}, {self.params.box})")
+        call("bsx_recon_insert", self.h, _ptr(imgs) if n else None,
+             particles_ctypes(parts) if n else None, n, _stream(stream))
+
+    def last_timing(self):
+        ms, st = C.c_double(), C.c_double()
+        nl = C.c_int64()
+        call("bsx_recon_last_timing", self.h, C.byref(ms), C.byref(nl), C.byref(st))
+        return ms.value, nl.value, st.value
+
+    def device_bytes(self) -> int:
+        b = C.c_int64()
+        call("bsx_recon_device_bytes", self.h, C.byref(b))
+        return b.value
+
+    def accumulators(self, half: int = 0, stream=None) -> torch.Tensor:
+        n = self.size
+        out = torch.empty((5, n, n, n), device="cuda", dtype=torch.float32)
+        call("bsx_recon_accumulators", self.h, half, _ptr(out), _stream(stream))
+        torch.cuda.synchronize()
+        return out
+
+    def finish(self, imap: int = 0, nmaps: int = 1, want_fom: bool = False,
+               want_ft: bool = False, want_map: bool = True, stream=None) -> FinishResult:
+        n = self.size
+        m = torch.empty((n, n, n), device="cuda", dtype=torch.float32) if want_map else None
+        fom = torch.empty((n, n, n), device="cuda", dtype=torch.float32) if want_fom else None
+        ft = torch.empty((n, n, n), device="cuda", dtype=torch.complex64) if want_ft else None
+        st = Stats()
+        call("bsx_recon_finish", self.h, imap, nmaps, _ptr(m), _ptr(fom), _ptr(ft), C.byref(st),
+             _stream(stream))
+        return FinishResult(m, fom, ft, st.coverage, st.friedel, st.particles, st.weight_sum)
+
+
+def particle_reconstruct(imgs: torch.Tensor, parts: np.ndarray, size: int, sampling: float,
+                         hi_res: float = 0.0, interp_type: int = 0, ctf_action: int = 0,
+                         wiener: float = 0.0, flags: int = 0, pad_factor: int = 2,
+                         nhalves: int = 1, batch: int = 0) -> Reconstruction:
+    """particle_reconstruct (mg_reconstruct.cpp:100-264): all particles of a
+    list into one set of accumulators, on the device."""
+    box = imgs.shape[-1]
+    ft = part_ft_size(max(box, size), 1, pad_factor)
+    rec = Reconstruction(size, box, ft, sampling, hi_res=hi_res, interp=interp_type,
+                         ctf_action=ctf_action, wiener=wiener, flags=flags, nhalves=nhalves,
+                         batch=batch)
+    rec.insert(imgs, parts)
+    return rec
+
+
+def img_reconstruction_sum_weigh(rec: Reconstruction, imap: int = 0, nmaps: int = 1,
+                                 **kw) -> FinishResult:
+    """img_reconstruction_sum_weigh (mg_reconstruct.cpp:266-361) followed by
+    fft_back and complex_to_real (breconstruct.cpp:340)."""
+    return rec.finish(imap, nmaps, **kw)
+
+
+def fft(data: torch.Tensor, dims: int, direction: int = -1, norm: int = 1, stream=None):
+    """In-place C2C transform of the last `dims` axes of a complex64 tensor
+    (fftw with the sign and norm_flag semantics of Bimage::fft)."""
+    if data.dtype != torch.complex64:
+        raise ValueError("complex64 tensor required")
+    shp = list(data.shape)
+    sizes = shp[-dims:][::-1] + [1] * (3 - dims)
+    batch = int(np.prod(shp[:-dims])) if len(shp) > dims else 1
+    call("bsx_fft", _ptr(data), sizes[0], sizes[1], sizes[2], batch, direction, norm,
+         _stream(stream))
+    return data
+
+
+def map_transform(vol: torch.Tensor, origin=None, stream=None) -> torch.Tensor:
+    """Forward FFT (x 1/sqrt(N^3)) and phase shift to the map origin — the
+    prologue of Bimage::project (Bimage_project.cpp:373-375)."""
+    n = vol.shape[0]
+    if origin is None:
+        origin = (n // 2, n // 2, n // 2)
+    o = (C.c_double * 3)(*origin)
+    out = torch.empty((n, n, n), device=vol.device, dtype=torch.complex64)
+    call("bsx_map_transform", _ptr(vol.contiguous()), n, o, _ptr(out), _stream(stream))
+    return out
+
+
+def central_sections(ft: torch.Tensor, mats: np.ndarray, sampling: float, resolution: float,
+                     kernel_width: int = 8, kernel_power: int = 2, center: bool = True,
+                     stream=None) -> torch.Tensor:
+    """Bimage::central_section for each rotation matrix (Bimage_project.cpp:288)
+    with the centre shift of Bimage::project."""
+    n = ft.shape[0]
+    mats = np.ascontiguousarray(np.asarray(mats, dtype=np.float64).reshape(-1, 9))
+    nv = mats.shape[0]
+    out = torch.empty((nv, n, n), device=ft.device, dtype=torch.complex64)
+    call("bsx_extract", _ptr(ft), n, float(sampling), mats.ctypes.data_as(C.POINTER(C.c_double)),
+         nv, float(resolution), int(kernel_width), int(kernel_power), int(center), _ptr(out),
+         _stream(stream))
+    return out
+
+
+def project(ft: torch.Tensor, views: np.ndarray, sampling: float, resolution: float,
+            kernel_width: int = 8, kernel_power: int = 2) -> torch.Tensor:
+    """Bimage::project over a list of views (x, y, z, angle)."""
+    return central_sections(ft, view_to_matrix(views), sampling, resolution, kernel_width,
+                            kernel_power, True)
+
+
+def fsc(a: torch.Tensor, b: torch.Tensor, sampling: float, hi_res: float) -> np.ndarray:
+    """Bimage::fsc (Bimage_resolution.cpp:330-406) of two complex volumes."""
+    n = a.shape[0]
+    cap = C.c_int(4 * n)
+    out = np.zeros(4 * n)
+    call("bsx_fsc", _ptr(a), _ptr(b), n, float(sampling), float(hi_res),
+         out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(cap), _stream(None))
+    return out[: cap.value]
+
+
+def img_recip_space_fom(parts: torch.Tensor, refs: torch.Tensor, pairs: np.ndarray,
+                        sampling: float, hi_res: float, lo_res: float = 0.0) -> np.ndarray:
+    """img_recip_space_fom, fom_type 0 (mg_refine.cpp:660-781), for
+    (particle, reference) index pairs."""
+    pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+    out = np.zeros(len(pairs))
+    call("bsx_score_frc", _ptr(parts), parts.shape[0], _ptr(refs), refs.shape[0],
+         parts.shape[-1], float(sampling),
+         float(hi_res), float(lo_res), pairs.ctypes.data_as(C.POINTER(C.c_int32)), len(pairs),
+         out.ctypes.data_as(C.POINTER(C.c_double)), _stream(None))
+    return out

@@ -1,0 +1,54 @@
+"""CPU checks of the C-ABI library: it loads, exports every entry point that
+include/bsx.h declares, and the host-only entry point matches the oracle."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from bsoft_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "bsx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bsx_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_core_api():
+    syms = declared_symbols()
+    for s in ["bsx_recon_create", "bsx_recon_insert", "bsx_recon_finish", "bsx_extract",
+              "bsx_fsc", "bsx_fft", "bsx_last_error"]:
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(abi.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    # the ctypes signature table covers the header
+    assert set(declared_symbols()) == set(abi.SIGNATURES)
+
+
+def test_struct_layouts():
+    assert C.sizeof(abi.Particle) == 144
+    assert C.sizeof(abi.ReconParams) == 72
+    assert C.sizeof(abi.Stats) == 32
+
+
+@pytest.mark.parametrize("x,scale,pad", [(64, 1, 2), (192, 1, 2), (100, 1.3, 3), (64, 1, 0), (64, 1, 12)])
+def test_part_ft_size_matches_oracle(oracle64, x, scale, pad):
+    assert abi.lib().bsx_part_ft_size(x, scale, pad) == oracle64.part_ft_size(x, scale, pad)
+
+
+def test_errors_are_status_codes():
+    L = abi.lib()
+    assert L.bsx_part_ft_size(0, 1.0, 2) == 0
+    assert b"size" in L.bsx_last_error()
+    h = C.c_void_p()
+    assert L.bsx_recon_create(C.byref(h), None) == abi.BSX_ERR_ARG
+    p = abi.ReconParams(size=64, box=64, ft_size=100, sampling=1.5, nhalves=1)
+    assert L.bsx_recon_create(C.byref(h), C.byref(p)) == abi.BSX_ERR_ARG
+    assert b"ft_size" in L.bsx_last_error()
