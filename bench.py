@@ -177,7 +177,8 @@ def cpu_baseline(args, ps, params, ft):
     g.insert(ps.images[:n].contiguous(), ps.parts[:n])
     gr = g.finish(want_ft=True)
     torch.cuda.synchronize()
-    fsc = Oracle(64).fsc(gr.ft.cpu().numpy(), ftc, rp.sampling, rp.hi_res)
+    fsc, sup = Oracle(64).fsc_with_support(gr.ft.cpu().numpy(), ftc, rp.sampling, rp.hi_res)
+    fsc = fsc[sup]
     err = float(np.abs(gr.map.cpu().numpy().astype(np.float64) - m).max() / np.abs(m).max())
     cpu = {"value": round(P / full, 3), "unit": "particles/s", "cores": threads, "kind": "port",
            "sample": f"FP32 CPU restatement, OpenMP over {threads} partial maps: {n} of the "

@@ -49,6 +49,10 @@ class ReconParams(C.Structure):
         ("nhalves", C.c_int32),
         ("batch", C.c_int32),
         ("mem_cap", C.c_int64),
+        ("method", C.c_int32),
+        ("order", C.c_int32),
+        ("merge", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -96,6 +100,7 @@ SIGNATURES = {
     "bsx_recon_accumulators": ([_vp, C.c_int, _vp, _vp], C.c_int),
     "bsx_recon_device_bytes": ([_vp, C.POINTER(C.c_int64)], C.c_int),
     "bsx_recon_last_timing": ([_vp, _dp, C.POINTER(C.c_int64), _dp], C.c_int),
+    "bsx_recon_kernel_times": ([_vp, _dp, C.c_int], C.c_int),
     "bsx_fft": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, _vp], C.c_int),
     "bsx_map_transform": ([_vp, C.c_int, _dp, _vp, _vp], C.c_int),
     "bsx_extract": ([_vp, C.c_int, C.c_double, _dp, C.c_int64, C.c_double, C.c_int, C.c_int,

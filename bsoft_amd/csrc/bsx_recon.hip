@@ -19,6 +19,7 @@
 //   K4 k_weigh       sum of half sets, FP64 weighing, SNR FOM, centre shift
 //   K5 3 axis FFTs   inverse 3D transform, x 1/sqrt(N^3), real part.
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -42,12 +43,14 @@ struct alignas(16) PartDev {
     double fscale;      // 1 / (ft * pixel_size)              img_ctf_calculate freq_scale
     double ew;          // Ewald offset lambda/2, 0 if off
     double hox, hoy;    // background-ellipse origin (image header)
+    long long src;      // index of the image in the caller's stack
     float wp;           // particle weight incl. pad ratio
     int half;           // accumulator set
 };
 
 struct ReconConst {
     int N, ft, box, stride, interp, ctf_mode, fom, nhalves;   // ctf_mode 0 none 1 flip 2 apply 3 wiener
+    int merge, ewald, NT, pad_;                               // NT: 16^3 tiles per edge
     double mins2, maxs2, mscale, shi2, wiener;
 };
 
@@ -57,7 +60,7 @@ __global__ __launch_bounds__(256) void k_background(const float* __restrict__ im
                                                     float* __restrict__ bkg,
                                                     int* __restrict__ nonfinite) {
     const int p = blockIdx.x;
-    const float* img = imgs + (size_t)p * box * box;
+    const float* img = imgs + (size_t)recs[p].src * box * box;
     const double ox = recs[p].hox, oy = recs[p].hoy;
     const long hx = box / 2, hy = box / 2;
     double sum = 0;
@@ -97,6 +100,7 @@ __global__ __launch_bounds__(256) void k_background(const float* __restrict__ im
 // complex line j). Output: inter[p][x][y], x in [0, FT/2], y in [0, box).
 template <int FT, int LINES>
 __global__ __launch_bounds__(256) void k_rows(const float* __restrict__ imgs,
+                                              const PartDev* __restrict__ recs,
                                               const float* __restrict__ bkg, int box,
                                               int tiles_per_part, float2* __restrict__ inter,
                                               const float2* __restrict__ twg) {
@@ -109,7 +113,7 @@ __global__ __launch_bounds__(256) void k_rows(const float* __restrict__ imgs,
     const int tid = threadIdx.x;
     const int p = blockIdx.x / tiles_per_part;
     const int y0 = (blockIdx.x % tiles_per_part) * 2 * LINES;
-    const float* img = imgs + (size_t)p * box * box;
+    const float* img = imgs + (size_t)recs[p].src * box * box;
     const float b = bkg[p];
     for (int i = tid; i < FT; i += NT) tw[i] = twg[i];
     // load: line j = rows (y0+2j, y0+2j+1), x < box from the image minus the
@@ -146,82 +150,189 @@ __global__ __launch_bounds__(256) void k_rows(const float* __restrict__ imgs,
 }
 
 // ------------------------------------------------------------------- K3
-__device__ __forceinline__ void acc_atomic(float* rec, float vr, float vi, double w, int fom) {
-    const float wf = (float)w;
-    unsafeAtomicAdd(rec + 0, (float)(vr * w));
-    unsafeAtomicAdd(rec + 1, (float)(vi * w));
-    unsafeAtomicAdd(rec + 2, wf);
-    if (fom) {
-        unsafeAtomicAdd(rec + 3, (float)(w * ((double)vr * vr + (double)vi * vi)));
-        unsafeAtomicAdd(rec + 4, (float)(w * w));
-    }
+// Accumulator layout: records of `stride` floats (sum w re, sum w im, sum w
+// [, sum w|F|^2, sum w^2]) in 16^3-voxel tiles; a tile is 4096 contiguous
+// records (48 KB lean). Key = half * NREC + record index, NREC = NT^3 * 4096,
+// so key >> 12 is the (half, tile) bin and key & 4095 the record in the tile.
+__host__ __device__ __forceinline__ long rec_index(long x, long y, long z, long NT) {
+    return ((((z >> 4) * NT + (y >> 4)) * NT + (x >> 4)) << 12) | ((z & 15) << 8) |
+           ((y & 15) << 4) | (x & 15);
 }
 
+// Periodic wrap (index_wrap, Bimage.h:418-426); the fast path covers the
+// in-range and single-step cases that the default geometry produces.
 __device__ __forceinline__ long wrapi(long c, long n) {
+    if (c >= 0 && c < n) return c;
+    if (c < 0 && c >= -n) return c + n;
+    if (c >= n && c < 2 * n) return c - n;
     c %= n;
     return c < 0 ? c + n : c;
 }
 
-// fspace_2D_interpolate (Bimage_reconstruct.cpp:25-88) for one sample.
-__device__ __forceinline__ void interpolate(float* __restrict__ acc, const ReconConst& K,
-                                            const double m[3], float vr, float vi, double w) {
-    const long N = K.N;
-    if (K.interp < 2) {
-        const double fx = floor(m[0] + 0.5), fy = floor(m[1] + 0.5), fz = floor(m[2] + 0.5);
-        if (K.interp == 1) {
-            const double dx = m[0] - fx, dy = m[1] - fy, dz = m[2] - fz;
-            w *= 1 - sqrt(dx * dx + dy * dy + dz * dz);
-        }
-        if (w > 0) {
-            const long cx = wrapi((long)fx, N), cy = wrapi((long)fy, N), cz = wrapi((long)fz, N);
-            acc_atomic(acc + ((cz * N + cy) * N + cx) * K.stride, vr, vi, w, K.fom);
-        }
-    } else {
-        const double fx = floor(m[0]), fy = floor(m[1]), fz = floor(m[2]);
-        const double dx = m[0] - fx, dy = m[1] - fy, dz = m[2] - fz;
-        const long cx = (long)fx, cy = (long)fy, cz = (long)fz;
+enum { MODE_DIRECT = 0, MODE_COUNT = 1, MODE_FILL = 2 };
+
+// One emission: a contribution to the record `key`; `kk` is the sample row
+// (runs are merged only along unbroken k steps); `incl` marks samples the
+// reference inserts (decided from geometry alone, so the count and fill
+// passes agree exactly).
+struct Emit {
+    int key, kk, incl;
+    float re, im, w, p, w2;
+};
+
+struct BinCtx {
+    int* ctr;        // per-bin counters: counts (count pass) / cursors (fill pass)
+    float4* bins;    // (re, im, w, record-in-tile)
+    int cap;         // entries allocated (the count pass bounds the fill pass)
+};
+
+// Merges runs of equal keys inside aligned groups of MG lanes (segmented
+// Hillis-Steele suffix sums); lanes l and l+d merge only if their keys are
+// equal and their sample rows differ by exactly d, so the samples between lie
+// on one straight segment and equal keys are contiguous (a periodic revisit
+// would need a segment longer than N-1 voxels, excluded on the host). The
+// first lane of each run then either adds to the volume with atomics
+// (MODE_DIRECT) or reserves a bin slot: lanes of a wavefront are grouped by
+// bin with ballots and one integer atomic per (wavefront, bin).
+template <int MG, bool FOM, int MODE>
+__device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool merge, Emit e,
+                                     const BinCtx& bc) {
+    bool head = true;
+    if (MG > 1 && merge) {
+        const int lg = __lane_id() & (MG - 1);
 #pragma unroll
-        for (int zz = 0; zz < 2; ++zz) {
-            const long jz = wrapi(cz + zz, N);
-            const double wz = zz ? dz : 1 - dz;
-#pragma unroll
-            for (int yy = 0; yy < 2; ++yy) {
-                const long jy = wrapi(cy + yy, N);
-                const double wy = yy ? dy : 1 - dy;
-#pragma unroll
-                for (int xx = 0; xx < 2; ++xx) {
-                    const long jx = wrapi(cx + xx, N);
-                    const double wx = xx ? dx : 1 - dx;
-                    const double ww = wx * wy * wz * w;
-                    acc_atomic(acc + ((jz * N + jy) * N + jx) * K.stride, vr, vi, ww, K.fom);
+        for (int d = 1; d < MG; d <<= 1) {
+            const int nk = __shfl_down(e.key, d, 64);
+            const int nkk = __shfl_down(e.kk, d, 64);
+            const int ninc = __shfl_down(e.incl, d, 64);
+            float r0 = 0.f, r1 = 0.f, r2 = 0.f, r3 = 0.f, r4 = 0.f;
+            if (MODE != MODE_COUNT) {
+                r0 = __shfl_down(e.re, d, 64);
+                r1 = __shfl_down(e.im, d, 64);
+                r2 = __shfl_down(e.w, d, 64);
+                if (FOM) {
+                    r3 = __shfl_down(e.p, d, 64);
+                    r4 = __shfl_down(e.w2, d, 64);
+                }
+            }
+            if (lg + d < MG && nk == e.key && abs(nkk - e.kk) == d) {
+                e.incl |= ninc;
+                if (MODE != MODE_COUNT) {
+                    e.re += r0;
+                    e.im += r1;
+                    e.w += r2;
+                    if (FOM) { e.p += r3; e.w2 += r4; }
                 }
             }
         }
+        const int pk = __shfl_up(e.key, 1, 64);
+        const int pkk = __shfl_up(e.kk, 1, 64);
+        head = !(lg != 0 && pk == e.key && abs(e.kk - pkk) == 1);
+    }
+    const bool out = head && e.incl;
+    if (MODE == MODE_DIRECT) {
+        if (!out) return;
+        float* rec = acc + (size_t)(unsigned)e.key * stride;
+        unsafeAtomicAdd(rec + 0, e.re);
+        unsafeAtomicAdd(rec + 1, e.im);
+        unsafeAtomicAdd(rec + 2, e.w);
+        if (FOM) {
+            unsafeAtomicAdd(rec + 3, e.p);
+            unsafeAtomicAdd(rec + 4, e.w2);
+        }
+        return;
+    }
+    const int bin = (int)((unsigned)e.key >> 12);
+    const int lane = __lane_id();
+    unsigned long long active = __ballot(out);
+    if (MODE == MODE_COUNT) {
+        // fire-and-forget: one counter increment per (wavefront, bin)
+        while (active) {
+            const int leader = __builtin_ctzll(active);
+            const int b0 = __shfl(bin, leader, 64);
+            const unsigned long long m = __ballot(out && bin == b0);
+            if (lane == leader) atomicAdd(bc.ctr + b0, __popcll(m));
+            active &= ~m;
+        }
+        return;
+    }
+    // fill: issue every reservation of this wavefront before waiting for any
+    int mybase = 0, myleader = lane, myrank = 0;
+    while (active) {
+        const int leader = __builtin_ctzll(active);
+        const int b0 = __shfl(bin, leader, 64);
+        const bool mine = out && bin == b0;
+        const unsigned long long m = __ballot(mine);
+        if (lane == leader) mybase = atomicAdd(bc.ctr + b0, __popcll(m));
+        if (mine) {
+            myleader = leader;
+            myrank = __popcll(m & ((1ull << lane) - 1ull));
+        }
+        active &= ~m;
+    }
+    const int base = __shfl(mybase, myleader, 64);
+    if (out) {
+        const int slot = base + myrank;
+        if (slot < bc.cap)
+            bc.bins[slot] = make_float4(e.re, e.im, e.w, __int_as_float((int)((unsigned)e.key & 4095u)));
     }
 }
 
-// One Fourier sample (h, k) of a particle with raw value F (already x 1/ft,
-// background DC included): phase shift, CTF, resolution band, taper,
-// rotation and interpolation — fspace_pack_2D's loop body.
-__device__ __forceinline__ void insert_sample(float* __restrict__ acc, const ReconConst& K,
-                                              const PartDev& P, long h, long k, float2 F) {
-    const double s0 = (double)h * P.invscale, s1 = (double)k * P.invscale;
-    const double s2 = s0 * s0 + s1 * s1;
-    if (!(s2 >= K.mins2 && s2 <= K.maxs2)) return;
-    double w = P.wp;
-    const double d = (K.maxs2 - s2) * K.mscale;
-    if (d < 1) w *= sqrt(d);
-    // phase shift to origin
-    const double phi = (double)h * P.phx + (double)k * P.phy;
-    double sn, cs;
-    sincos(phi, &sn, &cs);
-    double vr = F.x * cs - F.y * sn, vi = F.x * sn + F.y * cs;
-    vr = (float)vr;   // stored as FP32 after the shift (Bimage_complex.cpp:536)
-    vi = (float)vi;
+template <bool FOM>
+__device__ __forceinline__ Emit make_emit(unsigned key, int kk, bool incl, float fr, float fi,
+                                          double w) {
+    Emit e;
+    e.key = (int)key;
+    e.kk = kk;
+    e.incl = incl ? 1 : 0;
+    if (!incl) w = 0;
+    e.re = (float)(fr * w);
+    e.im = (float)(fi * w);
+    e.w = (float)w;
+    e.p = FOM ? (float)(w * ((double)fr * fr + (double)fi * fi)) : 0.f;
+    e.w2 = FOM ? (float)(w * w) : 0.f;
+    return e;
+}
+
+// sin and cos of a double-precision angle: exact reduction to [-pi, pi] in
+// FP64, then FP32 evaluation (absolute error ~1e-7, below the FP32 rounding
+// the reference applies when it stores the shifted / CTF-multiplied values).
+__device__ __forceinline__ void sincos_red(double a, float* s, float* c) {
+    const double r = a - 6.283185307179586 * rint(a * 0.15915494309189535);
+    sincosf((float)r, s, c);
+}
+__device__ __forceinline__ float sin_red(double a) {
+    const double r = a - 6.283185307179586 * rint(a * 0.15915494309189535);
+    return sinf((float)r);
+}
+
+// Band, taper, phase shift and CTF of one sample (fspace_pack_2D :147-157,
+// phase_shift_to_origin, img_ctf_calculate): returns whether the sample is in
+// the resolution band, its weight and (unless counting) its value.
+template <int MODE>
+__device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev& P, bool active,
+                                             long h, long k, float2 F, double* s0, double* s1,
+                                             double* s2, double* w, float* fr, float* fi) {
+    *s0 = (double)h * P.invscale;
+    *s1 = (double)k * P.invscale;
+    *s2 = *s0 * *s0 + *s1 * *s1;
+    const bool inband = active && *s2 >= K.mins2 && *s2 <= K.maxs2;
+    *w = 0;
+    *fr = 0.f;
+    *fi = 0.f;
+    if (!inband) return false;
+    double ww = P.wp;
+    const double d = (K.maxs2 - *s2) * K.mscale;
+    if (d < 1) ww *= sqrt(d);
+    *w = ww;
+    if (MODE == MODE_COUNT) return true;
+    float sn, cs;
+    sincos_red((double)h * P.phx + (double)k * P.phy, &sn, &cs);
+    float vr = F.x * cs - F.y * sn, vi = F.x * sn + F.y * cs;
     if (K.ctf_mode) {
         const double fx = (double)h * P.fscale, fy = (double)k * P.fscale;
         const double cs2 = fx * fx + fy * fy;
-        double c = 0;
+        float c = 0.f;
         if (cs2 <= K.shi2) {
             double c2a = 1, s2a = 0;
             if (cs2 > 0) {
@@ -233,73 +344,288 @@ __device__ __forceinline__ void insert_sample(float* __restrict__ acc, const Rec
             dphi += P.c20 * cs2;
             dphi += P.c22 * cs2 * c2a;
             dphi += P.c40 * cs2 * cs2;
-            c = sin(dphi);
-            if (K.ctf_mode == 1) c = (c < 0) ? -1 : 1;
-            else if (K.ctf_mode == 3) c = (K.wiener + 1) * c / (c * c + K.wiener);
+            c = sin_red(dphi);
+            if (K.ctf_mode == 1) c = (c < 0.f) ? -1.f : 1.f;
+            else if (K.ctf_mode == 3) c = (float)((K.wiener + 1) * c / ((double)c * c + K.wiener));
         }
         vr *= c;
         vi *= c;
     }
-    const float fr = (float)vr, fi = (float)vi;
+    *fr = vr;
+    *fi = vi;
+    return true;
+}
+
+// Interpolation of one sample at voxel coordinates m (fspace_2D_interpolate,
+// Bimage_reconstruct.cpp:25-88): nearest / weighted nearest / trilinear.
+template <int MG, bool FOM, int MODE>
+__device__ __forceinline__ void interp_emit(float* __restrict__ acc, const ReconConst& K,
+                                            const double m[3], bool inband, double w, float fr,
+                                            float fi, int kk, bool merge, unsigned keybase,
+                                            const BinCtx& bc) {
+    const long N = K.N;
+    if (K.interp < 2) {
+        const double fx = floor(m[0] + 0.5), fy = floor(m[1] + 0.5), fz = floor(m[2] + 0.5);
+        double ww = w;
+        if (K.interp == 1) {
+            const double dx = m[0] - fx, dy = m[1] - fy, dz = m[2] - fz;
+            ww *= 1 - sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        const long cx = wrapi((long)fx, N), cy = wrapi((long)fy, N), cz = wrapi((long)fz, N);
+        emit<MG, FOM, MODE>(acc, K.stride, merge,
+                            make_emit<FOM>(keybase + (unsigned)rec_index(cx, cy, cz, K.NT), kk,
+                                           inband && ww > 0, fr, fi, ww),
+                            bc);
+    } else {
+        const double fx = floor(m[0]), fy = floor(m[1]), fz = floor(m[2]);
+        const double dx = m[0] - fx, dy = m[1] - fy, dz = m[2] - fz;
+        const long cx = (long)fx, cy = (long)fy, cz = (long)fz;
+#pragma unroll
+        for (int corner = 0; corner < 8; ++corner) {
+            const int xx = corner & 1, yy = (corner >> 1) & 1, zz = corner >> 2;
+            const long jx = wrapi(cx + xx, N), jy = wrapi(cy + yy, N), jz = wrapi(cz + zz, N);
+            const double ww = (xx ? dx : 1 - dx) * (yy ? dy : 1 - dy) * (zz ? dz : 1 - dz) * w;
+            emit<MG, FOM, MODE>(acc, K.stride, merge,
+                                make_emit<FOM>(keybase + (unsigned)rec_index(jx, jy, jz, K.NT), kk,
+                                               inband, fr, fi, ww),
+                                bc);
+        }
+    }
+}
+
+__device__ __forceinline__ void coords(const ReconConst& K, const PartDev& P, double s0, double s1,
+                                       double sz, double m[3]) {
+    m[0] = (s0 * P.R[0] + s1 * P.R[1] + sz * P.R[2]) * K.mscale;
+    m[1] = (s0 * P.R[3] + s1 * P.R[4] + sz * P.R[5]) * K.mscale;
+    m[2] = (s0 * P.R[6] + s1 * P.R[7] + sz * P.R[8]) * K.mscale;
+}
+
+// One sample (h, k) with raw value F and the Friedel mate of its array
+// position, (-h, km) with value conj(F): fspace_pack_2D's loop body for both.
+// Away from the Nyquist row and without the Ewald offset the mate's
+// frequency is exactly -(h, k), so its coordinates are -m and its value the
+// conjugate: computed once, emitted twice. Every lane of a wavefront runs the
+// same emits (excluded samples carry incl = 0), so shuffles and ballots stay
+// uniform.
+template <int MG, bool FOM, int MODE>
+__device__ __forceinline__ void insert_pair(float* __restrict__ acc, const ReconConst& K,
+                                            const PartDev& P, bool active, bool mirror, long h,
+                                            long k, long km, float2 F, bool merge,
+                                            unsigned keybase, const BinCtx& bc) {
+    double s0, s1, s2, w;
+    float fr, fi;
+    const bool inband = sample_value<MODE>(K, P, active, h, k, F, &s0, &s1, &s2, &w, &fr, &fi);
     double m[3];
     const double mz0 = P.ew * s2;
-#pragma unroll
-    for (int e = 0; e < 2; ++e) {
-        const double sz = e ? -mz0 : mz0;
-        m[0] = (s0 * P.R[0] + s1 * P.R[1] + sz * P.R[2]) * K.mscale;
-        m[1] = (s0 * P.R[3] + s1 * P.R[4] + sz * P.R[5]) * K.mscale;
-        m[2] = (s0 * P.R[6] + s1 * P.R[7] + sz * P.R[8]) * K.mscale;
-        interpolate(acc, K, m, fr, fi, w);
-        if (P.ew == 0) break;
+    coords(K, P, s0, s1, mz0, m);
+    interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
+    if (K.ewald) {
+        coords(K, P, s0, s1, -mz0, m);
+        interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
+    }
+    // the mate
+    if (!K.ewald && km == -k) {
+        const double mm[3] = {-m[0], -m[1], -m[2]};
+        interp_emit<MG, FOM, MODE>(acc, K, mm, inband && mirror, w, fr, -fi, (int)km, merge,
+                                   keybase, bc);
+    } else {
+        double t0, t1, t2, tw;
+        float gr, gi;
+        const bool ib = sample_value<MODE>(K, P, mirror, -h, km, make_float2(F.x, -F.y), &t0,
+                                           &t1, &t2, &tw, &gr, &gi);
+        const double mz = P.ew * t2;
+        coords(K, P, t0, t1, mz, m);
+        interp_emit<MG, FOM, MODE>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
+        if (K.ewald) {
+            coords(K, P, t0, t1, -mz, m);
+            interp_emit<MG, FOM, MODE>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
+        }
     }
 }
 
 // Column tasks: task = (particle, x) with x in [0, FT/2]; G tasks per WG.
-template <int FT>
-__global__ __launch_bounds__(256) void k_cols_insert(const float2* __restrict__ inter,
-                                                     const float* __restrict__ bkg,
-                                                     const PartDev* __restrict__ recs, int nb,
-                                                     int box, ReconConst K,
-                                                     float* __restrict__ acc,
-                                                     const float2* __restrict__ twg) {
+// MODE_COUNT runs the geometry only (no transform) with the same lane layout
+// as MODE_FILL, so both passes make identical emission decisions.
+template <int FT, bool FOM, int MODE>
+__global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
+                                              const float* __restrict__ bkg,
+                                              const PartDev* __restrict__ recs, int nb, int box,
+                                              ReconConst K, float* __restrict__ acc,
+                                              const float2* __restrict__ twg, BinCtx bc) {
     constexpr int T = FftThreads<FT>::value;
     constexpr int G = 256 / T;
     constexpr int NT = G * T;
     constexpr int HX = FT / 2 + 1;
-    __shared__ float2 lds[G * FT + FT];
-    float2* tw = lds + G * FT;
+    constexpr int MG = T < 16 ? T : 16;   // run-merge group: lanes of one column
+    constexpr int LDSN = MODE == MODE_COUNT ? 1 : G * FT + FT;
+    __shared__ float2 lds[LDSN];
     const int tid = threadIdx.x;
-    for (int i = tid; i < FT; i += NT) tw[i] = twg[i];
     const int g = tid / T, t = tid - g * T;
     const long task = (long)blockIdx.x * G + g;
     const int p = (int)(task / HX);
     const int x = (int)(task - (long)p * HX);
     const bool valid = p < nb;
-    float2* line = lds + g * FT;
-    const float2* col = inter + ((size_t)p * HX + x) * box;
-    for (int i = t; i < FT; i += T) line[i] = (valid && i < box) ? col[i] : make_float2(0.f, 0.f);
-    __syncthreads();
-    FftLine<FT, -1, T>::run(line, tw, t);
-    if (!valid) return;
-    const PartDev& P = recs[p];
-    float* a = acc + (size_t)P.half * K.N * K.N * K.N * K.stride;
+    float2* line = lds + (MODE == MODE_COUNT ? 0 : g * FT);
+    if (MODE != MODE_COUNT) {
+        float2* tw = lds + G * FT;
+        for (int i = tid; i < FT; i += NT) tw[i] = twg[i];
+        const float2* col = inter + ((size_t)p * HX + x) * box;
+        for (int i = t; i < FT; i += T) line[i] = (valid && i < box) ? col[i] : make_float2(0.f, 0.f);
+        __syncthreads();
+        FftLine<FT, -1, T>::run(line, tw, t);
+    }
+    const PartDev& P = recs[valid ? p : 0];
+    const unsigned keybase = (unsigned)P.half * (unsigned)(K.NT * K.NT * K.NT) * 4096u;
     const float inv = 1.0f / FT;
-    const float b = bkg[p];
+    const float b = valid ? bkg[p] : 0.f;
     const long h = (x <= (FT - 1) / 2) ? x : x - FT;
-    const bool mirror = x >= 1 && x <= FT / 2 - 1;
+    const bool mirror = valid && x >= 1 && x <= FT / 2 - 1;
+    const bool merge = K.merge;
 #pragma unroll 1
     for (int r = 0; r < 8; ++r) {
         const int y = t + r * T;
-        float2 F = line[y];
-        F.x *= inv;
-        F.y *= inv;
-        if (x == 0 && y == 0) F.x += b * FT;
+        float2 F = make_float2(0.f, 0.f);
+        if (MODE != MODE_COUNT) {
+            F = line[y];
+            F.x *= inv;
+            F.y *= inv;
+            if (x == 0 && y == 0) F.x += b * FT;
+        }
         const long k = (y <= (FT - 1) / 2) ? y : y - FT;
-        insert_sample(a, K, P, h, k, F);
-        if (mirror) {
-            const int ym = (FT - y) & (FT - 1);
-            const long km = (ym <= (FT - 1) / 2) ? ym : ym - FT;
-            insert_sample(a, K, P, -h, km, make_float2(F.x, -F.y));
+        // Friedel mate of the column: array position (FT-x, FT-y), value conj
+        const int ym = (FT - y) & (FT - 1);
+        const long km = (ym <= (FT - 1) / 2) ? ym : ym - FT;
+        insert_pair<MG, FOM, MODE>(acc, K, P, valid, mirror, h, k, km, F, merge, keybase, bc);
+    }
+}
+
+// Bins are processed in pieces of at most PIECE entries so that the few hot
+// bins around the origin (every plane passes through the low-frequency core)
+// are spread over many workgroups.
+#define BSX_PIECE 16384
+
+// Exclusive scans of the per-bin counts (one workgroup): entry offsets
+// offsets[0..nbins] and piece offsets poff[0..nbins]; cursors = offsets;
+// counts cleared for the next batch.
+__global__ __launch_bounds__(1024) void k_scan(int* __restrict__ counts, int nbins,
+                                               int* __restrict__ offsets,
+                                               int* __restrict__ cursors,
+                                               int* __restrict__ poff) {
+    __shared__ int part[1024], ppart[1024];
+    const int tid = threadIdx.x;
+    const int per = (nbins + 1023) / 1024;
+    const int b0 = tid * per, b1 = min(nbins, b0 + per);
+    int sum = 0, psum = 0;
+    for (int i = b0; i < b1; ++i) {
+        sum += counts[i];
+        psum += (counts[i] + BSX_PIECE - 1) / BSX_PIECE;
+    }
+    part[tid] = sum;
+    ppart[tid] = psum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = tid >= o ? part[tid - o] : 0;
+        const int pv = tid >= o ? ppart[tid - o] : 0;
+        __syncthreads();
+        part[tid] += v;
+        ppart[tid] += pv;
+        __syncthreads();
+    }
+    int run = part[tid] - sum, prun = ppart[tid] - psum;
+    for (int i = b0; i < b1; ++i) {
+        const int c = counts[i];
+        offsets[i] = run;
+        cursors[i] = run;
+        poff[i] = prun;
+        run += c;
+        prun += (c + BSX_PIECE - 1) / BSX_PIECE;
+        counts[i] = 0;
+    }
+    if (tid == 1023) {
+        offsets[nbins] = part[1023];
+        poff[nbins] = ppart[1023];
+    }
+}
+
+// One workgroup per piece of a (half, tile) bin: sum its entries in LDS, then
+// add the tile partial to the volume — a plain read-modify-write when the bin
+// is a single piece, float atomics per touched record for multi-piece bins.
+// Each thread keeps 8 entry loads in flight before its LDS adds.
+__global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ offsets,
+                                                    const int* __restrict__ poff, int nbins,
+                                                    const float4* __restrict__ bins, int cap,
+                                                    float* __restrict__ acc) {
+    const int piece = blockIdx.x;
+    if (piece >= poff[nbins]) return;
+    int lo = 0, hi = nbins - 1;   // last bin with poff[bin] <= piece
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (poff[mid] <= piece) lo = mid;
+        else hi = mid - 1;
+    }
+    const int bin = lo;
+    const int j = piece - poff[bin];
+    const bool single = poff[bin + 1] - poff[bin] == 1;
+    const int beg = offsets[bin] + j * BSX_PIECE;
+    const int end = min(min(offsets[bin + 1], beg + BSX_PIECE), cap);
+    if (beg >= end) return;
+    __shared__ float sre[4096], sim[4096], sw[4096];
+    float* g = acc + (size_t)bin * 4096 * 3;
+    for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
+        sre[i] = 0.f;
+        sim[i] = 0.f;
+        sw[i] = 0.f;
+    }
+    __syncthreads();
+    constexpr int U = 8;
+    const int stride = blockDim.x;
+    for (int e0 = beg + threadIdx.x; e0 < end; e0 += U * stride) {
+        float4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * stride;
+            v[u] = e < end ? bins[e] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int li = __float_as_int(v[u].w);
+            if (li >= 0) {
+                atomicAdd(&sre[li], v[u].x);
+                atomicAdd(&sim[li], v[u].y);
+                atomicAdd(&sw[li], v[u].z);
+            }
+        }
+    }
+    __syncthreads();
+    if (single) {
+        // 12288 floats of the tile as 3072 float4: all loads, then all stores
+        float4* g4 = reinterpret_cast<float4*>(g);
+        constexpr int NV = 3072 / 512;
+        float4 cur[NV];
+#pragma unroll
+        for (int u = 0; u < NV; ++u) cur[u] = g4[threadIdx.x + u * 512];
+#pragma unroll
+        for (int u = 0; u < NV; ++u) {
+            const int f = 4 * (threadIdx.x + u * 512);
+            float add[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int li = (f + q) / 3, c = (f + q) - li * 3;
+                add[q] = c == 0 ? sre[li] : c == 1 ? sim[li] : sw[li];
+            }
+            cur[u].x += add[0];
+            cur[u].y += add[1];
+            cur[u].z += add[2];
+            cur[u].w += add[3];
+            g4[threadIdx.x + u * 512] = cur[u];
+        }
+    } else {
+        for (int li = threadIdx.x; li < 4096; li += blockDim.x) {
+            if (sw[li] != 0.f || sre[li] != 0.f || sim[li] != 0.f) {
+                unsafeAtomicAdd(g + 3 * li + 0, sre[li]);
+                unsafeAtomicAdd(g + 3 * li + 1, sim[li]);
+                unsafeAtomicAdd(g + 3 * li + 2, sw[li]);
+            }
         }
     }
 }
@@ -310,7 +636,8 @@ __global__ __launch_bounds__(256) void k_cols_insert(const float2* __restrict__ 
 // (phase_shift_to_center: x (-1)^(h+k+l) for even N, computed exactly via the
 // phase for odd N), output complex volume.
 __global__ __launch_bounds__(256) void k_weigh(const float* __restrict__ acc, long long n3, int N,
-                                               int stride, int hmask, int nhalves, int fom,
+                                               int NT, long long nrec, int stride, int hmask,
+                                               int nhalves, int fom,
                                                float2* __restrict__ out,
                                                float* __restrict__ fom_out,
                                                unsigned long long* __restrict__ cov,
@@ -320,9 +647,10 @@ __global__ __launch_bounds__(256) void k_weigh(const float* __restrict__ acc, lo
     int cloc = 0;
     for (; i < n3; i += (long long)gridDim.x * blockDim.x) {
         double fr = 0, fi = 0, W = 0, P = 0, W2 = 0;
+        const long long ri = rec_index(i % N, (i / N) % N, i / ((long long)N * N), NT);
         for (int hh = 0; hh < nhalves; ++hh) {
             if (!((hmask >> hh) & 1)) continue;
-            const float* r = acc + ((size_t)hh * n3 + i) * stride;
+            const float* r = acc + ((size_t)hh * nrec + ri) * stride;
             fr += r[0];
             fi += r[1];
             W += r[2];
@@ -419,13 +747,14 @@ __global__ __launch_bounds__(256) void k_friedel(const float2* __restrict__ v, i
         for (int q = 0; q < 3; ++q) atomicAdd(sums + q, sh[q][0]);
 }
 
-// Copy raw accumulator planes.
-__global__ void k_planes(const float* __restrict__ acc, long long n3, int stride,
-                         float* __restrict__ out) {
+// Raw accumulator planes of one half set in (z, y, x) order.
+__global__ void k_planes(const float* __restrict__ acc, long long n3, int N, int NT, int stride,
+                         int fom, float* __restrict__ out) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
          i += (long long)gridDim.x * blockDim.x) {
-        const float* r = acc + i * stride;
-        for (int q = 0; q < 5; ++q) out[q * n3 + i] = q < stride ? r[q] : 0.f;
+        const long long ri = rec_index(i % N, (i / N) % N, i / ((long long)N * N), NT);
+        for (int q = 0; q < 5; ++q)
+            out[q * n3 + i] = (q < 3 || fom) ? acc[ri * stride + q] : 0.f;
     }
 }
 
@@ -435,6 +764,7 @@ struct bsx_recon {
     ReconConst K;
     double hi_res;
     int dev;
+    bool binned = true;            // tile-binned insertion (else direct atomics)
     float* acc = nullptr;
     size_t acc_elems = 0;
     float2* inter = nullptr;
@@ -442,16 +772,43 @@ struct bsx_recon {
     PartDev* recs = nullptr;
     int* nonfinite = nullptr;
     int64_t recs_cap = 0;
+    int* counts = nullptr;         // per (half, tile) bin
+    int* offsets = nullptr;
+    int* cursors = nullptr;
+    int* poff = nullptr;           // piece offsets per bin
+    float4* bins = nullptr;
+    size_t bins_cap = 0;
+    int nbins = 0;
     int batch = 0;
     int64_t inserted[2] = {0, 0};
-    double last_insert_ms = 0, last_stage_ms = 0;
+    double ktime[7] = {0, 0, 0, 0, 0, 0, 0};   // background rows count scan fill tile direct
     int64_t last_launches = 0;
     std::vector<hipEvent_t> evs;
+    std::vector<int> evlab;
+    size_t nev = 0;
     size_t bytes = 0;
     PartDev* hrecs = nullptr;      // pinned staging of the records
     int64_t hrecs_cap = 0;
     hipEvent_t copied = nullptr;   // staging buffer reusable once this completes
+    bool timing = false;
 };
+
+enum { KT_BG = 0, KT_ROWS, KT_COUNT, KT_SCAN, KT_FILL, KT_TILE, KT_DIRECT, KT_END };
+
+// Timing marks: an event after each kernel, labelled with the kernel it closes.
+static int tmark(bsx_recon* h, hipStream_t s, int label) {
+    if (!h->timing) return BSX_OK;
+    if (h->nev >= h->evs.size()) {
+        hipEvent_t e;
+        BSX_HIP(hipEventCreate(&e));
+        h->evs.push_back(e);
+        h->evlab.push_back(0);
+    }
+    h->evlab[h->nev] = label;
+    BSX_HIP(hipEventRecord(h->evs[h->nev], s));
+    ++h->nev;
+    return BSX_OK;
+}
 
 static double check_resolution(double res, int n, double sampling) {
     // Bimage::check_resolution (Bimage.cpp:633-648), cubic 3D map
@@ -528,8 +885,7 @@ static void make_rec(const bsx_recon* h, const bsx_particle& q, PartDev& r) {
 }
 
 template <int FT>
-static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hipStream_t s,
-                        hipEvent_t e0, hipEvent_t e1) {
+static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hipStream_t s) {
     constexpr int T = FftThreads<FT>::value;
     constexpr int G = 256 / T;
     constexpr int LINES = G >= 8 ? G : 8;   // 16 rows per row-tile at FT = 512
@@ -538,27 +894,56 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
     const float2* tw = twiddles(FT, s);
     BSX_REQUIRE(tw, BSX_ERR_DEVICE, "twiddles(%d) failed", FT);
     const int tiles = (box + 2 * LINES - 1) / (2 * LINES);
-    hipLaunchKernelGGL((k_rows<FT, LINES>), dim3(nb * tiles), dim3(NT), 0, s,
-                       imgs + (size_t)p0 * box * box, h->bkg + p0, box, tiles, h->inter, tw);
+    hipLaunchKernelGGL((k_rows<FT, LINES>), dim3(nb * tiles), dim3(NT), 0, s, imgs,
+                       h->recs + p0, h->bkg + p0, box, tiles, h->inter, tw);
     BSX_HIP(hipGetLastError());
+    int r = tmark(h, s, KT_ROWS);
+    if (r) return r;
     const long ntask = (long)nb * (FT / 2 + 1);
-    const long nblk = (ntask + G - 1) / G;
-    if (e0) BSX_HIP(hipEventRecord(e0, s));
-    hipLaunchKernelGGL((k_cols_insert<FT>), dim3((unsigned)nblk), dim3(NT), 0, s, h->inter,
-                       h->bkg + p0, h->recs + p0, nb, box, h->K, h->acc, tw);
-    if (e1) BSX_HIP(hipEventRecord(e1, s));
-    BSX_HIP(hipGetLastError());
+    const dim3 grid((unsigned)((ntask + G - 1) / G));
+    const PartDev* rec = h->recs + p0;
+    const float* bk = h->bkg + p0;
+    if (h->binned) {
+        BinCtx bc{h->counts, nullptr, (int)h->bins_cap};
+        hipLaunchKernelGGL((k_cols<FT, false, MODE_COUNT>), grid, dim3(NT), 0, s, h->inter, bk,
+                           rec, nb, box, h->K, h->acc, tw, bc);
+        BSX_HIP(hipGetLastError());
+        if ((r = tmark(h, s, KT_COUNT))) return r;
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, h->counts, h->nbins, h->offsets,
+                           h->cursors, h->poff);
+        BSX_HIP(hipGetLastError());
+        if ((r = tmark(h, s, KT_SCAN))) return r;
+        BinCtx bf{h->cursors, h->bins, (int)h->bins_cap};
+        hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL>), grid, dim3(NT), 0, s, h->inter, bk,
+                           rec, nb, box, h->K, h->acc, tw, bf);
+        BSX_HIP(hipGetLastError());
+        if ((r = tmark(h, s, KT_FILL))) return r;
+        const unsigned maxpieces = (unsigned)(h->nbins + h->bins_cap / BSX_PIECE + 1);
+        hipLaunchKernelGGL(k_tile_accum, dim3(maxpieces), dim3(512), 0, s, h->offsets, h->poff,
+                           h->nbins, h->bins, (int)h->bins_cap, h->acc);
+        BSX_HIP(hipGetLastError());
+        if ((r = tmark(h, s, KT_TILE))) return r;
+    } else {
+        BinCtx bc{nullptr, nullptr, 0};
+        if (h->K.fom)
+            hipLaunchKernelGGL((k_cols<FT, true, MODE_DIRECT>), grid, dim3(NT), 0, s, h->inter,
+                               bk, rec, nb, box, h->K, h->acc, tw, bc);
+        else
+            hipLaunchKernelGGL((k_cols<FT, false, MODE_DIRECT>), grid, dim3(NT), 0, s, h->inter,
+                               bk, rec, nb, box, h->K, h->acc, tw, bc);
+        BSX_HIP(hipGetLastError());
+        if ((r = tmark(h, s, KT_DIRECT))) return r;
+    }
     return BSX_OK;
 }
 
-static int run_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hipStream_t s,
-                     hipEvent_t e0, hipEvent_t e1) {
+static int run_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hipStream_t s) {
     switch (h->p.ft_size) {
-        case 32: return launch_batch<32>(h, imgs, p0, nb, s, e0, e1);
-        case 64: return launch_batch<64>(h, imgs, p0, nb, s, e0, e1);
-        case 128: return launch_batch<128>(h, imgs, p0, nb, s, e0, e1);
-        case 256: return launch_batch<256>(h, imgs, p0, nb, s, e0, e1);
-        case 512: return launch_batch<512>(h, imgs, p0, nb, s, e0, e1);
+        case 32: return launch_batch<32>(h, imgs, p0, nb, s);
+        case 64: return launch_batch<64>(h, imgs, p0, nb, s);
+        case 128: return launch_batch<128>(h, imgs, p0, nb, s);
+        case 256: return launch_batch<256>(h, imgs, p0, nb, s);
+        case 512: return launch_batch<512>(h, imgs, p0, nb, s);
         default:
             set_error("ft_size %d not supported (32..512, power of 2)", h->p.ft_size);
             return BSX_ERR_ARG;
@@ -606,7 +991,8 @@ int bsx_part_ft_size(int xsize, double scale, int pad_factor) {
 int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     BSX_REQUIRE(out && p, BSX_ERR_ARG, "bsx_recon_create: null argument");
     *out = nullptr;
-    BSX_REQUIRE(p->size >= 2 && p->size <= 1024, BSX_ERR_ARG, "size %d out of range", p->size);
+    BSX_REQUIRE(p->size >= 2 && p->size <= 512, BSX_ERR_ARG, "size %d out of range 2..512",
+                p->size);
     BSX_REQUIRE(fft_length_supported(p->size), BSX_ERR_ARG,
                 "map size %d: inverse FFT length not supported", p->size);
     BSX_REQUIRE(p->box >= 2 && p->ft_size >= p->box, BSX_ERR_ARG, "box %d / ft_size %d invalid",
@@ -616,6 +1002,10 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     BSX_REQUIRE(p->interp >= 0 && p->interp <= 2, BSX_ERR_ARG, "interp %d invalid", p->interp);
     BSX_REQUIRE(p->nhalves == 1 || p->nhalves == 2, BSX_ERR_ARG, "nhalves must be 1 or 2");
     BSX_REQUIRE(p->sampling > 0, BSX_ERR_ARG, "sampling must be positive");
+    BSX_REQUIRE(p->method == 0 || p->method == 1, BSX_ERR_ARG,
+                "method must be 0 (binned) or 1 (direct atomics)");
+    BSX_REQUIRE(p->order >= 0 && p->order <= 2, BSX_ERR_ARG, "order must be 0, 1 or 2");
+    BSX_REQUIRE(p->merge >= 0 && p->merge <= 2, BSX_ERR_ARG, "merge must be 0, 1 or 2");
     bsx_recon* h = new bsx_recon();
     h->p = *p;
     BSX_HIP(hipGetDevice(&h->dev));
@@ -626,6 +1016,10 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     K.box = p->box;
     K.fom = (p->flags & BSX_FLAG_FOM) ? 1 : 0;
     K.stride = K.fom ? 5 : 3;
+    K.NT = (p->size + 15) / 16;
+    // tile-binned insertion for the lean accumulators; the FOM sums use the
+    // direct atomic path
+    h->binned = p->method == 0 && !K.fom;
     K.interp = p->interp;
     K.nhalves = p->nhalves;
     K.ctf_mode = p->ctf_action == 0 ? 0 : p->ctf_action == 1 ? 1
@@ -636,31 +1030,51 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     K.maxs2 = 1.0 / (hi * hi);
     K.mscale = p->size * p->sampling;
     K.shi2 = 1.0 / (h->hi_res * h->hi_res);          // img_ctf_calculate hires
-    const size_t n3 = (size_t)p->size * p->size * p->size;
-    h->acc_elems = n3 * K.stride * p->nhalves;
+    K.ewald = (p->flags & BSX_FLAG_EWALD) ? 1 : 0;
+    K.merge = 0;
+    const size_t ntile = (size_t)K.NT * K.NT * K.NT;
+    const size_t nrec = ntile * 4096;   // records incl. tile padding
+    h->acc_elems = nrec * K.stride * p->nhalves;
+    h->nbins = (int)(ntile * p->nhalves);
+    const size_t ft2 = (size_t)p->ft_size * p->ft_size;
+    const int mult = (K.ewald ? 2 : 1) * (p->interp == 2 ? 8 : 1);
+    const size_t inter_pp = (size_t)(p->ft_size / 2 + 1) * p->box * sizeof(float2);
+    const size_t bins_pp = h->binned ? ft2 * mult * sizeof(float4) : 0;
     int batch = p->batch;
     if (batch <= 0) {
-        // intermediate half-spectra of ~128 MB per batch
-        size_t per = (size_t)(p->ft_size / 2 + 1) * p->box * sizeof(float2);
-        batch = (int)std::max<size_t>(1, std::min<size_t>(4096, (128u << 20) / per));
+        // intermediate half-spectra within 512 MB and bin entries within 2 GB
+        size_t b = (512u << 20) / inter_pp;
+        if (bins_pp) b = std::min(b, (size_t)(2ull << 30) / bins_pp);
+        batch = (int)std::max<size_t>(1, std::min<size_t>(4096, b));
     }
     h->batch = batch;
-    size_t inter_bytes = (size_t)batch * (p->ft_size / 2 + 1) * p->box * sizeof(float2);
-    h->bytes = h->acc_elems * sizeof(float) + inter_bytes;
+    const size_t inter_bytes = (size_t)batch * inter_pp;
+    h->bins_cap = (size_t)batch * ft2 * mult;
+    BSX_REQUIRE(!h->binned || h->bins_cap < (1ull << 31), BSX_ERR_ARG,
+                "batch %d too large for 32-bit bin offsets", batch);
+    const size_t bins_bytes = h->binned ? h->bins_cap * sizeof(float4) : 0;
+    const size_t bin_idx_bytes = h->binned ? (4 * (size_t)h->nbins + 2) * sizeof(int) : 0;
+    h->bytes = h->acc_elems * sizeof(float) + inter_bytes + bins_bytes + bin_idx_bytes;
     if (p->mem_cap > 0 && (int64_t)h->bytes > p->mem_cap) {
         set_error("device memory %zu bytes exceeds the cap %lld", h->bytes, (long long)p->mem_cap);
         delete h;
         return BSX_ERR_NOMEM;
     }
-    if (hipMalloc(&h->acc, h->acc_elems * sizeof(float)) != hipSuccess ||
-        hipMalloc(&h->inter, inter_bytes) != hipSuccess ||
-        hipMalloc(&h->nonfinite, sizeof(int)) != hipSuccess) {
+    bool ok = hipMalloc(&h->acc, h->acc_elems * sizeof(float)) == hipSuccess &&
+              hipMalloc(&h->inter, inter_bytes) == hipSuccess &&
+              hipMalloc(&h->nonfinite, sizeof(int)) == hipSuccess;
+    if (ok && h->binned)
+        ok = hipMalloc(&h->bins, bins_bytes) == hipSuccess &&
+             hipMalloc(&h->counts, sizeof(int) * h->nbins) == hipSuccess &&
+             hipMalloc(&h->offsets, sizeof(int) * (h->nbins + 1)) == hipSuccess &&
+             hipMalloc(&h->cursors, sizeof(int) * h->nbins) == hipSuccess &&
+             hipMalloc(&h->poff, sizeof(int) * (h->nbins + 1)) == hipSuccess;
+    if (!ok) {
         set_error("device allocation of %zu bytes failed", h->bytes);
-        if (h->acc) (void)hipFree(h->acc);
-        if (h->inter) (void)hipFree(h->inter);
-        delete h;
+        bsx_recon_destroy(h);
         return BSX_ERR_NOMEM;
     }
+    if (h->binned) BSX_HIP(hipMemset(h->counts, 0, sizeof(int) * h->nbins));
     BSX_HIP(hipMemset(h->acc, 0, h->acc_elems * sizeof(float)));
     BSX_HIP(hipDeviceSynchronize());
     *out = h;
@@ -674,6 +1088,11 @@ int bsx_recon_destroy(bsx_recon* h) {
     if (h->bkg) (void)hipFree(h->bkg);
     if (h->recs) (void)hipFree(h->recs);
     if (h->nonfinite) (void)hipFree(h->nonfinite);
+    if (h->bins) (void)hipFree(h->bins);
+    if (h->counts) (void)hipFree(h->counts);
+    if (h->offsets) (void)hipFree(h->offsets);
+    if (h->cursors) (void)hipFree(h->cursors);
+    if (h->poff) (void)hipFree(h->poff);
     for (auto e : h->evs) (void)hipEventDestroy(e);
     if (h->copied) {
         (void)hipEventSynchronize(h->copied);
@@ -745,62 +1164,79 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
         BSX_HIP(hipHostMalloc((void**)&h->hrecs, sizeof(PartDev) * n, hipHostMallocDefault));
         h->hrecs_cap = n;
     }
-    for (int64_t i = 0; i < n; ++i) make_rec(h, parts[i], h->hrecs[i]);
+    // Insertion order: by plane orientation (normal folded to one hemisphere,
+    // 2-degree polar bands, snake order in azimuth) so that particles in flight
+    // together touch the same voxel lines; the result differs only by FP32
+    // summation order.
+    std::vector<int64_t> perm(n);
+    for (int64_t i = 0; i < n; ++i) perm[i] = i;
+    // auto: input order for the binned path (sorted orders concentrate the
+    // bin-counter increments of a batch on fewer counters), sorted for the
+    // direct path (neighbouring planes share cache lines)
+    const int order = h->p.order ? h->p.order : (h->binned ? 1 : 2);
+    if (order == 2 && n > 1) {
+        std::vector<double> key(n);
+        for (int64_t i = 0; i < n; ++i) {
+            double v0 = parts[i].view[0], v1 = parts[i].view[1], v2 = parts[i].view[2];
+            double nn = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+            if (nn < 1e-30) { v0 = v1 = 0; v2 = 1; nn = 1; }
+            v0 /= nn; v1 /= nn; v2 /= nn;
+            if (v2 < 0) { v0 = -v0; v1 = -v1; v2 = -v2; }
+            const double th = acos(std::min(1.0, v2)), ph = atan2(v1, v0);
+            const int band = (int)(th / (2 * M_PI / 180));
+            const double az = (band & 1) ? -ph : ph;
+            key[i] = band * 10.0 + (az + M_PI);
+        }
+        std::stable_sort(perm.begin(), perm.end(),
+                         [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+    }
+    double max_step = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        make_rec(h, parts[perm[i]], h->hrecs[i]);
+        h->hrecs[i].src = perm[i];
+        max_step = std::max(max_step, h->hrecs[i].invscale * h->K.mscale);
+    }
+    // run merge: equal voxels are contiguous within a merge group of up to 16
+    // consecutive samples when 15 steps stay shorter than N - 1 voxels
+    {
+        const int T = h->p.ft_size / 8, MG = T < 16 ? T : 16;
+        const bool safe = !h->K.ewald && (MG - 1) * max_step < h->K.N - 1;
+        h->K.merge = (h->p.merge != 2) && safe ? 1 : 0;
+    }
     BSX_HIP(hipMemcpyAsync(h->recs, h->hrecs, sizeof(PartDev) * n, hipMemcpyHostToDevice, s));
     BSX_HIP(hipEventRecord(h->copied, s));
     const bool check_nan = h->p.flags & BSX_FLAG_CHECK_NAN;
     if (check_nan) BSX_HIP(hipMemsetAsync(h->nonfinite, 0, sizeof(int), s));
-    const bool timing = h->p.flags & BSX_FLAG_TIMING;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    if (timing) {
-        while (h->evs.size() < 2) {
-            hipEvent_t e;
-            BSX_HIP(hipEventCreate(&e));
-            h->evs.push_back(e);
-        }
-        t0 = h->evs[0];
-        t1 = h->evs[1];
-        BSX_HIP(hipEventRecord(t0, s));
-    }
+    h->timing = h->p.flags & BSX_FLAG_TIMING;
+    h->nev = 0;
+    int r = tmark(h, s, -1);
+    if (r) return r;
     hipLaunchKernelGGL(k_background, dim3((unsigned)n), dim3(256), 0, s, imgs_dev, h->recs,
                        h->p.box, h->bkg, check_nan ? h->nonfinite : nullptr);
     BSX_HIP(hipGetLastError());
+    if ((r = tmark(h, s, KT_BG))) return r;
     if (check_nan) {
         int bad = 0;
         BSX_HIP(hipMemcpyAsync(&bad, h->nonfinite, sizeof(int), hipMemcpyDeviceToHost, s));
         BSX_HIP(hipStreamSynchronize(s));
         BSX_REQUIRE(!bad, BSX_ERR_NONFINITE, "non-finite values in the particle images");
     }
-    // per-batch kernel timing of the insertion kernel
     const int64_t nbatches = (n + h->batch - 1) / h->batch;
-    if (timing) {
-        while ((int64_t)h->evs.size() < 2 + 2 * nbatches) {
-            hipEvent_t e;
-            BSX_HIP(hipEventCreate(&e));
-            h->evs.push_back(e);
-        }
-    }
     for (int64_t b = 0; b < nbatches; ++b) {
         const int64_t p0 = b * h->batch;
         const int nb = (int)std::min<int64_t>(h->batch, n - p0);
-        int r = run_batch(h, imgs_dev, p0, nb, s, timing ? h->evs[2 + 2 * b] : nullptr,
-                          timing ? h->evs[3 + 2 * b] : nullptr);
-        if (r != BSX_OK) return r;
+        if ((r = run_batch(h, imgs_dev, p0, nb, s))) return r;
     }
     for (int64_t i = 0; i < n; ++i) h->inserted[h->p.nhalves > 1 ? (parts[i].half & 1) : 0]++;
-    if (timing) {
-        BSX_HIP(hipEventRecord(t1, s));
+    h->last_launches = nbatches;
+    if (h->timing) {
         BSX_HIP(hipStreamSynchronize(s));
-        float tot = 0, ins = 0;
-        BSX_HIP(hipEventElapsedTime(&tot, t0, t1));
-        for (int64_t b = 0; b < nbatches; ++b) {
+        for (int q = 0; q < KT_END; ++q) h->ktime[q] = 0;
+        for (size_t i = 1; i < h->nev; ++i) {
             float ms = 0;
-            BSX_HIP(hipEventElapsedTime(&ms, h->evs[2 + 2 * b], h->evs[3 + 2 * b]));
-            ins += ms;
+            BSX_HIP(hipEventElapsedTime(&ms, h->evs[i - 1], h->evs[i]));
+            h->ktime[h->evlab[i]] += ms;
         }
-        h->last_insert_ms = ins;
-        h->last_stage_ms = tot - ins;
-        h->last_launches = nbatches;
     }
     return BSX_OK;
 }
@@ -808,9 +1244,16 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
 int bsx_recon_last_timing(const bsx_recon* h, double* insert_ms, int64_t* launches,
                           double* stage_ms) {
     BSX_REQUIRE(h, BSX_ERR_ARG, "null handle");
-    if (insert_ms) *insert_ms = h->last_insert_ms;
+    const double* t = h->ktime;
+    if (insert_ms) *insert_ms = t[KT_COUNT] + t[KT_SCAN] + t[KT_FILL] + t[KT_TILE] + t[KT_DIRECT];
     if (launches) *launches = h->last_launches;
-    if (stage_ms) *stage_ms = h->last_stage_ms;
+    if (stage_ms) *stage_ms = t[KT_BG] + t[KT_ROWS];
+    return BSX_OK;
+}
+
+int bsx_recon_kernel_times(const bsx_recon* h, double* ms, int n) {
+    BSX_REQUIRE(h && ms && n >= 0, BSX_ERR_ARG, "null argument");
+    for (int q = 0; q < n && q < KT_END; ++q) ms[q] = h->ktime[q];
     return BSX_OK;
 }
 
@@ -818,8 +1261,10 @@ int bsx_recon_accumulators(bsx_recon* h, int half, float* planes_dev, void* stre
     BSX_REQUIRE(h && planes_dev, BSX_ERR_ARG, "null argument");
     BSX_REQUIRE(half >= 0 && half < h->p.nhalves, BSX_ERR_ARG, "half %d out of range", half);
     const long long n3 = (long long)h->K.N * h->K.N * h->K.N;
+    const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
     hipLaunchKernelGGL(k_planes, dim3(1024), dim3(256), 0, (hipStream_t)stream,
-                       h->acc + (size_t)half * n3 * h->K.stride, n3, h->K.stride, planes_dev);
+                       h->acc + (size_t)half * nrec * h->K.stride, n3, h->K.N, h->K.NT,
+                       h->K.stride, h->K.fom, planes_dev);
     BSX_HIP(hipGetLastError());
     return BSX_OK;
 }
@@ -852,8 +1297,9 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
     BSX_HIP(hipMalloc((void**)&dcov, sizeof(unsigned long long) + 4 * sizeof(double)));
     dsum = (double*)(dcov + 1);
     BSX_HIP(hipMemsetAsync(dcov, 0, sizeof(unsigned long long) + 4 * sizeof(double), s));
-    hipLaunchKernelGGL(k_weigh, dim3(2048), dim3(256), 0, s, h->acc, n3, N, h->K.stride, hmask,
-                       h->p.nhalves, h->K.fom, vol, fom_dev, dcov, dsum);
+    const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
+    hipLaunchKernelGGL(k_weigh, dim3(2048), dim3(256), 0, s, h->acc, n3, N, h->K.NT, nrec,
+                       h->K.stride, hmask, h->p.nhalves, h->K.fom, vol, fom_dev, dcov, dsum);
     BSX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_friedel, dim3(1024), dim3(256), 0, s, vol, N, dsum + 1);
     BSX_HIP(hipGetLastError());

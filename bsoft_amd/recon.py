@@ -71,7 +71,8 @@ class Reconstruction:
 
     def __init__(self, size: int, box: int, ft_size: int, sampling: float, hi_res: float = 0.0,
                  lo_res: float = 0.0, interp: int = 0, ctf_action: int = 0, wiener: float = 0.0,
-                 flags: int = 0, nhalves: int = 1, batch: int = 0, mem_cap: int = 0):
+                 flags: int = 0, nhalves: int = 1, batch: int = 0, mem_cap: int = 0,
+                 method: int = 0, order: int = 0, merge: int = 0):
         if ctf_action > 2 and wiener < 0.01:            # breconstruct.cpp:187
             wiener = 0.2
         if hi_res < 0.1:                                  # breconstruct.cpp:236
@@ -79,7 +80,8 @@ class Reconstruction:
         self.params = ReconParams(size=size, box=box, ft_size=ft_size, interp=interp,
                                   sampling=sampling, hi_res=hi_res, lo_res=lo_res,
                                   ctf_action=ctf_action, flags=flags, wiener=wiener,
-                                  nhalves=nhalves, batch=batch, mem_cap=mem_cap)
+                                  nhalves=nhalves, batch=batch, mem_cap=mem_cap,
+                                  method=method, order=order, merge=merge)
         self.h = C.c_void_p()
         call("bsx_recon_create", C.byref(self.h), C.byref(self.params))
         self.size = size
@@ -113,6 +115,13 @@ class Reconstruction:
         nl = C.c_int64()
         call("bsx_recon_last_timing", self.h, C.byref(ms), C.byref(nl), C.byref(st))
         return ms.value, nl.value, st.value
+
+    KERNELS = ("background", "rows", "count", "scan", "fill", "tile", "direct")
+
+    def kernel_times(self) -> dict:
+        out = (C.c_double * 7)()
+        call("bsx_recon_kernel_times", self.h, out, 7)
+        return dict(zip(self.KERNELS, list(out)))
 
     def device_bytes(self) -> int:
         b = C.c_int64()

@@ -63,6 +63,10 @@ typedef struct bsx_recon_params {
     int32_t nhalves;       /* 1: one accumulator; 2: half-set accumulators        */
     int32_t batch;         /* particles per internal batch (0 = autotuned)        */
     int64_t mem_cap;       /* device-memory cap for this handle (bytes, 0 = none) */
+    int32_t method;        /* 0: tile-binned insertion (FOM sums use 1); 1: direct atomics */
+    int32_t order;         /* 0 auto, 1 input order, 2 plane-orientation order     */
+    int32_t merge;         /* 0 auto, 1 merge equal-voxel runs before atomics, 2 off */
+    int32_t reserved;
 } bsx_recon_params;
 
 /* One particle (Bparticle fields used on the path, include/mg_processing.h:101,
@@ -142,10 +146,14 @@ int bsx_recon_accumulators(bsx_recon* h, int half, float* planes_dev, void* stre
 /* Device bytes held by a handle. */
 int bsx_recon_device_bytes(const bsx_recon* h, int64_t* bytes);
 
-/* Last kernel timings of the dominant insertion kernel (ms, summed over the
- * launches of the previous bsx_recon_insert) and the number of launches. */
+/* Kernel timings of the previous bsx_recon_insert (needs BSX_FLAG_TIMING):
+ * insertion kernels (count, scan, fill, tile accumulate or direct), staging
+ * kernels (background, row transforms) in ms, and the number of batches. */
 int bsx_recon_last_timing(const bsx_recon* h, double* insert_ms, int64_t* launches,
                           double* stage_ms);
+/* Per-kernel totals of the previous insert (ms): background, rows, count,
+ * scan, fill, tile accumulate, direct insertion (n <= 7 values). */
+int bsx_recon_kernel_times(const bsx_recon* h, double* ms, int n);
 
 /* ---- FFT (fft_setup_plan / fftw, src/util/fft.cpp:31-114) ---- */
 /* In-place batched C2C transform of `batch` contiguous nx*ny*nz complex64

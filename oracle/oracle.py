@@ -135,6 +135,21 @@ class Oracle:
         assert k >= 0
         return out[:k]
 
+    def fsc_with_support(self, a: np.ndarray, b: np.ndarray, sampling, hi_res):
+        """FSC plus a mask of the shells that hold power in both volumes (the
+        FSC of an empty shell is undefined; the reference prints 0)."""
+        f = self.fsc(a, b, sampling, hi_res)
+        n = a.shape[0]
+        h = np.fft.fftfreq(n, 1.0 / n)
+        r = np.sqrt(h[:, None, None] ** 2 + h[None, :, None] ** 2 + h[None, None, :] ** 2)
+        ir = np.floor(r).astype(np.int64)
+        pa = np.bincount(ir.ravel(), weights=(np.abs(a) ** 2).ravel(), minlength=len(f) + 2)
+        pb = np.bincount(ir.ravel(), weights=(np.abs(b) ** 2).ravel(), minlength=len(f) + 2)
+        # shell i receives voxels with floor(r) = i and i-1 (linear split)
+        sa = pa[: len(f)] + np.concatenate([[0], pa[: len(f) - 1]])
+        sb = pb[: len(f)] + np.concatenate([[0], pb[: len(f) - 1]])
+        return f, (sa > 0) & (sb > 0)
+
     def fsi_kernel(self, width, power) -> np.ndarray:
         w = 1 if power < 0 else width
         t = np.zeros(513 * w)

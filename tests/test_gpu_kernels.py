@@ -18,19 +18,16 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("shape", [(16,), (512,), (24,), (384,), (7, 128), (3, 64, 256),
-                                   (2, 512, 512), (96, 32), (4, 192, 96), (32, 32, 32),
-                                   (64, 64, 64), (2, 48, 48, 48)])
+@pytest.mark.parametrize("shape,dims", [((16,), 1), ((512,), 1), ((24,), 1), ((384,), 1),
+                                        ((7, 128), 1), ((3, 64, 256), 2), ((2, 512, 512), 2),
+                                        ((96, 32), 2), ((4, 192, 96), 2), ((32, 32, 32), 3),
+                                        ((64, 64, 64), 3), ((2, 48, 48, 48), 3),
+                                        ((128, 256, 64), 3)])
 @pytest.mark.parametrize("direction", [-1, 1])
-def test_fft_vs_textbook(gpu, shape, direction):
-    say(f"fft shape={shape} dir={direction}")
+def test_fft_vs_textbook(gpu, shape, dims, direction):
+    say(f"fft shape={shape} dims={dims} dir={direction}")
     rng = np.random.default_rng(sum(shape))
     x = (rng.normal(size=shape) + 1j * rng.normal(size=shape)).astype(np.complex64)
-    dims = min(3, len(shape)) if len(shape) <= 3 else 3
-    if len(shape) == 2 and shape[0] in (7,):
-        dims = 1
-    if len(shape) == 3 and shape[0] in (3, 4):
-        dims = 2
     t = torch.from_numpy(x).cuda()
     recon.fft(t, dims, direction, norm=1)
     axes = tuple(range(len(shape) - dims, len(shape)))
