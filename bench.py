@@ -44,6 +44,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 1 GiB device copy (reference kernel for PMC traffic scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
@@ -84,33 +86,58 @@ def main() -> int:
         step()
         torch.cuda.synchronize()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
-    ins_ms, launches = 0.0, 0
+    kt_sum: dict = {}
+    launches = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         r = step()
-        ms, nl, _ = rec.last_timing()
-        ins_ms += ms
+        _, nl, _ = rec.last_timing()
         launches += nl
+        for k, v in rec.kernel_times().items():
+            kt_sum[k] = kt_sum.get(k, 0.0) + v
     torch.cuda.synchronize()
     T = time.perf_counter() - t0
     log(f"[bench] {args.steps} steps in {T:.3f} s; coverage {r.coverage}, friedel {r.friedel:.2e}")
+    log(f"[bench] kernel ms over the timed steps: " +
+        ", ".join(f"{k} {v:.1f}" for k, v in kt_sum.items()))
 
-    # roofline of the dominant kernel (column FFT + phase/CTF + insertion):
-    # algorithmic bytes per particle = half-spectrum column input
-    # (ft/2+1)*box*8 + per inserted sample the 12-byte (re, im, w) record read
-    # and written (24 B); NN inserts all ft^2 samples at the default resolution.
+    # Roofline. The insertion (the hot path) runs as count -> scan -> fill ->
+    # tile-accumulate kernels; the dominant one by time is reported with its
+    # algorithmic bytes: fill reads the half-spectrum columns ((ft/2+1)*box*8
+    # per particle) and writes one 16-byte bin entry per merged sample run;
+    # tile-accumulate reads the entries and read-modify-writes each touched
+    # 48-KB tile; count reads only particle records (compute/atomic bound).
     n_s = ft * ft
-    bytes_pp = (ft // 2 + 1) * box * 8 + 24 * n_s
-    bpl = rec.params.batch if rec.params.batch else None
-    avg_launch_ms = ins_ms / max(1, launches)
-    per_launch_parts = P * args.steps / max(1, launches)
-    achieved = bytes_pp * per_launch_parts / (avg_launch_ms * 1e-3) / 1e9
+    entries = kt_sum.get("entries", 0.0)
+    nbatch = max(1, launches)
+    col_bytes = (ft // 2 + 1) * box * 8
+    kernel_bytes = {
+        "fill": col_bytes * P * args.steps + 16 * entries,
+        "tile": 16 * entries + 2 * 48 * 1024 * (N // 16) ** 3 * nbatch,
+        "count": 200 * P * args.steps,
+        "direct": 32 * n_s * P * args.steps,
+    }
+    cand = {k: kt_sum.get(k, 0.0) for k in kernel_bytes}
+    dom = max(cand, key=cand.get)
+    dom_ms = cand[dom]
+    achieved = kernel_bytes[dom] / (dom_ms * 1e-3) / 1e9
+    ins_ms = sum(kt_sum.get(k, 0.0) for k in ("count", "scan", "fill", "tile", "direct"))
+    pipe_bytes = 32 * n_s * P * args.steps     # K3 NN insertion, 32 B per sample
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"k_cols_insert<{ft}>", "avg_launch_ms": round(avg_launch_ms, 4),
-                "particles_per_launch": round(per_launch_parts, 1),
-                "algorithmic_bytes_per_particle": bytes_pp}
+                "kernel": f"k_cols<{ft}, false, {2 if dom == 'fill' else 1}>" if dom in ("fill", "count")
+                else ("k_tile_accum" if dom == "tile" else f"k_cols<{ft}, false, 0>"),
+                "stage": dom,
+                "avg_launch_ms": round(dom_ms / nbatch, 4),
+                "launches": nbatch,
+                "algorithmic_bytes_per_launch": round(kernel_bytes[dom] / nbatch),
+                "bin_entries_per_particle": round(entries / (P * args.steps), 1),
+                "kernel_ms": {k: round(v, 2) for k, v in kt_sum.items() if k != "entries"},
+                "insertion_pipeline": {
+                    "ms": round(ins_ms, 2),
+                    "algorithmic_bytes_per_particle": 32 * n_s,
+                    "achieved_GBs": round(pipe_bytes / (ins_ms * 1e-3) / 1e9, 1)}}
 
     result = {
         "metric": "particles/s (search+recon, 256^3) learned-proposal vs plain GPU; FSC vs CPU",
@@ -128,6 +155,12 @@ def main() -> int:
         "cpu_baseline": None,
     }
 
+    if args.calib_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_cpu:
         result["cpu_baseline"], result["fsc_vs_cpu"] = cpu_baseline(args, ps, rec.params, ft)
     print(json.dumps(result), flush=True)

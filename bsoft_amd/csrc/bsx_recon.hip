@@ -510,7 +510,8 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
 __global__ __launch_bounds__(1024) void k_scan(int* __restrict__ counts, int nbins,
                                                int* __restrict__ offsets,
                                                int* __restrict__ cursors,
-                                               int* __restrict__ poff) {
+                                               int* __restrict__ poff,
+                                               unsigned long long* __restrict__ total) {
     __shared__ int part[1024], ppart[1024];
     const int tid = threadIdx.x;
     const int per = (nbins + 1023) / 1024;
@@ -544,6 +545,7 @@ __global__ __launch_bounds__(1024) void k_scan(int* __restrict__ counts, int nbi
     if (tid == 1023) {
         offsets[nbins] = part[1023];
         poff[nbins] = ppart[1023];
+        *total += (unsigned long long)part[1023];   // entries binned since the last reset
     }
 }
 
@@ -776,12 +778,14 @@ struct bsx_recon {
     int* offsets = nullptr;
     int* cursors = nullptr;
     int* poff = nullptr;           // piece offsets per bin
+    unsigned long long* entries = nullptr;   // bin entries of the current insert
     float4* bins = nullptr;
     size_t bins_cap = 0;
     int nbins = 0;
     int batch = 0;
     int64_t inserted[2] = {0, 0};
     double ktime[7] = {0, 0, 0, 0, 0, 0, 0};   // background rows count scan fill tile direct
+    double last_entries = 0;
     int64_t last_launches = 0;
     std::vector<hipEvent_t> evs;
     std::vector<int> evlab;
@@ -910,7 +914,7 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_COUNT))) return r;
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, h->counts, h->nbins, h->offsets,
-                           h->cursors, h->poff);
+                           h->cursors, h->poff, h->entries);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_SCAN))) return r;
         BinCtx bf{h->cursors, h->bins, (int)h->bins_cap};
@@ -1068,7 +1072,8 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
              hipMalloc(&h->counts, sizeof(int) * h->nbins) == hipSuccess &&
              hipMalloc(&h->offsets, sizeof(int) * (h->nbins + 1)) == hipSuccess &&
              hipMalloc(&h->cursors, sizeof(int) * h->nbins) == hipSuccess &&
-             hipMalloc(&h->poff, sizeof(int) * (h->nbins + 1)) == hipSuccess;
+             hipMalloc(&h->poff, sizeof(int) * (h->nbins + 1)) == hipSuccess &&
+             hipMalloc(&h->entries, sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
         set_error("device allocation of %zu bytes failed", h->bytes);
         bsx_recon_destroy(h);
@@ -1093,6 +1098,7 @@ int bsx_recon_destroy(bsx_recon* h) {
     if (h->offsets) (void)hipFree(h->offsets);
     if (h->cursors) (void)hipFree(h->cursors);
     if (h->poff) (void)hipFree(h->poff);
+    if (h->entries) (void)hipFree(h->entries);
     for (auto e : h->evs) (void)hipEventDestroy(e);
     if (h->copied) {
         (void)hipEventSynchronize(h->copied);
@@ -1209,6 +1215,7 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
     if (check_nan) BSX_HIP(hipMemsetAsync(h->nonfinite, 0, sizeof(int), s));
     h->timing = h->p.flags & BSX_FLAG_TIMING;
     h->nev = 0;
+    if (h->binned) BSX_HIP(hipMemsetAsync(h->entries, 0, sizeof(unsigned long long), s));
     int r = tmark(h, s, -1);
     if (r) return r;
     hipLaunchKernelGGL(k_background, dim3((unsigned)n), dim3(256), 0, s, imgs_dev, h->recs,
@@ -1237,6 +1244,10 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
             BSX_HIP(hipEventElapsedTime(&ms, h->evs[i - 1], h->evs[i]));
             h->ktime[h->evlab[i]] += ms;
         }
+        unsigned long long ne = 0;
+        if (h->binned)
+            BSX_HIP(hipMemcpy(&ne, h->entries, sizeof(ne), hipMemcpyDeviceToHost));
+        h->last_entries = (double)ne;
     }
     return BSX_OK;
 }
@@ -1254,6 +1265,7 @@ int bsx_recon_last_timing(const bsx_recon* h, double* insert_ms, int64_t* launch
 int bsx_recon_kernel_times(const bsx_recon* h, double* ms, int n) {
     BSX_REQUIRE(h && ms && n >= 0, BSX_ERR_ARG, "null argument");
     for (int q = 0; q < n && q < KT_END; ++q) ms[q] = h->ktime[q];
+    if (n > KT_END) ms[KT_END] = h->last_entries;
     return BSX_OK;
 }
 

@@ -116,11 +116,13 @@ class Reconstruction:
         call("bsx_recon_last_timing", self.h, C.byref(ms), C.byref(nl), C.byref(st))
         return ms.value, nl.value, st.value
 
-    KERNELS = ("background", "rows", "count", "scan", "fill", "tile", "direct")
+    KERNELS = ("background", "rows", "count", "scan", "fill", "tile", "direct", "entries")
 
     def kernel_times(self) -> dict:
-        out = (C.c_double * 7)()
-        call("bsx_recon_kernel_times", self.h, out, 7)
+        """Per-kernel milliseconds of the previous insert (BSX_FLAG_TIMING) and the
+        number of bin entries written."""
+        out = (C.c_double * 8)()
+        call("bsx_recon_kernel_times", self.h, out, 8)
         return dict(zip(self.KERNELS, list(out)))
 
     def device_bytes(self) -> int:

@@ -152,7 +152,8 @@ int bsx_recon_device_bytes(const bsx_recon* h, int64_t* bytes);
 int bsx_recon_last_timing(const bsx_recon* h, double* insert_ms, int64_t* launches,
                           double* stage_ms);
 /* Per-kernel totals of the previous insert (ms): background, rows, count,
- * scan, fill, tile accumulate, direct insertion (n <= 7 values). */
+ * scan, fill, tile accumulate, direct insertion; an 8th value receives the
+ * number of bin entries written (n <= 8 values). */
 int bsx_recon_kernel_times(const bsx_recon* h, double* ms, int n);
 
 /* ---- FFT (fft_setup_plan / fftw, src/util/fft.cpp:31-114) ---- */
