@@ -180,10 +180,18 @@ struct Emit {
     float re, im, w, p, w2;
 };
 
+// Each bin's counter is replicated BSX_SLOTS times (slot = workgroup index
+// mod BSX_SLOTS) so that concurrent reservations on a hot bin spread over
+// several addresses; a bin's slots are adjacent, so its entries stay
+// contiguous after the scan. The count and fill passes run the same grid, so
+// a workgroup uses the same slot in both.
+#define BSX_SLOTS 8
+
 struct BinCtx {
-    int* ctr;        // per-bin counters: counts (count pass) / cursors (fill pass)
+    int* ctr;        // per-(bin, slot) counters: counts (count pass) / cursors (fill pass)
     float4* bins;    // (re, im, w, record-in-tile)
     int cap;         // entries allocated (the count pass bounds the fill pass)
+    int slot;        // this workgroup's counter slot
 };
 
 // Merges runs of equal keys inside aligned groups of MG lanes (segmented
@@ -251,7 +259,7 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
             const int leader = __builtin_ctzll(active);
             const int b0 = __shfl(bin, leader, 64);
             const unsigned long long m = __ballot(out && bin == b0);
-            if (lane == leader) atomicAdd(bc.ctr + b0, __popcll(m));
+            if (lane == leader) atomicAdd(bc.ctr + b0 * BSX_SLOTS + bc.slot, __popcll(m));
             active &= ~m;
         }
         return;
@@ -263,7 +271,7 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
         const int b0 = __shfl(bin, leader, 64);
         const bool mine = out && bin == b0;
         const unsigned long long m = __ballot(mine);
-        if (lane == leader) mybase = atomicAdd(bc.ctr + b0, __popcll(m));
+        if (lane == leader) mybase = atomicAdd(bc.ctr + b0 * BSX_SLOTS + bc.slot, __popcll(m));
         if (mine) {
             myleader = leader;
             myrank = __popcll(m & ((1ull << lane) - 1ull));
@@ -474,6 +482,7 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         __syncthreads();
         FftLine<FT, -1, T>::run(line, tw, t);
     }
+    bc.slot = (int)(blockIdx.x % BSX_SLOTS);
     const PartDev& P = recs[valid ? p : 0];
     const unsigned keybase = (unsigned)P.half * (unsigned)(K.NT * K.NT * K.NT) * 4096u;
     const float inv = 1.0f / FT;
@@ -504,9 +513,9 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
 // are spread over many workgroups.
 #define BSX_PIECE 16384
 
-// Exclusive scans of the per-bin counts (one workgroup): entry offsets
-// offsets[0..nbins] and piece offsets poff[0..nbins]; cursors = offsets;
-// counts cleared for the next batch.
+// Exclusive scans (one workgroup) of the per-(bin, slot) counts: entry
+// offsets offsets[0..nbins*SLOTS] (cursors = offsets), and per-bin piece
+// offsets poff[0..nbins]; counts cleared for the next batch.
 __global__ __launch_bounds__(1024) void k_scan(int* __restrict__ counts, int nbins,
                                                int* __restrict__ offsets,
                                                int* __restrict__ cursors,
@@ -514,12 +523,14 @@ __global__ __launch_bounds__(1024) void k_scan(int* __restrict__ counts, int nbi
                                                unsigned long long* __restrict__ total) {
     __shared__ int part[1024], ppart[1024];
     const int tid = threadIdx.x;
-    const int per = (nbins + 1023) / 1024;
+    const int per = (nbins + 1023) / 1024;   // bins per thread
     const int b0 = tid * per, b1 = min(nbins, b0 + per);
     int sum = 0, psum = 0;
     for (int i = b0; i < b1; ++i) {
-        sum += counts[i];
-        psum += (counts[i] + BSX_PIECE - 1) / BSX_PIECE;
+        int c = 0;
+        for (int q = 0; q < BSX_SLOTS; ++q) c += counts[i * BSX_SLOTS + q];
+        sum += c;
+        psum += (c + BSX_PIECE - 1) / BSX_PIECE;
     }
     part[tid] = sum;
     ppart[tid] = psum;
@@ -534,16 +545,21 @@ __global__ __launch_bounds__(1024) void k_scan(int* __restrict__ counts, int nbi
     }
     int run = part[tid] - sum, prun = ppart[tid] - psum;
     for (int i = b0; i < b1; ++i) {
-        const int c = counts[i];
-        offsets[i] = run;
-        cursors[i] = run;
+        int c = 0;
+        for (int q = 0; q < BSX_SLOTS; ++q) {
+            const int j = i * BSX_SLOTS + q;
+            const int cq = counts[j];
+            offsets[j] = run + c;
+            cursors[j] = run + c;
+            c += cq;
+            counts[j] = 0;
+        }
         poff[i] = prun;
         run += c;
         prun += (c + BSX_PIECE - 1) / BSX_PIECE;
-        counts[i] = 0;
     }
     if (tid == 1023) {
-        offsets[nbins] = part[1023];
+        offsets[nbins * BSX_SLOTS] = part[1023];
         poff[nbins] = ppart[1023];
         *total += (unsigned long long)part[1023];   // entries binned since the last reset
     }
@@ -568,8 +584,8 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ offs
     const int bin = lo;
     const int j = piece - poff[bin];
     const bool single = poff[bin + 1] - poff[bin] == 1;
-    const int beg = offsets[bin] + j * BSX_PIECE;
-    const int end = min(min(offsets[bin + 1], beg + BSX_PIECE), cap);
+    const int beg = offsets[bin * BSX_SLOTS] + j * BSX_PIECE;
+    const int end = min(min(offsets[(bin + 1) * BSX_SLOTS], beg + BSX_PIECE), cap);
     if (beg >= end) return;
     __shared__ float sre[4096], sim[4096], sw[4096];
     float* g = acc + (size_t)bin * 4096 * 3;
@@ -1057,7 +1073,8 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     BSX_REQUIRE(!h->binned || h->bins_cap < (1ull << 31), BSX_ERR_ARG,
                 "batch %d too large for 32-bit bin offsets", batch);
     const size_t bins_bytes = h->binned ? h->bins_cap * sizeof(float4) : 0;
-    const size_t bin_idx_bytes = h->binned ? (4 * (size_t)h->nbins + 2) * sizeof(int) : 0;
+    const size_t bin_idx_bytes =
+        h->binned ? ((3 * BSX_SLOTS + 1) * (size_t)h->nbins + 2) * sizeof(int) : 0;
     h->bytes = h->acc_elems * sizeof(float) + inter_bytes + bins_bytes + bin_idx_bytes;
     if (p->mem_cap > 0 && (int64_t)h->bytes > p->mem_cap) {
         set_error("device memory %zu bytes exceeds the cap %lld", h->bytes, (long long)p->mem_cap);
@@ -1069,9 +1086,9 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
               hipMalloc(&h->nonfinite, sizeof(int)) == hipSuccess;
     if (ok && h->binned)
         ok = hipMalloc(&h->bins, bins_bytes) == hipSuccess &&
-             hipMalloc(&h->counts, sizeof(int) * h->nbins) == hipSuccess &&
-             hipMalloc(&h->offsets, sizeof(int) * (h->nbins + 1)) == hipSuccess &&
-             hipMalloc(&h->cursors, sizeof(int) * h->nbins) == hipSuccess &&
+             hipMalloc(&h->counts, sizeof(int) * h->nbins * BSX_SLOTS) == hipSuccess &&
+             hipMalloc(&h->offsets, sizeof(int) * (h->nbins * BSX_SLOTS + 1)) == hipSuccess &&
+             hipMalloc(&h->cursors, sizeof(int) * h->nbins * BSX_SLOTS) == hipSuccess &&
              hipMalloc(&h->poff, sizeof(int) * (h->nbins + 1)) == hipSuccess &&
              hipMalloc(&h->entries, sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
@@ -1079,7 +1096,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
         bsx_recon_destroy(h);
         return BSX_ERR_NOMEM;
     }
-    if (h->binned) BSX_HIP(hipMemset(h->counts, 0, sizeof(int) * h->nbins));
+    if (h->binned) BSX_HIP(hipMemset(h->counts, 0, sizeof(int) * h->nbins * BSX_SLOTS));
     BSX_HIP(hipMemset(h->acc, 0, h->acc_elems * sizeof(float)));
     BSX_HIP(hipDeviceSynchronize());
     *out = h;
