@@ -75,6 +75,22 @@ class Particle(C.Structure):
     ]
 
 
+BSX_SEARCH_CTF = 1
+
+
+class SearchParams(C.Structure):
+    _fields_ = [
+        ("box", C.c_int32),
+        ("radius", C.c_int32),
+        ("sampling", C.c_double),
+        ("kernel_width", C.c_int32),
+        ("kernel_power", C.c_int32),
+        ("topk", C.c_int32),
+        ("flags", C.c_int32),
+        ("mem_cap", C.c_int64),
+    ]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("coverage", C.c_int64),
@@ -108,6 +124,23 @@ SIGNATURES = {
     "bsx_fsc": ([_vp, _vp, C.c_int, C.c_double, C.c_double, _dp, C.POINTER(C.c_int), _vp], C.c_int),
     "bsx_score_frc": ([_vp, C.c_int64, _vp, C.c_int64, C.c_int, C.c_double, C.c_double,
                        C.c_double, C.POINTER(C.c_int32), C.c_int64, _dp, _vp], C.c_int),
+    "bsx_search_create": ([C.POINTER(_vp), C.POINTER(SearchParams)], C.c_int),
+    "bsx_search_destroy": ([_vp], C.c_int),
+    "bsx_search_info": ([_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                         C.POINTER(C.c_int64)], C.c_int),
+    "bsx_search_points": ([_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
+    "bsx_search_set_reference": ([_vp, _vp, _dp, C.c_int64, _vp], C.c_int),
+    "bsx_search_stage": ([_vp, _vp, C.POINTER(Particle), C.c_int64, _vp], C.c_int),
+    "bsx_search_exhaustive": ([_vp, _vp, _vp, _vp], C.c_int),
+    "bsx_search_rescore": ([_vp, _vp, C.c_int32, _vp, _vp, _vp, _vp], C.c_int),
+    "bsx_search_score_mats": ([_vp, _vp, _vp, C.c_int32, _vp, _vp], C.c_int),
+    "bsx_search_particle_data": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
+    "bsx_search_library_data": ([_vp, _vp, _vp, _vp], C.c_int),
+    "bsx_search_view_mats": ([_vp, _vp, C.c_int64, _vp, _vp], C.c_int),
+    "bsx_search_refine": ([_vp, _vp, _vp, _vp, _dp, C.c_int32, _vp], C.c_int),
+    "bsx_view_matrices": ([_dp, C.c_int64, _dp], C.c_int),
+    "bsx_views_c1": ([C.c_double, C.c_double, C.c_int32, C.c_double, _dp, C.c_int64,
+                      C.POINTER(C.c_int64)], C.c_int),
 }
 
 _lib = None

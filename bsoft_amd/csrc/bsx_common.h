@@ -1,0 +1,175 @@
+// bsx_common.h — helpers shared by the reconstruction, extraction and search
+// sources: electron wavelength, view -> rotation matrix, CTF aberration
+// weights, the FSI interpolation kernel table and its device-side gather.
+#pragma once
+
+#include <cmath>
+#include <vector>
+
+#include "bsx_internal.h"
+
+namespace bsx {
+
+// CTFparam::lambda (include/ctf.h:309-317) with the constants of
+// include/utilities.h:55-58.
+inline double lambda_of(double av) {
+    const double PLANCK = 6.626070E-34, ECHARGE = 1.602177E-19;
+    const double EMASS = 9.109384E-31, LIGHTSPEED = 299792458.0;
+    if (av <= 0) return 0;
+    return PLANCK * 1e10 /
+           sqrt(2.0 * EMASS * ECHARGE * av *
+                (1.0 + ECHARGE * av / (2.0 * EMASS * LIGHTSPEED * LIGHTSPEED)));
+}
+
+// View::quaternion + Matrix3::from_quaternion (include/View.h:131-157,
+// include/Matrix3.h:30-42): row-major d[3*i+j].
+inline void view_matrix(const double view[4], double d[9]) {
+    double v0 = view[0], v1 = view[1], v2 = view[2], a = view[3];
+    double size = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    if (size < 1e-30) { v0 = v1 = 0; v2 = 1; }
+    else { v0 /= size; v1 /= size; v2 /= size; }
+    a = remainder(a, 2 * M_PI);
+    double ang = a / 2.0, ca = cos(ang), sa = sin(ang);
+    double q0 = ca, q1 = 0, q2 = 0, q3 = sa;
+    if (v2 < 1.0 - 1e-30) {
+        if (v2 <= 1e-30 - 1.0) { q0 = 0; q1 = sa; q2 = ca; q3 = 0; }
+        else {
+            double z1 = v2 + 1, f1 = sqrt(z1 / 2.0), f2 = sqrt(1.0 / (2 * z1));
+            q0 = f1 * ca; q1 = f2 * (v0 * sa - v1 * ca); q2 = f2 * (v0 * ca + v1 * sa); q3 = f1 * sa;
+        }
+    }
+    d[0] = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
+    d[1] = 2 * q1 * q2 - 2 * q0 * q3;
+    d[2] = 2 * q1 * q3 + 2 * q0 * q2;
+    d[3] = 2 * q1 * q2 + 2 * q0 * q3;
+    d[4] = q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3;
+    d[5] = 2 * q2 * q3 - 2 * q0 * q1;
+    d[6] = 2 * q1 * q3 - 2 * q0 * q2;
+    d[7] = 2 * q2 * q3 + 2 * q0 * q1;
+    d[8] = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
+    for (int i = 0; i < 9; i++) {
+        if (fabs(d[i]) < 1e-12) d[i] = 0;
+        if (d[i] > 1) d[i] = 1;
+        if (d[i] < -1) d[i] = -1;
+    }
+}
+
+// Even aberration weights of CTFparam (include/ctf.h:79-83, 321-335;
+// src/util/ctf.cpp:21-57): chi = c0 + c20 s^2 + c22 s^2 cos2a + c2m2 s^2 sin2a
+// + c40 s^4, with the particle's defocus, astigmatism and Cs.
+struct CtfCoef {
+    double c0, c20, c22, c2m2, c40;
+};
+
+inline CtfCoef ctf_coef(const bsx_particle& q) {
+    double wl = lambda_of(q.volt);
+    double t1 = M_PI_2 * wl * wl * wl, t2 = -M_PI * wl, dev = fabs(q.def_dev);
+    CtfCoef c;
+    c.c0 = -q.amp_shift;
+    c.c20 = t2 * q.def_avg;
+    c.c22 = t2 * dev * cos(2 * q.ast_angle);
+    c.c2m2 = t2 * dev * sin(2 * q.ast_angle);
+    c.c40 = t1 * q.cs;
+    return c;
+}
+
+// chi at spatial frequency (fx, fy) in 1/A, FP64.
+__host__ __device__ inline double ctf_chi(const CtfCoef& c, double fx, double fy) {
+    const double s2 = fx * fx + fy * fy;
+    double c2a = 1, s2a = 0;
+    if (s2 > 0) {
+        c2a = (fx * fx - fy * fy) / s2;
+        s2a = 2 * fx * fy / s2;
+    }
+    double dphi = c.c0;
+    dphi += c.c2m2 * s2 * s2a;
+    dphi += c.c20 * s2;
+    dphi += c.c22 * s2 * c2a;
+    dphi += c.c40 * s2 * s2;
+    return dphi;
+}
+
+// FSI_Kernel table (include/FSI_Kernel.h:28-68), computed in double and
+// stored FP32: (ndiv + 1) rows of `wd` weights, row = round(frac * 512).
+constexpr int FSI_NDIV = 512;
+
+inline std::vector<float> fsi_table(int wd, int power) {
+    const int ndiv = FSI_NDIV;
+    if (power < 0) wd = 1;
+    std::vector<double> data((size_t)(ndiv + 1) * wd, 0.0);
+    if (power < 0) {
+        for (auto& d : data) d = 1;
+    } else {
+        int wd_2 = (wd - 1) / 2;
+        double iwd = 1.0 / wd, piwd = M_PI * iwd, incr = 1.0 / ndiv, arg = incr;
+        data[wd_2] = 1;
+        long ik = wd;
+        for (long i = 1; i < ndiv; i++) {
+            double arg1 = M_PI * (arg + wd_2);
+            double s1 = iwd * sin(arg1);
+            double arg2 = arg1 * iwd;
+            for (int j = 0; j < wd; j++, ik++) {
+                double s2 = sin(arg2), c = cos(arg2);
+                data[ik] = (s1 / s2) * pow(c, power);
+                arg2 -= piwd;
+                s1 = -s1;
+            }
+            arg += incr;
+        }
+        ik += wd_2 + 1;
+        data[ik] = 1;
+    }
+    return std::vector<float>(data.begin(), data.end());
+}
+
+// Bimage::fspace_interpolate (src/img/Bimage_fspace.cpp:85-147): the value of
+// an N^3 complex transform at voxel coordinate m by WD^3 kernel taps around
+// floor(m) - (WD-1)/2, indices wrapped periodically (the reference wraps one
+// step; the total wrap here keeps every index inside the volume for any m).
+// K: the kernel table (in LDS). FP32 taps, per-plane sums added in double
+// into (vr, vi).
+template <int WD>
+__device__ inline void fsi_sample(const float2* __restrict__ ft, int N, const double m[3],
+                                  const float* K, double& vr, double& vi) {
+    constexpr int HW = (WD - 1) / 2;
+    long ix = (long)floor(m[0]), iy = (long)floor(m[1]), iz = (long)floor(m[2]);
+    const int ikx = ((int)((m[0] - ix) * FSI_NDIV + 0.5)) * WD;
+    const int iky = ((int)((m[1] - iy) * FSI_NDIV + 0.5)) * WD;
+    const int ikz = ((int)((m[2] - iz) * FSI_NDIV + 0.5)) * WD;
+    ix -= HW;
+    iy -= HW;
+    iz -= HW;
+    long jxs[WD];
+#pragma unroll
+    for (int q = 0; q < WD; ++q) {
+        long jx = (ix + q) % N;
+        if (jx < 0) jx += N;
+        jxs[q] = jx;
+    }
+    vr = 0;
+    vi = 0;
+#pragma unroll 1
+    for (int zz = 0; zz < WD; ++zz) {
+        long jz = (iz + zz) % N;
+        if (jz < 0) jz += N;
+        float pr = 0.f, pi = 0.f;
+#pragma unroll 1
+        for (int yy2 = 0; yy2 < WD; ++yy2) {
+            long jy = (iy + yy2) % N;
+            if (jy < 0) jy += N;
+            const float w = K[iky + yy2] * K[ikz + zz];
+            const float2* row = ft + (jz * N + jy) * (long)N;
+#pragma unroll
+            for (int q = 0; q < WD; ++q) {
+                const float kw = K[ikx + q] * w;
+                const float2 a = row[jxs[q]];
+                pr = fmaf(a.x, kw, pr);
+                pi = fmaf(a.y, kw, pi);
+            }
+        }
+        vr += pr;
+        vi += pi;
+    }
+}
+
+}  // namespace bsx

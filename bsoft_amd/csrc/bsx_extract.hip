@@ -10,7 +10,7 @@
 #include <cmath>
 #include <vector>
 
-#include "bsx_internal.h"
+#include "bsx_common.h"
 
 namespace bsx {
 int fft_axis_ex(float2* data, float* real_out, int nx, int ny, int nz, int64_t batch, int axis,
@@ -52,9 +52,8 @@ __global__ __launch_bounds__(256) void k_extract(const float2* __restrict__ ft, 
                                                  double rs, double maxs2, int center,
                                                  const float* __restrict__ ktab,
                                                  float2* __restrict__ out) {
-    constexpr int NDIV = 512;
-    __shared__ float K[(NDIV + 1) * WD];
-    for (int i = threadIdx.x; i < (NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
+    __shared__ float K[(FSI_NDIV + 1) * WD];
+    for (int i = threadIdx.x; i < (FSI_NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
     __syncthreads();
     const int v = blockIdx.y;
     const long long npix = (long long)N * N;
@@ -70,45 +69,7 @@ __global__ __launch_bounds__(256) void k_extract(const float2* __restrict__ ft, 
     if (s2 <= maxs2) {
         double m[3];
         for (int k = 0; k < 3; ++k) m[k] = (ivx * M[3 * k] + ivy * M[3 * k + 1] + 0.0 * M[3 * k + 2]) * rs;
-        constexpr int HW = (WD - 1) / 2;
-        long ix = (long)floor(m[0]), iy = (long)floor(m[1]), iz = (long)floor(m[2]);
-        const int ikx = ((int)((m[0] - ix) * NDIV + 0.5)) * WD;
-        const int iky = ((int)((m[1] - iy) * NDIV + 0.5)) * WD;
-        const int ikz = ((int)((m[2] - iz) * NDIV + 0.5)) * WD;
-        ix -= HW;
-        iy -= HW;
-        iz -= HW;
-        long jxs[WD];
-#pragma unroll
-        for (int q = 0; q < WD; ++q) {
-            // single-step wrap as the reference, made total so that no
-            // index can leave the volume for any coordinate
-            long jx = (ix + q) % N;
-            if (jx < 0) jx += N;
-            jxs[q] = jx;
-        }
-#pragma unroll 1
-        for (int zz = 0; zz < WD; ++zz) {
-            long jz = (iz + zz) % N;
-            if (jz < 0) jz += N;
-            float pr = 0.f, pi = 0.f;
-#pragma unroll 1
-            for (int yy2 = 0; yy2 < WD; ++yy2) {
-                long jy = (iy + yy2) % N;
-                if (jy < 0) jy += N;
-                const float w = K[iky + yy2] * K[ikz + zz];
-                const float2* row = ft + (jz * N + jy) * (long)N;
-#pragma unroll
-                for (int q = 0; q < WD; ++q) {
-                    const float kw = K[ikx + q] * w;
-                    const float2 a = row[jxs[q]];
-                    pr = fmaf(a.x, kw, pr);
-                    pi = fmaf(a.y, kw, pi);
-                }
-            }
-            vr += pr;
-            vi += pi;
-        }
+        fsi_sample<WD>(ft, N, m, K, vr, vi);
     }
     if (center) {
         // phase_shift_to_center of a 2D section: x (-1)^(h+k) for even N
@@ -223,36 +184,6 @@ static double check_res(double res, int n, double sampling, int ndim) {
     if (res > n * sampling) return n * sampling;
     double rm = 2 / sqrt(ndim / (sampling * sampling));
     return res < rm ? rm : res;
-}
-
-// FSI_Kernel table (include/FSI_Kernel.h:28-68), double precision, stored FP32.
-static std::vector<float> fsi_table(int wd, int power) {
-    const int ndiv = 512;
-    if (power < 0) wd = 1;
-    std::vector<double> data((size_t)(ndiv + 1) * wd, 0.0);
-    if (power < 0) {
-        for (auto& d : data) d = 1;
-    } else {
-        int wd_2 = (wd - 1) / 2;
-        double iwd = 1.0 / wd, piwd = M_PI * iwd, incr = 1.0 / ndiv, arg = incr;
-        data[wd_2] = 1;
-        long ik = wd;
-        for (long i = 1; i < ndiv; i++) {
-            double arg1 = M_PI * (arg + wd_2);
-            double s1 = iwd * sin(arg1);
-            double arg2 = arg1 * iwd;
-            for (int j = 0; j < wd; j++, ik++) {
-                double s2 = sin(arg2), c = cos(arg2);
-                data[ik] = (s1 / s2) * pow(c, power);
-                arg2 -= piwd;
-                s1 = -s1;
-            }
-            arg += incr;
-        }
-        ik += wd_2 + 1;
-        data[ik] = 1;
-    }
-    return std::vector<float>(data.begin(), data.end());
 }
 
 extern "C" {

@@ -24,7 +24,7 @@
 #include <vector>
 
 #include "bsx_fft_dev.h"
-#include "bsx_internal.h"
+#include "bsx_common.h"
 
 namespace bsx {
 int fft_axis_ex(float2* data, float* real_out, int nx, int ny, int nz, int64_t batch, int axis,
@@ -835,48 +835,6 @@ static double check_resolution(double res, int n, double sampling) {
     if (res > n * sampling) return n * sampling;
     double rm = 2 / sqrt(3.0 / (sampling * sampling));
     return res < rm ? rm : res;
-}
-
-static double lambda_of(double av) {
-    // CTFparam::lambda (include/ctf.h:309-317) with the constants of
-    // include/utilities.h:55-58
-    const double PLANCK = 6.626070E-34, ECHARGE = 1.602177E-19;
-    const double EMASS = 9.109384E-31, LIGHTSPEED = 299792458.0;
-    if (av <= 0) return 0;
-    return PLANCK * 1e10 /
-           sqrt(2.0 * EMASS * ECHARGE * av * (1.0 + ECHARGE * av / (2.0 * EMASS * LIGHTSPEED * LIGHTSPEED)));
-}
-
-// View::matrix (include/View.h:131-157; Matrix3.h:30-42)
-static void view_matrix(const double view[4], double d[9]) {
-    double v0 = view[0], v1 = view[1], v2 = view[2], a = view[3];
-    double size = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-    if (size < 1e-30) { v0 = v1 = 0; v2 = 1; }
-    else { v0 /= size; v1 /= size; v2 /= size; }
-    a = remainder(a, 2 * M_PI);
-    double ang = a / 2.0, ca = cos(ang), sa = sin(ang);
-    double q0 = ca, q1 = 0, q2 = 0, q3 = sa;
-    if (v2 < 1.0 - 1e-30) {
-        if (v2 <= 1e-30 - 1.0) { q0 = 0; q1 = sa; q2 = ca; q3 = 0; }
-        else {
-            double z1 = v2 + 1, f1 = sqrt(z1 / 2.0), f2 = sqrt(1.0 / (2 * z1));
-            q0 = f1 * ca; q1 = f2 * (v0 * sa - v1 * ca); q2 = f2 * (v0 * ca + v1 * sa); q3 = f1 * sa;
-        }
-    }
-    d[0] = q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3;
-    d[1] = 2 * q1 * q2 - 2 * q0 * q3;
-    d[2] = 2 * q1 * q3 + 2 * q0 * q2;
-    d[3] = 2 * q1 * q2 + 2 * q0 * q3;
-    d[4] = q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3;
-    d[5] = 2 * q2 * q3 - 2 * q0 * q1;
-    d[6] = 2 * q1 * q3 - 2 * q0 * q2;
-    d[7] = 2 * q2 * q3 + 2 * q0 * q1;
-    d[8] = q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3;
-    for (int i = 0; i < 9; i++) {
-        if (fabs(d[i]) < 1e-12) d[i] = 0;
-        if (d[i] > 1) d[i] = 1;
-        if (d[i] < -1) d[i] = -1;
-    }
 }
 
 static void make_rec(const bsx_recon* h, const bsx_particle& q, PartDev& r) {

@@ -1,0 +1,974 @@
+// bsx_search.hip — orientation search by projection matching on gfx950.
+//
+// The reference scores a particle transform against central sections of the
+// map transform with a ring-weighted Fourier ring correlation
+// (img_recip_space_fom, src/mg/mg_refine.cpp:660-781, CTF applied to the
+// section) over views from asymmetric_unit_views (src/util/symmetry.cpp:383-548)
+// expanded in-plane (view_list_expand_angles, src/util/View.cpp:519-556);
+// sections come from Bimage::central_section (src/img/Bimage_project.cpp:288-338).
+//
+// Here every comparison is restricted to a low-resolution disk of radius R
+// Fourier pixels (rings 0..R-1), stored as a list of half-plane points (the
+// transforms of real images are Hermitian, so the half plane carries every
+// ring sum up to a factor 2 that cancels in each ring correlation):
+//   k_part_stage   : particle FFT -> phase shift to its origin -> disk values,
+//                    CTF values, and the BF16 row of the coarse score
+//   k_lib_build<W> : library sections at the disk points by FSI-kernel
+//                    interpolation (FP32 values) and their BF16 rows
+//   k_gemm_topk<T> : BF16 MFMA (v_mfma_f32_32x32x16_bf16) GEMM of particle rows
+//                    against every library row with FP32 accumulation and a
+//                    fused per-particle top-T selection
+//   k_topk_merge<T>: merge of the per-segment top-T lists
+//   k_rescore      : FP32 FRC of the reference formula for candidate library
+//                    views, best candidate per particle
+//   k_score_mats<W>: FP32 FRC at arbitrary rotations with the section
+//                    interpolated on the fly (local refinement)
+//
+// Coarse score (BF16 GEMM): with a = F_p*CTF and b = F_ref on ring j,
+//   score = 1/(R-1) * sum_{j=1}^{R-1} sum_{i in j} Re(a_i conj b_i) / (|a|_j |b|_j),
+// i.e. the mean ring correlation with the CTF moved from the section to the
+// particle; it ranks candidates for the FP32 FRC, which decides.
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+
+#include "bsx_common.h"
+
+namespace bsx {
+int fft_axis_ex(float2* data, float* real_out, int nx, int ny, int nz, int64_t batch, int axis,
+                int sign, float scale, hipStream_t s);
+}
+using namespace bsx;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+namespace {
+
+constexpr int GM = 128;          // GEMM tile rows (particles)
+constexpr int GN = 128;          // GEMM tile columns (library views)
+constexpr int GK = 64;           // K step staged in LDS
+constexpr int LROW = GK * 2 + 16;  // bytes per staged row (16-byte pad: conflict-free b128 reads)
+constexpr int CROW = GN + 1;     // floats per epilogue row
+constexpr int GEMM_LDS = 2 * (GM + GN) * LROW;   // 73,728 B, aliased by the 66,048-B epilogue tile
+static_assert(GM * CROW * 4 <= GEMM_LDS, "epilogue tile must fit the staging buffers");
+
+__device__ inline uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------- staging
+struct SPart {
+    double phx, phy;     // 2 pi ori / N
+    double fscale;       // 1 / (N * pixel_size)
+    CtfCoef c;
+    int apply;           // CTF applied (flag and defocus >= 1, mg_refine.cpp:687-688)
+    int pad_;
+};
+
+// One workgroup per particle. spec: the particle's N x N forward transform
+// (sign -1, x 1/N). Writes the disk values F (phase-shifted to the origin),
+// the CTF values, and the BF16 coarse row (F*CTF normalised per ring, x 1/(R-1)).
+__global__ __launch_bounds__(256) void k_part_stage(
+    const float2* __restrict__ spec, int N, const SPart* __restrict__ sp,
+    const int2* __restrict__ hk, const int* __restrict__ ring_start, int R, int Nd, int Kp,
+    int64_t p0, float2* __restrict__ fp, float* __restrict__ ctf, uint16_t* __restrict__ arow) {
+    extern __shared__ float2 sv[];          // Nd values F*CTF
+    __shared__ float rn[65];
+    const int b = blockIdx.x;
+    const SPart P = sp[b];
+    const float2* S = spec + (size_t)b * N * N;
+    const int64_t p = p0 + b;
+    for (int i = threadIdx.x; i < Nd; i += blockDim.x) {
+        const int h = hk[i].x, k = hk[i].y;
+        const int xx = h < 0 ? h + N : h, yy = k < 0 ? k + N : k;
+        const float2 F = S[(size_t)yy * N + xx];
+        double sn, cs;
+        sincos((double)h * P.phx + (double)k * P.phy, &sn, &cs);
+        const float vr = (float)(F.x * cs - F.y * sn), vi = (float)(F.x * sn + F.y * cs);
+        float c = 1.f;
+        if (P.apply) c = (float)sin(ctf_chi(P.c, h * P.fscale, k * P.fscale));
+        fp[p * Nd + i] = make_float2(vr, vi);
+        ctf[p * Nd + i] = c;
+        sv[i] = make_float2(vr * c, vi * c);
+    }
+    __syncthreads();
+    if (threadIdx.x < R) {
+        const int j = threadIdx.x;
+        float s = 0.f;
+        for (int i = ring_start[j]; i < ring_start[j + 1]; ++i) s += sv[i].x * sv[i].x + sv[i].y * sv[i].y;
+        rn[j] = (s > 0.f && j > 0) ? 1.f / (sqrtf(s) * (float)(R - 1)) : 0.f;
+    }
+    __syncthreads();
+    uint16_t* A = arow + p * Kp;
+    for (int q = threadIdx.x; q < Kp / 2; q += blockDim.x) {
+        const int i = q + 1;
+        uint32_t w = 0;
+        if (i < Nd) {
+            int j = 1;
+            while (j < R && ring_start[j + 1] <= i) ++j;
+            const float s = rn[j];
+            w = (uint32_t)to_bf16(sv[i].x * s) | ((uint32_t)to_bf16(sv[i].y * s) << 16);
+        }
+        *(uint32_t*)(A + 2 * q) = w;
+    }
+}
+
+// ---------------------------------------------------------------- library
+// One workgroup per view. Section value at each disk point by FSI
+// interpolation, with the same coordinate arithmetic as k_extract.
+template <int WD>
+__global__ __launch_bounds__(256) void k_lib_build(
+    const float2* __restrict__ ft, int N, const double* __restrict__ mats, double inv, double rs,
+    const float* __restrict__ ktab, const int2* __restrict__ hk, const int* __restrict__ ring_start,
+    int R, int Nd, int Kp, int64_t v0, float2* __restrict__ lib, uint16_t* __restrict__ brow) {
+    __shared__ float K[(FSI_NDIV + 1) * WD];
+    __shared__ float rn[65];
+    extern __shared__ float2 sv[];
+    for (int i = threadIdx.x; i < (FSI_NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
+    __syncthreads();
+    const int64_t v = v0 + blockIdx.x;
+    const double* M = mats + 9 * (size_t)blockIdx.x;
+    for (int i = threadIdx.x; i < Nd; i += blockDim.x) {
+        const double ivx = (double)hk[i].x * inv, ivy = (double)hk[i].y * inv;
+        double m[3];
+        for (int c = 0; c < 3; ++c) m[c] = (ivx * M[3 * c] + ivy * M[3 * c + 1] + 0.0 * M[3 * c + 2]) * rs;
+        double vr, vi;
+        fsi_sample<WD>(ft, N, m, K, vr, vi);
+        const float2 val = make_float2((float)vr, (float)vi);
+        lib[v * Nd + i] = val;
+        sv[i] = val;
+    }
+    __syncthreads();
+    if (threadIdx.x < R) {
+        const int j = threadIdx.x;
+        float s = 0.f;
+        for (int i = ring_start[j]; i < ring_start[j + 1]; ++i) s += sv[i].x * sv[i].x + sv[i].y * sv[i].y;
+        rn[j] = (s > 0.f && j > 0) ? 1.f / sqrtf(s) : 0.f;
+    }
+    __syncthreads();
+    uint16_t* B = brow + v * Kp;
+    for (int q = threadIdx.x; q < Kp / 2; q += blockDim.x) {
+        const int i = q + 1;
+        uint32_t w = 0;
+        if (i < Nd) {
+            int j = 1;
+            while (j < R && ring_start[j + 1] <= i) ++j;
+            const float s = rn[j];
+            w = (uint32_t)to_bf16(sv[i].x * s) | ((uint32_t)to_bf16(sv[i].y * s) << 16);
+        }
+        *(uint32_t*)(B + 2 * q) = w;
+    }
+}
+
+// ------------------------------------------------------------ GEMM + top-k
+// Workgroup (256 threads, 4 waves) = one 128-row block of particles x one
+// segment of library column tiles. Per 128x128 tile: K loop in steps of 64
+// staged through double-buffered LDS, each wave a 64x64 quadrant as 2x2
+// v_mfma_f32_32x32x16_bf16 blocks; then the tile goes to LDS and thread t
+// folds row t%128, columns 64*(t/128)..+63 into its running top-T list.
+//
+// MFMA operand layout (32x32x16): lane l supplies row (A) / column (B) l%32 at
+// k = 8*(l/32)..+7; the result holds column l%32, rows 8*(r/4)+4*(l/32)+r%4 in
+// element r.
+template <int T>
+__global__ __launch_bounds__(256) void k_gemm_topk(const uint16_t* __restrict__ A,
+                                                   const uint16_t* __restrict__ B, int Kp,
+                                                   int ncols, int tiles_per_seg, int ntiles,
+                                                   float* __restrict__ pscore,
+                                                   int* __restrict__ pidx) {
+    __shared__ __attribute__((aligned(16))) char lds[GEMM_LDS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const size_t row0 = (size_t)blockIdx.x * GM;
+    const int seg = blockIdx.y, nseg = gridDim.y;
+    const int t0 = seg * tiles_per_seg;
+    const int t1 = min(ntiles, t0 + tiles_per_seg);
+    const int my_row = tid & (GM - 1), my_half = tid >> 7;
+
+    float best[T];
+    int bidx[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) { best[q] = -INFINITY; bidx[q] = -1; }
+
+    const int nk = Kp / GK;
+    // global -> LDS staging: 1024 16-byte chunks per operand tile, 4 per thread
+    int ld_row[4], ld_part[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = tid + 256 * q;
+        ld_row[q] = c >> 3;
+        ld_part[q] = c & 7;
+    }
+
+    for (int t = t0; t < t1; ++t) {
+        const size_t col0 = (size_t)t * GN;
+        f32x16 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+        // the staged chunks live in named registers (an array here ends up in
+        // scratch memory)
+        uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define BSX_GLOAD(K0)                                                               \
+    ra0 = *(const uint4*)(A + (row0 + ld_row[0]) * Kp + (K0) + ld_part[0] * 8);     \
+    ra1 = *(const uint4*)(A + (row0 + ld_row[1]) * Kp + (K0) + ld_part[1] * 8);     \
+    ra2 = *(const uint4*)(A + (row0 + ld_row[2]) * Kp + (K0) + ld_part[2] * 8);     \
+    ra3 = *(const uint4*)(A + (row0 + ld_row[3]) * Kp + (K0) + ld_part[3] * 8);     \
+    rb0 = *(const uint4*)(B + (col0 + ld_row[0]) * Kp + (K0) + ld_part[0] * 8);     \
+    rb1 = *(const uint4*)(B + (col0 + ld_row[1]) * Kp + (K0) + ld_part[1] * 8);     \
+    rb2 = *(const uint4*)(B + (col0 + ld_row[2]) * Kp + (K0) + ld_part[2] * 8);     \
+    rb3 = *(const uint4*)(B + (col0 + ld_row[3]) * Kp + (K0) + ld_part[3] * 8);
+#define BSX_LSTORE(BUF)                                                             \
+    *(uint4*)((BUF) + ld_row[0] * LROW + ld_part[0] * 16) = ra0;                    \
+    *(uint4*)((BUF) + ld_row[1] * LROW + ld_part[1] * 16) = ra1;                    \
+    *(uint4*)((BUF) + ld_row[2] * LROW + ld_part[2] * 16) = ra2;                    \
+    *(uint4*)((BUF) + ld_row[3] * LROW + ld_part[3] * 16) = ra3;                    \
+    *(uint4*)((BUF) + (GM + ld_row[0]) * LROW + ld_part[0] * 16) = rb0;             \
+    *(uint4*)((BUF) + (GM + ld_row[1]) * LROW + ld_part[1] * 16) = rb1;             \
+    *(uint4*)((BUF) + (GM + ld_row[2]) * LROW + ld_part[2] * 16) = rb2;             \
+    *(uint4*)((BUF) + (GM + ld_row[3]) * LROW + ld_part[3] * 16) = rb3;
+        BSX_GLOAD(0)
+        BSX_LSTORE(lds)
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const bool more = kt + 1 < nk;
+            if (more) {
+                const size_t k0 = (size_t)(kt + 1) * GK;
+                BSX_GLOAD(k0)
+            }
+            const char* buf = lds + (kt & 1) * (GM + GN) * LROW;
+#pragma unroll
+            for (int ks = 0; ks < GK / 16; ++ks) {
+                bf16x8 fa[2], fb[2];
+                const int off = ks * 32 + (lane >> 5) * 16;
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    fa[i] = *(const bf16x8*)(buf + (wm * 64 + i * 32 + (lane & 31)) * LROW + off);
+                    fb[i] = *(const bf16x8*)(buf + (GM + wn * 64 + i * 32 + (lane & 31)) * LROW + off);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            }
+            if (more) {
+                char* nb = lds + ((kt + 1) & 1) * (GM + GN) * LROW;
+                BSX_LSTORE(nb)
+            }
+            __syncthreads();
+        }
+#undef BSX_GLOAD
+#undef BSX_LSTORE
+        // epilogue: tile to LDS, then per-row top-T
+        float* C = (float*)lds;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int row = wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const int col = wn * 64 + j * 32 + (lane & 31);
+                    C[row * CROW + col] = acc[i][j][r];
+                }
+        __syncthreads();
+        const float* cr = C + my_row * CROW + my_half * 64;
+        const int cbase = (int)col0 + my_half * 64;
+        const int jmax = min(64, ncols - cbase);
+        for (int j = 0; j < jmax; ++j) {
+            const float v = cr[j];
+            if (v > best[T - 1]) {
+                best[T - 1] = v;
+                bidx[T - 1] = cbase + j;
+#pragma unroll
+                for (int q = T - 1; q > 0; --q) {
+                    if (best[q] > best[q - 1]) {
+                        const float tv = best[q]; best[q] = best[q - 1]; best[q - 1] = tv;
+                        const int ti = bidx[q]; bidx[q] = bidx[q - 1]; bidx[q - 1] = ti;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const size_t o = (((row0 + my_row) * nseg + seg) * 2 + my_half) * T;
+#pragma unroll
+    for (int q = 0; q < T; ++q) {
+        pscore[o + q] = best[q];
+        pidx[o + q] = bidx[q];
+    }
+}
+
+// One thread per particle: merge the 2*nseg sorted lists into the top T.
+template <int T>
+__global__ void k_topk_merge(const float* __restrict__ pscore, const int* __restrict__ pidx,
+                             int nlist, int64_t n, int topk, int* __restrict__ cand,
+                             float* __restrict__ cscore) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float best[T];
+    int bidx[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) { best[q] = -INFINITY; bidx[q] = -1; }
+    const float* S = pscore + (size_t)p * nlist * T;
+    const int* I = pidx + (size_t)p * nlist * T;
+    for (int e = 0; e < nlist * T; ++e) {
+        const float v = S[e];
+        const int id = I[e];
+        if (id >= 0 && (v > best[T - 1] || (v == best[T - 1] && bidx[T - 1] >= 0 && id < bidx[T - 1]) ||
+                        bidx[T - 1] < 0)) {
+            best[T - 1] = v;
+            bidx[T - 1] = id;
+#pragma unroll
+            for (int q = T - 1; q > 0; --q) {
+                const bool sw = bidx[q - 1] < 0 || best[q] > best[q - 1] ||
+                                (best[q] == best[q - 1] && bidx[q] < bidx[q - 1]);
+                if (sw) {
+                    const float tv = best[q]; best[q] = best[q - 1]; best[q - 1] = tv;
+                    const int ti = bidx[q]; bidx[q] = bidx[q - 1]; bidx[q - 1] = ti;
+                }
+            }
+        }
+    }
+    for (int q = 0; q < topk; ++q) {
+        cand[p * topk + q] = bidx[q];
+        cscore[p * topk + q] = best[q];
+    }
+}
+
+// ------------------------------------------------------------ FP32 FRC
+// Ring correlation of img_recip_space_fom (fom_type 0, unit ring weights,
+// CTF on the section) over rings 0..R-1, one wavefront per comparison; ring
+// sums reduced across the wavefront in a fixed order.
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline float frc_wave(const float2* __restrict__ P, const float* __restrict__ Cf,
+                                 const float2* __restrict__ S, const int* __restrict__ ring_start,
+                                 int R, int lane) {
+    float f = 0.f;
+    for (int j = 0; j < R; ++j) {
+        float num = 0.f, dp = 0.f, dr = 0.f;
+        for (int i = ring_start[j] + lane; i < ring_start[j + 1]; i += 64) {
+            const float2 a = P[i];
+            const float c = Cf[i];
+            const float2 r = make_float2(S[i].x * c, S[i].y * c);
+            num += a.x * r.x + a.y * r.y;
+            dp += a.x * a.x + a.y * a.y;
+            dr += r.x * r.x + r.y * r.y;
+        }
+        num = wave_sum(num);
+        dp = wave_sum(dp);
+        dr = wave_sum(dr);
+        const float den = dp * dr;
+        if (den > 0.f) f += num / sqrtf(den);
+    }
+    return f / (float)R;
+}
+
+// One workgroup per particle; wave w scores candidates w, w+4, ...
+// cand: n x ncand library indices (negative = unused slot).
+__global__ __launch_bounds__(256) void k_rescore(const float2* __restrict__ fp,
+                                                 const float* __restrict__ ctf,
+                                                 const float2* __restrict__ lib, int64_t nviews,
+                                                 const int* __restrict__ ring_start, int R, int Nd,
+                                                 const int* __restrict__ cand, int ncand,
+                                                 float* __restrict__ frc, int* __restrict__ best,
+                                                 float* __restrict__ best_frc) {
+    __shared__ float bs[4];
+    __shared__ int bi[4];
+    const int64_t p = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float2* P = fp + p * Nd;
+    const float* Cf = ctf + p * Nd;
+    float wb = -INFINITY;
+    int wi = -1;
+    for (int c = wave; c < ncand; c += 4) {
+        const int v = cand[p * ncand + c];
+        float f = -INFINITY;
+        if (v >= 0 && v < nviews) {
+            f = frc_wave(P, Cf, lib + (size_t)v * Nd, ring_start, R, lane);
+            if (f > wb) { wb = f; wi = c; }
+        }
+        if (frc && lane == 0) frc[p * ncand + c] = f;
+    }
+    if (lane == 0) { bs[wave] = wb; bi[wave] = wi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = -INFINITY;
+        int ci = -1;
+        for (int w = 0; w < 4; ++w)
+            if (bi[w] >= 0 && (bs[w] > b || (bs[w] == b && bi[w] < ci))) { b = bs[w]; ci = bi[w]; }
+        if (best) best[p] = ci >= 0 ? cand[p * ncand + ci] : -1;
+        if (best_frc) best_frc[p] = b;
+    }
+}
+
+// One workgroup per (particle, candidate rotation): the section at the disk
+// points is interpolated on the fly into LDS, then wave 0 forms the FRC.
+template <int WD>
+__global__ __launch_bounds__(256) void k_score_mats(
+    const float2* __restrict__ ft, int N, const double* __restrict__ mats, int ncand, double inv,
+    double rs, const float* __restrict__ ktab, const int2* __restrict__ hk,
+    const int* __restrict__ ring_start, int R, int Nd, const float2* __restrict__ fp,
+    const float* __restrict__ ctf, float* __restrict__ frc) {
+    __shared__ float K[(FSI_NDIV + 1) * WD];
+    extern __shared__ float2 sv[];
+    for (int i = threadIdx.x; i < (FSI_NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
+    __syncthreads();
+    const int64_t pc = blockIdx.x;
+    const int64_t p = pc / ncand;
+    const double* M = mats + 9 * pc;
+    for (int i = threadIdx.x; i < Nd; i += blockDim.x) {
+        const double ivx = (double)hk[i].x * inv, ivy = (double)hk[i].y * inv;
+        double m[3];
+        for (int c = 0; c < 3; ++c) m[c] = (ivx * M[3 * c] + ivy * M[3 * c + 1] + 0.0 * M[3 * c + 2]) * rs;
+        double vr, vi;
+        fsi_sample<WD>(ft, N, m, K, vr, vi);
+        sv[i] = make_float2((float)vr, (float)vi);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const float f = frc_wave(fp + p * Nd, ctf + p * Nd, sv, ring_start, R, threadIdx.x);
+        if (threadIdx.x == 0) frc[pc] = f;
+    }
+}
+
+// Local refinement grid: candidate c of particle p is R0 * Rot(delta) with
+// delta = step * ((c%3)-1, (c/3%3)-1, (c/9)-1) a rotation vector (Rodrigues).
+__global__ void k_local_mats(const double* __restrict__ r0, int64_t n, double step,
+                             double* __restrict__ mats) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * 27) return;
+    const int64_t p = g / 27;
+    const int c = (int)(g % 27);
+    const double wx = step * (c % 3 - 1), wy = step * (c / 3 % 3 - 1), wz = step * (c / 9 - 1);
+    const double th = sqrt(wx * wx + wy * wy + wz * wz);
+    double D[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (th > 0) {
+        const double ux = wx / th, uy = wy / th, uz = wz / th;
+        const double cs = cos(th), sn = sin(th), t = 1 - cs;
+        D[0] = cs + ux * ux * t;      D[1] = ux * uy * t - uz * sn; D[2] = ux * uz * t + uy * sn;
+        D[3] = uy * ux * t + uz * sn; D[4] = cs + uy * uy * t;      D[5] = uy * uz * t - ux * sn;
+        D[6] = uz * ux * t - uy * sn; D[7] = uz * uy * t + ux * sn; D[8] = cs + uz * uz * t;
+    }
+    const double* A = r0 + 9 * p;
+    double* M = mats + 9 * g;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            M[3 * i + j] = A[3 * i] * D[j] + A[3 * i + 1] * D[3 + j] + A[3 * i + 2] * D[6 + j];
+}
+
+// Best of the 27 per particle (ties keep the centre, then the lowest index);
+// the winner becomes the new centre.
+__global__ void k_local_pick(const double* __restrict__ mats, const float* __restrict__ frc,
+                             int64_t n, double* __restrict__ r0, float* __restrict__ best) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int bc = 13;
+    float bf = frc[p * 27 + 13];
+    for (int c = 0; c < 27; ++c) {
+        const float f = frc[p * 27 + c];
+        if (f > bf) { bf = f; bc = c; }
+    }
+    for (int i = 0; i < 9; ++i) r0[9 * p + i] = mats[(p * 27 + bc) * 9 + i];
+    if (best) best[p] = bf;
+}
+
+// Library view matrices of the chosen views (negative index: identity).
+__global__ void k_view_mats(const double* __restrict__ lm, int64_t nviews,
+                            const int* __restrict__ idx, int64_t n, double* __restrict__ out) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int v = idx[p];
+    for (int i = 0; i < 9; ++i)
+        out[9 * p + i] = (v >= 0 && v < nviews) ? lm[9 * (int64_t)v + i] : (i % 4 == 0 ? 1.0 : 0.0);
+}
+
+__global__ void k_real_to_complex_b(const float* __restrict__ in, float2* __restrict__ out,
+                                    long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        out[i] = make_float2(in[i], 0.f);
+}
+
+}  // namespace
+
+// ================================================================ host side
+struct bsx_search {
+    bsx_search_params p{};
+    int device = 0;
+    int R = 0, Nd = 0, Kp = 0;
+    std::vector<int2> hk;
+    std::vector<int> ring_start;
+    int2* d_hk = nullptr;
+    int* d_ring = nullptr;
+    // library
+    int64_t nviews = 0, lib_cap = 0, lbf_cap = 0;
+    float2* d_lib = nullptr;
+    uint16_t* d_lbf = nullptr;
+    // staged particles
+    int64_t nparts = 0, fp_cap = 0, ctf_cap = 0, pbf_cap = 0;
+    float2* d_fp = nullptr;
+    float* d_ctf = nullptr;
+    uint16_t* d_pbf = nullptr;
+    // scratch
+    float2* d_spec = nullptr;
+    int64_t spec_cap = 0;
+    SPart* d_sp = nullptr;
+    int64_t sp_cap = 0;
+    double* d_libmats = nullptr;     // nviews x 9, kept for the chosen views
+    int64_t libmats_cap = 0;
+    double* d_lmats = nullptr;       // local refinement: n x 27 x 9
+    int64_t lmats_cap = 0;
+    float* d_lfrc = nullptr;
+    int64_t lfrc_cap = 0;
+    float* d_ktab = nullptr;
+    float* d_pscore = nullptr;
+    int* d_pidx = nullptr;
+    int64_t pscore_cap = 0, pidx_cap = 0;
+    int64_t bytes = 0;
+};
+
+namespace {
+
+template <class T>
+int grow(bsx_search* h, T** ptr, int64_t* cap, int64_t need, const char* what) {
+    if (need <= *cap && *ptr) return BSX_OK;
+    const int64_t add = (int64_t)sizeof(T) * (need - *cap);
+    BSX_REQUIRE(h->p.mem_cap <= 0 || h->bytes + add <= h->p.mem_cap, BSX_ERR_NOMEM,
+                "bsx_search: %s needs %lld more bytes over the memory cap %lld", what,
+                (long long)add, (long long)h->p.mem_cap);
+    if (*ptr) {
+        (void)hipFree(*ptr);
+        h->bytes -= (int64_t)sizeof(T) * *cap;
+        *ptr = nullptr;
+        *cap = 0;
+    }
+    hipError_t e = hipMalloc((void**)ptr, sizeof(T) * (size_t)need);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        set_error("bsx_search: allocation of %lld bytes for %s failed: %s",
+                  (long long)(sizeof(T) * need), what, hipGetErrorString(e));
+        return BSX_ERR_NOMEM;
+    }
+    *cap = need;
+    h->bytes += (int64_t)sizeof(T) * need;
+    return BSX_OK;
+}
+
+inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace
+
+extern "C" {
+
+int bsx_search_create(bsx_search** out, const bsx_search_params* prm) {
+    BSX_REQUIRE(out && prm, BSX_ERR_ARG, "bsx_search_create: null argument");
+    *out = nullptr;
+    const bsx_search_params& p = *prm;
+    BSX_REQUIRE(fft_length_supported(p.box) && p.box >= 8 && p.box <= 1024, BSX_ERR_ARG,
+                "bsx_search_create: box %d unsupported", p.box);
+    BSX_REQUIRE(p.radius >= 2 && p.radius <= 64 && 2 * p.radius + 2 <= p.box, BSX_ERR_ARG,
+                "bsx_search_create: radius %d out of range 2..min(64, box/2-1)", p.radius);
+    BSX_REQUIRE(p.sampling > 0, BSX_ERR_ARG, "bsx_search_create: sampling must be > 0");
+    BSX_REQUIRE(p.kernel_width >= 1 && p.kernel_width <= 10, BSX_ERR_ARG,
+                "bsx_search_create: kernel width %d out of range 1..10", p.kernel_width);
+    BSX_REQUIRE(p.topk == 8 || p.topk == 16 || (p.topk >= 1 && p.topk < 8), BSX_ERR_ARG,
+                "bsx_search_create: topk %d out of range 1..8 or 16", p.topk);
+    bsx_search* h = new bsx_search();
+    h->p = p;
+    if (h->p.kernel_power < 0) h->p.kernel_width = 1;
+    (void)hipGetDevice(&h->device);
+    const int R = p.radius;
+    h->R = R;
+    // disk points: DC, then rings 1..R-1 of the half plane (k > 0, or k == 0
+    // and h > 0), each ring j = floor(|(h,k)|) contiguous
+    h->hk.push_back(make_int2(0, 0));
+    h->ring_start.push_back(0);
+    for (int j = 1; j < R; ++j) {
+        h->ring_start.push_back((int)h->hk.size());
+        for (int k = 0; k <= j + 1; ++k)
+            for (int x = -j - 1; x <= j + 1; ++x) {
+                if (k == 0 && x <= 0) continue;
+                const int d2 = x * x + k * k;
+                if (d2 >= j * j && d2 < (j + 1) * (j + 1)) h->hk.push_back(make_int2(x, k));
+            }
+    }
+    h->ring_start.push_back((int)h->hk.size());
+    h->Nd = (int)h->hk.size();
+    h->Kp = (int)round_up(2 * (h->Nd - 1), GK);
+    int rc = BSX_OK;
+    int64_t c1 = 0, c2 = 0, c3 = 0;
+    rc = grow(h, &h->d_hk, &c1, h->Nd, "disk table");
+    if (rc == BSX_OK) rc = grow(h, &h->d_ring, &c2, R + 1, "ring table");
+    std::vector<float> tab = fsi_table(h->p.kernel_width, h->p.kernel_power);
+    if (rc == BSX_OK) rc = grow(h, &h->d_ktab, &c3, (int64_t)tab.size(), "kernel table");
+    if (rc == BSX_OK &&
+        (hipMemcpy(h->d_hk, h->hk.data(), sizeof(int2) * h->Nd, hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(h->d_ring, h->ring_start.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice) != hipSuccess ||
+         hipMemcpy(h->d_ktab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+        set_error("bsx_search_create: table upload failed");
+        rc = BSX_ERR_DEVICE;
+    }
+    if (rc != BSX_OK) {
+        bsx_search_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return BSX_OK;
+}
+
+int bsx_search_destroy(bsx_search* h) {
+    if (!h) return BSX_OK;
+    void* ptrs[] = {h->d_hk,   h->d_ring,    h->d_lib,    h->d_lbf,    h->d_fp,
+                    h->d_ctf,  h->d_pbf,     h->d_spec,   h->d_sp,
+                    h->d_ktab, h->d_pscore,  h->d_pidx,   h->d_libmats, h->d_lmats,
+                    h->d_lfrc};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    delete h;
+    return BSX_OK;
+}
+
+int bsx_search_info(const bsx_search* h, int32_t* npoints, int32_t* kdim, int64_t* nviews,
+                    int64_t* nparts) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_search_info: null handle");
+    if (npoints) *npoints = h->Nd;
+    if (kdim) *kdim = h->Kp;
+    if (nviews) *nviews = h->nviews;
+    if (nparts) *nparts = h->nparts;
+    return BSX_OK;
+}
+
+int bsx_search_points(const bsx_search* h, int32_t* hk, int32_t* ring_start) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_search_points: null handle");
+    if (hk)
+        for (int i = 0; i < h->Nd; ++i) { hk[2 * i] = h->hk[i].x; hk[2 * i + 1] = h->hk[i].y; }
+    if (ring_start)
+        for (int j = 0; j <= h->R; ++j) ring_start[j] = h->ring_start[j];
+    return BSX_OK;
+}
+
+int bsx_search_set_reference(bsx_search* h, const float* ft_dev, const double* mats,
+                             int64_t nviews, void* stream) {
+    BSX_REQUIRE(h && ft_dev && (mats || nviews == 0), BSX_ERR_ARG,
+                "bsx_search_set_reference: null argument");
+    BSX_REQUIRE(nviews >= 0 && nviews < (1ll << 31) - GN, BSX_ERR_ARG,
+                "bsx_search_set_reference: nviews %lld out of range", (long long)nviews);
+    for (int64_t i = 0; i < 9 * nviews; ++i)
+        BSX_REQUIRE(std::isfinite(mats[i]) && fabs(mats[i]) <= 1.0 + 1e-9, BSX_ERR_ARG,
+                    "bsx_search_set_reference: matrix %lld is not a rotation", (long long)(i / 9));
+    hipStream_t s = (hipStream_t)stream;
+    const int N = h->p.box, Nd = h->Nd, Kp = h->Kp;
+    const int64_t vpad = round_up(std::max<int64_t>(nviews, 1), GN);
+    int rc = grow(h, &h->d_lib, &h->lib_cap, vpad * Nd, "library values");
+    if (!rc) rc = grow(h, &h->d_lbf, &h->lbf_cap, vpad * Kp, "library rows");
+    if (rc) return rc;
+    // padded rows score zero and are never selected (column < nviews)
+    BSX_HIP(hipMemsetAsync(h->d_lbf, 0, sizeof(uint16_t) * vpad * Kp, s));
+    h->nviews = nviews;
+    rc = grow(h, &h->d_libmats, &h->libmats_cap, 9 * std::max<int64_t>(nviews, 1), "view matrices");
+    if (rc) return rc;
+    if (nviews > 0)
+        BSX_HIP(hipMemcpyAsync(h->d_libmats, mats, sizeof(double) * 9 * nviews, hipMemcpyHostToDevice, s));
+    const int64_t chunk = 65536;
+    const double inv = 1.0 / (N * h->p.sampling), rs = N * h->p.sampling;
+    const size_t shm = sizeof(float2) * Nd;
+    for (int64_t v0 = 0; v0 < nviews; v0 += chunk) {
+        const int64_t nv = std::min(chunk, nviews - v0);
+        switch (h->p.kernel_width) {
+#define BSX_LIB(W) case W: hipLaunchKernelGGL((k_lib_build<W>), dim3((unsigned)nv), dim3(256), shm, s, \
+            (const float2*)ft_dev, N, h->d_libmats + 9 * v0, inv, rs, h->d_ktab, h->d_hk, h->d_ring, h->R, \
+            Nd, Kp, v0, h->d_lib, h->d_lbf); break;
+            BSX_LIB(1) BSX_LIB(2) BSX_LIB(3) BSX_LIB(4) BSX_LIB(5)
+            BSX_LIB(6) BSX_LIB(7) BSX_LIB(8) BSX_LIB(9) BSX_LIB(10)
+#undef BSX_LIB
+        }
+        BSX_HIP(hipGetLastError());
+    }
+    BSX_HIP(hipStreamSynchronize(s));   // the host matrices may be released on return
+    return BSX_OK;
+}
+
+int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* parts, int64_t n,
+                     void* stream) {
+    BSX_REQUIRE(h && (n == 0 || (imgs_dev && parts)), BSX_ERR_ARG, "bsx_search_stage: null argument");
+    BSX_REQUIRE(n >= 0 && n < (1ll << 31) - GM, BSX_ERR_ARG, "bsx_search_stage: n %lld out of range",
+                (long long)n);
+    const int N = h->p.box, Nd = h->Nd, Kp = h->Kp;
+    std::vector<SPart> sp((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        const bsx_particle& q = parts[i];
+        BSX_REQUIRE(std::isfinite(q.ori[0]) && std::isfinite(q.ori[1]) && q.pixel_size > 0 &&
+                        std::isfinite(q.def_avg) && std::isfinite(q.def_dev) &&
+                        std::isfinite(q.ast_angle) && std::isfinite(q.amp_shift),
+                    BSX_ERR_ARG, "bsx_search_stage: particle %lld has invalid parameters", (long long)i);
+        SPart& r = sp[i];
+        r.phx = q.ori[0] / N * 2 * M_PI;
+        r.phy = q.ori[1] / N * 2 * M_PI;
+        r.fscale = 1.0 / (N * q.pixel_size);
+        r.c = ctf_coef(q);
+        r.apply = (h->p.flags & BSX_SEARCH_CTF) && q.def_avg >= 1 && q.volt > 0;
+        r.pad_ = 0;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t ppad = round_up(std::max<int64_t>(n, 1), GM);
+    int rc = grow(h, &h->d_fp, &h->fp_cap, ppad * Nd, "particle values");
+    if (!rc) rc = grow(h, &h->d_ctf, &h->ctf_cap, ppad * Nd, "particle CTF");
+    if (!rc) rc = grow(h, &h->d_pbf, &h->pbf_cap, ppad * Kp, "particle rows");
+    if (rc) return rc;
+    BSX_HIP(hipMemsetAsync(h->d_pbf, 0, sizeof(uint16_t) * ppad * Kp, s));
+    h->nparts = n;
+    // transform in chunks of at most 256 MB of spectra
+    const int64_t per = (int64_t)N * N;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n, (256ll << 20) / (8 * per)));
+    rc = grow(h, &h->d_spec, &h->spec_cap, chunk * per, "spectra");
+    if (!rc) rc = grow(h, &h->d_sp, &h->sp_cap, std::max<int64_t>(chunk, 1), "particle records");
+    if (rc) return rc;
+    for (int64_t p0 = 0; p0 < n; p0 += chunk) {
+        const int64_t nb = std::min(chunk, n - p0);
+        BSX_HIP(hipMemcpyAsync(h->d_sp, sp.data() + p0, sizeof(SPart) * nb, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_real_to_complex_b, dim3(2048), dim3(256), 0, s, imgs_dev + p0 * per,
+                           h->d_spec, (long long)(nb * per));
+        BSX_HIP(hipGetLastError());
+        rc = fft_axis_ex(h->d_spec, nullptr, N, N, 1, nb, 0, -1, 1.0f, s);
+        if (!rc) rc = fft_axis_ex(h->d_spec, nullptr, N, N, 1, nb, 1, -1, 1.0f / N, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_part_stage, dim3((unsigned)nb), dim3(256), sizeof(float2) * Nd, s,
+                           h->d_spec, N, h->d_sp, h->d_hk, h->d_ring, h->R, Nd, Kp, p0, h->d_fp,
+                           h->d_ctf, h->d_pbf);
+        BSX_HIP(hipGetLastError());
+        BSX_HIP(hipStreamSynchronize(s));   // records and spectra are reused
+    }
+    return BSX_OK;
+}
+
+int bsx_search_exhaustive(bsx_search* h, int32_t* cand_dev, float* score_dev, void* stream) {
+    BSX_REQUIRE(h && cand_dev && score_dev, BSX_ERR_ARG, "bsx_search_exhaustive: null argument");
+    BSX_REQUIRE(h->nviews > 0, BSX_ERR_STATE, "bsx_search_exhaustive: no reference views");
+    if (h->nparts == 0) return BSX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int T = h->p.topk <= 8 ? 8 : 16;
+    const int64_t rblocks = round_up(h->nparts, GM) / GM;
+    const int ntiles = (int)(round_up(h->nviews, GN) / GN);
+    // enough segments to give the chip >= 2048 workgroups, at least 4 tiles each
+    int nseg = (int)std::max<int64_t>(1, std::min<int64_t>(ntiles / 4 > 0 ? ntiles / 4 : 1,
+                                                           (2048 + rblocks - 1) / rblocks));
+    nseg = std::min(nseg, 64);
+    const int tps = (ntiles + nseg - 1) / nseg;
+    nseg = (ntiles + tps - 1) / tps;
+    const int64_t need = rblocks * GM * nseg * 2 * T;
+    int rc = grow(h, &h->d_pscore, &h->pscore_cap, need, "top-k partials");
+    if (!rc) rc = grow(h, &h->d_pidx, &h->pidx_cap, need, "top-k partials");
+    if (rc) return rc;
+    BSX_REQUIRE(rblocks < 65536 * 32768ll, BSX_ERR_ARG, "bsx_search_exhaustive: too many particles");
+    dim3 grid((unsigned)rblocks, (unsigned)nseg);
+    if (T == 8)
+        hipLaunchKernelGGL((k_gemm_topk<8>), grid, dim3(256), 0, s, h->d_pbf, h->d_lbf, h->Kp,
+                           (int)h->nviews, tps, ntiles, h->d_pscore, h->d_pidx);
+    else
+        hipLaunchKernelGGL((k_gemm_topk<16>), grid, dim3(256), 0, s, h->d_pbf, h->d_lbf, h->Kp,
+                           (int)h->nviews, tps, ntiles, h->d_pscore, h->d_pidx);
+    BSX_HIP(hipGetLastError());
+    const unsigned mg = (unsigned)((h->nparts + 255) / 256);
+    if (T == 8)
+        hipLaunchKernelGGL((k_topk_merge<8>), dim3(mg), dim3(256), 0, s, h->d_pscore, h->d_pidx,
+                           2 * nseg, h->nparts, h->p.topk, cand_dev, score_dev);
+    else
+        hipLaunchKernelGGL((k_topk_merge<16>), dim3(mg), dim3(256), 0, s, h->d_pscore, h->d_pidx,
+                           2 * nseg, h->nparts, h->p.topk, cand_dev, score_dev);
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_search_rescore(bsx_search* h, const int32_t* cand_dev, int32_t ncand, float* frc_dev,
+                       int32_t* best_dev, float* best_frc_dev, void* stream) {
+    BSX_REQUIRE(h && cand_dev, BSX_ERR_ARG, "bsx_search_rescore: null argument");
+    BSX_REQUIRE(ncand >= 1 && ncand <= 4096, BSX_ERR_ARG, "bsx_search_rescore: ncand %d out of range",
+                ncand);
+    BSX_REQUIRE(h->nviews > 0, BSX_ERR_STATE, "bsx_search_rescore: no reference views");
+    if (h->nparts == 0) return BSX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_rescore, dim3((unsigned)h->nparts), dim3(256), 0, s, h->d_fp, h->d_ctf,
+                       h->d_lib, h->nviews, h->d_ring, h->R, h->Nd, cand_dev, ncand, frc_dev,
+                       best_dev, best_frc_dev);
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_search_score_mats(bsx_search* h, const float* ft_dev, const double* mats_dev,
+                          int32_t ncand, float* frc_dev, void* stream) {
+    BSX_REQUIRE(h && ft_dev && mats_dev && frc_dev, BSX_ERR_ARG,
+                "bsx_search_score_mats: null argument");
+    BSX_REQUIRE(ncand >= 1 && (int64_t)ncand * std::max<int64_t>(h->nparts, 1) < (1ll << 31),
+                BSX_ERR_ARG, "bsx_search_score_mats: ncand %d out of range", ncand);
+    if (h->nparts == 0) return BSX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int N = h->p.box;
+    const double inv = 1.0 / (N * h->p.sampling), rs = N * h->p.sampling;
+    const unsigned g = (unsigned)(h->nparts * ncand);
+    const size_t shm = sizeof(float2) * h->Nd;
+    switch (h->p.kernel_width) {
+#define BSX_SM(W) case W: hipLaunchKernelGGL((k_score_mats<W>), dim3(g), dim3(256), shm, s, \
+        (const float2*)ft_dev, N, mats_dev, ncand, inv, rs, h->d_ktab, h->d_hk, h->d_ring, h->R, \
+        h->Nd, h->d_fp, h->d_ctf, frc_dev); break;
+        BSX_SM(1) BSX_SM(2) BSX_SM(3) BSX_SM(4) BSX_SM(5)
+        BSX_SM(6) BSX_SM(7) BSX_SM(8) BSX_SM(9) BSX_SM(10)
+#undef BSX_SM
+    }
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_search_particle_data(const bsx_search* h, float* values_dev, float* ctf_dev,
+                             uint16_t* rows_dev, void* stream) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_search_particle_data: null handle");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = h->nparts;
+    if (n == 0) return BSX_OK;
+    if (values_dev)
+        BSX_HIP(hipMemcpyAsync(values_dev, h->d_fp, sizeof(float2) * n * h->Nd, hipMemcpyDeviceToDevice, s));
+    if (ctf_dev)
+        BSX_HIP(hipMemcpyAsync(ctf_dev, h->d_ctf, sizeof(float) * n * h->Nd, hipMemcpyDeviceToDevice, s));
+    if (rows_dev)
+        BSX_HIP(hipMemcpyAsync(rows_dev, h->d_pbf, sizeof(uint16_t) * n * h->Kp, hipMemcpyDeviceToDevice, s));
+    return BSX_OK;
+}
+
+int bsx_search_library_data(const bsx_search* h, float* values_dev, uint16_t* rows_dev,
+                            void* stream) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_search_library_data: null handle");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = h->nviews;
+    if (n == 0) return BSX_OK;
+    if (values_dev)
+        BSX_HIP(hipMemcpyAsync(values_dev, h->d_lib, sizeof(float2) * n * h->Nd, hipMemcpyDeviceToDevice, s));
+    if (rows_dev)
+        BSX_HIP(hipMemcpyAsync(rows_dev, h->d_lbf, sizeof(uint16_t) * n * h->Kp, hipMemcpyDeviceToDevice, s));
+    return BSX_OK;
+}
+
+int bsx_search_view_mats(const bsx_search* h, const int32_t* views_dev, int64_t n,
+                         double* mats_dev, void* stream) {
+    BSX_REQUIRE(h && views_dev && mats_dev, BSX_ERR_ARG, "bsx_search_view_mats: null argument");
+    BSX_REQUIRE(h->nviews > 0, BSX_ERR_STATE, "bsx_search_view_mats: no reference views");
+    if (n <= 0) return BSX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_view_mats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, h->d_libmats,
+                       h->nviews, views_dev, n, mats_dev);
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_search_refine(bsx_search* h, const float* ft_dev, double* mats_dev, float* frc_dev,
+                      const double* steps, int32_t nsteps, void* stream) {
+    BSX_REQUIRE(h && ft_dev && mats_dev && (steps || nsteps == 0), BSX_ERR_ARG,
+                "bsx_search_refine: null argument");
+    BSX_REQUIRE(nsteps >= 0 && nsteps <= 16, BSX_ERR_ARG, "bsx_search_refine: nsteps %d out of range",
+                nsteps);
+    for (int i = 0; i < nsteps; ++i)
+        BSX_REQUIRE(std::isfinite(steps[i]) && steps[i] > 0 && steps[i] < 1, BSX_ERR_ARG,
+                    "bsx_search_refine: step %d (%g rad) out of range (0, 1)", i, steps[i]);
+    const int64_t n = h->nparts;
+    if (n == 0 || nsteps == 0) return BSX_OK;
+    BSX_REQUIRE(n * 27 < (1ll << 31), BSX_ERR_ARG, "bsx_search_refine: too many particles");
+    int rc = grow(h, &h->d_lmats, &h->lmats_cap, n * 27 * 9, "refinement matrices");
+    if (!rc) rc = grow(h, &h->d_lfrc, &h->lfrc_cap, n * 27, "refinement scores");
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < nsteps; ++i) {
+        hipLaunchKernelGGL(k_local_mats, dim3((unsigned)((n * 27 + 255) / 256)), dim3(256), 0, s,
+                           mats_dev, n, steps[i], h->d_lmats);
+        BSX_HIP(hipGetLastError());
+        rc = bsx_search_score_mats(h, ft_dev, h->d_lmats, 27, h->d_lfrc, stream);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_local_pick, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           h->d_lmats, h->d_lfrc, n, mats_dev, frc_dev);
+        BSX_HIP(hipGetLastError());
+    }
+    return BSX_OK;
+}
+
+int bsx_view_matrices(const double* views, int64_t n, double* mats) {
+    BSX_REQUIRE((views && mats) || n == 0, BSX_ERR_ARG, "bsx_view_matrices: null argument");
+    for (int64_t i = 0; i < n; ++i) {
+        BSX_REQUIRE(std::isfinite(views[4 * i]) && std::isfinite(views[4 * i + 1]) &&
+                        std::isfinite(views[4 * i + 2]) && std::isfinite(views[4 * i + 3]),
+                    BSX_ERR_ARG, "bsx_view_matrices: view %lld is not finite", (long long)i);
+        view_matrix(views + 4 * i, mats + 9 * i);
+    }
+    return BSX_OK;
+}
+
+// asymmetric_unit_views for point group C1 (src/util/symmetry.cpp:383-470)
+// followed by view_list_expand_angles (src/util/View.cpp:519-556) over
+// [0, 2 pi) in steps of angle_step (0 = no expansion).
+int bsx_views_c1(double theta_step, double phi_step, int32_t flag, double angle_step,
+                 double* views, int64_t cap, int64_t* count) {
+    BSX_REQUIRE(count, BSX_ERR_ARG, "bsx_views_c1: null count");
+    BSX_REQUIRE(angle_step >= 0 && std::isfinite(theta_step) && std::isfinite(phi_step), BSX_ERR_ARG,
+                "bsx_views_c1: bad steps");
+    if (theta_step < M_PI / 1800) theta_step = M_PI / 180;
+    if (phi_step < M_PI / 1800) phi_step = M_PI / 180;
+    const bool full = flag & 1, inplane = !(flag & 2);
+    const double max_theta = full ? M_PI + 1e-6 : M_PI_2 + 1e-6;
+    const int nphi = std::max(1, (int)(2 * M_PI / phi_step + 0.5));
+    std::vector<double> base;
+    auto add = [&](double x, double y, double z, double a) {
+        const double r = sqrt(x * x + y * y + z * z);
+        base.push_back(x / r);
+        base.push_back(y / r);
+        base.push_back(z / r);
+        base.push_back(a);
+    };
+    add(0, 0, 1, 0);
+    for (double th = theta_step; th <= max_theta; th += theta_step) {
+        int nr = (int)(nphi * sin(th) / 2 + 0.5);
+        double dphi = 0;
+        int j0 = 0;
+        if (nr) {
+            dphi = M_PI / nr;
+            j0 = -nr;
+            nr -= 1;
+        }
+        for (int j = j0; j <= nr; ++j)
+            add(sin(th) * cos(j * dphi), sin(th) * sin(j * dphi), cos(th), inplane ? j * dphi : 0);
+    }
+    if (full && base[base.size() - 2] > -0.999999) add(0, 0, -1, 0);
+    const int64_t nb = (int64_t)base.size() / 4;
+    int64_t na = 1;
+    if (angle_step > 0) {
+        na = 0;
+        for (double a = 0; a <= 2 * M_PI - angle_step * 0.5; a += angle_step) ++na;
+    }
+    *count = nb * na;
+    if (!views) return BSX_OK;
+    int64_t o = 0;
+    for (int64_t i = 0; i < nb; ++i) {
+        double a = 0;
+        for (int64_t q = 0; q < na; ++q, a += angle_step) {
+            if (o < cap) {
+                for (int c = 0; c < 3; ++c) views[4 * o + c] = base[4 * i + c];
+                views[4 * o + 3] = base[4 * i + 3] + a;
+            }
+            ++o;
+        }
+    }
+    return BSX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+"""Orientation search by projection matching on the GPU (host side of
+bsx_search_* in include/bsx.h).
+
+The reference's exhaustive search compares each particle transform with
+central sections of the map over an asymmetric-unit view grid expanded
+in-plane, scoring by the ring FRC of img_recip_space_fom
+(src/mg/mg_refine.cpp:660-781; driver part_determine_orientation,
+src/mg/mg_orient.cpp:545-798; grid asymmetric_unit_views,
+src/util/symmetry.cpp:383-548, and view_list_expand_angles,
+src/util/View.cpp:519-556). Here:
+
+* ``Search`` wraps one device handle: the view library of a map, the staged
+  particles, the BF16 MFMA GEMM with per-particle top-k, the FP32 FRC
+  rescoring and the on-the-fly local refinement;
+* ``views_c1`` / ``view_matrices`` give the view grid and its rotations;
+* ``exhaustive_search`` is the plain pipeline: GEMM over every view -> FP32
+  FRC of the top-k -> local refinement.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import abi
+from .abi import SearchParams, call, lib
+from .recon import _ptr as _dev_ptr, _stream
+from .synth import PARTICLE_DTYPE, particles_ctypes
+
+
+def views_c1(theta_step_deg: float, phi_step_deg: float | None = None, flag: int = 1,
+             angle_step_deg: float = 0.0) -> np.ndarray:
+    """asymmetric_unit_views(C1) expanded by in-plane angles q * angle_step in
+    [0, 360): an (n, 4) array of (x, y, z, angle [rad])."""
+    phi = theta_step_deg if phi_step_deg is None else phi_step_deg
+    args = (math.radians(theta_step_deg), math.radians(phi), int(flag), math.radians(angle_step_deg))
+    n = C.c_int64()
+    call("bsx_views_c1", *args, None, 0, C.byref(n))
+    out = np.zeros((n.value, 4))
+    call("bsx_views_c1", *args, out.ctypes.data_as(C.POINTER(C.c_double)), n.value, C.byref(n))
+    return out
+
+
+def view_matrices(views: np.ndarray) -> np.ndarray:
+    """View::matrix of each (x, y, z, angle) row: (n, 3, 3) FP64."""
+    v = np.ascontiguousarray(views, dtype=np.float64).reshape(-1, 4)
+    m = np.zeros((len(v), 9))
+    call("bsx_view_matrices", v.ctypes.data_as(C.POINTER(C.c_double)), len(v),
+         m.ctypes.data_as(C.POINTER(C.c_double)))
+    return m.reshape(-1, 3, 3)
+
+
+def rotation_angle_deg(a, b) -> np.ndarray:
+    """Geodesic angle (degrees) between rotation matrices a and b (..., 3, 3)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    tr = np.einsum("...ij,...ij->...", a, b)
+    return np.degrees(np.arccos(np.clip((tr - 1) / 2, -1.0, 1.0)))
+
+
+class Search:
+    """One search handle: a view library of one map and a staged particle set."""
+
+    def __init__(self, box: int, radius: int, sampling: float, *, kernel_width: int = 8,
+                 kernel_power: int = 2, topk: int = 8, ctf: bool = True, mem_cap: int = 0,
+                 device: str | torch.device = "cuda"):
+        self.device = torch.device(device)
+        self.params = SearchParams(box=box, radius=radius, sampling=sampling,
+                                   kernel_width=kernel_width, kernel_power=kernel_power,
+                                   topk=topk, flags=abi.BSX_SEARCH_CTF if ctf else 0,
+                                   mem_cap=mem_cap)
+        self.h = C.c_void_p()
+        call("bsx_search_create", C.byref(self.h), C.byref(self.params))
+        nd, kd = C.c_int32(), C.c_int32()
+        call("bsx_search_info", self.h, C.byref(nd), C.byref(kd), None, None)
+        self.npoints, self.kdim = nd.value, kd.value
+        self.box, self.radius, self.topk = box, radius, topk
+        self.nviews = 0
+        self.nparts = 0
+        self._ft = None
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            try:
+                lib().bsx_search_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def points(self):
+        """Disk points (Nd, 2) as (h, k) and ring starts (R + 1)."""
+        hk = np.zeros((self.npoints, 2), dtype=np.int32)
+        rs = np.zeros(self.radius + 1, dtype=np.int32)
+        call("bsx_search_points", self.h, hk.ctypes.data_as(C.POINTER(C.c_int32)),
+             rs.ctypes.data_as(C.POINTER(C.c_int32)))
+        return hk, rs
+
+    def set_reference(self, ft: torch.Tensor, mats: np.ndarray) -> None:
+        """ft: the N^3 complex64 map transform (bsx_map_transform); mats:
+        (nviews, 3, 3) rotations. The transform is kept for local refinement."""
+        assert ft.is_cuda and ft.dtype == torch.complex64 and ft.is_contiguous()
+        assert ft.shape == (self.box,) * 3
+        m = np.ascontiguousarray(mats, dtype=np.float64).reshape(-1, 9)
+        call("bsx_search_set_reference", self.h, _dev_ptr(ft),
+             m.ctypes.data_as(C.POINTER(C.c_double)), len(m), _stream())
+        self.nviews = len(m)
+        self._ft = ft
+
+    def stage(self, images: torch.Tensor, parts) -> None:
+        """images: (n, N, N) float32 on the device; parts: PARTICLE_DTYPE records."""
+        arr = np.ascontiguousarray(parts, dtype=PARTICLE_DTYPE)
+        n = len(arr)
+        if n and (images.dtype != torch.float32 or tuple(images.shape) != (n, self.box, self.box)):
+            raise ValueError(f"images must be float32 ({n}, {self.box}, {self.box})")
+        call("bsx_search_stage", self.h, _dev_ptr(images) if n else None,
+             particles_ctypes(arr) if n else None, n, _stream())
+        self.nparts = n
+
+    def exhaustive(self):
+        """BF16 GEMM against every view: (cand, score), each (n, topk)."""
+        cand = torch.empty((self.nparts, self.topk), dtype=torch.int32, device=self.device)
+        score = torch.empty((self.nparts, self.topk), dtype=torch.float32, device=self.device)
+        call("bsx_search_exhaustive", self.h, _dev_ptr(cand), _dev_ptr(score), _stream())
+        return cand, score
+
+    def rescore(self, cand: torch.Tensor):
+        """FP32 FRC of candidate views (n, c): (frc (n, c), best view (n), best frc (n))."""
+        cand = cand.to(device=self.device, dtype=torch.int32).contiguous()
+        n, c = cand.shape
+        assert n == self.nparts
+        frc = torch.empty((n, c), dtype=torch.float32, device=self.device)
+        best = torch.empty(n, dtype=torch.int32, device=self.device)
+        bf = torch.empty(n, dtype=torch.float32, device=self.device)
+        call("bsx_search_rescore", self.h, _dev_ptr(cand), c, _dev_ptr(frc), _dev_ptr(best),
+             _dev_ptr(bf), _stream())
+        return frc, best, bf
+
+    def score_mats(self, mats: torch.Tensor, ft: torch.Tensor | None = None):
+        """FP32 FRC at arbitrary rotations: mats (n, c, 3, 3) float64 -> (n, c)."""
+        ft = self._ft if ft is None else ft
+        mats = mats.to(device=self.device, dtype=torch.float64).contiguous()
+        n, c = mats.shape[:2]
+        assert n == self.nparts
+        frc = torch.empty((n, c), dtype=torch.float32, device=self.device)
+        call("bsx_search_score_mats", self.h, _dev_ptr(ft), _dev_ptr(mats), c, _dev_ptr(frc),
+             _stream())
+        return frc
+
+    def view_mats(self, views: torch.Tensor) -> torch.Tensor:
+        views = views.to(device=self.device, dtype=torch.int32).contiguous()
+        out = torch.empty((len(views), 3, 3), dtype=torch.float64, device=self.device)
+        call("bsx_search_view_mats", self.h, _dev_ptr(views), len(views), _dev_ptr(out), _stream())
+        return out
+
+    def refine(self, mats: torch.Tensor, steps_deg=(2.0, 1.0), ft: torch.Tensor | None = None):
+        """Local 3x3x3 grid refinement at each step; returns (mats, frc)."""
+        ft = self._ft if ft is None else ft
+        mats = mats.to(device=self.device, dtype=torch.float64).contiguous().clone()
+        frc = torch.empty(self.nparts, dtype=torch.float32, device=self.device)
+        st = np.radians(np.asarray(steps_deg, dtype=np.float64))
+        call("bsx_search_refine", self.h, _dev_ptr(ft), _dev_ptr(mats), _dev_ptr(frc),
+             st.ctypes.data_as(C.POINTER(C.c_double)), len(st), _stream())
+        return mats, frc
+
+    def particle_data(self):
+        v = torch.empty((self.nparts, self.npoints), dtype=torch.complex64, device=self.device)
+        c = torch.empty((self.nparts, self.npoints), dtype=torch.float32, device=self.device)
+        r = torch.empty((self.nparts, self.kdim), dtype=torch.int16, device=self.device)
+        call("bsx_search_particle_data", self.h, _dev_ptr(v), _dev_ptr(c), _dev_ptr(r), _stream())
+        return v, c, r
+
+    def library_data(self):
+        v = torch.empty((self.nviews, self.npoints), dtype=torch.complex64, device=self.device)
+        r = torch.empty((self.nviews, self.kdim), dtype=torch.int16, device=self.device)
+        call("bsx_search_library_data", self.h, _dev_ptr(v), _dev_ptr(r), _stream())
+        return v, r
+
+
+@dataclass
+class SearchResult:
+    mats: torch.Tensor        # (n, 3, 3) float64 final rotations
+    frc: torch.Tensor         # (n,) final FRC
+    grid_view: torch.Tensor   # (n,) best library view before refinement
+    grid_frc: torch.Tensor    # (n,) its FRC
+
+
+def exhaustive_search(s: Search, steps_deg=(2.0, 1.0)) -> SearchResult:
+    """Plain pipeline on the staged particles: BF16 GEMM over every library
+    view -> FP32 FRC of the top-k -> local refinement."""
+    cand, _ = s.exhaustive()
+    _, best, bf = s.rescore(cand)
+    mats = s.view_mats(best)
+    if steps_deg:
+        mats, frc = s.refine(mats, steps_deg)
+    else:
+        frc = bf
+    return SearchResult(mats=mats, frc=frc, grid_view=best, grid_frc=bf)

@@ -198,6 +198,108 @@ int bsx_score_frc(const float* part_dev, int64_t nparts, const float* ref_dev, i
                   int n, double sampling, double hi_res, double lo_res, const int32_t* pairs,
                   int64_t npairs, double* out, void* stream);
 
+/* ==== orientation search by projection matching ====
+ * The reference compares a particle transform with central sections of the
+ * map transform by img_recip_space_fom (src/mg/mg_refine.cpp:660-781,
+ * fom_type 0, CTF applied to the section) over views of
+ * asymmetric_unit_views (src/util/symmetry.cpp:383-548) expanded in-plane
+ * (view_list_expand_angles, src/util/View.cpp:519-556); the exhaustive
+ * driver is part_determine_orientation (src/mg/mg_orient.cpp:545-798).
+ * Here comparisons use the rings 0..R-1 of a low-resolution disk (R =
+ * radius in Fourier pixels, i.e. hi_res = box * sampling / R), held as
+ * half-plane point lists: a BF16 MFMA GEMM ranks every library view by a
+ * ring-normalised correlation, the top-k go to the FP32 FRC of the
+ * reference formula, and bsx_search_score_mats evaluates the same FRC at
+ * arbitrary rotations for local refinement. */
+#define BSX_SEARCH_CTF 1   /* apply each particle's CTF (defocus >= 1 A) */
+
+typedef struct bsx_search_params {
+    int32_t box;           /* particle / section edge N (map N^3), 8..1024       */
+    int32_t radius;        /* disk radius R (Fourier pixels), 2..min(64, N/2-1)   */
+    double  sampling;      /* map voxel and particle pixel size (A)               */
+    int32_t kernel_width;  /* FSI kernel width for sections (brefine: 8), 1..10   */
+    int32_t kernel_power;  /* FSI kernel power (2); < 0 = nearest-neighbour table */
+    int32_t topk;          /* candidates kept by the GEMM: 1..8 or 16             */
+    int32_t flags;         /* BSX_SEARCH_*                                        */
+    int64_t mem_cap;       /* device byte cap for the handle, 0 = none            */
+} bsx_search_params;
+
+typedef struct bsx_search bsx_search;  /* opaque handle: library + staged particles */
+
+int bsx_search_create(bsx_search** h, const bsx_search_params* p);
+int bsx_search_destroy(bsx_search* h);
+/* Disk point count Nd (complex), GEMM row length (BF16 elements), views, particles. */
+int bsx_search_info(const bsx_search* h, int32_t* npoints, int32_t* kdim, int64_t* nviews,
+                    int64_t* nparts);
+/* Host copies of the disk points (Nd x 2 int32: h, k) and ring starts (R+1). */
+int bsx_search_points(const bsx_search* h, int32_t* hk, int32_t* ring_start);
+
+/* Library from a map transform (bsx_map_transform output, N^3 complex64) at
+ * nviews row-major FP64 rotation matrices (host): sections as
+ * Bimage::central_section (Bimage_project.cpp:288-338) without the centre
+ * shift, at the disk points. Replaces any previous library. */
+int bsx_search_set_reference(bsx_search* h, const float* ft_dev, const double* mats,
+                             int64_t nviews, void* stream);
+
+/* Stage n particle images (n x N x N FP32, device): forward FFT x 1/N, phase
+ * shift to each particle's origin `ori` (phase_shift_to_origin,
+ * Bimage_complex.cpp:490-543), disk values, CTF values (sin chi of the
+ * particle's parameters, include/ctf.h:321-365) and BF16 GEMM rows.
+ * Replaces any previously staged set. */
+int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* parts, int64_t n,
+                     void* stream);
+
+/* Exhaustive coarse search: BF16 GEMM of every staged particle against every
+ * library view with FP32 accumulation and per-particle top-k (k = topk).
+ * cand_dev: n x topk int32 view indices, score_dev: n x topk FP32 coarse
+ * scores, best first. Asynchronous on `stream`. */
+int bsx_search_exhaustive(bsx_search* h, int32_t* cand_dev, float* score_dev, void* stream);
+
+/* FP32 FRC (img_recip_space_fom, fom_type 0, CTF on the section, rings
+ * 0..R-1) of candidate library views: cand_dev n x ncand int32 (negative =
+ * empty slot). frc_dev (n x ncand, optional), best_dev / best_frc_dev (n,
+ * optional): the best candidate's view index and FRC. Asynchronous. */
+int bsx_search_rescore(bsx_search* h, const int32_t* cand_dev, int32_t ncand, float* frc_dev,
+                       int32_t* best_dev, float* best_frc_dev, void* stream);
+
+/* The same FRC at arbitrary rotations, sections interpolated on the fly from
+ * ft_dev (N^3 complex64): mats_dev n x ncand x 9 FP64 (device), frc_dev
+ * n x ncand. Asynchronous. */
+int bsx_search_score_mats(bsx_search* h, const float* ft_dev, const double* mats_dev,
+                          int32_t ncand, float* frc_dev, void* stream);
+
+/* Rotation matrices (n x 9 FP64, device) of library views views_dev (n int32,
+ * device; a negative index gives the identity). Asynchronous. */
+int bsx_search_view_mats(const bsx_search* h, const int32_t* views_dev, int64_t n,
+                         double* mats_dev, void* stream);
+
+/* Local refinement of every staged particle's orientation, in the manner of
+ * views_for_refinement (src/util/View.cpp:371-436) on a 3x3x3 grid: for each
+ * step s (radians, host array), the 27 rotations R * Rot(s * (i,j,k)),
+ * i,j,k in {-1,0,1}, are scored by the FRC of bsx_search_score_mats and the
+ * best becomes R. mats_dev: n x 9 FP64 (device, in/out); frc_dev (n,
+ * optional): the final FRC. Asynchronous. */
+int bsx_search_refine(bsx_search* h, const float* ft_dev, double* mats_dev, float* frc_dev,
+                      const double* steps, int32_t nsteps, void* stream);
+
+/* View::matrix (include/View.h:131-157, Matrix3.h:30-42) of n views (x, y,
+ * z, angle) into n row-major 3x3 FP64 matrices. Host only. */
+int bsx_view_matrices(const double* views, int64_t n, double* mats);
+
+/* Device copies of the staged data (any pointer may be NULL): values n x Nd
+ * complex64, CTF n x Nd FP32, GEMM rows n x kdim BF16 (raw uint16). */
+int bsx_search_particle_data(const bsx_search* h, float* values_dev, float* ctf_dev,
+                             uint16_t* rows_dev, void* stream);
+int bsx_search_library_data(const bsx_search* h, float* values_dev, uint16_t* rows_dev,
+                            void* stream);
+
+/* asymmetric_unit_views for C1 (src/util/symmetry.cpp:383-470; flag bit 0
+ * full sphere, bit 1 no in-plane alignment) then view_list_expand_angles
+ * (src/util/View.cpp:519-556) with angles q*angle_step in [0, 2 pi)
+ * (angle_step 0: none). views: cap x 4 (x, y, z, angle); *count = total. */
+int bsx_views_c1(double theta_step, double phi_step, int32_t flag, double angle_step,
+                 double* views, int64_t cap, int64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts():
     assert C.sizeof(abi.Particle) == 144
     assert C.sizeof(abi.ReconParams) == 88
+    assert C.sizeof(abi.SearchParams) == 40
     assert C.sizeof(abi.Stats) == 32
 
 

@@ -1,0 +1,194 @@
+"""GPU parity of the projection-matching path (bsx_search_* through the
+C-ABI) against the FP64 restatement in oracle/search_ref.py and the C oracle.
+
+Tolerances: disk values and CTF values of the staging within FP32 rounding
+(1e-5 of the maximum); BF16 rows within one BF16 rounding (2^-8 relative);
+GEMM scores within FP32 accumulation of the exact BF16 products (1e-4
+absolute); FP32 FRC within 1e-4 of the FP64 reference formula."""
+import math
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from bsoft_amd import search, synth
+from bsoft_amd.recon import central_sections, map_transform
+from oracle import search_ref as sr
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bf16_to_f32(x: torch.Tensor) -> np.ndarray:
+    u = x.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    return u.view(np.float32).astype(np.float64)
+
+
+@pytest.fixture(scope="module")
+def setup64():
+    n, R = 64, 14
+    rng = np.random.default_rng(21)
+    bm = synth.make_blob_map(n, 30, rng)
+    vol = bm.volume().cuda()
+    ft = map_transform(vol)
+    ps = synth.make_particles(bm, 40, n, rng, snr=0.5, seed=21, astig=(0.0, 1500.0))
+    return n, R, bm, vol, ft, ps
+
+
+def test_stage_matches_oracle(setup64):
+    n, R, bm, vol, ft, ps = setup64
+    say(f"stage n={n} R={R} particles={len(ps.parts)}")
+    s = search.Search(n, R, 1.5)
+    hk, rs = s.points()
+    ohk, ors = sr.disk_points(R)
+    assert np.array_equal(hk, ohk) and np.array_equal(rs, ors)
+    s.stage(ps.images.cuda().contiguous(), ps.parts)
+    v, c, rows = s.particle_data()
+    v, c = v.cpu().numpy(), c.cpu().numpy()
+    rows = bf16_to_f32(rows)
+    o = Oracle(64)
+    parts = synth.particles_list(ps.parts)
+    for i in range(0, len(parts), 7):
+        ov, oc = sr.particle_disk(ps.images[i].numpy(), parts[i], ohk, o)
+        assert np.abs(v[i] - ov).max() <= 1e-5 * np.abs(ov).max()
+        assert np.abs(c[i] - oc).max() <= 1e-5
+        orow = sr.coarse_row(ov * oc, ors)
+        k = len(orow)
+        assert np.all(np.abs(rows[i, :k] - orow) <= 2.0 ** -8 * np.abs(orow) + 1e-7)
+        assert np.all(rows[i, k:] == 0)
+
+
+def test_library_matches_extraction_and_oracle(setup64):
+    n, R, bm, vol, ft, ps = setup64
+    views = search.views_c1(15.0, angle_step_deg=60.0)
+    mats = search.view_matrices(views)
+    say(f"library views={len(views)}")
+    s = search.Search(n, R, 1.5, kernel_width=8, kernel_power=2)
+    s.set_reference(ft, mats)
+    lv, lrows = s.library_data()
+    lv = lv.cpu().numpy()
+    hk, rs = s.points()
+    sec = central_sections(ft, mats.reshape(-1, 9), 1.5, 1.5, kernel_width=8, kernel_power=2,
+                           center=False).cpu().numpy()
+    # same interpolation code: identical values at the disk points
+    ex = sec[:, hk[:, 1] % n, hk[:, 0] % n]
+    assert np.array_equal(lv, ex.astype(np.complex64))
+    o = Oracle(64)
+    ftc = ft.cpu().numpy()
+    for i in range(0, len(mats), 11):
+        osec = o.central_section(ftc, 1.5, mats[i], 1.5, 8, 2, 0)
+        ov = osec[hk[:, 1] % n, hk[:, 0] % n]
+        assert np.abs(lv[i] - ov).max() <= 1e-5 * np.abs(ov).max()
+        orow = sr.coarse_row(ov, rs, scale_rings=False)
+        lr = bf16_to_f32(lrows[i])[: len(orow)]
+        assert np.all(np.abs(lr - orow) <= 2.0 ** -8 * np.abs(orow) + 1e-7)
+
+
+@pytest.mark.parametrize("topk", [8, 16, 3])
+def test_gemm_topk_exact_on_bf16_rows(setup64, topk):
+    """The GEMM scores are the FP32-accumulated dot products of the BF16 rows
+    and the selected views are the top-k of those products."""
+    n, R, bm, vol, ft, ps = setup64
+    views = search.views_c1(6.0, angle_step_deg=30.0)   # 1154 x 12 views: ragged last tile
+    mats = search.view_matrices(views)
+    say(f"gemm views={len(mats)} topk={topk}")
+    s = search.Search(n, R, 1.5, topk=topk)
+    s.set_reference(ft, mats)
+    s.stage(ps.images.cuda().contiguous(), ps.parts)
+    cand, score = s.exhaustive()
+    torch.cuda.synchronize()
+    A = bf16_to_f32(s.particle_data()[2])
+    B = bf16_to_f32(s.library_data()[1])
+    full = A @ B.T                                   # exact products, FP64 sums
+    cand, score = cand.cpu().numpy(), score.cpu().numpy()
+    for i in range(len(A)):
+        assert len(set(cand[i])) == topk and cand[i].min() >= 0 and cand[i].max() < len(mats)
+        np.testing.assert_allclose(score[i], full[i, cand[i]], atol=1e-4)
+        kth = np.sort(full[i])[::-1][topk - 1]
+        assert np.all(full[i, cand[i]] >= kth - 2e-4)
+        assert np.all(np.diff(score[i]) <= 0)
+
+
+def test_rescore_and_score_mats_match_reference_frc(setup64):
+    n, R, bm, vol, ft, ps = setup64
+    views = search.views_c1(10.0, angle_step_deg=45.0)
+    mats = search.view_matrices(views)
+    s = search.Search(n, R, 1.5)
+    s.set_reference(ft, mats)
+    s.stage(ps.images.cuda().contiguous(), ps.parts)
+    rng = np.random.default_rng(2)
+    P, C = len(ps.parts), 12
+    cand = rng.integers(0, len(mats), size=(P, C)).astype(np.int32)
+    cand[3, 5] = -1
+    frc, best, bf = s.rescore(torch.as_tensor(cand))
+    fm = s.score_mats(torch.as_tensor(mats[np.maximum(cand, 0)]))
+    torch.cuda.synchronize()
+    frc, best, bf, fm = frc.cpu().numpy(), best.cpu().numpy(), bf.cpu().numpy(), fm.cpu().numpy()
+    assert frc[3, 5] == -np.inf
+    # on-the-fly sections reproduce the library values exactly
+    ok = cand >= 0
+    assert np.array_equal(frc[ok], fm[ok])
+    o = Oracle(64)
+    hk, rs = s.points()
+    parts = synth.particles_list(ps.parts)
+    ftc = ft.cpu().numpy()
+    h = np.fft.fftfreq(n, 1 / n)
+    fs = 1.0 / (n * 1.5)
+    for i in range(0, P, 9):
+        pv, pc = sr.particle_disk(ps.images[i].numpy(), parts[i], hk, o)
+        F = np.fft.fft2(ps.images[i].numpy().astype(np.float64)) / n
+        ph = np.exp(2j * np.pi * (h[None, :] * parts[i].ori[0] + h[:, None] * parts[i].ori[1]) / n)
+        ctf = np.array([[o.ctf_value(parts[i], x * fs, y * fs) for x in h] for y in h])
+        for c in range(0, C, 4):
+            if cand[i, c] < 0:
+                continue
+            osec = o.central_section(ftc, 1.5, mats[cand[i, c]], 1.5, 8, 2, 0)
+            ref = sr.frc_full(F * ph, osec * ctf, n, 1.5, R, o)
+            assert frc[i, c] == pytest.approx(ref, abs=1e-4)
+        j = int(np.argmax(np.where(ok[i], frc[i], -np.inf)))
+        assert best[i] == cand[i, j] and bf[i] == frc[i, j]
+
+
+def test_search_recovers_ground_truth():
+    """Exhaustive search (4-degree grid with in-plane angles) plus 2/1-degree
+    local refinement recovers the generating orientations; the GPU's choice
+    among 50 views matches the FP64 oracle's."""
+    n, R = 64, 16
+    rng = np.random.default_rng(33)
+    bm = synth.make_blob_map(n, 40, rng)
+    ft = map_transform(bm.volume().cuda())
+    ps = synth.make_particles(bm, 48, n, rng, snr=1.0, seed=33)
+    views = search.views_c1(4.0, angle_step_deg=4.0)
+    mats = search.view_matrices(views)
+    say(f"recovery views={len(mats)}")
+    s = search.Search(n, R, 1.5, topk=8)
+    s.set_reference(ft, mats)
+    s.stage(ps.images.cuda().contiguous(), ps.parts)
+    res = search.exhaustive_search(s, steps_deg=(2.0, 1.0))
+    truth = synth.view_to_matrix(ps.parts["view"])
+    grid_err = search.rotation_angle_deg(mats[res.grid_view.cpu().numpy()], truth)
+    err = search.rotation_angle_deg(res.mats.cpu().numpy(), truth)
+    say(f"  grid median {np.median(grid_err):.2f} deg, refined median {np.median(err):.2f} deg, "
+        f"max {err.max():.2f}")
+    assert np.median(grid_err) < 3.0
+    assert np.median(err) < 1.0
+    # the FP64 reference formula picks the same view among 50 candidates
+    o = Oracle(64)
+    hk, rs = s.points()
+    parts = synth.particles_list(ps.parts)
+    ftc = ft.cpu().numpy()
+    for i in range(0, 48, 16):
+        c50 = np.concatenate([[res.grid_view[i].item()], rng.integers(0, len(mats), 49)])
+        frc, best, _ = s.rescore(torch.as_tensor(np.tile(c50, (48, 1)).astype(np.int32)))
+        pv, pc = sr.particle_disk(ps.images[i].numpy(), parts[i], hk, o)
+        of = []
+        for v in c50:
+            osec = o.central_section(ftc, 1.5, mats[v], 1.5, 8, 2, 0)
+            of.append(sr.frc_half(pv, pc, osec[hk[:, 1] % n, hk[:, 0] % n], rs))
+        assert c50[int(np.argmax(of))] == best[i].item()

@@ -1,0 +1,119 @@
+"""Pins of the FP64 projection-matching restatement (oracle/search_ref.py) and
+of the host-side view grid of libbsx (no GPU needed)."""
+import math
+
+import numpy as np
+import pytest
+
+from bsoft_amd import search, synth
+from bsoft_amd.abi import Particle
+from oracle import search_ref as sr
+
+
+def test_disk_points_bruteforce():
+    for R in (2, 5, 16):
+        hk, rs = sr.disk_points(R)
+        assert rs[0] == 0 and rs[1] == 1 and tuple(hk[0]) == (0, 0)
+        pts = {(h, k) for h in range(-R - 1, R + 2) for k in range(-R - 1, R + 2)
+               if math.isqrt(h * h + k * k) < R and (k > 0 or (k == 0 and h > 0))}
+        assert set(map(tuple, hk[1:])) == pts and len(hk) == len(pts) + 1
+        for j in range(R):
+            r = np.floor(np.hypot(hk[rs[j]:rs[j + 1], 0], hk[rs[j]:rs[j + 1], 1]))
+            assert np.all(r == j)
+        # every full-plane point of the disk is a half-plane point or its mate
+        full = {(h, k) for h in range(-R, R + 1) for k in range(-R, R + 1)
+                if math.isqrt(h * h + k * k) < R}
+        assert full == pts | {(-h, -k) for h, k in pts} | {(0, 0)}
+
+
+def _part(def_avg=1.5e4, def_dev=0.0, ast=0.0, ori=(0.0, 0.0)):
+    p = Particle()
+    p.view[:] = (0, 0, 1, 0)
+    p.ori[:] = ori
+    p.hdr_ori[:] = ori
+    p.pixel_size, p.mag, p.volt, p.cs = 1.5, 1.0, 3e5, 2.7e7
+    p.amp_shift = math.asin(0.07)
+    p.def_avg, p.def_dev, p.ast_angle = def_avg, def_dev, ast
+    p.sel = 1.0
+    return p
+
+
+@pytest.mark.parametrize("R", [6, 15])
+def test_frc_half_plane_equals_reference_full_plane(oracle64, R):
+    """The half-plane ring sums give the reference FRC of full planes (the C
+    oracle's img_recip_space_fom with the CTF multiplied into the section)."""
+    rng = np.random.default_rng(R)
+    n, pix = 32, 1.5
+    img = rng.normal(size=(n, n))
+    ref = 0.7 * img + rng.normal(size=(n, n))
+    part = _part(def_avg=1.2e4, def_dev=800.0, ast=0.4, ori=(n // 2, n // 2))
+    hk, rs = sr.disk_points(R)
+    pv, pc = sr.particle_disk(img, part, hk, oracle64)
+    sv, _ = sr.particle_disk(ref, part, hk, oracle64, ctf=False)
+    # full planes with the same phase shift, CTF on the section
+    F = np.fft.fft2(img) / n
+    G = np.fft.fft2(ref) / n
+    h = np.fft.fftfreq(n, 1 / n)
+    ph = np.exp(2j * np.pi * (h[None, :] * part.ori[0] + h[:, None] * part.ori[1]) / n)
+    fs = 1.0 / (n * pix)
+    ctf = np.array([[oracle64.ctf_value(part, x * fs, y * fs) for x in h] for y in h])
+    f_full = sr.frc_full(F * ph, G * ph * ctf, n, pix, R, oracle64)
+    f_half = sr.frc_half(pv, pc, sv, rs)
+    assert f_half == pytest.approx(f_full, abs=1e-12)
+    assert -1 <= f_half <= 1
+
+
+def test_frc_and_coarse_self_match(oracle64):
+    rng = np.random.default_rng(3)
+    n, R = 32, 12
+    hk, rs = sr.disk_points(R)
+    img = rng.normal(size=(n, n))
+    v, _ = sr.particle_disk(img, _part(), hk, oracle64, ctf=False)
+    ones = np.ones(len(v))
+    assert sr.frc_half(v, ones, v, rs) == pytest.approx(1.0, abs=1e-12)
+    assert sr.coarse_score(v, ones, v, rs) == pytest.approx(1.0, abs=1e-12)
+    assert sr.coarse_score(v, ones, -v, rs) == pytest.approx(-1.0, abs=1e-12)
+    # with a CTF the coarse score of the CTF-modulated section against the
+    # particle multiplied by the same CTF is |corr| of the products: still 1
+    c = np.cos(np.arange(len(v)) * 0.37)
+    assert sr.coarse_score(v, c, v * c, rs) == pytest.approx(1.0, abs=1e-12)
+    # a sign flip in one ring lowers the mean by 2/(R-1)
+    w = v.copy()
+    w[rs[3]:rs[4]] *= -1
+    assert sr.coarse_score(v, ones, w, rs) == pytest.approx(1 - 2 / (R - 1), abs=1e-12)
+
+
+def test_particle_disk_shift_theorem(oracle64):
+    """An impulse at the particle origin becomes a constant after the phase
+    shift to the origin (phase_shift_to_origin)."""
+    n = 32
+    img = np.zeros((n, n))
+    img[13, 9] = 1.0
+    hk, _ = sr.disk_points(10)
+    v, _ = sr.particle_disk(img, _part(ori=(9, 13)), hk, oracle64, ctf=False)
+    np.testing.assert_allclose(v, np.full(len(v), 1.0 / n), atol=1e-14)
+
+
+def test_views_c1_matches_oracle(oracle64):
+    v = search.views_c1(4.0)
+    ov = oracle64.asu_views_c1(math.radians(4), math.radians(4), 1)
+    assert v.shape == ov.shape == (2566, 4)
+    np.testing.assert_allclose(v, ov, atol=1e-15)
+    # in-plane expansion (view_list_expand_angles): 90 angles of 4 degrees
+    ve = search.views_c1(4.0, angle_step_deg=4.0)
+    assert len(ve) == 2566 * 90
+    np.testing.assert_allclose(ve[90 * 7 + 11, :3], v[7, :3])
+    assert ve[90 * 7 + 11, 3] == pytest.approx(v[7, 3] + math.radians(44), abs=1e-12)
+    m = search.view_matrices(v[::37])
+    om = np.array([oracle64.view_matrix(x) for x in v[::37]])
+    np.testing.assert_allclose(m, om, atol=1e-15)
+
+
+def test_rotation_angle():
+    rng = np.random.default_rng(5)
+    q = synth.uniform_quaternions(20, rng)
+    m = synth.view_to_matrix(synth.quat_to_view(q))
+    assert np.allclose(search.rotation_angle_deg(m, m), 0, atol=1e-5)
+    th = math.radians(3.0)
+    rz = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    assert np.allclose(search.rotation_angle_deg(m, m @ rz), 3.0, atol=1e-9)
