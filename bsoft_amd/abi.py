@@ -132,12 +132,24 @@ SIGNATURES = {
     "bsx_search_set_reference": ([_vp, _vp, _dp, C.c_int64, _vp], C.c_int),
     "bsx_search_stage": ([_vp, _vp, C.POINTER(Particle), C.c_int64, _vp], C.c_int),
     "bsx_search_exhaustive": ([_vp, _vp, _vp, _vp], C.c_int),
+    "bsx_search_exhaustive_rows": ([_vp, _vp, C.c_int64, _vp, _vp, _vp], C.c_int),
     "bsx_search_rescore": ([_vp, _vp, C.c_int32, _vp, _vp, _vp, _vp], C.c_int),
     "bsx_search_score_mats": ([_vp, _vp, _vp, C.c_int32, _vp, _vp], C.c_int),
     "bsx_search_particle_data": ([_vp, _vp, _vp, _vp, _vp], C.c_int),
     "bsx_search_library_data": ([_vp, _vp, _vp, _vp], C.c_int),
     "bsx_search_view_mats": ([_vp, _vp, C.c_int64, _vp, _vp], C.c_int),
     "bsx_search_refine": ([_vp, _vp, _vp, _vp, _dp, C.c_int32, _vp], C.c_int),
+    "bsx_search_cnn_input": ([_vp, C.c_int32, _vp, _vp], C.c_int),
+    "bsx_cnn_create": ([C.POINTER(_vp), C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                        C.c_uint64], C.c_int),
+    "bsx_cnn_destroy": ([_vp], C.c_int),
+    "bsx_cnn_nparams": ([_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "bsx_cnn_params": ([_vp, _vp, _vp, _vp], C.c_int),
+    "bsx_cnn_forward": ([_vp, _vp, C.c_int32, _vp, _vp], C.c_int),
+    "bsx_cnn_gradients": ([_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp], C.c_int),
+    "bsx_cnn_train_step": ([_vp, _vp, _vp, C.c_int32, C.c_float, C.c_float, _vp, _vp], C.c_int),
+    "bsx_cnn_infer_topk": ([_vp, _vp, C.c_int64, C.c_int32, _vp, _vp, _vp], C.c_int),
+    "bsx_matrix_views": ([_dp, C.c_int64, _dp], C.c_int),
     "bsx_view_matrices": ([_dp, C.c_int64, _dp], C.c_int),
     "bsx_views_c1": ([C.c_double, C.c_double, C.c_int32, C.c_double, _dp, C.c_int64,
                       C.POINTER(C.c_int64)], C.c_int),

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(CSRC, "_build")
 LIB = os.path.join(HERE, "libbsx.so")
-SOURCES = ["bsx_fft.hip", "bsx_recon.hip", "bsx_extract.hip", "bsx_search.hip"]
+SOURCES = ["bsx_fft.hip", "bsx_recon.hip", "bsx_extract.hip", "bsx_search.hip",
+           "bsx_cnn.hip"]
 HEADERS = ["bsx_internal.h", "bsx_fft_dev.h", "bsx_common.h"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BSX_ARCH", "gfx950")

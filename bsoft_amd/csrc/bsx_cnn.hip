@@ -1,0 +1,655 @@
+// bsx_cnn.hip — the orientation-proposal network: a 5-layer CNN trained and
+// run by hand-written HIP kernels (no library GEMMs).
+//
+// Architecture (input S x S, one channel; S a multiple of 8):
+//   conv3x3 stride 1 (1 -> c1) + ReLU    S   x S
+//   conv3x3 stride 2 (c1 -> c2) + ReLU   S/2 x S/2
+//   conv3x3 stride 2 (c2 -> c3) + ReLU   S/4 x S/4
+//   conv3x3 stride 2 (c3 -> c4) + ReLU   S/8 x S/8
+//   fully connected (c4 (S/8)^2 -> nclasses), softmax cross-entropy
+// Parameters are one flat FP32 array (W1 b1 W2 b2 W3 b3 W4 b4 Wfc bfc; conv
+// weights [cout][cin][3][3], Wfc [class][feature]). Training: Adam on the
+// FP32 master weights. Inference: BF16 copies of the weights with FP32
+// accumulation and a fused per-row top-k of the softmax.
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+
+#include "bsx_common.h"
+
+using namespace bsx;
+
+namespace {
+
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__host__ __device__ inline uint16_t f_to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <typename T> __device__ inline float ldw(const T* p, long i);
+template <> __device__ inline float ldw<float>(const float* p, long i) { return p[i]; }
+template <> __device__ inline float ldw<uint16_t>(const uint16_t* p, long i) { return bf16_to_f(p[i]); }
+
+// ---------------------------------------------------------------- conv fwd
+// y[b][co][yo][xo] = relu(bias[co] + sum W[co][ci][ky][kx] x[b][ci][yo*s-1+ky][xo*s-1+kx]);
+// one thread per (b, 4 output channels, yo, xo).
+template <typename TW>
+__global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ x, const TW* __restrict__ W,
+                                                  const TW* __restrict__ bias, float* __restrict__ y,
+                                                  int B, int Cin, int H, int Wd, int Cout, int Ho,
+                                                  int Wo, int stride) {
+    const long total = (long)B * (Cout / 4) * Ho * Wo;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int xo = (int)(g % Wo);
+    const int yo = (int)((g / Wo) % Ho);
+    const int c4 = (int)((g / ((long)Wo * Ho)) % (Cout / 4));
+    const int b = (int)(g / ((long)Wo * Ho * (Cout / 4)));
+    const int co = 4 * c4;
+    float acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = ldw(bias, co + q);
+    const float* xb = x + (long)b * Cin * H * Wd;
+    for (int ci = 0; ci < Cin; ++ci) {
+        const float* xc = xb + (long)ci * H * Wd;
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) {
+            const int yi = yo * stride - 1 + ky;
+            if (yi < 0 || yi >= H) continue;
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const int xi = xo * stride - 1 + kx;
+                if (xi < 0 || xi >= Wd) continue;
+                const float v = xc[yi * Wd + xi];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    acc[q] = fmaf(ldw(W, (((long)(co + q) * Cin + ci) * 3 + ky) * 3 + kx), v, acc[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        y[(((long)b * Cout + co + q) * Ho + yo) * Wo + xo] = fmaxf(acc[q], 0.f);
+}
+
+// g <- g * (y > 0): the ReLU backward, in place.
+__global__ void k_relu_bwd(float* __restrict__ g, const float* __restrict__ y, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        if (!(y[i] > 0.f)) g[i] = 0.f;
+}
+
+// dx[b][ci][yi][xi] = sum_{co,ky,kx} W[co][ci][ky][kx] g[b][co][yo][xo],
+// yo = (yi + 1 - ky) / s when divisible and in range.
+__global__ __launch_bounds__(256) void k_conv_bwd_data(const float* __restrict__ g,
+                                                       const float* __restrict__ W,
+                                                       float* __restrict__ dx, int B, int Cin, int H,
+                                                       int Wd, int Cout, int Ho, int Wo, int stride) {
+    const long total = (long)B * Cin * H * Wd;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int xi = (int)(t % Wd);
+    const int yi = (int)((t / Wd) % H);
+    const int ci = (int)((t / ((long)Wd * H)) % Cin);
+    const int b = (int)(t / ((long)Wd * H * Cin));
+    float acc = 0.f;
+    for (int co = 0; co < Cout; ++co) {
+        const float* gc = g + ((long)b * Cout + co) * Ho * Wo;
+        const float* wc = W + ((long)co * Cin + ci) * 9;
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) {
+            const int ny = yi + 1 - ky;
+            if (ny < 0 || ny % stride) continue;
+            const int yo = ny / stride;
+            if (yo >= Ho) continue;
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const int nx = xi + 1 - kx;
+                if (nx < 0 || nx % stride) continue;
+                const int xo = nx / stride;
+                if (xo >= Wo) continue;
+                acc = fmaf(wc[ky * 3 + kx], gc[yo * Wo + xo], acc);
+            }
+        }
+    }
+    dx[t] = acc;
+}
+
+// dW[co][ci][9] = sum_{b,yo,xo} g[b][co][yo][xo] x[b][ci][yo*s-1+ky][xo*s-1+kx];
+// one workgroup per (co, ci); blockIdx.y == Cin gives db[co] = sum g.
+__global__ __launch_bounds__(256) void k_conv_bwd_weight(const float* __restrict__ g,
+                                                         const float* __restrict__ x,
+                                                         float* __restrict__ dW, float* __restrict__ db,
+                                                         int B, int Cin, int H, int Wd, int Cout,
+                                                         int Ho, int Wo, int stride) {
+    __shared__ float red[10][256];
+    const int co = blockIdx.x, ci = blockIdx.y;
+    float acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = 0.f;
+    float sb = 0.f;
+    const long per = (long)Ho * Wo;
+    for (long e = threadIdx.x; e < (long)B * per; e += blockDim.x) {
+        const int b = (int)(e / per);
+        const int r = (int)(e % per);
+        const int yo = r / Wo, xo = r % Wo;
+        const float gv = g[((long)b * Cout + co) * per + r];
+        if (ci == Cin) {
+            sb += gv;
+            continue;
+        }
+        const float* xc = x + ((long)b * Cin + ci) * H * Wd;
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky) {
+            const int yi = yo * stride - 1 + ky;
+            if (yi < 0 || yi >= H) continue;
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const int xi = xo * stride - 1 + kx;
+                if (xi < 0 || xi >= Wd) continue;
+                acc[ky * 3 + kx] = fmaf(gv, xc[yi * Wd + xi], acc[ky * 3 + kx]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) red[q][threadIdx.x] = acc[q];
+    red[9][threadIdx.x] = sb;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int q = 0; q < 10; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (ci == Cin) db[co] = red[9][0];
+        else
+            for (int q = 0; q < 9; ++q) dW[((long)co * Cin + ci) * 9 + q] = red[q][0];
+    }
+}
+
+// ------------------------------------------------------------------ GEMM
+// C[m][n] = sum_k A[m*am + k*ak] * B[k*bk + n*bn] (+ bias[n]); 64x64 tiles,
+// K step 16 through LDS, 4x4 outputs per thread, edges guarded.
+template <typename TB>
+__global__ __launch_bounds__(256) void k_sgemm(int M, int N, int K, const float* __restrict__ A,
+                                               long am, long ak, const TB* __restrict__ Bm, long bk,
+                                               long bn, const TB* __restrict__ bias,
+                                               float* __restrict__ Cm, long ldc) {
+    __shared__ float As[16][64 + 4];
+    __shared__ float Bs[16][64 + 4];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = threadIdx.x + 256 * q;          // 0..1023
+            // A tile: 64 rows x 16 k; consecutive threads walk k when ak == 1
+            const int ar = ak == 1 ? e / 16 : e % 64, ac = ak == 1 ? e % 16 : e / 64;
+            const int gm = m0 + ar, gk = k0 + ac;
+            As[ac][ar] = (gm < M && gk < K) ? A[gm * am + gk * ak] : 0.f;
+            const int br = bn == 1 ? e / 64 : e % 16, bc = bn == 1 ? e % 64 : e / 16;
+            const int gk2 = k0 + br, gn = n0 + bc;
+            Bs[br][bc] = (gk2 < K && gn < N) ? ldw(Bm, gk2 * bk + gn * bn) : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int gm = m0 + ty * 4 + i;
+        if (gm >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gn = n0 + tx * 4 + j;
+            if (gn >= N) continue;
+            Cm[(long)gm * ldc + gn] = acc[i][j] + (bias ? ldw(bias, gn) : 0.f);
+        }
+    }
+}
+
+// db[n] = sum_m g[m][n]
+__global__ void k_colsum(const float* __restrict__ g, int M, int N, float* __restrict__ db) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.f;
+    for (int m = 0; m < M; ++m) s += g[(long)m * N + n];
+    db[n] = s;
+}
+
+// ------------------------------------------------------ softmax / top-k
+// One workgroup per row: loss[b] = logsumexp - logit[label]; dl = (p - onehot) / B.
+__global__ __launch_bounds__(256) void k_softmax_ce(const float* __restrict__ logits,
+                                                    const int* __restrict__ labels, int B, int C,
+                                                    float* __restrict__ dl, float* __restrict__ loss) {
+    __shared__ float red[256];
+    const int b = blockIdx.x;
+    const float* l = logits + (long)b * C;
+    float mx = -INFINITY;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) mx = fmaxf(mx, l[c]);
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    mx = red[0];
+    __syncthreads();
+    float se = 0.f;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) se += expf(l[c] - mx);
+    red[threadIdx.x] = se;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    se = red[0];
+    const int t = labels[b];
+    const float inv = 1.f / (float)B;
+    for (int c = threadIdx.x; c < C; c += blockDim.x)
+        dl[(long)b * C + c] = (expf(l[c] - mx) / se - (c == t ? 1.f : 0.f)) * inv;
+    if (threadIdx.x == 0 && loss) loss[b] = logf(se) + mx - l[t];
+}
+
+// One workgroup per row: the k largest logits (k <= 32) as classes and
+// softmax probabilities, best first (ties: lower class).
+__global__ __launch_bounds__(256) void k_row_topk(const float* __restrict__ logits, int C, int k,
+                                                  int* __restrict__ cls, float* __restrict__ prob) {
+    __shared__ float sv[256];
+    __shared__ int si[256];
+    __shared__ float red[256];
+    const long b = blockIdx.x;
+    const float* l = logits + b * C;
+    constexpr int T = 32;
+    float best[T];
+    int bi[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) { best[q] = -INFINITY; bi[q] = 0x7fffffff; }
+    float mx = -INFINITY;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+        const float v = l[c];
+        mx = fmaxf(mx, v);
+        if (v > best[T - 1]) {
+            best[T - 1] = v;
+            bi[T - 1] = c;
+#pragma unroll
+            for (int q = T - 1; q > 0; --q)
+                if (best[q] > best[q - 1]) {
+                    const float tv = best[q]; best[q] = best[q - 1]; best[q - 1] = tv;
+                    const int ti = bi[q]; bi[q] = bi[q - 1]; bi[q - 1] = ti;
+                }
+        }
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    mx = red[0];
+    __syncthreads();
+    float se = 0.f;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) se += expf(l[c] - mx);
+    red[threadIdx.x] = se;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    se = red[0];
+    // k rounds of a block-wide arg-max over the heads of the per-thread lists
+    int head = 0;
+    for (int r = 0; r < k; ++r) {
+        float hv = -INFINITY;
+        int hi = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < T; ++q)
+            if (q == head) { hv = best[q]; hi = bi[q]; }
+        sv[threadIdx.x] = hv;
+        si[threadIdx.x] = hi;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                const float ov = sv[threadIdx.x + s];
+                const int oi = si[threadIdx.x + s];
+                if (ov > sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                    sv[threadIdx.x] = ov;
+                    si[threadIdx.x] = oi;
+                }
+            }
+            __syncthreads();
+        }
+        const int wi = si[0];
+        const float wv = sv[0];
+        if (threadIdx.x == 0) {
+            cls[b * k + r] = wi < C ? wi : -1;
+            prob[b * k + r] = wi < C ? expf(wv - mx) / se : 0.f;
+        }
+        if (hi == wi && head < T) ++head;
+        __syncthreads();
+    }
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, float wd) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float gi = g[i] + wd * p[i];
+        const float mi = b1 * m[i] + (1 - b1) * gi;
+        const float vi = b2 * v[i] + (1 - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ p, uint16_t* __restrict__ q, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        q[i] = f_to_bf16(p[i]);
+}
+
+unsigned nblk(long n, int t = 256) { return (unsigned)std::min<long>((n + t - 1) / t, 1l << 30); }
+
+}  // namespace
+
+// ================================================================ host
+struct bsx_cnn {
+    int S = 0, ch[5] = {1, 0, 0, 0, 0}, nclass = 0, maxb = 0;
+    int hw[5];                 // spatial edge per activation level
+    long woff[5], boff[5];     // offsets of W/b of layers 1..4 and the FC (index 4)
+    long nparam = 0, feat = 0;
+    float* d_p = nullptr;      // parameters
+    float* d_g = nullptr;      // gradients
+    float* d_m = nullptr;
+    float* d_v = nullptr;
+    uint16_t* d_pb = nullptr;  // BF16 copy for inference
+    bool pb_valid = false;
+    float* d_act[5] = {};      // activations 1..4 (index 1..4), logits at [0]
+    float* d_gact[5] = {};
+    float* d_loss = nullptr;
+    long step = 0;
+};
+
+namespace {
+
+void free_cnn(bsx_cnn* h) {
+    void* ptrs[] = {h->d_p, h->d_g, h->d_m, h->d_v, h->d_pb, h->d_loss};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (int i = 0; i < 5; ++i) {
+        if (h->d_act[i]) (void)hipFree(h->d_act[i]);
+        if (h->d_gact[i]) (void)hipFree(h->d_gact[i]);
+    }
+}
+
+long act_size(const bsx_cnn* h, int l, int B) {
+    if (l == 0) return (long)B * h->nclass;
+    return (long)B * h->ch[l] * h->hw[l] * h->hw[l];
+}
+
+// splitmix64 + Box-Muller: reproducible He-normal initialisation
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double normal() { return sqrt(-2 * log(uniform())) * cos(2 * M_PI * uniform()); }
+};
+
+template <typename TW>
+int forward(bsx_cnn* h, const float* x, int B, const TW* P, hipStream_t s) {
+    const float* in = x;
+    for (int l = 1; l <= 4; ++l) {
+        const int Cin = h->ch[l - 1], Cout = h->ch[l], H = h->hw[l - 1], Ho = h->hw[l];
+        const int stride = l == 1 ? 1 : 2;
+        const long tot = (long)B * (Cout / 4) * Ho * Ho;
+        hipLaunchKernelGGL((k_conv_fwd<TW>), dim3(nblk(tot)), dim3(256), 0, s, in, P + h->woff[l - 1],
+                           P + h->boff[l - 1], h->d_act[l], B, Cin, H, H, Cout, Ho, Ho, stride);
+        BSX_HIP(hipGetLastError());
+        in = h->d_act[l];
+    }
+    const int F = (int)h->feat, C = h->nclass;
+    dim3 g((C + 63) / 64, (B + 63) / 64);
+    hipLaunchKernelGGL((k_sgemm<TW>), g, dim3(256), 0, s, B, C, F, h->d_act[4], (long)F, 1l,
+                       P + h->woff[4], 1l, (long)F, P + h->boff[4], h->d_act[0], (long)C);
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int backward(bsx_cnn* h, const float* x, const int* labels, int B, hipStream_t s) {
+    const int F = (int)h->feat, C = h->nclass;
+    hipLaunchKernelGGL(k_softmax_ce, dim3(B), dim3(256), 0, s, h->d_act[0], labels, B, C,
+                       h->d_gact[0], h->d_loss);
+    BSX_HIP(hipGetLastError());
+    float* P = h->d_p;
+    float* G = h->d_g;
+    // FC: dW = dl^T x, db = colsum(dl), dx = dl W
+    {
+        dim3 g((F + 63) / 64, (C + 63) / 64);
+        hipLaunchKernelGGL((k_sgemm<float>), g, dim3(256), 0, s, C, F, B, h->d_gact[0], 1l, (long)C,
+                           h->d_act[4], (long)F, 1l, (const float*)nullptr, G + h->woff[4], (long)F);
+        BSX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_colsum, dim3((C + 255) / 256), dim3(256), 0, s, h->d_gact[0], B, C,
+                           G + h->boff[4]);
+        BSX_HIP(hipGetLastError());
+        dim3 g2((F + 63) / 64, (B + 63) / 64);
+        hipLaunchKernelGGL((k_sgemm<float>), g2, dim3(256), 0, s, B, F, C, h->d_gact[0], (long)C, 1l,
+                           P + h->woff[4], (long)F, 1l, (const float*)nullptr, h->d_gact[4], (long)F);
+        BSX_HIP(hipGetLastError());
+    }
+    for (int l = 4; l >= 1; --l) {
+        const int Cin = h->ch[l - 1], Cout = h->ch[l], H = h->hw[l - 1], Ho = h->hw[l];
+        const int stride = l == 1 ? 1 : 2;
+        const long na = act_size(h, l, B);
+        hipLaunchKernelGGL(k_relu_bwd, dim3(nblk(na)), dim3(256), 0, s, h->d_gact[l], h->d_act[l], na);
+        BSX_HIP(hipGetLastError());
+        const float* in = l == 1 ? x : h->d_act[l - 1];
+        hipLaunchKernelGGL(k_conv_bwd_weight, dim3(Cout, Cin + 1), dim3(256), 0, s, h->d_gact[l], in,
+                           G + h->woff[l - 1], G + h->boff[l - 1], B, Cin, H, H, Cout, Ho, Ho, stride);
+        BSX_HIP(hipGetLastError());
+        if (l > 1) {
+            const long nx = (long)B * Cin * H * H;
+            hipLaunchKernelGGL(k_conv_bwd_data, dim3(nblk(nx)), dim3(256), 0, s, h->d_gact[l],
+                               P + h->woff[l - 1], h->d_gact[l - 1], B, Cin, H, H, Cout, Ho, Ho, stride);
+            BSX_HIP(hipGetLastError());
+        }
+    }
+    return BSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsx_cnn_create(bsx_cnn** out, int32_t in_size, const int32_t* channels, int32_t nclasses,
+                   int32_t max_batch, uint64_t seed) {
+    BSX_REQUIRE(out && channels, BSX_ERR_ARG, "bsx_cnn_create: null argument");
+    *out = nullptr;
+    BSX_REQUIRE(in_size >= 8 && in_size <= 256 && in_size % 8 == 0, BSX_ERR_ARG,
+                "bsx_cnn_create: input size %d must be a multiple of 8 in 8..256", in_size);
+    for (int i = 0; i < 4; ++i)
+        BSX_REQUIRE(channels[i] >= 4 && channels[i] <= 1024 && channels[i] % 4 == 0, BSX_ERR_ARG,
+                    "bsx_cnn_create: channel count %d must be a multiple of 4 in 4..1024", channels[i]);
+    BSX_REQUIRE(nclasses >= 2 && nclasses <= (1 << 20), BSX_ERR_ARG,
+                "bsx_cnn_create: %d classes out of range", nclasses);
+    BSX_REQUIRE(max_batch >= 1 && max_batch <= 65536, BSX_ERR_ARG,
+                "bsx_cnn_create: max_batch %d out of range", max_batch);
+    bsx_cnn* h = new bsx_cnn();
+    h->S = in_size;
+    for (int i = 0; i < 4; ++i) h->ch[i + 1] = channels[i];
+    h->nclass = nclasses;
+    h->maxb = max_batch;
+    h->hw[0] = h->hw[1] = in_size;
+    h->hw[2] = in_size / 2;
+    h->hw[3] = in_size / 4;
+    h->hw[4] = in_size / 8;
+    h->feat = (long)h->ch[4] * h->hw[4] * h->hw[4];
+    long o = 0;
+    for (int l = 0; l < 4; ++l) {
+        h->woff[l] = o;
+        o += (long)h->ch[l + 1] * h->ch[l] * 9;
+        h->boff[l] = o;
+        o += h->ch[l + 1];
+    }
+    h->woff[4] = o;
+    o += h->feat * nclasses;
+    h->boff[4] = o;
+    o += nclasses;
+    h->nparam = o;
+    // He-normal initialisation, zero biases
+    std::vector<float> p((size_t)o, 0.f);
+    Rng r{seed ? seed : 0x5eedull};
+    for (int l = 0; l < 5; ++l) {
+        const long fan_in = l < 4 ? (long)h->ch[l] * 9 : h->feat;
+        const long nw = l < 4 ? (long)h->ch[l + 1] * h->ch[l] * 9 : h->feat * nclasses;
+        const double sd = sqrt(2.0 / fan_in) * (l == 4 ? 0.5 : 1.0);
+        for (long i = 0; i < nw; ++i) p[h->woff[l] + i] = (float)(sd * r.normal());
+    }
+    auto fail = [&](hipError_t e, const char* what) {
+        set_error("bsx_cnn_create: %s: %s", what, hipGetErrorString(e));
+        free_cnn(h);
+        delete h;
+        return BSX_ERR_NOMEM;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&h->d_p, sizeof(float) * o)) != hipSuccess) return fail(e, "parameters");
+    if ((e = hipMalloc(&h->d_g, sizeof(float) * o)) != hipSuccess) return fail(e, "gradients");
+    if ((e = hipMalloc(&h->d_m, sizeof(float) * o)) != hipSuccess) return fail(e, "moments");
+    if ((e = hipMalloc(&h->d_v, sizeof(float) * o)) != hipSuccess) return fail(e, "moments");
+    if ((e = hipMalloc(&h->d_pb, sizeof(uint16_t) * o)) != hipSuccess) return fail(e, "bf16 copy");
+    if ((e = hipMalloc(&h->d_loss, sizeof(float) * max_batch)) != hipSuccess) return fail(e, "loss");
+    for (int l = 0; l < 5; ++l) {
+        const long n = act_size(h, l, max_batch);
+        if ((e = hipMalloc(&h->d_act[l], sizeof(float) * n)) != hipSuccess) return fail(e, "activations");
+        if ((e = hipMalloc(&h->d_gact[l], sizeof(float) * n)) != hipSuccess) return fail(e, "activations");
+    }
+    if ((e = hipMemcpy(h->d_p, p.data(), sizeof(float) * o, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(h->d_m, 0, sizeof(float) * o)) != hipSuccess ||
+        (e = hipMemset(h->d_v, 0, sizeof(float) * o)) != hipSuccess ||
+        (e = hipMemset(h->d_g, 0, sizeof(float) * o)) != hipSuccess)
+        return fail(e, "initialisation");
+    *out = h;
+    return BSX_OK;
+}
+
+int bsx_cnn_destroy(bsx_cnn* h) {
+    if (!h) return BSX_OK;
+    free_cnn(h);
+    delete h;
+    return BSX_OK;
+}
+
+int bsx_cnn_nparams(const bsx_cnn* h, int64_t* n, int64_t* feature_size) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_cnn_nparams: null handle");
+    if (n) *n = h->nparam;
+    if (feature_size) *feature_size = h->feat;
+    return BSX_OK;
+}
+
+int bsx_cnn_params(bsx_cnn* h, float* get_dev, const float* set_dev, void* stream) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_cnn_params: null handle");
+    hipStream_t s = (hipStream_t)stream;
+    if (get_dev)
+        BSX_HIP(hipMemcpyAsync(get_dev, h->d_p, sizeof(float) * h->nparam, hipMemcpyDeviceToDevice, s));
+    if (set_dev) {
+        BSX_HIP(hipMemcpyAsync(h->d_p, set_dev, sizeof(float) * h->nparam, hipMemcpyDeviceToDevice, s));
+        h->pb_valid = false;
+    }
+    return BSX_OK;
+}
+
+int bsx_cnn_forward(bsx_cnn* h, const float* x_dev, int32_t batch, float* logits_dev, void* stream) {
+    BSX_REQUIRE(h && x_dev, BSX_ERR_ARG, "bsx_cnn_forward: null argument");
+    BSX_REQUIRE(batch >= 1 && batch <= h->maxb, BSX_ERR_ARG, "bsx_cnn_forward: batch %d out of 1..%d",
+                batch, h->maxb);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = forward<float>(h, x_dev, batch, h->d_p, s);
+    if (rc) return rc;
+    if (logits_dev)
+        BSX_HIP(hipMemcpyAsync(logits_dev, h->d_act[0], sizeof(float) * batch * h->nclass,
+                               hipMemcpyDeviceToDevice, s));
+    return BSX_OK;
+}
+
+int bsx_cnn_gradients(bsx_cnn* h, const float* x_dev, const int32_t* labels_dev, int32_t batch,
+                      float* grad_dev, float* loss_dev, void* stream) {
+    BSX_REQUIRE(h && x_dev && labels_dev, BSX_ERR_ARG, "bsx_cnn_gradients: null argument");
+    BSX_REQUIRE(batch >= 1 && batch <= h->maxb, BSX_ERR_ARG,
+                "bsx_cnn_gradients: batch %d out of 1..%d", batch, h->maxb);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = forward<float>(h, x_dev, batch, h->d_p, s);
+    if (!rc) rc = backward(h, x_dev, labels_dev, batch, s);
+    if (rc) return rc;
+    if (grad_dev)
+        BSX_HIP(hipMemcpyAsync(grad_dev, h->d_g, sizeof(float) * h->nparam, hipMemcpyDeviceToDevice, s));
+    if (loss_dev)
+        BSX_HIP(hipMemcpyAsync(loss_dev, h->d_loss, sizeof(float) * batch, hipMemcpyDeviceToDevice, s));
+    return BSX_OK;
+}
+
+int bsx_cnn_train_step(bsx_cnn* h, const float* x_dev, const int32_t* labels_dev, int32_t batch,
+                       float lr, float weight_decay, float* loss_dev, void* stream) {
+    BSX_REQUIRE(lr > 0 && std::isfinite(lr) && weight_decay >= 0, BSX_ERR_ARG,
+                "bsx_cnn_train_step: bad learning rate / weight decay");
+    int rc = bsx_cnn_gradients(h, x_dev, labels_dev, batch, nullptr, loss_dev, stream);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    h->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)h->step), c2 = 1.f - powf(b2, (float)h->step);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(h->nparam)), dim3(256), 0, s, h->d_p, h->d_g, h->d_m, h->d_v,
+                       h->nparam, lr, b1, b2, 1e-8f, c1, c2, weight_decay);
+    BSX_HIP(hipGetLastError());
+    h->pb_valid = false;
+    return BSX_OK;
+}
+
+int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int32_t* cls_dev,
+                       float* prob_dev, void* stream) {
+    BSX_REQUIRE(h && (n == 0 || (x_dev && cls_dev && prob_dev)), BSX_ERR_ARG,
+                "bsx_cnn_infer_topk: null argument");
+    BSX_REQUIRE(k >= 1 && k <= 32 && k <= h->nclass, BSX_ERR_ARG, "bsx_cnn_infer_topk: k %d out of range", k);
+    hipStream_t s = (hipStream_t)stream;
+    if (!h->pb_valid) {
+        hipLaunchKernelGGL(k_to_bf16, dim3(nblk(h->nparam)), dim3(256), 0, s, h->d_p, h->d_pb, h->nparam);
+        BSX_HIP(hipGetLastError());
+        h->pb_valid = true;
+    }
+    const long in = (long)h->S * h->S;
+    for (int64_t b0 = 0; b0 < n; b0 += h->maxb) {
+        const int nb = (int)std::min<int64_t>(h->maxb, n - b0);
+        int rc = forward<uint16_t>(h, x_dev + b0 * in, nb, h->d_pb, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_row_topk, dim3(nb), dim3(256), 0, s, h->d_act[0], h->nclass, k,
+                           cls_dev + b0 * k, prob_dev + b0 * k);
+        BSX_HIP(hipGetLastError());
+    }
+    return BSX_OK;
+}
+
+}  // extern "C"

@@ -28,6 +28,7 @@
 //   score = 1/(R-1) * sum_{j=1}^{R-1} sum_{i in j} Re(a_i conj b_i) / (|a|_j |b|_j),
 // i.e. the mean ring correlation with the CTF moved from the section to the
 // particle; it ranks candidates for the FP32 FRC, which decides.
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <algorithm>
@@ -498,6 +499,62 @@ __global__ void k_view_mats(const double* __restrict__ lm, int64_t nviews,
         out[9 * p + i] = (v >= 0 && v < nviews) ? lm[9 * (int64_t)v + i] : (i % 4 == 0 ? 1.0 : 0.0);
 }
 
+// Input image of the proposal network: disk values phase-flipped by the
+// sign of the CTF on an S x S grid (mates filled by Hermitian symmetry,
+// centre shifted to S/2). gidx[g]: disk index, -(i+1) for the conjugate
+// mate, INT_MIN outside the disk.
+__global__ void k_cnn_grid(const float2* __restrict__ fp, const float* __restrict__ ctf, int Nd,
+                           const int* __restrict__ gidx, int S, int64_t p0, int64_t nb,
+                           float2* __restrict__ grid) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t S2 = (int64_t)S * S;
+    if (t >= nb * S2) return;
+    const int64_t p = p0 + t / S2;
+    const int g = (int)(t % S2);
+    const int code = gidx[g];
+    float2 v = make_float2(0.f, 0.f);
+    if (code != INT_MIN) {
+        const int i = code >= 0 ? code : -code - 1;
+        const float2 F = fp[p * Nd + i];
+        const float sg = ctf[p * Nd + i] < 0.f ? -1.f : 1.f;
+        v = make_float2(F.x * sg, code >= 0 ? F.y * sg : -F.y * sg);
+        const int xx = g % S, yy = g / S;
+        if ((xx + yy) & 1) { v.x = -v.x; v.y = -v.y; }
+    }
+    grid[t] = v;
+}
+
+// Real part of the inverse transform, normalised to zero mean and unit
+// variance per image (one workgroup per image, FP64 sums).
+__global__ __launch_bounds__(256) void k_cnn_norm(const float2* __restrict__ grid, int S,
+                                                  int64_t p0, float* __restrict__ out) {
+    __shared__ double rs[256], rq[256];
+    const int64_t b = blockIdx.x;
+    const int S2 = S * S;
+    const float2* G = grid + b * S2;
+    double s = 0, q = 0;
+    for (int i = threadIdx.x; i < S2; i += blockDim.x) {
+        const double v = G[i].x;
+        s += v;
+        q += v * v;
+    }
+    rs[threadIdx.x] = s;
+    rq[threadIdx.x] = q;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            rs[threadIdx.x] += rs[threadIdx.x + st];
+            rq[threadIdx.x] += rq[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    const double mean = rs[0] / S2;
+    const double var = rq[0] / S2 - mean * mean;
+    const float inv = var > 0 ? (float)(1.0 / sqrt(var)) : 0.f;
+    float* O = out + (p0 + b) * S2;
+    for (int i = threadIdx.x; i < S2; i += blockDim.x) O[i] = (float)(G[i].x - mean) * inv;
+}
+
 __global__ void k_real_to_complex_b(const float* __restrict__ in, float2* __restrict__ out,
                                     long long n) {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -536,6 +593,8 @@ struct bsx_search {
     int64_t lmats_cap = 0;
     float* d_lfrc = nullptr;
     int64_t lfrc_cap = 0;
+    uint16_t* d_sub = nullptr;       // gathered rows for a partial exhaustive search
+    int64_t sub_cap = 0;
     float* d_ktab = nullptr;
     float* d_pscore = nullptr;
     int* d_pidx = nullptr;
@@ -637,7 +696,7 @@ int bsx_search_destroy(bsx_search* h) {
     void* ptrs[] = {h->d_hk,   h->d_ring,    h->d_lib,    h->d_lbf,    h->d_fp,
                     h->d_ctf,  h->d_pbf,     h->d_spec,   h->d_sp,
                     h->d_ktab, h->d_pscore,  h->d_pidx,   h->d_libmats, h->d_lmats,
-                    h->d_lfrc};
+                    h->d_lfrc, h->d_sub};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     delete h;
@@ -757,13 +816,14 @@ int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* p
     return BSX_OK;
 }
 
-int bsx_search_exhaustive(bsx_search* h, int32_t* cand_dev, float* score_dev, void* stream) {
-    BSX_REQUIRE(h && cand_dev && score_dev, BSX_ERR_ARG, "bsx_search_exhaustive: null argument");
-    BSX_REQUIRE(h->nviews > 0, BSX_ERR_STATE, "bsx_search_exhaustive: no reference views");
-    if (h->nparts == 0) return BSX_OK;
-    hipStream_t s = (hipStream_t)stream;
+namespace {
+
+// GEMM + top-k of `nrows` BF16 rows at A (allocated to a multiple of GM rows)
+// against the whole library.
+int run_exhaustive(bsx_search* h, const uint16_t* A, int64_t nrows, int32_t* cand_dev,
+                   float* score_dev, hipStream_t s) {
     const int T = h->p.topk <= 8 ? 8 : 16;
-    const int64_t rblocks = round_up(h->nparts, GM) / GM;
+    const int64_t rblocks = round_up(nrows, GM) / GM;
     const int ntiles = (int)(round_up(h->nviews, GN) / GN);
     // enough segments to give the chip >= 2048 workgroups, at least 4 tiles each
     int nseg = (int)std::max<int64_t>(1, std::min<int64_t>(ntiles / 4 > 0 ? ntiles / 4 : 1,
@@ -775,24 +835,64 @@ int bsx_search_exhaustive(bsx_search* h, int32_t* cand_dev, float* score_dev, vo
     int rc = grow(h, &h->d_pscore, &h->pscore_cap, need, "top-k partials");
     if (!rc) rc = grow(h, &h->d_pidx, &h->pidx_cap, need, "top-k partials");
     if (rc) return rc;
-    BSX_REQUIRE(rblocks < 65536 * 32768ll, BSX_ERR_ARG, "bsx_search_exhaustive: too many particles");
+    BSX_REQUIRE(rblocks < (1ll << 31), BSX_ERR_ARG, "bsx_search_exhaustive: too many particles");
     dim3 grid((unsigned)rblocks, (unsigned)nseg);
     if (T == 8)
-        hipLaunchKernelGGL((k_gemm_topk<8>), grid, dim3(256), 0, s, h->d_pbf, h->d_lbf, h->Kp,
+        hipLaunchKernelGGL((k_gemm_topk<8>), grid, dim3(256), 0, s, A, h->d_lbf, h->Kp,
                            (int)h->nviews, tps, ntiles, h->d_pscore, h->d_pidx);
     else
-        hipLaunchKernelGGL((k_gemm_topk<16>), grid, dim3(256), 0, s, h->d_pbf, h->d_lbf, h->Kp,
+        hipLaunchKernelGGL((k_gemm_topk<16>), grid, dim3(256), 0, s, A, h->d_lbf, h->Kp,
                            (int)h->nviews, tps, ntiles, h->d_pscore, h->d_pidx);
     BSX_HIP(hipGetLastError());
-    const unsigned mg = (unsigned)((h->nparts + 255) / 256);
+    const unsigned mg = (unsigned)((nrows + 255) / 256);
     if (T == 8)
         hipLaunchKernelGGL((k_topk_merge<8>), dim3(mg), dim3(256), 0, s, h->d_pscore, h->d_pidx,
-                           2 * nseg, h->nparts, h->p.topk, cand_dev, score_dev);
+                           2 * nseg, nrows, h->p.topk, cand_dev, score_dev);
     else
         hipLaunchKernelGGL((k_topk_merge<16>), dim3(mg), dim3(256), 0, s, h->d_pscore, h->d_pidx,
-                           2 * nseg, h->nparts, h->p.topk, cand_dev, score_dev);
+                           2 * nseg, nrows, h->p.topk, cand_dev, score_dev);
     BSX_HIP(hipGetLastError());
     return BSX_OK;
+}
+
+__global__ void k_gather_rows(const uint16_t* __restrict__ src, const int* __restrict__ rows,
+                              int64_t m, int64_t nsrc, int Kp, uint16_t* __restrict__ dst) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = Kp / 8;
+    if (t >= m * per) return;
+    const int64_t r = t / per, c = t % per;
+    const int64_t q = rows[r];
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (q >= 0 && q < nsrc) v = *(const uint4*)(src + q * Kp + c * 8);
+    *(uint4*)(dst + r * Kp + c * 8) = v;
+}
+
+}  // namespace
+
+int bsx_search_exhaustive(bsx_search* h, int32_t* cand_dev, float* score_dev, void* stream) {
+    BSX_REQUIRE(h && cand_dev && score_dev, BSX_ERR_ARG, "bsx_search_exhaustive: null argument");
+    BSX_REQUIRE(h->nviews > 0, BSX_ERR_STATE, "bsx_search_exhaustive: no reference views");
+    if (h->nparts == 0) return BSX_OK;
+    return run_exhaustive(h, h->d_pbf, h->nparts, cand_dev, score_dev, (hipStream_t)stream);
+}
+
+int bsx_search_exhaustive_rows(bsx_search* h, const int32_t* rows_dev, int64_t m,
+                               int32_t* cand_dev, float* score_dev, void* stream) {
+    BSX_REQUIRE(h && (m == 0 || (rows_dev && cand_dev && score_dev)), BSX_ERR_ARG,
+                "bsx_search_exhaustive_rows: null argument");
+    BSX_REQUIRE(h->nviews > 0, BSX_ERR_STATE, "bsx_search_exhaustive_rows: no reference views");
+    BSX_REQUIRE(m >= 0 && m <= h->nparts, BSX_ERR_ARG,
+                "bsx_search_exhaustive_rows: %lld rows of %lld staged", (long long)m,
+                (long long)h->nparts);
+    if (m == 0) return BSX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = grow(h, &h->d_sub, &h->sub_cap, round_up(m, GM) * h->Kp, "row subset");
+    if (rc) return rc;
+    const int64_t tot = m * (h->Kp / 8);
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, h->d_pbf,
+                       rows_dev, m, h->nparts, h->Kp, h->d_sub);
+    BSX_HIP(hipGetLastError());
+    return run_exhaustive(h, h->d_sub, m, cand_dev, score_dev, s);
 }
 
 int bsx_search_rescore(bsx_search* h, const int32_t* cand_dev, int32_t ncand, float* frc_dev,
@@ -903,6 +1003,50 @@ int bsx_search_refine(bsx_search* h, const float* ft_dev, double* mats_dev, floa
     return BSX_OK;
 }
 
+int bsx_search_cnn_input(bsx_search* h, int32_t S, float* out_dev, void* stream) {
+    BSX_REQUIRE(h && (out_dev || h->nparts == 0), BSX_ERR_ARG, "bsx_search_cnn_input: null argument");
+    BSX_REQUIRE(S >= 8 && S <= 256 && S % 2 == 0 && fft_length_supported(S), BSX_ERR_ARG,
+                "bsx_search_cnn_input: size %d unsupported", S);
+    const int64_t n = h->nparts;
+    if (n == 0) return BSX_OK;
+    // grid point -> disk point table
+    std::vector<int> gidx((size_t)S * S, INT_MIN);
+    for (int i = 0; i < h->Nd; ++i) {
+        const int hh = h->hk[i].x, kk = h->hk[i].y;
+        if (2 * std::abs(hh) >= S || 2 * std::abs(kk) >= S) continue;
+        gidx[(size_t)((kk + S) % S) * S + (hh + S) % S] = i;
+        if (i > 0) gidx[(size_t)((-kk + S) % S) * S + (-hh + S) % S] = -i - 1;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int* dg = nullptr;
+    int64_t gcap = 0;
+    int rc = grow(h, &dg, &gcap, (int64_t)S * S, "network input table");
+    if (rc) return rc;
+    const int64_t S2 = (int64_t)S * S;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n, (256ll << 20) / (8 * S2)));
+    rc = grow(h, &h->d_spec, &h->spec_cap, chunk * S2, "network input grids");
+    if (rc == BSX_OK &&
+        hipMemcpyAsync(dg, gidx.data(), sizeof(int) * S2, hipMemcpyHostToDevice, s) != hipSuccess) {
+        set_error("bsx_search_cnn_input: table upload failed");
+        rc = BSX_ERR_DEVICE;
+    }
+    for (int64_t p0 = 0; rc == BSX_OK && p0 < n; p0 += chunk) {
+        const int64_t nb = std::min(chunk, n - p0);
+        hipLaunchKernelGGL(k_cnn_grid, dim3((unsigned)((nb * S2 + 255) / 256)), dim3(256), 0, s,
+                           h->d_fp, h->d_ctf, h->Nd, dg, S, p0, nb, h->d_spec);
+        if (hipGetLastError() != hipSuccess) { set_error("bsx_search_cnn_input: launch failed"); rc = BSX_ERR_DEVICE; break; }
+        rc = fft_axis_ex(h->d_spec, nullptr, S, S, 1, nb, 0, +1, 1.0f, s);
+        if (!rc) rc = fft_axis_ex(h->d_spec, nullptr, S, S, 1, nb, 1, +1, 1.0f, s);
+        if (rc) break;
+        hipLaunchKernelGGL(k_cnn_norm, dim3((unsigned)nb), dim3(256), 0, s, h->d_spec, S, p0, out_dev);
+        if (hipGetLastError() != hipSuccess) { set_error("bsx_search_cnn_input: launch failed"); rc = BSX_ERR_DEVICE; break; }
+    }
+    (void)hipStreamSynchronize(s);   // the table and the grids are released / reused
+    h->bytes -= sizeof(int) * gcap;
+    (void)hipFree(dg);
+    return rc;
+}
+
 int bsx_view_matrices(const double* views, int64_t n, double* mats) {
     BSX_REQUIRE((views && mats) || n == 0, BSX_ERR_ARG, "bsx_view_matrices: null argument");
     for (int64_t i = 0; i < n; ++i) {
@@ -910,6 +1054,48 @@ int bsx_view_matrices(const double* views, int64_t n, double* mats) {
                         std::isfinite(views[4 * i + 2]) && std::isfinite(views[4 * i + 3]),
                     BSX_ERR_ARG, "bsx_view_matrices: view %lld is not finite", (long long)i);
         view_matrix(views + 4 * i, mats + 9 * i);
+    }
+    return BSX_OK;
+}
+
+// Matrix3::quaternion (include/Matrix3.h:129-166, the trace-based
+// conversion) followed by View(Quaternion&) (include/View.h:52-63).
+int bsx_matrix_views(const double* mats, int64_t n, double* views) {
+    BSX_REQUIRE((mats && views) || n == 0, BSX_ERR_ARG, "bsx_matrix_views: null argument");
+    for (int64_t i = 0; i < n; ++i) {
+        const double* d = mats + 9 * i;
+        for (int j = 0; j < 9; ++j)
+            BSX_REQUIRE(std::isfinite(d[j]), BSX_ERR_ARG, "bsx_matrix_views: matrix %lld is not finite",
+                        (long long)i);
+        double q[4];
+        const double tr = 1 + d[0] + d[4] + d[8];
+        if (tr > 1) {
+            const double t = 0.5 * sqrt(tr), u = 0.25 / t;
+            q[0] = t; q[1] = (d[7] - d[5]) * u; q[2] = (d[2] - d[6]) * u; q[3] = (d[3] - d[1]) * u;
+        } else if (d[0] > d[4] && d[0] > d[8]) {
+            const double t = 0.5 * sqrt(1 + d[0] - d[4] - d[8]), u = 0.25 / t;
+            q[1] = t; q[0] = (d[7] - d[5]) * u; q[2] = (d[1] + d[3]) * u; q[3] = (d[2] + d[6]) * u;
+        } else if (d[4] > d[8]) {
+            const double t = 0.5 * sqrt(1 - d[0] + d[4] - d[8]), u = 0.25 / t;
+            q[2] = t; q[0] = (d[2] - d[6]) * u; q[1] = (d[1] + d[3]) * u; q[3] = (d[5] + d[7]) * u;
+        } else {
+            const double t = 0.5 * sqrt(1 - d[0] - d[4] + d[8]), u = 0.25 / t;
+            q[3] = t; q[0] = (d[3] - d[1]) * u; q[1] = (d[2] + d[6]) * u; q[2] = (d[5] + d[7]) * u;
+        }
+        const double qn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        for (double& x : q) x /= qn;
+        double ang = atan2(q[3], q[0]);
+        const double ca = cos(ang), sa = sin(ang);
+        const double q2 = q[0] * q[0] + q[3] * q[3];
+        const double f2 = 2 * sqrt(q2);
+        if (q2 < 1e-30) ang = atan2(q[1], q[2]);
+        double v0 = f2 * (q[2] * ca + q[1] * sa), v1 = f2 * (q[2] * sa - q[1] * ca), v2 = 2 * q2 - 1;
+        const double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+        if (r > 0) { v0 /= r; v1 /= r; v2 /= r; }
+        views[4 * i] = v0;
+        views[4 * i + 1] = v1;
+        views[4 * i + 2] = v2;
+        views[4 * i + 3] = remainder(2 * ang, 2 * M_PI);
     }
     return BSX_OK;
 }

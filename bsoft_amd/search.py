@@ -53,6 +53,16 @@ def view_matrices(views: np.ndarray) -> np.ndarray:
     return m.reshape(-1, 3, 3)
 
 
+def matrix_views(mats) -> np.ndarray:
+    """Views (x, y, z, angle) of rotation matrices (..., 3, 3): the inverse of
+    view_matrices (Matrix3::quaternion, then View(Quaternion&))."""
+    m = np.ascontiguousarray(np.asarray(mats, dtype=np.float64).reshape(-1, 9))
+    v = np.zeros((len(m), 4))
+    call("bsx_matrix_views", m.ctypes.data_as(C.POINTER(C.c_double)), len(m),
+         v.ctypes.data_as(C.POINTER(C.c_double)))
+    return v
+
+
 def rotation_angle_deg(a, b) -> np.ndarray:
     """Geodesic angle (degrees) between rotation matrices a and b (..., 3, 3)."""
     a = np.asarray(a, dtype=np.float64)
@@ -126,6 +136,24 @@ class Search:
         score = torch.empty((self.nparts, self.topk), dtype=torch.float32, device=self.device)
         call("bsx_search_exhaustive", self.h, _dev_ptr(cand), _dev_ptr(score), _stream())
         return cand, score
+
+    def exhaustive_rows(self, rows: torch.Tensor):
+        """BF16 GEMM for a subset of the staged particles (rows: indices)."""
+        rows = rows.to(device=self.device, dtype=torch.int32).contiguous()
+        m = len(rows)
+        cand = torch.empty((m, self.topk), dtype=torch.int32, device=self.device)
+        score = torch.empty((m, self.topk), dtype=torch.float32, device=self.device)
+        if m:
+            call("bsx_search_exhaustive_rows", self.h, _dev_ptr(rows), m, _dev_ptr(cand),
+                 _dev_ptr(score), _stream())
+        return cand, score
+
+    def cnn_input(self, size: int) -> torch.Tensor:
+        """(n, size, size) low-pass, CTF-flipped, normalised images of the staged particles."""
+        out = torch.empty((self.nparts, size, size), dtype=torch.float32, device=self.device)
+        call("bsx_search_cnn_input", self.h, int(size), _dev_ptr(out) if self.nparts else None,
+             _stream())
+        return out
 
     def rescore(self, cand: torch.Tensor):
         """FP32 FRC of candidate views (n, c): (frc (n, c), best view (n), best frc (n))."""

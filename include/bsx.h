@@ -255,6 +255,11 @@ int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* p
  * scores, best first. Asynchronous on `stream`. */
 int bsx_search_exhaustive(bsx_search* h, int32_t* cand_dev, float* score_dev, void* stream);
 
+/* The same over a subset: rows_dev (m int32, device) staged-particle indices;
+ * cand_dev / score_dev m x topk. */
+int bsx_search_exhaustive_rows(bsx_search* h, const int32_t* rows_dev, int64_t m,
+                               int32_t* cand_dev, float* score_dev, void* stream);
+
 /* FP32 FRC (img_recip_space_fom, fom_type 0, CTF on the section, rings
  * 0..R-1) of candidate library views: cand_dev n x ncand int32 (negative =
  * empty slot). frc_dev (n x ncand, optional), best_dev / best_frc_dev (n,
@@ -282,6 +287,16 @@ int bsx_search_view_mats(const bsx_search* h, const int32_t* views_dev, int64_t 
 int bsx_search_refine(bsx_search* h, const float* ft_dev, double* mats_dev, float* frc_dev,
                       const double* steps, int32_t nsteps, void* stream);
 
+/* Rotation matrices back to views: Matrix3::quaternion (include/Matrix3.h:
+ * 129-166) then View(Quaternion&) (include/View.h:52-63). Host only. */
+int bsx_matrix_views(const double* mats, int64_t n, double* views);
+
+/* Input images of the orientation-proposal network for the staged particles:
+ * the disk values phase-flipped by the sign of their CTF on an S x S grid
+ * (centre at S/2), inverse transformed, real part normalised to zero mean and
+ * unit variance. out_dev: n x S x S FP32. No counterpart in the reference. */
+int bsx_search_cnn_input(bsx_search* h, int32_t S, float* out_dev, void* stream);
+
 /* View::matrix (include/View.h:131-157, Matrix3.h:30-42) of n views (x, y,
  * z, angle) into n row-major 3x3 FP64 matrices. Host only. */
 int bsx_view_matrices(const double* views, int64_t n, double* mats);
@@ -292,6 +307,37 @@ int bsx_search_particle_data(const bsx_search* h, float* values_dev, float* ctf_
                              uint16_t* rows_dev, void* stream);
 int bsx_search_library_data(const bsx_search* h, float* values_dev, uint16_t* rows_dev,
                             void* stream);
+
+/* ==== orientation-proposal network (no counterpart in the reference) ====
+ * A 5-layer CNN: conv3x3 s1 (1->c1), conv3x3 s2 (c1->c2), conv3x3 s2 (c2->c3),
+ * conv3x3 s2 (c3->c4), each + ReLU, then a fully connected layer to nclasses
+ * coarse orientation classes with softmax cross-entropy. Trained with Adam on
+ * FP32 weights by the library's own forward and backward kernels; inference
+ * uses BF16 copies of the weights with FP32 accumulation. Inputs: batch x S x S
+ * FP32 (bsx_search_cnn_input), S a multiple of 8. */
+typedef struct bsx_cnn bsx_cnn;
+
+int bsx_cnn_create(bsx_cnn** h, int32_t in_size, const int32_t channels[4], int32_t nclasses,
+                   int32_t max_batch, uint64_t seed);
+int bsx_cnn_destroy(bsx_cnn* h);
+/* Parameter count (flat FP32: W1 b1 W2 b2 W3 b3 W4 b4 Wfc bfc) and FC input size. */
+int bsx_cnn_nparams(const bsx_cnn* h, int64_t* n, int64_t* feature_size);
+/* Copy the parameters out (get_dev) and/or in (set_dev), device arrays. */
+int bsx_cnn_params(bsx_cnn* h, float* get_dev, const float* set_dev, void* stream);
+/* FP32 forward: logits_dev batch x nclasses (optional). */
+int bsx_cnn_forward(bsx_cnn* h, const float* x_dev, int32_t batch, float* logits_dev, void* stream);
+/* Forward + backward without an update: grad_dev (nparams, optional) holds
+ * the gradient of the mean cross-entropy, loss_dev (batch, optional) the
+ * per-sample losses. */
+int bsx_cnn_gradients(bsx_cnn* h, const float* x_dev, const int32_t* labels_dev, int32_t batch,
+                      float* grad_dev, float* loss_dev, void* stream);
+/* One Adam step (b1 0.9, b2 0.999, eps 1e-8; L2 weight decay). */
+int bsx_cnn_train_step(bsx_cnn* h, const float* x_dev, const int32_t* labels_dev, int32_t batch,
+                       float lr, float weight_decay, float* loss_dev, void* stream);
+/* BF16-weight inference over n inputs (batched internally): the k <= 32 most
+ * probable classes per input (cls_dev n x k) and their softmax probabilities. */
+int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int32_t* cls_dev,
+                       float* prob_dev, void* stream);
 
 /* asymmetric_unit_views for C1 (src/util/symmetry.cpp:383-470; flag bit 0
  * full sphere, bit 1 no in-plane alignment) then view_list_expand_angles

@@ -117,3 +117,36 @@ def test_rotation_angle():
     th = math.radians(3.0)
     rz = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
     assert np.allclose(search.rotation_angle_deg(m, m @ rz), 3.0, atol=1e-9)
+
+
+def test_cnn_oracle_gradient_finite_differences():
+    """The numpy backward of the proposal network equals central finite
+    differences of its loss (pins oracle/cnn_ref.py)."""
+    from oracle import cnn_ref
+    S, ch, ncls, B = 8, (4, 4, 8, 4), 5, 3
+    off, n, feat, chans = cnn_ref.layout(S, ch, ncls)
+    rng = np.random.default_rng(0)
+    p = rng.normal(size=n) * 0.4
+    x = rng.normal(size=(B, S, S))
+    y = rng.integers(0, ncls, B)
+    L, g, _ = cnn_ref.loss_and_grad(p, x, y, S, ch, ncls)
+    idx = np.concatenate([rng.choice(np.arange(o, ob + c), size=6, replace=False)
+                          for o, ob, c in off])
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(n)
+        e[i] = h
+        lp = cnn_ref.loss_and_grad(p + e, x, y, S, ch, ncls)[0]
+        lm = cnn_ref.loss_and_grad(p - e, x, y, S, ch, ncls)[0]
+        assert g[i] == pytest.approx((lp - lm) / (2 * h), rel=1e-5, abs=1e-8)
+
+
+def test_matrix_view_round_trip():
+    rng = np.random.default_rng(9)
+    v = synth.uniform_views(500, rng)
+    v = np.concatenate([v, [[0, 0, 1, 0.3], [0, 0, -1, -1.2], [1, 0, 0, 2.0]]])
+    m = search.view_matrices(v)
+    back = search.matrix_views(m)
+    np.testing.assert_allclose(search.view_matrices(back), m, atol=1e-12)
+    ok = np.abs(v[:, 2]) < 0.999
+    np.testing.assert_allclose(back[ok], v[ok], atol=1e-9)
