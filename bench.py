@@ -1,15 +1,27 @@
 #!/usr/bin/env python3
-"""Reconstruction throughput on one MI355X at the 256^3 configuration:
-a 256^3 map of 400 Gaussian blobs, 100k particles of 256^2 FP32 at known
-uniform views, per-particle defocus U[0.8,3.5] um and astigmatism U[0,0.2] um,
-SNR 0.05, NN insertion, pad 2 (ft 512), CTF flip.
+"""Search + reconstruction throughput on one MI355X at a 256^3 map.
 
-A step = one complete reconstruction of all particles already resident in HBM:
-reset the accumulators, background + pad + 2D FFT + phase shift + CTF +
-central-section insertion of every particle, then sum/weigh/centre/3D inverse
-FFT to the real-space map. Prints one JSON line.
+Workload: a 256^3 map of 400 Gaussian blobs; P particles (default 100k) of
+256^2 FP32 projected at uniform random orientations, per-particle defocus
+U[0.8,3.5] um and astigmatism U[0,0.2] um, SNR 0.05 (synthetic).
 
-    python bench.py [--steps K] [--warmup W] [--particles P] [--no-cpu]
+A step is one complete pass of a pipeline over all P particles, resident in
+HBM when the timed region starts, from images to the finished map:
+  plain   : search staging -> BF16 MFMA GEMM against every view of the 4-degree
+            grid (2566 directions x 90 in-plane angles) with top-8 -> FP32 FRC
+            of the 8 -> 2/1-degree local refinement -> insertion of every
+            particle at its found orientation -> sum/weigh/inverse FFT
+  learned : search staging -> CNN top-16 orientation classes -> FP32 FRC of
+            their library views (exhaustive top-8 added for particles whose
+            top class is not confident) -> the same refinement, insertion and
+            finish
+`value` is the learned pipeline's particles/s; the plain pipeline, the
+reconstruction alone at the known orientations (with its kernel roofline),
+angular errors and the FSC against the CPU restatement are reported beside it.
+One-time work per map (library sections, training set, network training) is
+timed separately and not part of a step.
+
+    python bench.py [--steps K] [--warmup W] [--particles P] [--no-cpu] [--recon-only]
 """
 from __future__ import annotations
 
@@ -26,24 +38,53 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class StageTimer:
+    """HIP events on torch's current stream (the stream every libbsx call of
+    this script is launched on)."""
+
+    def __init__(self):
+        import torch
+        self.torch = torch
+        self.marks = []
+
+    def __call__(self, name):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.marks.append((name, e))
+
+    def totals(self, into: dict):
+        self.torch.cuda.synchronize()
+        for (_, a), (name, b) in zip(self.marks, self.marks[1:]):
+            into[name] = into.get(name, 0.0) + a.elapsed_time(b)
+        self.marks = []
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--particles", type=int, default=100000)
     ap.add_argument("--box", type=int, default=256)
     ap.add_argument("--blobs", type=int, default=400)
-    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--radius", type=int, default=24, help="search disk radius (Fourier pixels)")
+    ap.add_argument("--kernel-width", type=int, default=4, help="FSI kernel width of the search")
+    ap.add_argument("--grid-step", type=float, default=4.0)
+    ap.add_argument("--class-step", type=float, default=15.0)
+    ap.add_argument("--train", type=int, default=60000, help="training projections")
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--min-prob", type=float, default=0.05)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--recon-only", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for PMC traffic scaling)")
     args = ap.parse_args()
@@ -53,124 +94,243 @@ def main() -> int:
         return 2
 
     import torch
-    from bsoft_amd import abi, synth
-    from bsoft_amd.recon import Reconstruction, part_ft_size
+    from bsoft_amd import abi, search, synth
+    from bsoft_amd.proposal import OrientationClasses, ProposalNet, learned_search, training_set
+    from bsoft_amd.recon import Reconstruction, map_transform, part_ft_size
 
     dev = torch.device("cuda:0")
-    box, N = args.box, args.box
+    box = N = args.box
+    pix = 1.5
     ft = part_ft_size(box, 1, 2)
     P = args.particles
     log(f"[bench] generating {P} particles {box}^2, map {N}^3 ({args.blobs} blobs), ft {ft}")
     t0 = time.perf_counter()
     rng = np.random.default_rng(0)
     bm = synth.make_blob_map(N, args.blobs, rng)
-    ps = synth.make_particles(bm, P, box, rng, pix=1.5, defocus=(8e3, 3.5e4), astig=(0.0, 2e3),
+    ps = synth.make_particles(bm, P, box, rng, pix=pix, defocus=(8e3, 3.5e4), astig=(0.0, 2e3),
                               snr=0.05, device=dev, seed=0, chunk=1024)
+    truth = synth.view_to_matrix(ps.parts["view"])
     torch.cuda.synchronize()
     log(f"[bench] data ready in {time.perf_counter() - t0:.1f} s")
 
-    flags = abi.BSX_FLAG_TIMING
-    rec = Reconstruction(N, box, ft, 1.5, ctf_action=abi.BSX_CTF_FLIP, flags=flags,
-                         batch=args.batch)
-    out_map = None
+    rec = Reconstruction(N, box, ft, pix, ctf_action=abi.BSX_CTF_FLIP, flags=abi.BSX_FLAG_TIMING)
+    result = {
+        "metric": "particles/s (search+recon, 256^3) learned-proposal vs plain GPU; FSC vs CPU",
+        "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+    }
 
-    def step():
-        nonlocal out_map
+    # ---------------------------------------------- reconstruction at known views
+    def recon_step(parts):
         rec.reset()
-        rec.insert(ps.images, ps.parts)
-        r = rec.finish()
-        out_map = r.map
-        return r
+        rec.insert(ps.images, parts)
+        return rec.finish()
 
-    for i in range(args.warmup):
-        step()
-        torch.cuda.synchronize()
-        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    for _ in range(args.warmup):
+        recon_step(ps.parts)
+    torch.cuda.synchronize()
     kt_sum: dict = {}
     launches = 0
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        r = step()
-        _, nl, _ = rec.last_timing()
-        launches += nl
+    for _ in range(args.steps):
+        r = recon_step(ps.parts)
+        launches += rec.last_timing()[1]
         for k, v in rec.kernel_times().items():
             kt_sum[k] = kt_sum.get(k, 0.0) + v
     torch.cuda.synchronize()
-    T = time.perf_counter() - t0
-    log(f"[bench] {args.steps} steps in {T:.3f} s; coverage {r.coverage}, friedel {r.friedel:.2e}")
-    log(f"[bench] kernel ms over the timed steps: " +
+    T_rec = (time.perf_counter() - t0) / args.steps
+    log(f"[bench] reconstruction at known views: {T_rec * 1e3:.1f} ms/step; kernels " +
         ", ".join(f"{k} {v:.1f}" for k, v in kt_sum.items()))
+    recon_roof = insertion_roofline(kt_sum, launches, P * args.steps, box, ft, N)
 
-    # Roofline. The insertion (the hot path) runs as count -> scan -> fill ->
-    # tile-accumulate kernels; the dominant one by time is reported with its
-    # algorithmic bytes: fill reads the half-spectrum columns ((ft/2+1)*box*8
-    # per particle) and writes one 16-byte bin entry per merged sample run;
-    # tile-accumulate reads the entries and read-modify-writes each touched
-    # 48-KB tile; count reads only particle records (compute/atomic bound).
-    n_s = ft * ft
+    if args.recon_only:
+        result.update({"value": round(P / T_rec, 1), "ms_per_step": round(1e3 * T_rec, 3),
+                       "config": {"workload": f"reconstruction {N}^3 from {P} particles {box}^2 "
+                                              f"at known views, ft {ft}, NN, CTF flip, SNR 0.05"},
+                       "roofline": recon_roof})
+        finish_result(args, result, ps, rec, ft, dev)
+        return 0
+
+    # ---------------------------------------------- one-time set-up per map
+    setup = {}
+    t0 = time.perf_counter()
+    mapft = map_transform(bm.volume(device=dev))
+    views = search.views_c1(args.grid_step, angle_step_deg=args.grid_step)
+    lib_mats = search.view_matrices(views)
+    s = search.Search(box, args.radius, pix, kernel_width=args.kernel_width, topk=8)
+    s.set_reference(mapft, lib_mats)
+    torch.cuda.synchronize()
+    setup["library_s"] = round(time.perf_counter() - t0, 2)
+    log(f"[bench] library: {len(lib_mats)} views, {s.npoints} disk points, GEMM K {s.kdim}, "
+        f"{setup['library_s']} s")
+    t0 = time.perf_counter()
+    classes = OrientationClasses(args.class_step)
+    members = torch.as_tensor(classes.members(lib_mats), device=dev)
+    S = 32
+    x, y, _ = training_set(mapft, box, args.train, classes, s, S, np.random.default_rng(1),
+                           snr=0.05, defocus=(8e3, 3.5e4))
+    torch.cuda.synchronize()
+    setup["training_set_s"] = round(time.perf_counter() - t0, 2)
+    t0 = time.perf_counter()
+    net = ProposalNet(S, classes.n, (32, 64, 128, 128), max_batch=512, seed=1)
+    hist = net.fit(x, y, epochs=args.epochs, batch=256, lr=2e-3, log=log)
+    torch.cuda.synchronize()
+    setup["training_s"] = round(time.perf_counter() - t0, 2)
+    setup["train_loss"] = [round(h, 4) for h in hist]
+    setup["classes"] = classes.n
+    del x, y
+    log(f"[bench] set-up: {setup}")
+
+    # ---------------------------------------------- the two pipelines
+    def pipeline(kind, timer):
+        s.stage(ps.images, ps.parts)
+        timer("stage")
+        if kind == "plain":
+            cand, _ = s.exhaustive()
+            timer("gemm")
+            _, best, bf = s.rescore(cand)
+            timer("rescore")
+            stats = None
+            mats0 = s.view_mats(best)
+            mats, frc = s.refine(mats0, (2.0, 1.0))
+            timer("refine")
+        else:
+            res, stats, _, _ = learned_search(s, net, members, k=16, min_prob=args.min_prob,
+                                              steps_deg=(2.0, 1.0), timer=timer)
+            mats = res.mats
+        found = mats.cpu().numpy()
+        parts = ps.parts.copy()
+        parts["view"] = search.matrix_views(found)
+        timer("views")
+        rec.reset()
+        rec.insert(ps.images, parts)
+        timer("insert")
+        fin = rec.finish()
+        timer("finish")
+        return found, stats, fin
+
+    out = {}
+    for kind in ("plain", "learned"):
+        for _ in range(args.warmup):
+            pipeline(kind, lambda name: None)
+        torch.cuda.synchronize()
+        stages: dict = {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tm = StageTimer()
+            tm("start")
+            found, stats, fin = pipeline(kind, tm)
+            tm.totals(stages)
+        torch.cuda.synchronize()
+        T = (time.perf_counter() - t0) / args.steps
+        err = search.rotation_angle_deg(found, truth)
+        out[kind] = {"T": T, "found": found, "stages": {k: round(v / args.steps, 2) for k, v in stages.items()},
+                     "median_err_vs_truth_deg": round(float(np.median(err)), 3), "stats": stats}
+        log(f"[bench] {kind}: {T * 1e3:.1f} ms/step ({P / T:.0f} particles/s); median error vs "
+            f"truth {np.median(err):.2f} deg; stages ms {out[kind]['stages']}")
+    d = search.rotation_angle_deg(out["learned"]["found"], out["plain"]["found"])
+    T_plain, T_learn = out["plain"]["T"], out["learned"]["T"]
+    st = out["learned"]["stats"]
+    search_ms = {k: sum(v for n, v in out[k]["stages"].items() if n not in ("insert", "finish", "views"))
+                 for k in out}
+
+    result.update({
+        "value": round(P / T_learn, 1),
+        "ms_per_step": round(1e3 * T_learn, 3),
+        "config": {"workload": f"search + reconstruction, {N}^3 blob map ({args.blobs} blobs), "
+                               f"{P} particles {box}^2, SNR 0.05, defocus U[0.8,3.5]um, astig "
+                               f"U[0,0.2]um; {len(lib_mats)}-view 4-degree grid, disk radius "
+                               f"{args.radius}, FSI width {args.kernel_width}; ft {ft} NN CTF flip",
+                   "particles": P, "box": box, "map": N, "ft": ft, "views": len(lib_mats)},
+        "plain_gpu": {"value": round(P / T_plain, 1), "ms_per_step": round(1e3 * T_plain, 3),
+                      "stages_ms": out["plain"]["stages"],
+                      "median_err_vs_truth_deg": out["plain"]["median_err_vs_truth_deg"]},
+        "learned": {"stages_ms": out["learned"]["stages"],
+                    "median_err_vs_truth_deg": out["learned"]["median_err_vs_truth_deg"],
+                    "median_err_vs_exhaustive_deg": round(float(np.median(d)), 3),
+                    "fallback_particles": st.fallback, "candidate_slots": st.candidates},
+        "speedup_learned_vs_plain": {"search_plus_recon": round(T_plain / T_learn, 3),
+                                     "search_only": round(search_ms["plain"] / search_ms["learned"], 3)},
+        "recon_known_views": {"value": round(P / T_rec, 1), "ms_per_step": round(1e3 * T_rec, 3)},
+        "setup_s": setup,
+    })
+    result["roofline"] = pipeline_roofline(out, s, P, args, recon_roof)
+    finish_result(args, result, ps, rec, ft, dev)
+    return 0
+
+
+def insertion_roofline(kt_sum, launches, nparticles, box, ft, N):
+    """Roofline of the dominant insertion kernel: fill reads the half-spectrum
+    columns ((ft/2+1)*box*8 B per particle) and writes a 16-B bin entry per
+    merged sample run; tile-accumulate reads the entries and read-modify-writes
+    the touched 48-KB tiles; count reads particle records only."""
     entries = kt_sum.get("entries", 0.0)
     nbatch = max(1, launches)
     col_bytes = (ft // 2 + 1) * box * 8
     kernel_bytes = {
-        "fill": col_bytes * P * args.steps + 16 * entries,
+        "fill": col_bytes * nparticles + 16 * entries,
         "tile": 16 * entries + 2 * 48 * 1024 * (N // 16) ** 3 * nbatch,
-        "count": 200 * P * args.steps,
-        "direct": 32 * n_s * P * args.steps,
+        "count": 200 * nparticles,
+        "direct": 32 * ft * ft * nparticles,
     }
     cand = {k: kt_sum.get(k, 0.0) for k in kernel_bytes}
     dom = max(cand, key=cand.get)
-    dom_ms = cand[dom]
+    dom_ms = max(cand[dom], 1e-9)
     achieved = kernel_bytes[dom] / (dom_ms * 1e-3) / 1e9
     ins_ms = sum(kt_sum.get(k, 0.0) for k in ("count", "scan", "fill", "tile", "direct"))
-    pipe_bytes = 32 * n_s * P * args.steps     # K3 NN insertion, 32 B per sample
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": f"k_cols<{ft}, false, {2 if dom == 'fill' else 1}>" if dom in ("fill", "count")
-                else ("k_tile_accum" if dom == "tile" else f"k_cols<{ft}, false, 0>"),
-                "stage": dom,
-                "avg_launch_ms": round(dom_ms / nbatch, 4),
-                "launches": nbatch,
-                "algorithmic_bytes_per_launch": round(kernel_bytes[dom] / nbatch),
-                "bin_entries_per_particle": round(entries / (P * args.steps), 1),
-                "kernel_ms": {k: round(v, 2) for k, v in kt_sum.items() if k != "entries"},
-                "insertion_pipeline": {
-                    "ms": round(ins_ms, 2),
-                    "algorithmic_bytes_per_particle": 32 * n_s,
-                    "achieved_GBs": round(pipe_bytes / (ins_ms * 1e-3) / 1e9, 1)}}
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": {"fill": f"k_cols<{ft}, false, 2>", "count": f"k_cols<{ft}, false, 1>",
+                       "tile": "k_tile_accum", "direct": f"k_cols<{ft}, false, 0>"}[dom],
+            "stage": dom, "avg_launch_ms": round(dom_ms / nbatch, 4), "launches": nbatch,
+            "algorithmic_bytes_per_launch": round(kernel_bytes[dom] / nbatch),
+            "bin_entries_per_particle": round(entries / max(1, nparticles), 1),
+            "kernel_ms": {k: round(v, 2) for k, v in kt_sum.items() if k != "entries"},
+            "insertion_ms": round(ins_ms, 2)}
 
-    result = {
-        "metric": "particles/s (search+recon, 256^3) learned-proposal vs plain GPU; FSC vs CPU",
-        "value": round(P * args.steps / T, 1),
-        "unit": "particles/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * T / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"reconstruction {N}^3 map from {P} particles {box}^2, "
-                               f"ft {ft}, NN, CTF flip, defocus U[0.8,3.5]um, astig U[0,0.2]um, "
-                               f"SNR 0.05, {args.blobs}-blob map",
-                   "particles": P, "box": box, "map": N, "ft": ft},
-        "roofline": roofline,
-        "cpu_baseline": None,
-    }
 
+def pipeline_roofline(out, s, P, args, recon_roof):
+    """Roofline of the GEMM (MFMA-bound: 2 * particles * views * K flops per
+    launch) and, for the record, the other stages of the learned pipeline."""
+    gemm_ms = out["plain"]["stages"].get("gemm", 0.0)
+    kreal = 2 * (s.npoints - 1)
+    flops = 2.0 * P * s.nviews * kreal
+    gemm_tf = flops / max(gemm_ms, 1e-9) / 1e9
+    refine_ms = out["learned"]["stages"].get("refine", 0.0)
+    # refinement: 2 steps x 27 rotations, each a section of npoints with w^3 taps of 8 B
+    refine_bytes = 2 * 27 * P * s.npoints * args.kernel_width ** 3 * 8
+    return {"bound": "mfma", "achieved": round(gemm_tf, 1), "peak": BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(gemm_tf / BF16_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "k_gemm_topk<8> (+ k_topk_merge<8>)", "stage": "gemm",
+            "avg_launch_ms": round(gemm_ms, 3), "launches": 1,
+            "algorithmic_flops_per_launch": flops,
+            "other_stages": {
+                "refine_k_score_mats": {"ms": round(refine_ms, 2), "gathered_GBs":
+                                        round(refine_bytes / max(refine_ms, 1e-9) / 1e6, 1)},
+                "insertion": {k: recon_roof[k] for k in ("kernel", "achieved", "frac", "avg_launch_ms")}},
+            }
+
+
+def finish_result(args, result, ps, rec, ft, dev):
+    import torch
     if args.calib_copy:
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    result["cpu_baseline"] = None
     if not args.no_cpu:
         result["cpu_baseline"], result["fsc_vs_cpu"] = cpu_baseline(args, ps, rec.params, ft)
     print(json.dumps(result), flush=True)
-    return 0
 
 
 def cpu_baseline(args, ps, params, ft):
-    """The FP32 CPU restatement (OpenMP over partial maps, like breconstruct
-    -threads T) timed on a bounded sample of the same workload, plus the GPU
-    result on the same sample for the FSC comparison."""
+    """The FP32 CPU restatement of the reconstruction (OpenMP over partial
+    maps, like breconstruct -threads T) timed on a bounded sample of the same
+    particles at their known orientations, plus the GPU result on the same
+    sample for the FSC comparison."""
     import torch
     from bsoft_amd import synth
     from bsoft_amd.recon import Reconstruction
@@ -205,9 +365,8 @@ def cpu_baseline(args, ps, params, ft):
     m, _, ftc, st = o.finish(want_ft=True)
     tfin = time.perf_counter() - t0
     P = len(ps.parts)
-    full = tins / nsamp * P + tfin
+    full = tins / max(nsamp, 1) * P + tfin
     log(f"[bench] cpu: insert {nsamp} in {tins:.1f} s, finish {tfin:.1f} s -> {P / full:.2f} particles/s")
-    # GPU on the same ncal + nsamp particles, FSC against the CPU map
     n = ncal + nsamp
     g = Reconstruction(rp.size, rp.box, rp.ft_size, rp.sampling, hi_res=rp.hi_res,
                        ctf_action=rp.ctf_action, wiener=rp.wiener)
@@ -218,10 +377,10 @@ def cpu_baseline(args, ps, params, ft):
     fsc = fsc[sup]
     err = float(np.abs(gr.map.cpu().numpy().astype(np.float64) - m).max() / np.abs(m).max())
     cpu = {"value": round(P / full, 3), "unit": "particles/s", "cores": threads, "kind": "port",
-           "sample": f"FP32 CPU restatement, OpenMP over {threads} partial maps: {nsamp} of the "
-                     f"{P} particles timed, inserted in {tins:.1f} s (after a {ncal}-particle "
-                     f"warm-up), finish {tfin:.1f} s; value = {P} / (per-particle time x {P} "
-                     f"+ finish)"}
+           "sample": f"reconstruction only (the CPU path has no search): FP32 CPU restatement, "
+                     f"OpenMP over {threads} partial maps, {nsamp} of the {P} particles at their "
+                     f"known views inserted in {tins:.1f} s (after a {ncal}-particle warm-up), "
+                     f"finish {tfin:.1f} s; value = {P} / (per-particle time x {P} + finish)"}
     fsc_info = {"particles": n, "min_fsc": round(float(fsc.min()), 7),
                 "max_rel_voxel_err": err, "shells": len(fsc)}
     return cpu, fsc_info
