@@ -166,7 +166,7 @@ def main() -> int:
         f"{setup['library_s']} s")
     t0 = time.perf_counter()
     classes = OrientationClasses(args.class_step)
-    members = torch.as_tensor(classes.members(lib_mats), device=dev)
+    s.set_classes(classes.members(lib_mats))
     S = 32
     x, y, _ = training_set(mapft, box, args.train, classes, s, S, np.random.default_rng(1),
                            snr=0.05, defocus=(8e3, 3.5e4))
@@ -196,7 +196,7 @@ def main() -> int:
             mats, frc = s.refine(mats0, (2.0, 1.0))
             timer("refine")
         else:
-            res, stats, _, _ = learned_search(s, net, members, k=16, min_prob=args.min_prob,
+            res, stats, _, _ = learned_search(s, net, k=16, min_prob=args.min_prob,
                                               steps_deg=(2.0, 1.0), timer=timer)
             mats = res.mats
         found = mats.cpu().numpy()

@@ -133,6 +133,8 @@ SIGNATURES = {
     "bsx_search_stage": ([_vp, _vp, C.POINTER(Particle), C.c_int64, _vp], C.c_int),
     "bsx_search_exhaustive": ([_vp, _vp, _vp, _vp], C.c_int),
     "bsx_search_exhaustive_rows": ([_vp, _vp, C.c_int64, _vp, _vp, _vp], C.c_int),
+    "bsx_search_set_classes": ([_vp, C.POINTER(C.c_int32), C.c_int32, C.c_int32], C.c_int),
+    "bsx_search_proposed": ([_vp, _vp, C.c_int32, _vp, _vp, _vp], C.c_int),
     "bsx_search_rescore": ([_vp, _vp, C.c_int32, _vp, _vp, _vp, _vp], C.c_int),
     "bsx_search_score_mats": ([_vp, _vp, _vp, C.c_int32, _vp, _vp], C.c_int),
     "bsx_search_particle_data": ([_vp, _vp, _vp, _vp, _vp], C.c_int),

@@ -310,6 +310,213 @@ __global__ __launch_bounds__(256) void k_gemm_topk(const uint16_t* __restrict__ 
     }
 }
 
+// ------------------------------------------------- proposal-restricted GEMM
+// Routing of (particle, proposal slot) pairs to their proposed class, so
+// that one class's member views are scored against all particles that
+// proposed it with one MFMA tile (a grouped GEMM).
+__global__ void k_route_count(const int* __restrict__ cls, int64_t n, int k, int nclass,
+                              int* __restrict__ counts) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const int c = cls[t];
+    if (c >= 0 && c < nclass) atomicAdd(counts + c, 1);
+}
+
+__global__ void k_route_fill(const int* __restrict__ cls, int64_t n, int k, int nclass,
+                             int* __restrict__ cursors, int* __restrict__ pairs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const int c = cls[t];
+    if (c >= 0 && c < nclass) pairs[atomicAdd(cursors + c, 1)] = (int)t;   // t = particle * k + slot
+}
+
+// Unrouted slots (no valid class) still need empty lists for the merge.
+__global__ void k_route_empty(const int* __restrict__ cls, int64_t n, int k, int nclass, int T,
+                              float* __restrict__ pscore, int* __restrict__ pidx) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * k) return;
+    const int c = cls[t];
+    if (c >= 0 && c < nclass) return;
+    for (int q = 0; q < T; ++q) { pscore[t * T + q] = -INFINITY; pidx[t * T + q] = -1; }
+}
+
+struct GroupItem {
+    int cls, p0, np, pad_;
+};
+
+// One workgroup per work item: up to 128 routed pairs of one class against
+// the class's member views (128 per column tile). Rows are gathered by
+// index (zero row for padding); the per-pair top-T over the members goes to
+// pscore/pidx[(particle * k + slot) * T + q].
+template <int T>
+__global__ __launch_bounds__(256) void k_gemm_grouped(
+    const uint16_t* __restrict__ A, const uint16_t* __restrict__ B, const uint16_t* __restrict__ zrow,
+    int Kp, const GroupItem* __restrict__ items, const int* __restrict__ pairs, int k,
+    const int* __restrict__ members, const int* __restrict__ mcount, int maxm,
+    float* __restrict__ pscore, int* __restrict__ pidx) {
+    __shared__ __attribute__((aligned(16))) char lds[GEMM_LDS];
+    __shared__ int srow[GM];
+    __shared__ int scol[GN];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const GroupItem it = items[blockIdx.x];
+    const int my_row = tid & (GM - 1), my_half = tid >> 7;
+    const int ncls_members = mcount[it.cls];
+    if (tid < GM) srow[tid] = tid < it.np ? pairs[it.p0 + tid] : -1;
+    float best[T];
+    int bidx[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) { best[q] = -INFINITY; bidx[q] = -1; }
+    const int nk = Kp / GK;
+    int ld_row[4], ld_part[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = tid + 256 * q;
+        ld_row[q] = c >> 3;
+        ld_part[q] = c & 7;
+    }
+    const int ntile = (ncls_members + GN - 1) / GN;
+    for (int t = 0; t < ntile; ++t) {
+        __syncthreads();
+        if (tid < GN) {
+            const int j = t * GN + tid;
+            scol[tid] = j < ncls_members ? members[(size_t)it.cls * maxm + j] : -1;
+        }
+        __syncthreads();
+        const uint16_t* pa[4];
+        const uint16_t* pb[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int pr = srow[ld_row[q]];
+            pa[q] = pr >= 0 ? A + (size_t)(pr / k) * Kp : zrow;
+            const int vc = scol[ld_row[q]];
+            pb[q] = vc >= 0 ? B + (size_t)vc * Kp : zrow;
+        }
+        f32x16 acc[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+        uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
+#define BSX_GGLOAD(K0)                                               \
+    ra0 = *(const uint4*)(pa[0] + (K0) + ld_part[0] * 8);            \
+    ra1 = *(const uint4*)(pa[1] + (K0) + ld_part[1] * 8);            \
+    ra2 = *(const uint4*)(pa[2] + (K0) + ld_part[2] * 8);            \
+    ra3 = *(const uint4*)(pa[3] + (K0) + ld_part[3] * 8);            \
+    rb0 = *(const uint4*)(pb[0] + (K0) + ld_part[0] * 8);            \
+    rb1 = *(const uint4*)(pb[1] + (K0) + ld_part[1] * 8);            \
+    rb2 = *(const uint4*)(pb[2] + (K0) + ld_part[2] * 8);            \
+    rb3 = *(const uint4*)(pb[3] + (K0) + ld_part[3] * 8);
+#define BSX_GLSTORE(BUF)                                                            \
+    *(uint4*)((BUF) + ld_row[0] * LROW + ld_part[0] * 16) = ra0;                    \
+    *(uint4*)((BUF) + ld_row[1] * LROW + ld_part[1] * 16) = ra1;                    \
+    *(uint4*)((BUF) + ld_row[2] * LROW + ld_part[2] * 16) = ra2;                    \
+    *(uint4*)((BUF) + ld_row[3] * LROW + ld_part[3] * 16) = ra3;                    \
+    *(uint4*)((BUF) + (GM + ld_row[0]) * LROW + ld_part[0] * 16) = rb0;             \
+    *(uint4*)((BUF) + (GM + ld_row[1]) * LROW + ld_part[1] * 16) = rb1;             \
+    *(uint4*)((BUF) + (GM + ld_row[2]) * LROW + ld_part[2] * 16) = rb2;             \
+    *(uint4*)((BUF) + (GM + ld_row[3]) * LROW + ld_part[3] * 16) = rb3;
+        BSX_GGLOAD(0)
+        BSX_GLSTORE(lds)
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const bool more = kt + 1 < nk;
+            if (more) {
+                const size_t k0 = (size_t)(kt + 1) * GK;
+                BSX_GGLOAD(k0)
+            }
+            const char* buf = lds + (kt & 1) * (GM + GN) * LROW;
+#pragma unroll
+            for (int ks = 0; ks < GK / 16; ++ks) {
+                bf16x8 fa[2], fb[2];
+                const int off = ks * 32 + (lane >> 5) * 16;
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    fa[i] = *(const bf16x8*)(buf + (wm * 64 + i * 32 + (lane & 31)) * LROW + off);
+                    fb[i] = *(const bf16x8*)(buf + (GM + wn * 64 + i * 32 + (lane & 31)) * LROW + off);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            }
+            if (more) {
+                char* nb = lds + ((kt + 1) & 1) * (GM + GN) * LROW;
+                BSX_GLSTORE(nb)
+            }
+            __syncthreads();
+        }
+#undef BSX_GGLOAD
+#undef BSX_GLSTORE
+        float* C = (float*)lds;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int row = wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const int col = wn * 64 + j * 32 + (lane & 31);
+                    C[row * CROW + col] = acc[i][j][r];
+                }
+        __syncthreads();
+        const float* cr = C + my_row * CROW + my_half * 64;
+        for (int j = 0; j < 64; ++j) {
+            const int vc = scol[my_half * 64 + j];
+            if (vc < 0) continue;
+            const float v = cr[j];
+            if (v > best[T - 1]) {
+                best[T - 1] = v;
+                bidx[T - 1] = vc;
+#pragma unroll
+                for (int q = T - 1; q > 0; --q) {
+                    if (best[q] > best[q - 1]) {
+                        const float tv = best[q]; best[q] = best[q - 1]; best[q - 1] = tv;
+                        const int ti = bidx[q]; bidx[q] = bidx[q - 1]; bidx[q - 1] = ti;
+                    }
+                }
+            }
+        }
+    }
+    // combine the two column halves of each row into the row's list
+    __syncthreads();
+    float* xs = (float*)lds;
+    int* xi = (int*)(lds + GM * 2 * T * sizeof(float));
+#pragma unroll
+    for (int q = 0; q < T; ++q) {
+        xs[(my_row * 2 + my_half) * T + q] = best[q];
+        xi[(my_row * 2 + my_half) * T + q] = bidx[q];
+    }
+    __syncthreads();
+    if (my_half == 0 && my_row < it.np) {
+#pragma unroll
+        for (int q = 0; q < T; ++q) {
+            const float v = xs[(my_row * 2 + 1) * T + q];
+            const int id = xi[(my_row * 2 + 1) * T + q];
+            if (id >= 0 && v > best[T - 1]) {
+                best[T - 1] = v;
+                bidx[T - 1] = id;
+#pragma unroll
+                for (int z = T - 1; z > 0; --z) {
+                    if (best[z] > best[z - 1]) {
+                        const float tv = best[z]; best[z] = best[z - 1]; best[z - 1] = tv;
+                        const int ti = bidx[z]; bidx[z] = bidx[z - 1]; bidx[z - 1] = ti;
+                    }
+                }
+            }
+        }
+        const size_t o = (size_t)srow[my_row] * T;
+#pragma unroll
+        for (int q = 0; q < T; ++q) {
+            pscore[o + q] = best[q];
+            pidx[o + q] = bidx[q];
+        }
+    }
+}
+
 // One thread per particle: merge the 2*nseg sorted lists into the top T.
 template <int T>
 __global__ void k_topk_merge(const float* __restrict__ pscore, const int* __restrict__ pidx,
@@ -595,6 +802,22 @@ struct bsx_search {
     int64_t lfrc_cap = 0;
     uint16_t* d_sub = nullptr;       // gathered rows for a partial exhaustive search
     int64_t sub_cap = 0;
+    // proposal classes: member views per class
+    int nclass = 0, maxm = 0;
+    int* d_members = nullptr;
+    int64_t members_cap = 0;
+    int* d_mcount = nullptr;
+    int64_t mcount_cap = 0;
+    int* d_counts = nullptr;
+    int64_t counts_cap = 0;
+    int* d_cursors = nullptr;
+    int64_t cursors_cap = 0;
+    int* d_pairs = nullptr;
+    int64_t pairs_cap = 0;
+    GroupItem* d_items = nullptr;
+    int64_t items_cap = 0;
+    uint16_t* d_zrow = nullptr;
+    int64_t zrow_cap = 0;
     float* d_ktab = nullptr;
     float* d_pscore = nullptr;
     int* d_pidx = nullptr;
@@ -696,7 +919,8 @@ int bsx_search_destroy(bsx_search* h) {
     void* ptrs[] = {h->d_hk,   h->d_ring,    h->d_lib,    h->d_lbf,    h->d_fp,
                     h->d_ctf,  h->d_pbf,     h->d_spec,   h->d_sp,
                     h->d_ktab, h->d_pscore,  h->d_pidx,   h->d_libmats, h->d_lmats,
-                    h->d_lfrc, h->d_sub};
+                    h->d_lfrc, h->d_sub,     h->d_members, h->d_mcount, h->d_counts,
+                    h->d_cursors, h->d_pairs, h->d_items, h->d_zrow};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     delete h;
@@ -893,6 +1117,104 @@ int bsx_search_exhaustive_rows(bsx_search* h, const int32_t* rows_dev, int64_t m
                        rows_dev, m, h->nparts, h->Kp, h->d_sub);
     BSX_HIP(hipGetLastError());
     return run_exhaustive(h, h->d_sub, m, cand_dev, score_dev, s);
+}
+
+int bsx_search_set_classes(bsx_search* h, const int32_t* members, int32_t nclasses,
+                           int32_t maxm) {
+    BSX_REQUIRE(h && members, BSX_ERR_ARG, "bsx_search_set_classes: null argument");
+    BSX_REQUIRE(nclasses >= 1 && maxm >= 1 && (int64_t)nclasses * maxm < (1ll << 31), BSX_ERR_ARG,
+                "bsx_search_set_classes: %d classes x %d members out of range", nclasses, maxm);
+    BSX_REQUIRE(h->nviews > 0, BSX_ERR_STATE, "bsx_search_set_classes: set the reference views first");
+    std::vector<int> cnt(nclasses);
+    for (int c = 0; c < nclasses; ++c) {
+        int m = 0;
+        while (m < maxm && members[(size_t)c * maxm + m] >= 0) ++m;
+        for (int j = 0; j < maxm; ++j) {
+            const int v = members[(size_t)c * maxm + j];
+            BSX_REQUIRE(j < m ? v < h->nviews : v < 0, BSX_ERR_ARG,
+                        "bsx_search_set_classes: class %d entry %d (%d) is not a view index of a "
+                        "leading run", c, j, v);
+        }
+        cnt[c] = m;
+    }
+    int rc = grow(h, &h->d_members, &h->members_cap, (int64_t)nclasses * maxm, "class members");
+    if (!rc) rc = grow(h, &h->d_mcount, &h->mcount_cap, nclasses, "class sizes");
+    if (!rc) rc = grow(h, &h->d_zrow, &h->zrow_cap, h->Kp, "zero row");
+    if (rc) return rc;
+    BSX_HIP(hipMemcpy(h->d_members, members, sizeof(int) * nclasses * maxm, hipMemcpyHostToDevice));
+    BSX_HIP(hipMemcpy(h->d_mcount, cnt.data(), sizeof(int) * nclasses, hipMemcpyHostToDevice));
+    BSX_HIP(hipMemset(h->d_zrow, 0, sizeof(uint16_t) * h->Kp));
+    h->nclass = nclasses;
+    h->maxm = maxm;
+    return BSX_OK;
+}
+
+int bsx_search_proposed(bsx_search* h, const int32_t* cls_dev, int32_t k, int32_t* cand_dev,
+                        float* score_dev, void* stream) {
+    BSX_REQUIRE(h && (h->nparts == 0 || (cls_dev && cand_dev && score_dev)), BSX_ERR_ARG,
+                "bsx_search_proposed: null argument");
+    BSX_REQUIRE(h->nclass > 0, BSX_ERR_STATE, "bsx_search_proposed: no classes (bsx_search_set_classes)");
+    BSX_REQUIRE(k >= 1 && k <= 64, BSX_ERR_ARG, "bsx_search_proposed: k %d out of range 1..64", k);
+    const int64_t n = h->nparts;
+    if (n == 0) return BSX_OK;
+    BSX_REQUIRE(n * k < (1ll << 31), BSX_ERR_ARG, "bsx_search_proposed: too many particles");
+    hipStream_t s = (hipStream_t)stream;
+    const int T = h->p.topk <= 8 ? 8 : 16;
+    int rc = grow(h, &h->d_counts, &h->counts_cap, h->nclass, "routing counts");
+    if (!rc) rc = grow(h, &h->d_cursors, &h->cursors_cap, h->nclass, "routing cursors");
+    if (!rc) rc = grow(h, &h->d_pairs, &h->pairs_cap, n * k, "routing pairs");
+    if (!rc) rc = grow(h, &h->d_pscore, &h->pscore_cap, n * k * T, "proposal partials");
+    if (!rc) rc = grow(h, &h->d_pidx, &h->pidx_cap, n * k * T, "proposal partials");
+    if (rc) return rc;
+    const unsigned g = (unsigned)((n * k + 255) / 256);
+    BSX_HIP(hipMemsetAsync(h->d_counts, 0, sizeof(int) * h->nclass, s));
+    hipLaunchKernelGGL(k_route_count, dim3(g), dim3(256), 0, s, cls_dev, n, k, h->nclass, h->d_counts);
+    BSX_HIP(hipGetLastError());
+    std::vector<int> cnt(h->nclass), off(h->nclass);
+    BSX_HIP(hipMemcpyAsync(cnt.data(), h->d_counts, sizeof(int) * h->nclass, hipMemcpyDeviceToHost, s));
+    BSX_HIP(hipStreamSynchronize(s));
+    std::vector<GroupItem> items;
+    int acc = 0;
+    for (int c = 0; c < h->nclass; ++c) {
+        off[c] = acc;
+        for (int p0 = 0; p0 < cnt[c]; p0 += GM)
+            items.push_back(GroupItem{c, acc + p0, std::min(GM, cnt[c] - p0), 0});
+        acc += cnt[c];
+    }
+    rc = grow(h, &h->d_items, &h->items_cap, std::max<int64_t>(1, (int64_t)items.size()), "work items");
+    if (rc) return rc;
+    BSX_HIP(hipMemcpyAsync(h->d_cursors, off.data(), sizeof(int) * h->nclass, hipMemcpyHostToDevice, s));
+    if (!items.empty())
+        BSX_HIP(hipMemcpyAsync(h->d_items, items.data(), sizeof(GroupItem) * items.size(),
+                               hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_route_fill, dim3(g), dim3(256), 0, s, cls_dev, n, k, h->nclass, h->d_cursors,
+                       h->d_pairs);
+    BSX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_route_empty, dim3(g), dim3(256), 0, s, cls_dev, n, k, h->nclass, T,
+                       h->d_pscore, h->d_pidx);
+    BSX_HIP(hipGetLastError());
+    if (!items.empty()) {
+        if (T == 8)
+            hipLaunchKernelGGL((k_gemm_grouped<8>), dim3((unsigned)items.size()), dim3(256), 0, s,
+                               h->d_pbf, h->d_lbf, h->d_zrow, h->Kp, h->d_items, h->d_pairs, k,
+                               h->d_members, h->d_mcount, h->maxm, h->d_pscore, h->d_pidx);
+        else
+            hipLaunchKernelGGL((k_gemm_grouped<16>), dim3((unsigned)items.size()), dim3(256), 0, s,
+                               h->d_pbf, h->d_lbf, h->d_zrow, h->Kp, h->d_items, h->d_pairs, k,
+                               h->d_members, h->d_mcount, h->maxm, h->d_pscore, h->d_pidx);
+        BSX_HIP(hipGetLastError());
+    }
+    const unsigned mg = (unsigned)((n + 255) / 256);
+    if (T == 8)
+        hipLaunchKernelGGL((k_topk_merge<8>), dim3(mg), dim3(256), 0, s, h->d_pscore, h->d_pidx, k,
+                           n, h->p.topk, cand_dev, score_dev);
+    else
+        hipLaunchKernelGGL((k_topk_merge<16>), dim3(mg), dim3(256), 0, s, h->d_pscore, h->d_pidx, k,
+                           n, h->p.topk, cand_dev, score_dev);
+    BSX_HIP(hipGetLastError());
+    // the host work-item list is released on return
+    BSX_HIP(hipStreamSynchronize(s));
+    return BSX_OK;
 }
 
 int bsx_search_rescore(bsx_search* h, const int32_t* cand_dev, int32_t ncand, float* frc_dev,

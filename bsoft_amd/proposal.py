@@ -237,29 +237,30 @@ class LearnedStats:
     times: dict = field(default_factory=dict)
 
 
-def learned_search(s: Search, net: ProposalNet, members: torch.Tensor, *, k: int = 16,
-                   min_prob: float = 0.0, steps_deg=(2.0, 1.0), timer=None):
-    """Learned pipeline on the staged particles: CNN top-k classes -> FP32 FRC
-    of their library views -> local refinement; particles whose top class
-    probability is below min_prob also get the exhaustive top-k candidates.
-    members: (C, maxm) int32 device table of library views per class."""
+def learned_search(s: Search, net: ProposalNet, *, k: int = 16, min_prob: float = 0.0,
+                   steps_deg=(2.0, 1.0), timer=None):
+    """Learned pipeline on the staged particles: CNN top-k classes -> BF16
+    GEMM against the member views of those classes only (routed, grouped) ->
+    FP32 FRC of the best candidates -> local refinement. Particles whose top
+    class probability is below min_prob also get the exhaustive candidates.
+    The classes' member table must be registered with s.set_classes."""
     st = LearnedStats()
     t = timer or (lambda name: None)
     x = s.cnn_input(net.size)
     t("cnn_input")
     cls, prob = net.topk(x, k)
     t("cnn")
-    cand = members[cls.long()].reshape(s.nparts, -1)
+    cand, _ = s.proposed(cls)
+    t("proposed_gemm")
     if min_prob > 0:
         low = torch.nonzero(prob[:, 0] < min_prob).flatten()
         st.fallback = int(low.numel())
         if st.fallback:
             ec, _ = s.exhaustive_rows(low.to(torch.int32))
-            extra = torch.full((s.nparts, ec.shape[1]), -1, dtype=torch.int32, device=cand.device)
+            extra = torch.full_like(cand, -1)
             extra[low] = ec
-            cand = torch.cat([extra, cand], dim=1)
+            cand = torch.cat([cand, extra], dim=1).contiguous()
         t("fallback")
-    cand = cand.contiguous()
     st.candidates = cand.shape[1]
     _, best, bf = s.rescore(cand)
     t("rescore")

@@ -148,6 +148,22 @@ class Search:
                  _dev_ptr(score), _stream())
         return cand, score
 
+    def set_classes(self, members: np.ndarray) -> None:
+        """members: (nclasses, maxm) library view indices per class, -1 padded."""
+        m = np.ascontiguousarray(members, dtype=np.int32)
+        call("bsx_search_set_classes", self.h, m.ctypes.data_as(C.POINTER(C.c_int32)), m.shape[0],
+             m.shape[1])
+
+    def proposed(self, cls: torch.Tensor):
+        """BF16 GEMM restricted to the members of each particle's proposed
+        classes (cls: (n, k)): (cand, score), each (n, topk)."""
+        cls = cls.to(device=self.device, dtype=torch.int32).contiguous()
+        cand = torch.empty((self.nparts, self.topk), dtype=torch.int32, device=self.device)
+        score = torch.empty((self.nparts, self.topk), dtype=torch.float32, device=self.device)
+        call("bsx_search_proposed", self.h, _dev_ptr(cls) if self.nparts else None, cls.shape[1],
+             _dev_ptr(cand), _dev_ptr(score), _stream())
+        return cand, score
+
     def cnn_input(self, size: int) -> torch.Tensor:
         """(n, size, size) low-pass, CTF-flipped, normalised images of the staged particles."""
         out = torch.empty((self.nparts, size, size), dtype=torch.float32, device=self.device)

@@ -260,6 +260,19 @@ int bsx_search_exhaustive(bsx_search* h, int32_t* cand_dev, float* score_dev, vo
 int bsx_search_exhaustive_rows(bsx_search* h, const int32_t* rows_dev, int64_t m,
                                int32_t* cand_dev, float* score_dev, void* stream);
 
+/* Proposal classes for bsx_search_proposed: members (host, nclasses x maxm)
+ * holds each class's library view indices as a leading run, -1 padded. Call
+ * after bsx_search_set_reference. No counterpart in the reference. */
+int bsx_search_set_classes(bsx_search* h, const int32_t* members, int32_t nclasses, int32_t maxm);
+
+/* Coarse search restricted to proposed classes: cls_dev (n x k int32,
+ * device; negative = none) routes each (particle, proposal) to its class, and
+ * a grouped BF16 MFMA GEMM scores the particle against that class's member
+ * views only. Output as bsx_search_exhaustive: n x topk candidates and
+ * scores. Synchronises `stream` once (work-item list). */
+int bsx_search_proposed(bsx_search* h, const int32_t* cls_dev, int32_t k, int32_t* cand_dev,
+                        float* score_dev, void* stream);
+
 /* FP32 FRC (img_recip_space_fom, fom_type 0, CTF on the section, rings
  * 0..R-1) of candidate library views: cand_dev n x ncand int32 (negative =
  * empty slot). frc_dev (n x ncand, optional), best_dev / best_frc_dev (n,

@@ -119,9 +119,9 @@ def test_trained_proposal_beats_untrained():
     views = search.views_c1(4.0, angle_step_deg=4.0)
     lm = search.view_matrices(views)
     s.set_reference(ft, lm)
-    members = torch.as_tensor(classes.members(lm)).cuda()
+    s.set_classes(classes.members(lm))
     ex = search.exhaustive_search(s)
-    le, st, _, _ = learned_search(s, net, members, k=16, min_prob=0.0)
+    le, st, _, _ = learned_search(s, net, k=16, min_prob=0.0)
     d = search.rotation_angle_deg(le.mats.cpu().numpy(), ex.mats.cpu().numpy())
     et = search.rotation_angle_deg(ex.mats.cpu().numpy(), truth)
     say(f"  learned vs exhaustive median {np.median(d):.2f} deg (exhaustive vs truth "

@@ -192,3 +192,39 @@ def test_search_recovers_ground_truth():
             osec = o.central_section(ftc, 1.5, mats[v], 1.5, 8, 2, 0)
             of.append(sr.frc_half(pv, pc, osec[hk[:, 1] % n, hk[:, 0] % n], rs))
         assert c50[int(np.argmax(of))] == best[i].item()
+
+
+def test_proposed_grouped_gemm_matches_restricted_topk(setup64):
+    """The routed, grouped GEMM returns the top-k of the exact BF16 products
+    over the member views of each particle's proposed classes."""
+    from bsoft_amd.proposal import OrientationClasses
+    n, R, bm, vol, ft, ps = setup64
+    views = search.views_c1(6.0, angle_step_deg=6.0)
+    mats = search.view_matrices(views)
+    classes = OrientationClasses(20.0)
+    members = classes.members(mats)
+    say(f"proposed views={len(mats)} classes={classes.n} maxm={members.shape[1]}")
+    s = search.Search(n, R, 1.5, topk=8)
+    s.set_reference(ft, mats)
+    s.set_classes(members)
+    s.stage(ps.images.cuda().contiguous(), ps.parts)
+    rng = np.random.default_rng(4)
+    P, k = len(ps.parts), 5
+    cls = rng.integers(0, classes.n, size=(P, k)).astype(np.int32)
+    cls[0, 1] = -1                       # an empty slot
+    cls[1, :] = cls[1, 0]                # the same class proposed five times
+    cand, score = s.proposed(torch.as_tensor(cls))
+    torch.cuda.synchronize()
+    A = bf16_to_f32(s.particle_data()[2])
+    B = bf16_to_f32(s.library_data()[1])
+    cand, score = cand.cpu().numpy(), score.cpu().numpy()
+    for i in range(P):
+        allowed = np.unique(np.concatenate([members[c][members[c] >= 0] for c in cls[i] if c >= 0]))
+        full = A[i] @ B[allowed].T
+        kk = min(8, len(allowed))
+        valid = cand[i] >= 0
+        assert valid.sum() == kk
+        assert set(cand[i][valid]) <= set(allowed)
+        np.testing.assert_allclose(score[i][valid], A[i] @ B[cand[i][valid]].T, atol=1e-4)
+        kth = np.sort(full)[::-1][kk - 1]
+        assert np.all(A[i] @ B[cand[i][valid]].T >= kth - 2e-4)
