@@ -85,6 +85,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--recon-only", action="store_true")
+    ap.add_argument("--mixed", action="store_true",
+                    help="mixed-size run: boxes 128/192/256 in a 50/30/20 ratio")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for PMC traffic scaling)")
     args = ap.parse_args()
@@ -98,6 +100,8 @@ def main() -> int:
     from bsoft_amd.proposal import OrientationClasses, ProposalNet, learned_search, training_set
     from bsoft_amd.recon import Reconstruction, map_transform, part_ft_size
 
+    if args.mixed:
+        return run_mixed(args)
     dev = torch.device("cuda:0")
     box = N = args.box
     pix = 1.5
@@ -183,6 +187,12 @@ def main() -> int:
     log(f"[bench] set-up: {setup}")
 
     # ---------------------------------------------- the two pipelines
+    # their reconstruction handle replays the insertion from a hipGraph and
+    # uses the autotuned batch size
+    recp = Reconstruction(N, box, ft, pix, ctf_action=abi.BSX_CTF_FLIP, flags=abi.BSX_FLAG_GRAPH)
+    tb = recp.autotune(ps.images[:8192].contiguous(), ps.parts[:8192])
+    setup["autotuned_batch"] = tb
+
     def pipeline(kind, timer):
         s.stage(ps.images, ps.parts)
         timer("stage")
@@ -203,10 +213,10 @@ def main() -> int:
         parts = ps.parts.copy()
         parts["view"] = search.matrix_views(found)
         timer("views")
-        rec.reset()
-        rec.insert(ps.images, parts)
+        recp.reset()
+        recp.insert(ps.images, parts)
         timer("insert")
-        fin = rec.finish()
+        fin = recp.finish()
         timer("finish")
         return found, stats, fin
 
@@ -254,9 +264,109 @@ def main() -> int:
                                      "search_only": round(search_ms["plain"] / search_ms["learned"], 3)},
         "recon_known_views": {"value": round(P / T_rec, 1), "ms_per_step": round(1e3 * T_rec, 3)},
         "setup_s": setup,
+        "graph_replays": recp.graph_launches(),
     })
     result["roofline"] = pipeline_roofline(out, s, P, args, recon_roof)
     finish_result(args, result, ps, rec, ft, dev)
+    return 0
+
+
+def run_mixed(args) -> int:
+    """Boxes {128, 192, 256} in a 50/30/20 ratio (default 60k particles) of one
+    object (a 384-A blob model sampled at 384/box A/px), each size with its own
+    map, view library and reconstruction volume; one proposal network (its
+    32x32 input is the same 24-A low-pass disk at every size) serves all."""
+    import torch
+    from bsoft_amd import abi, search, synth
+    from bsoft_amd.proposal import OrientationClasses, ProposalNet, learned_search, training_set
+    from bsoft_amd.recon import Reconstruction, map_transform, part_ft_size
+
+    dev = torch.device("cuda:0")
+    total = args.particles if args.particles != 100000 else 60000
+    boxes = {128: 0.5, 192: 0.3, 256: 0.2}
+    rng = np.random.default_rng(3)
+    base = synth.make_blob_map(256, args.blobs, rng)
+    classes = OrientationClasses(args.class_step)
+    S = 32
+    groups = {}
+    for box, frac in boxes.items():
+        f = box / 256
+        bm = synth.BlobMap(centers=base.centers * f, sigmas=base.sigmas * f, amps=base.amps, n=box)
+        pix = 384.0 / box
+        n = int(round(total * frac))
+        ps = synth.make_particles(bm, n, box, rng, pix=pix, defocus=(8e3, 3.5e4),
+                                  astig=(0.0, 2e3), snr=0.05, device=dev, seed=box, chunk=1024)
+        mapft = map_transform(bm.volume(device=dev))
+        R = 16 * box // 128
+        s = search.Search(box, R, pix, kernel_width=args.kernel_width, topk=8)
+        lib_mats = search.view_matrices(search.views_c1(args.grid_step, angle_step_deg=args.grid_step))
+        s.set_reference(mapft, lib_mats)
+        s.set_classes(classes.members(lib_mats))
+        ft = part_ft_size(box, 1, 2)
+        rec = Reconstruction(box, box, ft, pix, ctf_action=abi.BSX_CTF_FLIP, flags=abi.BSX_FLAG_GRAPH)
+        rec.autotune(ps.images[:4096].contiguous(), ps.parts[:4096])
+        groups[box] = dict(ps=ps, s=s, rec=rec, mapft=mapft, pix=pix,
+                           truth=synth.view_to_matrix(ps.parts["view"]))
+        log(f"[bench] box {box}: {n} particles, pix {pix:.2f}, disk radius {R}, batch {rec.batch()}")
+    # one network, trained on extracted projections of every size in proportion
+    xs, ys = [], []
+    for box, frac in boxes.items():
+        g = groups[box]
+        x, y, _ = training_set(g["mapft"], box, int(args.train * frac), classes, g["s"], S,
+                               np.random.default_rng(box), pix=g["pix"], snr=0.05,
+                               defocus=(8e3, 3.5e4))
+        xs.append(x)
+        ys.append(y)
+    net = ProposalNet(S, classes.n, (32, 64, 128, 128), max_batch=512, seed=1)
+    net.fit(torch.cat(xs), torch.cat(ys), epochs=args.epochs, batch=256, lr=2e-3, log=log)
+    del xs, ys
+
+    def step():
+        found = {}
+        for box, g in groups.items():
+            s, ps = g["s"], g["ps"]
+            s.stage(ps.images, ps.parts)
+            res, st, _, _ = learned_search(s, net, k=16, min_prob=args.min_prob)
+            m = res.mats.cpu().numpy()
+            parts = ps.parts.copy()
+            parts["view"] = search.matrix_views(m)
+            g["rec"].reset()
+            g["rec"].insert(ps.images, parts)
+            g["fin"] = g["rec"].finish(want_ft=True)
+            found[box] = m
+        return found
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        found = step()
+    torch.cuda.synchronize()
+    T = (time.perf_counter() - t0) / args.steps
+    per_box = {}
+    for box, g in groups.items():
+        err = search.rotation_angle_deg(found[box], g["truth"])
+        # FSC of the map from the found orientations against the map from the true ones
+        ref = Reconstruction(box, box, part_ft_size(box, 1, 2), g["pix"], ctf_action=abi.BSX_CTF_FLIP)
+        ref.insert(g["ps"].images, g["ps"].parts)
+        rf = ref.finish(want_ft=True)
+        from bsoft_amd.recon import fsc
+        f = fsc(g["fin"].ft, rf.ft, g["pix"], 2 * g["pix"])
+        per_box[box] = {"particles": len(g["ps"].parts),
+                        "median_err_vs_truth_deg": round(float(np.median(err)), 3),
+                        "fsc_found_vs_true_views_min_to_half_nyquist":
+                            round(float(np.min(f[1:len(f) // 2])), 4)}
+    result = {"metric": "particles/s (search+recon, mixed boxes 128/192/256) learned proposal",
+              "value": round(total / T, 1), "unit": "particles/s", "n_gpus": 1,
+              "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * T, 3),
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+              "data": "synthetic",
+              "config": {"workload": f"mixed sizes: {total} particles, boxes 128/192/256 at "
+                                     f"50/30/20, SNR 0.05, learned proposal + refinement + "
+                                     f"graph-replayed insertion into per-box volumes"},
+              "per_box": per_box, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(result), flush=True)
     return 0
 
 

@@ -28,6 +28,7 @@ BSX_FLAG_EWALD = 8
 BSX_FLAG_FOM = 16
 BSX_FLAG_CHECK_NAN = 32
 BSX_FLAG_TIMING = 64
+BSX_FLAG_GRAPH = 128
 
 BSX_INTERP_NEAREST = 0
 BSX_INTERP_WEIGHTED = 1
@@ -116,6 +117,9 @@ SIGNATURES = {
     "bsx_recon_accumulators": ([_vp, C.c_int, _vp, _vp], C.c_int),
     "bsx_recon_device_bytes": ([_vp, C.POINTER(C.c_int64)], C.c_int),
     "bsx_recon_last_timing": ([_vp, _dp, C.POINTER(C.c_int64), _dp], C.c_int),
+    "bsx_recon_autotune": ([_vp, _vp, C.POINTER(Particle), C.c_int64, C.POINTER(C.c_int32), _dp,
+                            _vp], C.c_int),
+    "bsx_recon_graph_launches": ([_vp, C.POINTER(C.c_int64)], C.c_int),
     "bsx_recon_kernel_times": ([_vp, _dp, C.c_int], C.c_int),
     "bsx_fft": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, _vp], C.c_int),
     "bsx_map_transform": ([_vp, C.c_int, _dp, _vp, _vp], C.c_int),

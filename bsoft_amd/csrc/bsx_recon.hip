@@ -811,6 +811,15 @@ struct bsx_recon {
     int64_t hrecs_cap = 0;
     hipEvent_t copied = nullptr;   // staging buffer reusable once this completes
     bool timing = false;
+    // BSX_FLAG_GRAPH: the batch loop of an insert, captured once per
+    // (images, count, batch) and replayed while the same set is inserted again
+    hipGraphExec_t gexec = nullptr;
+    const float* g_imgs = nullptr;
+    int64_t g_n = -1;
+    int g_batch = 0;
+    int g_merge = -1;              // kernel constants baked into the graph
+    int g_seen = 0;                // identical calls seen (capture on the second)
+    int64_t graph_launches = 0;
 };
 
 enum { KT_BG = 0, KT_ROWS, KT_COUNT, KT_SCAN, KT_FILL, KT_TILE, KT_DIRECT, KT_END };
@@ -1080,6 +1089,7 @@ int bsx_recon_destroy(bsx_recon* h) {
         (void)hipEventDestroy(h->copied);
     }
     if (h->hrecs) (void)hipHostFree(h->hrecs);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     delete h;
     return BSX_OK;
 }
@@ -1204,10 +1214,53 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
         BSX_REQUIRE(!bad, BSX_ERR_NONFINITE, "non-finite values in the particle images");
     }
     const int64_t nbatches = (n + h->batch - 1) / h->batch;
-    for (int64_t b = 0; b < nbatches; ++b) {
-        const int64_t p0 = b * h->batch;
-        const int nb = (int)std::min<int64_t>(h->batch, n - p0);
-        if ((r = run_batch(h, imgs_dev, p0, nb, s))) return r;
+    auto batch_loop = [&]() -> int {
+        for (int64_t b = 0; b < nbatches; ++b) {
+            const int64_t p0 = b * h->batch;
+            const int nb = (int)std::min<int64_t>(h->batch, n - p0);
+            const int rr = run_batch(h, imgs_dev, p0, nb, s);
+            if (rr) return rr;
+        }
+        return BSX_OK;
+    };
+    const bool graph = (h->p.flags & BSX_FLAG_GRAPH) && !h->timing;
+    const bool same = imgs_dev == h->g_imgs && n == h->g_n && h->batch == h->g_batch &&
+                      h->K.merge == h->g_merge;
+    if (graph && same && h->gexec) {
+        BSX_HIP(hipGraphLaunch(h->gexec, s));
+        h->graph_launches++;
+    } else if (graph && same && h->g_seen >= 1) {
+        // second identical call: every lazily created resource (twiddles)
+        // exists, so the loop can be captured and replayed from now on
+        hipGraph_t g = nullptr;
+        BSX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        r = batch_loop();
+        hipError_t ce = hipStreamEndCapture(s, &g);
+        if (r) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        BSX_HIP(ce);
+        if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+        hipError_t ie = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        BSX_HIP(ie);
+        BSX_HIP(hipGraphLaunch(h->gexec, s));
+        h->graph_launches++;
+    } else {
+        if ((r = batch_loop())) return r;
+        if (graph) {
+            h->g_seen = same ? h->g_seen + 1 : 1;
+            if (!same && h->gexec) {
+                (void)hipGraphExecDestroy(h->gexec);
+                h->gexec = nullptr;
+            }
+            h->g_imgs = imgs_dev;
+            h->g_n = n;
+            h->g_batch = h->batch;
+            h->g_merge = h->K.merge;
+        }
     }
     for (int64_t i = 0; i < n; ++i) h->inserted[h->p.nhalves > 1 ? (parts[i].half & 1) : 0]++;
     h->last_launches = nbatches;
@@ -1224,6 +1277,59 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
             BSX_HIP(hipMemcpy(&ne, h->entries, sizeof(ne), hipMemcpyDeviceToHost));
         h->last_entries = (double)ne;
     }
+    return BSX_OK;
+}
+
+// Batch-size autotuner: inserts the sample at each candidate batch (the
+// allocated batch and its halvings down to 1/8, at least 32) and keeps the
+// fastest. The handle must be empty; it is left empty (accumulators reset).
+int bsx_recon_autotune(bsx_recon* h, const float* imgs_dev, const bsx_particle* parts, int64_t n,
+                       int32_t* chosen_batch, double* best_ms, void* stream) {
+    BSX_REQUIRE(h && imgs_dev && parts && n > 0, BSX_ERR_ARG, "bsx_recon_autotune: bad arguments");
+    BSX_REQUIRE(h->inserted[0] == 0 && h->inserted[1] == 0, BSX_ERR_STATE,
+                "bsx_recon_autotune: call it before inserting particles");
+    hipStream_t s = (hipStream_t)stream;
+    const int cap = h->batch;
+    const int saved_flags = h->p.flags;
+    h->p.flags &= ~(BSX_FLAG_TIMING | BSX_FLAG_GRAPH);
+    hipEvent_t e0, e1;
+    BSX_HIP(hipEventCreate(&e0));
+    BSX_HIP(hipEventCreate(&e1));
+    int best = cap, rc = BSX_OK;
+    double bms = 1e300;
+    for (int div = 8; div >= 1 && rc == BSX_OK; div /= 2) {
+        const int b = std::max(32, cap / div);
+        if (b > cap || (div > 1 && b == std::max(32, cap / (div / 2)))) continue;
+        h->batch = b;
+        rc = bsx_recon_insert(h, imgs_dev, parts, n, stream);      // warm-up
+        if (rc) break;
+        float ms = 0;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = BSX_ERR_DEVICE; break; }
+        rc = bsx_recon_insert(h, imgs_dev, parts, n, stream);
+        if (rc) break;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            set_error("bsx_recon_autotune: timing failed");
+            rc = BSX_ERR_DEVICE;
+            break;
+        }
+        if (ms < bms) { bms = ms; best = b; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->p.flags = saved_flags;
+    h->batch = best;
+    h->inserted[0] = h->inserted[1] = 0;
+    const int r2 = bsx_recon_reset(h, stream);
+    if (rc) return rc;
+    if (chosen_batch) *chosen_batch = best;
+    if (best_ms) *best_ms = bms;
+    return r2;
+}
+
+int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n) {
+    BSX_REQUIRE(h && n, BSX_ERR_ARG, "bsx_recon_graph_launches: null argument");
+    *n = h->graph_launches;
     return BSX_OK;
 }
 

@@ -110,6 +110,24 @@ class Reconstruction:
         call("bsx_recon_insert", self.h, _ptr(imgs) if n else None,
              particles_ctypes(parts) if n else None, n, _stream(stream))
 
+    def autotune(self, imgs: torch.Tensor, parts: np.ndarray, stream=None) -> int:
+        """Pick the fastest insertion batch size on a sample (bsx_recon_autotune);
+        the handle must be empty and is left empty."""
+        parts = np.ascontiguousarray(parts, dtype=PARTICLE_DTYPE)
+        b, ms = C.c_int32(), C.c_double()
+        call("bsx_recon_autotune", self.h, _ptr(imgs.contiguous()), particles_ctypes(parts),
+             len(parts), C.byref(b), C.byref(ms), _stream(stream))
+        self._batch = b.value
+        return b.value
+
+    def batch(self) -> int:
+        return getattr(self, "_batch", self.params.batch)
+
+    def graph_launches(self) -> int:
+        n = C.c_int64()
+        call("bsx_recon_graph_launches", self.h, C.byref(n))
+        return n.value
+
     def last_timing(self):
         ms, st = C.c_double(), C.c_double()
         nl = C.c_int64()

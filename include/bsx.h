@@ -42,6 +42,9 @@ extern "C" {
 #define BSX_FLAG_CHECK_NAN  32   /* scan particles for non-finite values */
 #define BSX_FLAG_TIMING     64   /* time the insertion kernel with HIP events
                                     (bsx_recon_insert then synchronises)   */
+#define BSX_FLAG_GRAPH     128   /* replay the insertion kernels of a repeated
+                                    insert (same images, count and batch) from
+                                    a captured hipGraph                      */
 
 /* ---- interpolation: Bimage_reconstruct.cpp:25 (0 NN, 1 weighted NN, 2 trilinear) ---- */
 #define BSX_INTERP_NEAREST   0
@@ -149,6 +152,15 @@ int bsx_recon_device_bytes(const bsx_recon* h, int64_t* bytes);
 /* Kernel timings of the previous bsx_recon_insert (needs BSX_FLAG_TIMING):
  * insertion kernels (count, scan, fill, tile accumulate or direct), staging
  * kernels (background, row transforms) in ms, and the number of batches. */
+/* Batch-size autotuner (no counterpart in the reference): times the insertion
+ * of n sample particles at the allocated batch and its halvings (down to 1/8,
+ * at least 32) and keeps the fastest for later inserts. Requires an empty
+ * handle and leaves it empty (accumulators reset). */
+int bsx_recon_autotune(bsx_recon* h, const float* imgs_dev, const bsx_particle* parts, int64_t n,
+                       int32_t* chosen_batch, double* best_ms, void* stream);
+/* Number of inserts served by graph replay so far (BSX_FLAG_GRAPH). */
+int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n);
+
 int bsx_recon_last_timing(const bsx_recon* h, double* insert_ms, int64_t* launches,
                           double* stage_ms);
 /* Per-kernel totals of the previous insert (ms): background, rows, count,

@@ -207,3 +207,40 @@ def test_variants_agree(interp, ft, n):
         for q in range(3):
             e = ((a[q] - a0[q]).abs().max() / a0[q].abs().max()).item()
             assert e < 2e-5, (method, order, merge, q, e)
+
+
+def test_graph_replay_and_autotune():
+    """Inserts replayed from a captured hipGraph give the eager result (the
+    replay counter proves the graph ran); the autotuner leaves the handle
+    empty and the chosen batch gives the same accumulators."""
+    import ctypes as C
+    from bsoft_amd.abi import call
+    from bsoft_amd.recon import _ptr
+    bm, ps = dataset(32, 10, 300, 32, seed=10)
+    d = ps.images.cuda()
+    ref = Reconstruction(32, 32, 64, 1.5, ctf_action=1, batch=64)
+    ref.insert(d, ps.parts)
+    a0 = ref.accumulators(0)
+    g = Reconstruction(32, 32, 64, 1.5, ctf_action=1, batch=64, flags=abi.BSX_FLAG_GRAPH)
+    for rep in range(4):
+        g.reset()
+        g.insert(d, ps.parts)
+        torch.cuda.synchronize()
+        a = g.accumulators(0)
+        assert (a[:3] - a0[:3]).abs().max() <= 2e-5 * a0[:3].abs().max(), rep
+        torch.cuda.synchronize()
+    nl = C.c_int64()
+    call("bsx_recon_graph_launches", g.h, C.byref(nl))
+    say(f"  graph replays {nl.value}")
+    assert nl.value == 3            # the first insert runs eagerly, the second is captured
+    t = Reconstruction(32, 32, 64, 1.5, ctf_action=1, batch=256)
+    cb, ms = C.c_int32(), C.c_double()
+    parts = np.ascontiguousarray(ps.parts)
+    call("bsx_recon_autotune", t.h, _ptr(d), synth.particles_ctypes(parts), len(parts),
+         C.byref(cb), C.byref(ms), None)
+    say(f"  autotune chose batch {cb.value} ({ms.value:.3f} ms)")
+    assert cb.value in (32, 64, 128, 256)
+    assert t.accumulators(0).abs().max().item() == 0
+    t.insert(d, ps.parts)
+    a = t.accumulators(0)
+    assert (a[:3] - a0[:3]).abs().max() <= 2e-5 * a0[:3].abs().max()
