@@ -820,6 +820,8 @@ struct bsx_recon {
     int g_merge = -1;              // kernel constants baked into the graph
     int g_seen = 0;                // identical calls seen (capture on the second)
     int64_t graph_launches = 0;
+    hipStream_t cap_stream = nullptr;   // capture happens here (the caller's stream may be
+                                        // the legacy default stream, which cannot capture)
 };
 
 enum { KT_BG = 0, KT_ROWS, KT_COUNT, KT_SCAN, KT_FILL, KT_TILE, KT_DIRECT, KT_END };
@@ -1090,6 +1092,7 @@ int bsx_recon_destroy(bsx_recon* h) {
     }
     if (h->hrecs) (void)hipHostFree(h->hrecs);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     delete h;
     return BSX_OK;
 }
@@ -1214,11 +1217,11 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
         BSX_REQUIRE(!bad, BSX_ERR_NONFINITE, "non-finite values in the particle images");
     }
     const int64_t nbatches = (n + h->batch - 1) / h->batch;
-    auto batch_loop = [&]() -> int {
+    auto batch_loop = [&](hipStream_t st) -> int {
         for (int64_t b = 0; b < nbatches; ++b) {
             const int64_t p0 = b * h->batch;
             const int nb = (int)std::min<int64_t>(h->batch, n - p0);
-            const int rr = run_batch(h, imgs_dev, p0, nb, s);
+            const int rr = run_batch(h, imgs_dev, p0, nb, st);
             if (rr) return rr;
         }
         return BSX_OK;
@@ -1233,9 +1236,10 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
         // second identical call: every lazily created resource (twiddles)
         // exists, so the loop can be captured and replayed from now on
         hipGraph_t g = nullptr;
-        BSX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        r = batch_loop();
-        hipError_t ce = hipStreamEndCapture(s, &g);
+        if (!h->cap_stream) BSX_HIP(hipStreamCreateWithFlags(&h->cap_stream, hipStreamNonBlocking));
+        BSX_HIP(hipStreamBeginCapture(h->cap_stream, hipStreamCaptureModeThreadLocal));
+        r = batch_loop(h->cap_stream);
+        hipError_t ce = hipStreamEndCapture(h->cap_stream, &g);
         if (r) {
             if (g) (void)hipGraphDestroy(g);
             return r;
@@ -1249,7 +1253,7 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
         BSX_HIP(hipGraphLaunch(h->gexec, s));
         h->graph_launches++;
     } else {
-        if ((r = batch_loop())) return r;
+        if ((r = batch_loop(s))) return r;
         if (graph) {
             h->g_seen = same ? h->g_seen + 1 : 1;
             if (!same && h->gexec) {
