@@ -369,6 +369,160 @@ __global__ void k_to_bf16(const float* __restrict__ p, uint16_t* __restrict__ q,
         q[i] = f_to_bf16(p[i]);
 }
 
+// ------------------------------------------------------ MFMA inference
+// Activations are NHWC BF16. Layer 1 (one input channel) runs on the VALU;
+// layers 2-4 and the fully connected layer are implicit GEMMs on
+// v_mfma_f32_32x32x16_bf16: rows = output positions (b, yo, xo), columns =
+// output channels, k = (tap, input channel) with the channel fastest, so
+// that 8 consecutive k are one 16-byte load from the NHWC input.
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+constexpr int IM = 128, IN = 64, IK = 32, IROW = IK * 2 + 16;
+
+// y[b][yo][xo][co] = bf16(relu(bias + sum_{ky,kx} W[co][0][ky][kx] x[b][yo-1+ky][xo-1+kx]))
+__global__ __launch_bounds__(256) void k_conv1_nhwc(const float* __restrict__ x,
+                                                    const uint16_t* __restrict__ W,
+                                                    const uint16_t* __restrict__ bias, int B, int S,
+                                                    int C1, uint16_t* __restrict__ y) {
+    __shared__ float w[1024 * 10];   // C1 <= 1024 weights x 9 taps + biases
+    for (int i = threadIdx.x; i < C1 * 9; i += blockDim.x) w[i] = bf16_to_f(W[i]);
+    for (int i = threadIdx.x; i < C1; i += blockDim.x) w[C1 * 9 + i] = bf16_to_f(bias[i]);
+    __syncthreads();
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)B * S * S) return;
+    const int xo = (int)(t % S), yo = (int)((t / S) % S);
+    const long b = t / ((long)S * S);
+    float v[9];
+#pragma unroll
+    for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) {
+            const int yi = yo - 1 + ky, xi = xo - 1 + kx;
+            v[ky * 3 + kx] = (yi >= 0 && yi < S && xi >= 0 && xi < S) ? x[(b * S + yi) * S + xi] : 0.f;
+        }
+    uint16_t* o = y + t * C1;
+    for (int c = 0; c < C1; c += 2) {
+        float a0 = w[C1 * 9 + c], a1 = w[C1 * 9 + c + 1];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            a0 = fmaf(w[c * 9 + q], v[q], a0);
+            a1 = fmaf(w[(c + 1) * 9 + q], v[q], a1);
+        }
+        *(uint32_t*)(o + c) = (uint32_t)f_to_bf16(fmaxf(a0, 0.f)) |
+                              ((uint32_t)f_to_bf16(fmaxf(a1, 0.f)) << 16);
+    }
+}
+
+// Implicit-GEMM convolution (ks = 3: pad 1; ks = 1: a plain GEMM over
+// H = W = 1) of NHWC BF16 input with BF16 weights [Cout][ks*ks][Cin] and
+// BF16 biases. OUTF32: FP32 output rows of stride ldo without ReLU (logits);
+// else BF16 NHWC output with ReLU. Cin must be a multiple of 32.
+template <bool OUTF32>
+__global__ __launch_bounds__(256) void k_conv_mfma(const uint16_t* __restrict__ x, int B, int H,
+                                                   int Wd, int Cin, int ks, int stride, int Ho,
+                                                   int Wo, const uint16_t* __restrict__ Wt,
+                                                   const uint16_t* __restrict__ bias, int Cout,
+                                                   long ldo, void* __restrict__ out) {
+    __shared__ __attribute__((aligned(16))) char lds[2 * (IM + IN) * IROW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long M = (long)B * Ho * Wo;
+    const int K = ks * ks * Cin;
+    const int nk = K / IK;
+    const long m0 = (long)blockIdx.x * IM;
+    const int n0 = blockIdx.y * IN;
+    const int pad = ks / 2;
+    // A staging: chunks tid and tid + 256 (row = c >> 2, part = c & 3); the
+    // positions live in named scalars (arrays captured by the loaders end up
+    // in scratch memory)
+    const long am0 = m0 + (tid >> 2), am1 = m0 + ((tid + 256) >> 2);
+    const long mm0 = am0 < M ? am0 : 0, mm1 = am1 < M ? am1 : 0;
+    const int ax0 = (int)(mm0 % Wo), ay0 = (int)((mm0 / Wo) % Ho), ab0 = (int)(mm0 / ((long)Wo * Ho));
+    const int ax1 = (int)(mm1 % Wo), ay1 = (int)((mm1 / Wo) % Ho), ab1 = (int)(mm1 / ((long)Wo * Ho));
+    const int apart0 = tid & 3;
+    const int brow = tid >> 2, bpart = tid & 3;
+    const int bco = n0 + brow;
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) { acc0[r] = 0.f; acc1[r] = 0.f; }
+    uint4 ra0, ra1, rb;
+#define BSX_CGLOAD(KT)                                                                        \
+    {                                                                                          \
+        const int k0 = (KT) * IK;                                                              \
+        const int tap = k0 / Cin;                                                              \
+        const int ky = tap / ks, kx = tap % ks;                                                \
+        const int ci0 = k0 % Cin + apart0 * 8;                                                 \
+        const int yi0 = ay0 * stride - pad + ky, xi0 = ax0 * stride - pad + kx;                \
+        const int yi1 = ay1 * stride - pad + ky, xi1 = ax1 * stride - pad + kx;                \
+        ra0 = make_uint4(0, 0, 0, 0);                                                          \
+        ra1 = make_uint4(0, 0, 0, 0);                                                          \
+        rb = make_uint4(0, 0, 0, 0);                                                           \
+        if (am0 < M && yi0 >= 0 && yi0 < H && xi0 >= 0 && xi0 < Wd)                            \
+            ra0 = *(const uint4*)(x + (((long)ab0 * H + yi0) * Wd + xi0) * Cin + ci0);         \
+        if (am1 < M && yi1 >= 0 && yi1 < H && xi1 >= 0 && xi1 < Wd)                            \
+            ra1 = *(const uint4*)(x + (((long)ab1 * H + yi1) * Wd + xi1) * Cin + ci0);         \
+        if (bco < Cout) rb = *(const uint4*)(Wt + (long)bco * K + k0 + bpart * 8);             \
+    }
+#define BSX_CLSTORE(BUF)                                                                      \
+    {                                                                                          \
+        *(uint4*)((BUF) + (tid >> 2) * IROW + apart0 * 16) = ra0;                              \
+        *(uint4*)((BUF) + ((tid + 256) >> 2) * IROW + apart0 * 16) = ra1;                      \
+        *(uint4*)((BUF) + (IM + brow) * IROW + bpart * 16) = rb;                               \
+    }
+    BSX_CGLOAD(0)
+    BSX_CLSTORE(lds)
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const bool more = kt + 1 < nk;
+        if (more) BSX_CGLOAD(kt + 1)
+        const char* buf = lds + (kt & 1) * (IM + IN) * IROW;
+#pragma unroll
+        for (int sub = 0; sub < IK / 16; ++sub) {
+            const int off = sub * 32 + (lane >> 5) * 16;
+            const bf16x8 fa = *(const bf16x8*)(buf + (wave * 32 + (lane & 31)) * IROW + off);
+            const bf16x8 fb0 = *(const bf16x8*)(buf + (IM + (lane & 31)) * IROW + off);
+            const bf16x8 fb1 = *(const bf16x8*)(buf + (IM + 32 + (lane & 31)) * IROW + off);
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb1, acc1, 0, 0, 0);
+        }
+        if (more) BSX_CLSTORE(lds + ((kt + 1) & 1) * (IM + IN) * IROW)
+        __syncthreads();
+    }
+#undef BSX_CGLOAD
+#undef BSX_CLSTORE
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const long m = m0 + wave * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const int co = n0 + j * 32 + (lane & 31);
+            if (m >= M || co >= Cout) continue;
+            const float v = (j == 0 ? acc0[r] : acc1[r]) + bf16_to_f(bias[co]);
+            if (OUTF32) ((float*)out)[m * ldo + co] = v;
+            else ((uint16_t*)out)[m * Cout + co] = f_to_bf16(fmaxf(v, 0.f));
+        }
+}
+
+// BF16 inference weights: conv [co][ci][ky][kx] -> [co][ky*3+kx][ci]; FC
+// [class][c*hw + pos] -> [class][pos*C + c] (NHWC flattening).
+__global__ void k_perm_conv(const float* __restrict__ W, int Cout, int Cin, uint16_t* __restrict__ o) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)Cout * Cin * 9) return;
+    const int tap = (int)(t % 9);
+    const int ci = (int)((t / 9) % Cin);
+    const long co = t / (9l * Cin);
+    o[(co * 9 + tap) * Cin + ci] = f_to_bf16(W[t]);
+}
+
+__global__ void k_perm_fc(const float* __restrict__ W, int ncls, int C, int hw, uint16_t* __restrict__ o) {
+    const long F = (long)C * hw;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ncls * F) return;
+    const long cls = t / F;
+    const int f = (int)(t % F);
+    const int c = f / hw, pos = f % hw;
+    o[cls * F + (long)pos * C + c] = f_to_bf16(W[t]);
+}
+
 unsigned nblk(long n, int t = 256) { return (unsigned)std::min<long>((n + t - 1) / t, 1l << 30); }
 
 }  // namespace
@@ -389,6 +543,10 @@ struct bsx_cnn {
     float* d_gact[5] = {};
     float* d_loss = nullptr;
     long step = 0;
+    // MFMA inference: permuted BF16 weights of layers 2-4 and the FC, NHWC
+    // BF16 activations of layers 1-4
+    uint16_t* d_wi[5] = {};
+    uint16_t* d_ia[5] = {};
 };
 
 namespace {
@@ -400,6 +558,8 @@ void free_cnn(bsx_cnn* h) {
     for (int i = 0; i < 5; ++i) {
         if (h->d_act[i]) (void)hipFree(h->d_act[i]);
         if (h->d_gact[i]) (void)hipFree(h->d_gact[i]);
+        if (h->d_wi[i]) (void)hipFree(h->d_wi[i]);
+        if (h->d_ia[i]) (void)hipFree(h->d_ia[i]);
     }
 }
 
@@ -495,6 +655,8 @@ int bsx_cnn_create(bsx_cnn** out, int32_t in_size, const int32_t* channels, int3
     for (int i = 0; i < 4; ++i)
         BSX_REQUIRE(channels[i] >= 4 && channels[i] <= 1024 && channels[i] % 4 == 0, BSX_ERR_ARG,
                     "bsx_cnn_create: channel count %d must be a multiple of 4 in 4..1024", channels[i]);
+    BSX_REQUIRE(channels[0] <= 1024 && channels[0] % 2 == 0, BSX_ERR_ARG,
+                "bsx_cnn_create: first-layer channel count must be even");
     BSX_REQUIRE(nclasses >= 2 && nclasses <= (1 << 20), BSX_ERR_ARG,
                 "bsx_cnn_create: %d classes out of range", nclasses);
     BSX_REQUIRE(max_batch >= 1 && max_batch <= 65536, BSX_ERR_ARG,
@@ -547,7 +709,14 @@ int bsx_cnn_create(bsx_cnn** out, int32_t in_size, const int32_t* channels, int3
         const long n = act_size(h, l, max_batch);
         if ((e = hipMalloc(&h->d_act[l], sizeof(float) * n)) != hipSuccess) return fail(e, "activations");
         if ((e = hipMalloc(&h->d_gact[l], sizeof(float) * n)) != hipSuccess) return fail(e, "activations");
+        if (l >= 1 && (e = hipMalloc(&h->d_ia[l], sizeof(uint16_t) * n)) != hipSuccess)
+            return fail(e, "inference activations");
     }
+    for (int l = 1; l < 4; ++l)
+        if ((e = hipMalloc(&h->d_wi[l], sizeof(uint16_t) * h->ch[l + 1] * h->ch[l] * 9)) != hipSuccess)
+            return fail(e, "inference weights");
+    if ((e = hipMalloc(&h->d_wi[4], sizeof(uint16_t) * h->feat * nclasses)) != hipSuccess)
+        return fail(e, "inference weights");
     if ((e = hipMemcpy(h->d_p, p.data(), sizeof(float) * o, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemset(h->d_m, 0, sizeof(float) * o)) != hipSuccess ||
         (e = hipMemset(h->d_v, 0, sizeof(float) * o)) != hipSuccess ||
@@ -635,16 +804,53 @@ int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int
                 "bsx_cnn_infer_topk: null argument");
     BSX_REQUIRE(k >= 1 && k <= 32 && k <= h->nclass, BSX_ERR_ARG, "bsx_cnn_infer_topk: k %d out of range", k);
     hipStream_t s = (hipStream_t)stream;
+    // the MFMA layers need input channel counts (c1..c3 and the FC input) in multiples of 32
+    const bool mfma = h->ch[1] % 32 == 0 && h->ch[2] % 32 == 0 && h->ch[3] % 32 == 0 &&
+                      h->feat % 32 == 0;
     if (!h->pb_valid) {
         hipLaunchKernelGGL(k_to_bf16, dim3(nblk(h->nparam)), dim3(256), 0, s, h->d_p, h->d_pb, h->nparam);
         BSX_HIP(hipGetLastError());
+        if (mfma) {
+            for (int l = 1; l < 4; ++l) {
+                const long nw = (long)h->ch[l + 1] * h->ch[l] * 9;
+                hipLaunchKernelGGL(k_perm_conv, dim3(nblk(nw)), dim3(256), 0, s, h->d_p + h->woff[l],
+                                   h->ch[l + 1], h->ch[l], h->d_wi[l]);
+                BSX_HIP(hipGetLastError());
+            }
+            const long nf = h->feat * h->nclass;
+            hipLaunchKernelGGL(k_perm_fc, dim3(nblk(nf)), dim3(256), 0, s, h->d_p + h->woff[4],
+                               h->nclass, h->ch[4], h->hw[4] * h->hw[4], h->d_wi[4]);
+            BSX_HIP(hipGetLastError());
+        }
         h->pb_valid = true;
     }
     const long in = (long)h->S * h->S;
     for (int64_t b0 = 0; b0 < n; b0 += h->maxb) {
         const int nb = (int)std::min<int64_t>(h->maxb, n - b0);
-        int rc = forward<uint16_t>(h, x_dev + b0 * in, nb, h->d_pb, s);
-        if (rc) return rc;
+        if (!mfma) {
+            int rc = forward<uint16_t>(h, x_dev + b0 * in, nb, h->d_pb, s);
+            if (rc) return rc;
+        } else {
+            const uint16_t* P = h->d_pb;
+            const long pos1 = (long)nb * in;
+            hipLaunchKernelGGL(k_conv1_nhwc, dim3(nblk(pos1)), dim3(256), 0, s, x_dev + b0 * in,
+                               P + h->woff[0], P + h->boff[0], nb, h->S, h->ch[1], h->d_ia[1]);
+            BSX_HIP(hipGetLastError());
+            for (int l = 2; l <= 4; ++l) {
+                const long M = (long)nb * h->hw[l] * h->hw[l];
+                dim3 g((unsigned)((M + IM - 1) / IM), (unsigned)((h->ch[l] + IN - 1) / IN));
+                hipLaunchKernelGGL((k_conv_mfma<false>), g, dim3(256), 0, s, h->d_ia[l - 1], nb,
+                                   h->hw[l - 1], h->hw[l - 1], h->ch[l - 1], 3, 2, h->hw[l], h->hw[l],
+                                   h->d_wi[l - 1], P + h->boff[l - 1], h->ch[l], (long)h->ch[l],
+                                   (void*)h->d_ia[l]);
+                BSX_HIP(hipGetLastError());
+            }
+            dim3 g((unsigned)((nb + IM - 1) / IM), (unsigned)((h->nclass + IN - 1) / IN));
+            hipLaunchKernelGGL((k_conv_mfma<true>), g, dim3(256), 0, s, h->d_ia[4], nb, 1, 1,
+                               (int)h->feat, 1, 1, 1, 1, h->d_wi[4], P + h->boff[4], h->nclass,
+                               (long)h->nclass, (void*)h->d_act[0]);
+            BSX_HIP(hipGetLastError());
+        }
         hipLaunchKernelGGL(k_row_topk, dim3(nb), dim3(256), 0, s, h->d_act[0], h->nclass, k,
                            cls_dev + b0 * k, prob_dev + b0 * k);
         BSX_HIP(hipGetLastError());

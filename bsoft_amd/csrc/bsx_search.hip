@@ -625,33 +625,43 @@ __global__ __launch_bounds__(256) void k_rescore(const float2* __restrict__ fp,
     }
 }
 
-// One workgroup per (particle, candidate rotation): the section at the disk
-// points is interpolated on the fly into LDS, then wave 0 forms the FRC.
+// One workgroup per particle: the kernel table is loaded once and each wave
+// takes candidate rotations w, w+4, ...: the section at the disk points is
+// interpolated on the fly into the wave's LDS slice, then the wave forms the
+// FRC with the same fixed-order ring reduction as k_rescore.
 template <int WD>
 __global__ __launch_bounds__(256) void k_score_mats(
     const float2* __restrict__ ft, int N, const double* __restrict__ mats, int ncand, double inv,
     double rs, const float* __restrict__ ktab, const int2* __restrict__ hk,
     const int* __restrict__ ring_start, int R, int Nd, const float2* __restrict__ fp,
-    const float* __restrict__ ctf, float* __restrict__ frc) {
+    const float* __restrict__ ctf, int slices, float* __restrict__ frc) {
     __shared__ float K[(FSI_NDIV + 1) * WD];
-    extern __shared__ float2 sv[];
+    extern __shared__ float2 sv[];            // one Nd slice per working wave
     for (int i = threadIdx.x; i < (FSI_NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
     __syncthreads();
-    const int64_t pc = blockIdx.x;
-    const int64_t p = pc / ncand;
-    const double* M = mats + 9 * pc;
-    for (int i = threadIdx.x; i < Nd; i += blockDim.x) {
-        const double ivx = (double)hk[i].x * inv, ivy = (double)hk[i].y * inv;
-        double m[3];
-        for (int c = 0; c < 3; ++c) m[c] = (ivx * M[3 * c] + ivy * M[3 * c + 1] + 0.0 * M[3 * c + 2]) * rs;
-        double vr, vi;
-        fsi_sample<WD>(ft, N, m, K, vr, vi);
-        sv[i] = make_float2((float)vr, (float)vi);
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const float f = frc_wave(fp + p * Nd, ctf + p * Nd, sv, ring_start, R, threadIdx.x);
-        if (threadIdx.x == 0) frc[pc] = f;
+    const int64_t p = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave >= slices) return;
+    float2* my = sv + (size_t)wave * Nd;
+    for (int c = wave; c < ncand; c += slices) {
+        const double* M = mats + 9 * (p * ncand + c);
+        double m0[3], m1[3];
+        for (int q = 0; q < 3; ++q) { m0[q] = M[3 * q]; m1[q] = M[3 * q + 1]; }
+        for (int i = lane; i < Nd; i += 64) {
+            const double ivx = (double)hk[i].x * inv, ivy = (double)hk[i].y * inv;
+            double m[3];
+            for (int q = 0; q < 3; ++q) m[q] = (ivx * m0[q] + ivy * m1[q] + 0.0 * M[3 * q + 2]) * rs;
+            double vr, vi;
+            fsi_sample<WD>(ft, N, m, K, vr, vi);
+            my[i] = make_float2((float)vr, (float)vi);
+        }
+        // the wave's own LDS writes are visible to its lanes after this fence
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const float f = frc_wave(fp + p * Nd, ctf + p * Nd, my, ring_start, R, lane);
+        if (lane == 0) frc[p * ncand + c] = f;
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -1242,12 +1252,14 @@ int bsx_search_score_mats(bsx_search* h, const float* ft_dev, const double* mats
     hipStream_t s = (hipStream_t)stream;
     const int N = h->p.box;
     const double inv = 1.0 / (N * h->p.sampling), rs = N * h->p.sampling;
-    const unsigned g = (unsigned)(h->nparts * ncand);
-    const size_t shm = sizeof(float2) * h->Nd;
+    const unsigned g = (unsigned)h->nparts;
+    // LDS slices of Nd values, one per working wave, within 64 KB
+    const int slices = std::max(1, std::min(4, 8192 / h->Nd));
+    const size_t shm = sizeof(float2) * h->Nd * slices;
     switch (h->p.kernel_width) {
 #define BSX_SM(W) case W: hipLaunchKernelGGL((k_score_mats<W>), dim3(g), dim3(256), shm, s, \
         (const float2*)ft_dev, N, mats_dev, ncand, inv, rs, h->d_ktab, h->d_hk, h->d_ring, h->R, \
-        h->Nd, h->d_fp, h->d_ctf, frc_dev); break;
+        h->Nd, h->d_fp, h->d_ctf, slices, frc_dev); break;
         BSX_SM(1) BSX_SM(2) BSX_SM(3) BSX_SM(4) BSX_SM(5)
         BSX_SM(6) BSX_SM(7) BSX_SM(8) BSX_SM(9) BSX_SM(10)
 #undef BSX_SM

@@ -51,8 +51,10 @@ def _col2im(dcols, shape, stride):
     return dxp[:, :, 1:-1, 1:-1]
 
 
-def forward(p: np.ndarray, x: np.ndarray, S: int, ch, ncls: int):
-    """x: (B, S, S). Returns logits (B, ncls) and the cache for backward."""
+def forward(p: np.ndarray, x: np.ndarray, S: int, ch, ncls: int, round_act=None):
+    """x: (B, S, S). Returns logits (B, ncls) and the cache for backward.
+    round_act, if given, is applied to each conv layer's output (the storage
+    rounding of an inference path)."""
     off, n, feat, chans = layout(S, ch, ncls)
     a = x.reshape(len(x), 1, S, S).astype(np.float64)
     cache = []
@@ -64,6 +66,8 @@ def forward(p: np.ndarray, x: np.ndarray, S: int, ch, ncls: int):
         cols = _im2col(a, stride)
         z = np.einsum("oikl,bikluv->bouv", W, cols) + b[None, :, None, None]
         y = np.maximum(z, 0)
+        if round_act is not None:
+            y = round_act(y)
         cache.append((a, cols, y, stride))
         a = y
     o, ob, _ = off[4]

@@ -56,8 +56,13 @@ def test_forward_and_gradients_match_oracle(S, ch, ncls, B):
             assert err <= 1e-4 * np.abs(ref).max() + 1e-7
 
 
-def test_training_reduces_loss_and_bf16_inference():
-    S, ch, ncls, B = 16, (8, 8, 16, 8), 20, 64
+@pytest.mark.parametrize("ch", [(8, 8, 16, 8), (32, 64, 32, 64)])
+def test_training_reduces_loss_and_bf16_inference(ch):
+    """Adam memorises a fixed batch; BF16 inference (VALU path for small
+    channel counts, MFMA implicit GEMM when the channels are multiples of
+    32) equals the FP64 forward of the BF16-rounded weights with BF16
+    activations."""
+    S, ncls, B = 16, 20, 64
     net = ProposalNet(S, ncls, ch, max_batch=B, seed=5)
     rng = np.random.default_rng(1)
     x = torch.as_tensor(rng.normal(size=(B, S, S)).astype(np.float32)).cuda()
@@ -69,7 +74,9 @@ def test_training_reduces_loss_and_bf16_inference():
     assert last < 0.2 * first
     cls, prob = net.topk(x, 5)
     p = bf16_round(net.params().cpu().numpy())
-    ol, _ = cnn_ref.forward(p, x.cpu().numpy().astype(np.float64), S, ch, ncls)
+    mfma = all(c % 32 == 0 for c in ch[:3])
+    ol, _ = cnn_ref.forward(p, x.cpu().numpy().astype(np.float64), S, ch, ncls,
+                            round_act=bf16_round if mfma else None)
     e = np.exp(ol - ol.max(axis=1, keepdims=True))
     pr = e / e.sum(axis=1, keepdims=True)
     order = np.argsort(-pr, axis=1, kind="stable")[:, :5]
