@@ -187,12 +187,86 @@ struct Emit {
 // a workgroup uses the same slot in both.
 #define BSX_SLOTS 8
 
+// Fill-pass staging in LDS (nearest / weighted-nearest insertion without
+// Ewald): a workgroup appends its merged entries to an LDS list and counts
+// them per bin in a small LDS hash table; at each flush one integer atomic
+// per (workgroup, bin) reserves the bin range and the entries are written
+// out. This replaces one returning atomic per (wavefront, bin, emission) and
+// its latency wait. The per-bin totals equal the count pass's, so the ranges
+// tile the bins exactly.
+#define BSX_STAGE_CAP 1024     // entries per flush: 2 rounds x 2 emissions x 256 threads
+#define BSX_STAGE_HASH 128     // bins per flush (more fall back to direct reservations)
+
+struct Stage {
+    float4* e;       // (re, im, w, key bits)
+    int* r;          // (hash slot << 20) | rank in the bin, -1 = written directly
+    int* hkey;       // bin per hash slot, -1 = free
+    int* hcnt;       // entries per hash slot
+    int* hbase;      // reserved base per hash slot
+    int* count;      // entries in the list
+    int on;
+};
+
 struct BinCtx {
     int* ctr;        // per-(bin, slot) counters: counts (count pass) / cursors (fill pass)
     float4* bins;    // (re, im, w, record-in-tile)
     int cap;         // entries allocated (the count pass bounds the fill pass)
     int slot;        // this workgroup's counter slot
+    Stage st;        // fill-pass staging (st.on = 0: reserve per wavefront)
 };
+
+__device__ __forceinline__ void stage_entry(const BinCtx& bc, int key, float re, float im,
+                                            float w) {
+    const int bin = (int)((unsigned)key >> 12);
+    const int i = atomicAdd(bc.st.count, 1);
+    int slot = -1;
+    if (i < BSX_STAGE_CAP) {
+        int hs = (int)(((unsigned)bin * 2654435761u) >> 25) & (BSX_STAGE_HASH - 1);
+        for (int t = 0; t < BSX_STAGE_HASH; ++t) {
+            const int prev = atomicCAS(&bc.st.hkey[hs], -1, bin);
+            if (prev == -1 || prev == bin) { slot = hs; break; }
+            hs = (hs + 1) & (BSX_STAGE_HASH - 1);
+        }
+    }
+    if (slot < 0) {
+        // list or table full: reserve this entry on its own
+        if (i < BSX_STAGE_CAP) bc.st.r[i] = -1;
+        const int s = atomicAdd(bc.ctr + bin * BSX_SLOTS + bc.slot, 1);
+        if (s < bc.cap) bc.bins[s] = make_float4(re, im, w, __int_as_float((int)((unsigned)key & 4095u)));
+        return;
+    }
+    const int rank = atomicAdd(&bc.st.hcnt[slot], 1);
+    bc.st.e[i] = make_float4(re, im, w, __int_as_float(key));
+    bc.st.r[i] = (slot << 20) | rank;
+}
+
+// Workgroup-uniform: reserve every staged bin, write the staged entries,
+// clear the list and the table.
+__device__ __forceinline__ void stage_flush(const BinCtx& bc) {
+    __syncthreads();
+    const int tid = threadIdx.x;
+    if (tid < BSX_STAGE_HASH) {
+        const int b = bc.st.hkey[tid];
+        if (b >= 0) bc.st.hbase[tid] = atomicAdd(bc.ctr + b * BSX_SLOTS + bc.slot, bc.st.hcnt[tid]);
+    }
+    __syncthreads();
+    const int n = min(*bc.st.count, BSX_STAGE_CAP);
+    for (int i = tid; i < n; i += blockDim.x) {
+        const int pr = bc.st.r[i];
+        if (pr < 0) continue;
+        const float4 v = bc.st.e[i];
+        const int sl = bc.st.hbase[pr >> 20] + (pr & 0xFFFFF);
+        if (sl < bc.cap)
+            bc.bins[sl] = make_float4(v.x, v.y, v.z, __int_as_float(__float_as_int(v.w) & 4095));
+    }
+    __syncthreads();
+    if (tid < BSX_STAGE_HASH) {
+        bc.st.hkey[tid] = -1;
+        bc.st.hcnt[tid] = 0;
+    }
+    if (tid == 0) *bc.st.count = 0;
+    __syncthreads();
+}
 
 // Merges runs of equal keys inside aligned groups of MG lanes (segmented
 // Hillis-Steele suffix sums); lanes l and l+d merge only if their keys are
@@ -262,6 +336,10 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
             if (lane == leader) atomicAdd(bc.ctr + b0 * BSX_SLOTS + bc.slot, __popcll(m));
             active &= ~m;
         }
+        return;
+    }
+    if (MODE == MODE_FILL && bc.st.on) {
+        if (out) stage_entry(bc, e.key, e.re, e.im, e.w);
         return;
     }
     // fill: issue every reservation of this wavefront before waiting for any
@@ -466,7 +544,13 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     constexpr int HX = FT / 2 + 1;
     constexpr int MG = T < 16 ? T : 16;   // run-merge group: lanes of one column
     constexpr int LDSN = MODE == MODE_COUNT ? 1 : G * FT + FT;
+    constexpr int SCAP = MODE == MODE_FILL && !FOM ? BSX_STAGE_CAP : 1;
+    constexpr int SHASH = MODE == MODE_FILL && !FOM ? BSX_STAGE_HASH : 1;
     __shared__ float2 lds[LDSN];
+    __shared__ float4 st_e[SCAP];
+    __shared__ int st_r[SCAP];
+    __shared__ int st_hkey[SHASH], st_hcnt[SHASH], st_hbase[SHASH];
+    __shared__ int st_count;
     const int tid = threadIdx.x;
     const int g = tid / T, t = tid - g * T;
     const long task = (long)blockIdx.x * G + g;
@@ -483,6 +567,18 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         FftLine<FT, -1, T>::run(line, tw, t);
     }
     bc.slot = (int)(blockIdx.x % BSX_SLOTS);
+    // staging: 2 rounds of at most 2 emissions per thread fit the list
+    const bool staged = MODE == MODE_FILL && !FOM && K.interp < 2 && !K.ewald && NT == 256 &&
+                        2 * 2 * NT <= BSX_STAGE_CAP;
+    bc.st = Stage{st_e, st_r, st_hkey, st_hcnt, st_hbase, &st_count, staged ? 1 : 0};
+    if (staged) {
+        if (tid < BSX_STAGE_HASH) {
+            st_hkey[tid] = -1;
+            st_hcnt[tid] = 0;
+        }
+        if (tid == 0) st_count = 0;
+        __syncthreads();
+    }
     const PartDev& P = recs[valid ? p : 0];
     const unsigned keybase = (unsigned)P.half * (unsigned)(K.NT * K.NT * K.NT) * 4096u;
     const float inv = 1.0f / FT;
@@ -505,6 +601,7 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         const int ym = (FT - y) & (FT - 1);
         const long km = (ym <= (FT - 1) / 2) ? ym : ym - FT;
         insert_pair<MG, FOM, MODE>(acc, K, P, valid, mirror, h, k, km, F, merge, keybase, bc);
+        if (MODE == MODE_FILL && staged && (r & 1)) stage_flush(bc);
     }
 }
 
