@@ -122,46 +122,52 @@ inline std::vector<float> fsi_table(int wd, int power) {
     return std::vector<float>(data.begin(), data.end());
 }
 
+// Periodic index in [0, N): one compare in the usual in-range case, a
+// modulo otherwise.
+__device__ __forceinline__ int wrap_index(int i, int N) {
+    if ((unsigned)i < (unsigned)N) return i;
+    const int r = i % N;
+    return r < 0 ? r + N : r;
+}
+
 // Bimage::fspace_interpolate (src/img/Bimage_fspace.cpp:85-147): the value of
 // an N^3 complex transform at voxel coordinate m by WD^3 kernel taps around
 // floor(m) - (WD-1)/2, indices wrapped periodically (the reference wraps one
 // step; the total wrap here keeps every index inside the volume for any m).
-// K: the kernel table (in LDS). FP32 taps, per-plane sums added in double
-// into (vr, vi).
+// K: the kernel table (in LDS); the x and y weights of a sample are read once.
+// FP32 taps, per-plane sums added in double into (vr, vi).
 template <int WD>
 __device__ inline void fsi_sample(const float2* __restrict__ ft, int N, const double m[3],
                                   const float* K, double& vr, double& vi) {
     constexpr int HW = (WD - 1) / 2;
-    long ix = (long)floor(m[0]), iy = (long)floor(m[1]), iz = (long)floor(m[2]);
-    const int ikx = ((int)((m[0] - ix) * FSI_NDIV + 0.5)) * WD;
-    const int iky = ((int)((m[1] - iy) * FSI_NDIV + 0.5)) * WD;
-    const int ikz = ((int)((m[2] - iz) * FSI_NDIV + 0.5)) * WD;
-    ix -= HW;
-    iy -= HW;
-    iz -= HW;
-    long jxs[WD];
+    const double fx = floor(m[0]), fy = floor(m[1]), fz = floor(m[2]);
+    const int ikx = ((int)((m[0] - fx) * FSI_NDIV + 0.5)) * WD;
+    const int iky = ((int)((m[1] - fy) * FSI_NDIV + 0.5)) * WD;
+    const int ikz = ((int)((m[2] - fz) * FSI_NDIV + 0.5)) * WD;
+    const int ix = (int)fx - HW, iy = (int)fy - HW, iz = (int)fz - HW;
+    float kx[WD], ky[WD];
+    int jxs[WD];
 #pragma unroll
     for (int q = 0; q < WD; ++q) {
-        long jx = (ix + q) % N;
-        if (jx < 0) jx += N;
-        jxs[q] = jx;
+        kx[q] = K[ikx + q];
+        ky[q] = K[iky + q];
+        jxs[q] = wrap_index(ix + q, N);
     }
     vr = 0;
     vi = 0;
 #pragma unroll 1
     for (int zz = 0; zz < WD; ++zz) {
-        long jz = (iz + zz) % N;
-        if (jz < 0) jz += N;
+        const long jz = wrap_index(iz + zz, N);
+        const float kzz = K[ikz + zz];
         float pr = 0.f, pi = 0.f;
-#pragma unroll 1
+#pragma unroll
         for (int yy2 = 0; yy2 < WD; ++yy2) {
-            long jy = (iy + yy2) % N;
-            if (jy < 0) jy += N;
-            const float w = K[iky + yy2] * K[ikz + zz];
+            const long jy = wrap_index(iy + yy2, N);
+            const float w = ky[yy2] * kzz;
             const float2* row = ft + (jz * N + jy) * (long)N;
 #pragma unroll
             for (int q = 0; q < WD; ++q) {
-                const float kw = K[ikx + q] * w;
+                const float kw = kx[q] * w;
                 const float2 a = row[jxs[q]];
                 pr = fmaf(a.x, kw, pr);
                 pi = fmaf(a.y, kw, pi);
