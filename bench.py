@@ -389,10 +389,12 @@ def insertion_roofline(kt_sum, launches, nparticles, box, ft, N):
     dom_ms = max(cand[dom], 1e-9)
     achieved = kernel_bytes[dom] / (dom_ms * 1e-3) / 1e9
     ins_ms = sum(kt_sum.get(k, 0.0) for k in ("count", "scan", "fill", "tile", "direct"))
+    kname = {"fill": f"k_cols<{ft}, false, 2>", "count": f"k_cols<{ft}, false, 1>",
+             "tile": "k_tile_accum", "direct": f"k_cols<{ft}, false, 0>"}[dom]
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": {"fill": f"k_cols<{ft}, false, 2>", "count": f"k_cols<{ft}, false, 1>",
-                       "tile": "k_tile_accum", "direct": f"k_cols<{ft}, false, 0>"}[dom],
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(kname, nparticles / nbatch) if ft == 512 else None,
+            "kernel": kname,
             "stage": dom, "avg_launch_ms": round(dom_ms / nbatch, 4), "launches": nbatch,
             "algorithmic_bytes_per_launch": round(kernel_bytes[dom] / nbatch),
             "bin_entries_per_particle": round(entries / max(1, nparticles), 1),
@@ -401,8 +403,11 @@ def insertion_roofline(kt_sum, launches, nparticles, box, ft, N):
 
 
 def pipeline_roofline(out, s, P, args, recon_roof):
-    """Roofline of the GEMM (MFMA-bound: 2 * particles * views * K flops per
-    launch) and, for the record, the other stages of the learned pipeline."""
+    """The learned step's dominant kernel is the insertion's fill pass
+    (k_cols<512,false,2>): its HBM roofline, timed with HIP events in the
+    known-view reconstruction (same kernel, same particles), is `roofline`.
+    The GEMM (MFMA-bound: 2 * particles * views * K flops) and the
+    refinement's gathers are reported beside it."""
     gemm_ms = out["plain"]["stages"].get("gemm", 0.0)
     kreal = 2 * (s.npoints - 1)
     flops = 2.0 * P * s.nviews * kreal
@@ -410,16 +415,30 @@ def pipeline_roofline(out, s, P, args, recon_roof):
     refine_ms = out["learned"]["stages"].get("refine", 0.0)
     # refinement: 2 steps x 27 rotations, each a section of npoints with w^3 taps of 8 B
     refine_bytes = 2 * 27 * P * s.npoints * args.kernel_width ** 3 * 8
-    return {"bound": "mfma", "achieved": round(gemm_tf, 1), "peak": BF16_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(gemm_tf / BF16_PEAK_TFLOPS, 4), "traffic": None,
-            "kernel": "k_gemm_topk<8> (+ k_topk_merge<8>)", "stage": "gemm",
-            "avg_launch_ms": round(gemm_ms, 3), "launches": 1,
-            "algorithmic_flops_per_launch": flops,
-            "other_stages": {
-                "refine_k_score_mats": {"ms": round(refine_ms, 2), "gathered_GBs":
-                                        round(refine_bytes / max(refine_ms, 1e-9) / 1e6, 1)},
-                "insertion": {k: recon_roof[k] for k in ("kernel", "achieved", "frac", "avg_launch_ms")}},
-            }
+    roof = dict(recon_roof)
+    roof["other_kernels"] = {
+        "k_gemm_topk<8> (plain search)": {
+            "bound": "mfma", "achieved_TFLOPs": round(gemm_tf, 1), "peak_TFLOPs": BF16_PEAK_TFLOPS,
+            "frac": round(gemm_tf / BF16_PEAK_TFLOPS, 4), "ms": round(gemm_ms, 3),
+            "algorithmic_flops": flops},
+        "k_score_mats (refinement)": {"ms": round(refine_ms, 2), "gathered_GBs":
+                                      round(refine_bytes / max(refine_ms, 1e-9) / 1e6, 1)}}
+    return roof
+
+
+def measured_traffic(kernel: str, particles_per_launch: float):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE / WRITE_SIZE per particle, already scaled by the
+    counters' readings for a 1 GiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "recon_256_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t["kernels"][kernel]
+        return round((k["fetch_bytes_per_particle"] + k["write_bytes_per_particle"]) *
+                     particles_per_launch)
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def finish_result(args, result, ps, rec, ft, dev):
