@@ -75,7 +75,7 @@ def main() -> int:
     ap.add_argument("--box", type=int, default=256)
     ap.add_argument("--blobs", type=int, default=400)
     ap.add_argument("--radius", type=int, default=24, help="search disk radius (Fourier pixels)")
-    ap.add_argument("--kernel-width", type=int, default=4, help="FSI kernel width of the search")
+    ap.add_argument("--kernel-width", type=int, default=3, help="FSI kernel width of the search")
     ap.add_argument("--grid-step", type=float, default=4.0)
     ap.add_argument("--class-step", type=float, default=15.0)
     ap.add_argument("--train", type=int, default=60000, help="training projections")
