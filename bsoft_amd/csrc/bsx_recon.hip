@@ -608,7 +608,9 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
 // Bins are processed in pieces of at most PIECE entries so that the few hot
 // bins around the origin (every plane passes through the low-frequency core)
 // are spread over many workgroups.
+#ifndef BSX_PIECE
 #define BSX_PIECE 16384
+#endif
 
 // Exclusive scans (one workgroup) of the per-(bin, slot) counts: entry
 // offsets offsets[0..nbins*SLOTS] (cursors = offsets), and per-bin piece
