@@ -1,0 +1,342 @@
+"""The on-disk formats either side of the reconstruction path: MRC images and
+stacks, and Bsoft STAR project files (micrograph blocks with CTF parameters
+and particle loops).
+
+Behaviour follows the reference's readers and writers:
+  readMRC / writeMRC      src/rwimg/rwMRC.cpp:58-195, 283-356 (header layout
+                          include/rwMRC.h:83-124, the MRC2014 standard)
+  STAR project reading    src/rwparam/rwmgSTAR.cpp:189-270 (particles),
+                          :455-517 (CTF), tags include/mg_tags.h
+"""
+from __future__ import annotations
+
+import math
+import os
+import shlex
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .synth import PARTICLE_DTYPE
+
+# ------------------------------------------------------------------- MRC
+# MRC2014 header: 1024 bytes (the standard's field order).
+MRC_HEADER = np.dtype([
+    ("nx", "<i4"), ("ny", "<i4"), ("nz", "<i4"), ("mode", "<i4"),
+    ("nxstart", "<i4"), ("nystart", "<i4"), ("nzstart", "<i4"),
+    ("mx", "<i4"), ("my", "<i4"), ("mz", "<i4"),
+    ("cella", "<f4", 3), ("cellb", "<f4", 3),
+    ("mapc", "<i4"), ("mapr", "<i4"), ("maps", "<i4"),
+    ("dmin", "<f4"), ("dmax", "<f4"), ("dmean", "<f4"),
+    ("ispg", "<i4"), ("nsymbt", "<i4"),
+    ("extra1", "<f4", 2), ("exttype", "S4"), ("nversion", "<i4"),
+    ("extra2", "<f4", 10), ("imod_stamp", "<i4"), ("imod_flags", "<i4"),
+    ("extra3", "<f4", 9), ("origin", "<f4", 3),
+    ("map", "S4"), ("machst", "u1", 4), ("rms", "<f4"), ("nlabl", "<i4"),
+    ("labels", "S800"),
+])
+assert MRC_HEADER.itemsize == 1024
+
+_MODES = {0: np.uint8, 1: np.int16, 2: np.float32, 6: np.uint16, 7: np.int32, 12: np.float16}
+
+
+@dataclass
+class MrcImage:
+    data: np.ndarray              # (images, z, y, x) float32
+    sampling: tuple               # (sx, sy, sz) A/voxel
+    origin: tuple                 # (ox, oy, oz) voxels, as the reference sets it
+    header: np.ndarray = field(repr=False, default=None)
+
+    @property
+    def images(self) -> int:
+        return self.data.shape[0]
+
+
+def read_mrc(path: str, img_select: int = -1) -> MrcImage:
+    """An MRC file as float32 (images, z, y, x). Stacks follow readMRC: files
+    before MRC2014 are stacks when the map field says "STK ", the extension
+    is st/stk/mrcs or contains "ali", or the extended type is SERI; MRC2014
+    files hold nz / mz volumes of mz sections. img_select >= 0 reads one image."""
+    with open(path, "rb") as f:
+        raw = f.read(1024)
+        if len(raw) < 1024:
+            raise ValueError(f"{path}: not an MRC file (short header)")
+        h = np.frombuffer(raw, dtype=MRC_HEADER)[0].copy()
+        swap = abs(int(h["mode"])) > 65535 or abs(int(h["mapc"])) > 65535
+        if swap:
+            h = np.frombuffer(raw, dtype=MRC_HEADER.newbyteorder(">"))[0].copy()
+        nx, ny, nz, mode = int(h["nx"]), int(h["ny"]), int(h["nz"]), int(h["mode"])
+        if nx < 1 or ny < 1 or nz < 1 or mode < 0:
+            raise ValueError(f"{path}: not a valid MRC image (nx {nx} ny {ny} nz {nz} mode {mode})")
+        if mode not in _MODES:
+            raise ValueError(f"{path}: MRC mode {mode} not supported")
+        exttype = bytes(h["exttype"]).decode("latin1")
+        labels = bytes(h["labels"])
+        if (not exttype or exttype[0] == " " or exttype[0] == "\0") and labels.startswith(b"Seri"):
+            exttype = "SERI"
+        ext = os.path.splitext(path)[1].lstrip(".").lower()
+        mz = int(h["mz"])
+        nver = int(h["nversion"])
+        if nver < 20140 or nver > 20209:
+            if bytes(h["map"]) == b"STK " or ext in ("st", "stk", "mrcs") or "ali" in ext or \
+                    exttype == "SERI":
+                nimg, sz = nz, 1
+            elif mz and mz < nz:
+                nimg, sz = nz // mz, mz
+            else:
+                nimg, sz = 1, nz
+        else:
+            if exttype == "SERI":
+                nimg, sz = nz, 1
+            else:
+                sz = mz if mz > 0 else nz
+                nimg = nz // sz
+        dt = np.dtype(_MODES[mode])
+        if mode == 0 and int(h["imod_flags"]) & 1:
+            dt = np.dtype(np.int8)
+        if swap:
+            dt = dt.newbyteorder(">")
+        per = nx * ny * sz
+        f.seek(1024 + int(h["nsymbt"]))
+        first, count = 0, nimg
+        if img_select >= 0:
+            first, count = min(img_select, nimg - 1), 1
+        f.seek(first * per * dt.itemsize, 1)
+        buf = np.fromfile(f, dtype=dt, count=count * per)
+        if buf.size != count * per:
+            raise ValueError(f"{path}: truncated data")
+    data = buf.astype(np.float32).reshape(count, sz, ny, nx)
+    mx, my = int(h["mx"]), int(h["my"])
+    a, b, c = (float(v) for v in h["cella"])
+    samp = (a / mx if mx else 1.0, b / my if my else 1.0, c / mz if mz else 1.0)
+    o = h["origin"]
+    origin = tuple(-float(o[i]) / samp[i] for i in range(3))
+    return MrcImage(data=data, sampling=samp, origin=origin, header=h)
+
+
+def write_mrc(path: str, data: np.ndarray, sampling=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0),
+              images: int = 1) -> None:
+    """Write float32 data as an MRC2014 file as writeMRC does: data (images*z,
+    y, x) with `images` volumes of z sections (a 2D stack: images = n, z = 1)."""
+    d = np.ascontiguousarray(data, dtype=np.float32)
+    if d.ndim == 2:
+        d = d[None]
+    nzt, ny, nx = d.shape
+    if nzt % images:
+        raise ValueError("section count is not a multiple of the image count")
+    sz = nzt // images
+    samp = tuple(float(s) for s in (list(sampling) + [1.0, 1.0])[:3])
+    h = np.zeros((), dtype=MRC_HEADER)
+    h["map"] = b"MAP "
+    h["nversion"] = 20140
+    h["machst"] = (0x44, 0x44, 0, 0)
+    h["nx"], h["ny"], h["nz"], h["mode"] = nx, ny, nzt, 2
+    for key, o in zip(("nxstart", "nystart", "nzstart"), origin):
+        h[key] = int(-(o + 0.5)) if o >= 0 else int(-(o - 0.5))
+    h["mx"], h["my"], h["mz"] = nx, ny, sz
+    h["cella"] = (nx * samp[0], ny * samp[1], sz * samp[2])
+    h["cellb"] = (90.0, 90.0, 90.0)
+    h["mapc"], h["mapr"], h["maps"] = 1, 2, 3
+    h["dmin"], h["dmax"], h["dmean"] = float(d.min()), float(d.max()), float(d.mean())
+    h["rms"] = float(d.std())
+    h["ispg"] = 0 if images > 1 or sz == 1 else 1
+    h["origin"] = tuple(-float(o) * s for o, s in zip(origin, samp))
+    h["nlabl"] = 1
+    h["labels"] = b"bsoft_amd".ljust(800)
+    with open(path, "wb") as f:
+        f.write(h.tobytes())
+        f.write(d.tobytes())
+
+
+# ------------------------------------------------------------------- STAR
+@dataclass
+class StarBlock:
+    name: str
+    items: dict = field(default_factory=dict)      # tag -> string value
+    loops: list = field(default_factory=list)      # [(tags, rows)]
+
+    def loop_with(self, tag: str):
+        for tags, rows in self.loops:
+            if tag in tags:
+                return tags, rows
+        return None
+
+
+def _tokens(text: str):
+    """STAR tokens: words, quoted strings, and ;-delimited text fields."""
+    lines = text.splitlines()
+    i = 0
+    while i < len(lines):
+        line = lines[i]
+        if line.startswith(";"):
+            buf = [line[1:]]
+            i += 1
+            while i < len(lines) and not lines[i].startswith(";"):
+                buf.append(lines[i])
+                i += 1
+            i += 1
+            yield "\n".join(buf).strip()
+            continue
+        try:
+            toks = shlex.split(line, comments=True, posix=True)
+        except ValueError:            # an unbalanced quote inside a value
+            toks = line.split("#", 1)[0].split()
+        yield from toks
+        i += 1
+
+
+def read_star(path: str) -> list[StarBlock]:
+    """Parse a STAR file into data blocks of single items and loops (tags
+    without the leading underscore)."""
+    with open(path, "r", encoding="latin1") as f:
+        toks = list(_tokens(f.read()))
+    blocks: list[StarBlock] = []
+    cur = None
+    i = 0
+    while i < len(toks):
+        t = toks[i]
+        if t.startswith("data_"):
+            cur = StarBlock(name=t[5:])
+            blocks.append(cur)
+            i += 1
+        elif t == "loop_":
+            if cur is None:
+                cur = StarBlock(name="")
+                blocks.append(cur)
+            i += 1
+            tags = []
+            while i < len(toks) and toks[i].startswith("_"):
+                tags.append(toks[i][1:])
+                i += 1
+            vals = []
+            while i < len(toks) and not (toks[i].startswith("_") or toks[i] == "loop_" or
+                                         toks[i].startswith("data_")):
+                vals.append(toks[i])
+                i += 1
+            if tags and len(vals) % len(tags):
+                raise ValueError(f"{path}: loop of {len(tags)} tags has {len(vals)} values")
+            rows = [vals[r:r + len(tags)] for r in range(0, len(vals), len(tags))] if tags else []
+            cur.loops.append((tags, rows))
+        elif t.startswith("_"):
+            if cur is None:
+                cur = StarBlock(name="")
+                blocks.append(cur)
+            if i + 1 >= len(toks):
+                raise ValueError(f"{path}: tag {t} without a value")
+            cur.items[t[1:]] = toks[i + 1]
+            i += 2
+        else:
+            i += 1
+    return blocks
+
+
+def _q(v) -> str:
+    s = str(v)
+    return f"'{s}'" if (" " in s or not s) else s
+
+
+def write_star(path: str, blocks: list[StarBlock]) -> None:
+    with open(path, "w") as f:
+        for b in blocks:
+            f.write(f"data_{b.name}\n\n")
+            for k, v in b.items.items():
+                f.write(f"_{k} {_q(v)}\n")
+            for tags, rows in b.loops:
+                f.write("\nloop_\n")
+                for t in tags:
+                    f.write(f"_{t}\n")
+                for r in rows:
+                    f.write(" ".join(_q(v) for v in r) + "\n")
+            f.write("\n")
+
+
+# ------------------------------------------------------------------- project
+@dataclass
+class ParticleSource:
+    file: str
+    index: int                    # image index in the file
+
+
+def _real(d: dict, *tags, default=None):
+    for t in tags:
+        if t in d:
+            return float(d[t])
+    return default
+
+
+def project_particles(path: str):
+    """The selected particles of a Bsoft STAR project: a PARTICLE_DTYPE record
+    per particle (view, origin, CTF of its micrograph with the particle's own
+    defocus average, pixel size, selection) and where its image lives.
+    Micrograph CTF: ctf.voltage (V), ctf.Cs (A), ctf.amplitude_phase_shift or
+    asin(ctf.amp_contrast), ctf.defocus_average / ctf.defocus_deviation (A),
+    ctf.astigmatism_angle (deg). Particle loop: particle.id, particle.file_name,
+    particle.defocus, particle.origin_x/y, particle.view_x/y/z,
+    particle.view_angle (deg), particle.select, particle.pixel_size_x."""
+    base = os.path.dirname(os.path.abspath(path))
+    recs, srcs = [], []
+    for b in read_star(path):
+        it = b.items
+        lp = b.loop_with("particle.id")
+        if lp is None:
+            continue
+        tags, rows = lp
+        col = {t: k for k, t in enumerate(tags)}
+        volt = _real(it, "ctf.voltage", "micrograph.voltage", default=0.0)
+        cs = _real(it, "ctf.Cs", "micrograph.ctf_Cs", default=0.0)
+        if "ctf.amplitude_phase_shift" in it:
+            amp_shift = float(it["ctf.amplitude_phase_shift"])
+        else:
+            amp_shift = math.asin(_real(it, "ctf.amp_contrast", "micrograph.ctf_amp_contrast",
+                                        default=0.0))
+        if "ctf.defocus_average" in it or "micrograph.ctf_defocus_average" in it:
+            def_avg = _real(it, "ctf.defocus_average", "micrograph.ctf_defocus_average")
+            def_dev = _real(it, "ctf.defocus_deviation", default=0.0)
+        else:
+            lo = _real(it, "ctf.defocus_min", default=0.0)
+            hi = _real(it, "ctf.defocus_max", default=0.0)
+            def_avg, def_dev = (hi + lo) / 2, abs(hi - lo) / 2
+        ast = math.radians(_real(it, "ctf.astigmatism_angle", default=0.0))
+        pix = _real(it, "micrograph.pixel_size_x", "micrograph.pixel_size", default=1.0)
+        mg_file = it.get("particle.file_name", it.get("micrograph_particle.file_name", ""))
+        for r in rows:
+            g = lambda t, d=None: float(r[col[t]]) if t in col else d
+            pid = int(float(r[col["particle.id"]]))
+            rec = np.zeros((), dtype=PARTICLE_DTYPE)
+            view = [g("particle.view_x", 0.0), g("particle.view_y", 0.0), g("particle.view_z", 1.0),
+                    math.radians(g("particle.view_angle", 0.0))]
+            if "particle.view_x" not in col and ("particle.psi" in col or "particle.theta" in col):
+                raise ValueError(f"{path}: Euler-angle particles are not supported (use view tags)")
+            rec["view"] = view
+            rec["ori"] = (g("particle.origin_x", 0.0), g("particle.origin_y", 0.0))
+            rec["pixel_size"] = g("particle.pixel_size_x", g("particle.pixel_size", pix))
+            rec["mag"] = g("particle.magnification", 1.0)
+            rec["volt"], rec["cs"], rec["amp_shift"] = volt, cs, amp_shift
+            rec["def_avg"] = g("particle.defocus", def_avg)
+            rec["def_dev"], rec["ast_angle"] = def_dev, ast
+            rec["sel"] = g("particle.select", 1.0)
+            if "particle.file_name" in col:
+                src = ParticleSource(os.path.join(base, r[col["particle.file_name"]]), 0)
+            else:
+                src = ParticleSource(os.path.join(base, mg_file), pid - 1)
+            recs.append(rec)
+            srcs.append(src)
+    arr = np.array(recs, dtype=PARTICLE_DTYPE) if recs else np.zeros(0, dtype=PARTICLE_DTYPE)
+    return arr, srcs
+
+
+def load_particle_images(srcs: list[ParticleSource]):
+    """Images of the particles (n, y, x) float32 and the header origins of
+    their files (used for the background ellipse)."""
+    cache: dict = {}
+    out, hdr = [], []
+    for s in srcs:
+        m = cache.get(s.file)
+        if m is None:
+            m = read_mrc(s.file)
+            cache[s.file] = m
+        if s.index >= m.images:
+            raise ValueError(f"{s.file}: image {s.index + 1} requested, {m.images} present")
+        out.append(m.data[s.index, 0])
+        hdr.append(m.origin[:2])
+    return np.stack(out) if out else np.zeros((0, 1, 1), np.float32), np.asarray(hdr, float)
