@@ -374,23 +374,22 @@ def insertion_roofline(kt_sum, launches, nparticles, box, ft, N):
     """Roofline of the dominant insertion kernel: fill reads the half-spectrum
     columns ((ft/2+1)*box*8 B per particle) and writes a 16-B bin entry per
     merged sample run; tile-accumulate reads the entries and read-modify-writes
-    the touched 48-KB tiles; count reads particle records only."""
+    the touched 48-KB tiles."""
     entries = kt_sum.get("entries", 0.0)
     nbatch = max(1, launches)
     col_bytes = (ft // 2 + 1) * box * 8
     kernel_bytes = {
         "fill": col_bytes * nparticles + 16 * entries,
         "tile": 16 * entries + 2 * 48 * 1024 * (N // 16) ** 3 * nbatch,
-        "count": 200 * nparticles,
         "direct": 32 * ft * ft * nparticles,
     }
     cand = {k: kt_sum.get(k, 0.0) for k in kernel_bytes}
     dom = max(cand, key=cand.get)
     dom_ms = max(cand[dom], 1e-9)
     achieved = kernel_bytes[dom] / (dom_ms * 1e-3) / 1e9
-    ins_ms = sum(kt_sum.get(k, 0.0) for k in ("count", "scan", "fill", "tile", "direct"))
-    kname = {"fill": f"k_cols<{ft}, false, 2>", "count": f"k_cols<{ft}, false, 1>",
-             "tile": "k_tile_accum", "direct": f"k_cols<{ft}, false, 0>"}[dom]
+    ins_ms = sum(kt_sum.get(k, 0.0) for k in ("scan", "fill", "tile", "direct"))
+    kname = {"fill": f"k_cols<{ft}, false, 2>", "tile": "k_tile_accum",
+             "direct": f"k_cols<{ft}, false, 0>"}[dom]
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(kname, nparticles / nbatch) if ft == 512 else None,

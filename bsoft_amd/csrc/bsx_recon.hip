@@ -169,7 +169,7 @@ __device__ __forceinline__ long wrapi(long c, long n) {
     return c < 0 ? c + n : c;
 }
 
-enum { MODE_DIRECT = 0, MODE_COUNT = 1, MODE_FILL = 2 };
+enum { MODE_DIRECT = 0, MODE_FILL = 2 };
 
 // One emission: a contribution to the record `key`; `kk` is the sample row
 // (runs are merged only along unbroken k steps); `incl` marks samples the
@@ -180,22 +180,27 @@ struct Emit {
     float re, im, w, p, w2;
 };
 
-// Each bin's counter is replicated BSX_SLOTS times (slot = workgroup index
-// mod BSX_SLOTS) so that concurrent reservations on a hot bin spread over
-// several addresses; a bin's slots are adjacent, so its entries stay
-// contiguous after the scan. The count and fill passes run the same grid, so
-// a workgroup uses the same slot in both.
-#define BSX_SLOTS 8
+// Bins live in chunks of BSX_CHUNK entries taken from a pool while the fill
+// pass reserves them: a bin's reservation counter gives each entry a position,
+// position / BSX_CHUNK indexes the bin's chunk table, and the reservation that
+// holds a chunk's first position allocates it (one pool atomic, then a
+// compare-and-swap into the table) while the other writers of that chunk wait
+// for the table entry. No counting pass is needed: after the fill the counters are the bin
+// sizes, k_piece_scan turns them into work pieces, and k_bin_reset clears the
+// counters and the used table entries for the next batch. An entry whose
+// chunk cannot be had (table or pool exhausted) is added to the volume with
+// float atomics instead, so capacity only costs speed.
+#define BSX_CHUNK 4096
+#define BSX_SPIN_LIMIT 4096    // polls of a chunk-table entry before the atomic fallback
 
 // Fill-pass staging in LDS (nearest / weighted-nearest insertion without
 // Ewald): a workgroup appends its merged entries to an LDS list and counts
 // them per bin in a small LDS hash table; at each flush one integer atomic
-// per (workgroup, bin) reserves the bin range and the entries are written
-// out. This replaces one returning atomic per (wavefront, bin, emission) and
-// its latency wait. The per-bin totals equal the count pass's, so the ranges
-// tile the bins exactly.
+// per (workgroup, bin) reserves the bin range (at most two chunks) and the
+// entries are written out.
 #define BSX_STAGE_CAP 1024     // entries per flush: 2 rounds x 2 emissions x 256 threads
 #define BSX_STAGE_HASH 128     // bins per flush (more fall back to direct reservations)
+static_assert(BSX_STAGE_CAP <= BSX_CHUNK, "a flushed bin range must span at most two chunks");
 
 struct Stage {
     float4* e;       // (re, im, w, key bits)
@@ -203,17 +208,93 @@ struct Stage {
     int* hkey;       // bin per hash slot, -1 = free
     int* hcnt;       // entries per hash slot
     int* hbase;      // reserved base per hash slot
+    int* hc0;        // pool chunk of the range start
+    int* hc1;        // pool chunk of the range end
     int* count;      // entries in the list
     int on;
 };
 
 struct BinCtx {
-    int* ctr;        // per-(bin, slot) counters: counts (count pass) / cursors (fill pass)
-    float4* bins;    // (re, im, w, record-in-tile)
-    int cap;         // entries allocated (the count pass bounds the fill pass)
-    int slot;        // this workgroup's counter slot
+    int* fill;       // per-bin reservation counters (bin sizes after the fill)
+    int* chunks;     // [nbins][maxc] pool chunk of each bin chunk, -1 = not yet, -2 = none
+    int* pool;       // [0] next free pool chunk, [1] fallback flags (1 table, 2 pool, 4 wait)
+    float4* bins;    // the pool: (re, im, w, record-in-tile) entries
+    float* acc;      // volume records (3 floats), target of the atomic fallback
+    int npool;       // chunks in the pool
+    int maxc;        // chunks per bin
     Stage st;        // fill-pass staging (st.on = 0: reserve per wavefront)
 };
+
+// Takes a pool chunk (-2 when the pool is exhausted) and installs it as chunk
+// ci of bin b unless one is there already; returns the installed value.
+__device__ __forceinline__ int chunk_take(const BinCtx& bc, int b, int ci) {
+    int nc = atomicAdd(bc.pool, 1);
+    if (nc >= bc.npool) nc = -2;
+    const int prev = atomicCAS(bc.chunks + (size_t)b * bc.maxc + ci, -1, nc);
+    return prev == -1 ? nc : prev;
+}
+
+// Called by the reservation holding position ci * BSX_CHUNK of bin b.
+__device__ __forceinline__ void chunk_alloc(const BinCtx& bc, int b, int ci) {
+    if (ci < bc.maxc) (void)chunk_take(bc, b, ci);
+}
+
+// Pool chunk of chunk ci of bin b once its allocator has installed it. A wait
+// past its bound installs a chunk itself (the allocator's chunk then goes
+// unused), so every writer of a bin chunk agrees on it. -1 selects the atomic
+// fallback (beyond the table or pool exhausted; then all writers of the chunk
+// fall back and the tile pass skips it).
+__device__ __forceinline__ int chunk_wait(const BinCtx& bc, int b, int ci) {
+    if (ci >= bc.maxc) {
+        atomicOr(bc.pool + 1, 1);
+        return -1;
+    }
+    const int* slot = bc.chunks + (size_t)b * bc.maxc + ci;
+    int c = -1;
+    for (int i = 0; i < BSX_SPIN_LIMIT && c == -1; ++i) {
+        c = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (c == -1) __builtin_amdgcn_s_sleep(2);
+    }
+    if (c == -1) {
+        atomicOr(bc.pool + 1, 4);
+        c = chunk_take(bc, b, ci);
+    }
+    if (c == -2) {
+        atomicOr(bc.pool + 1, 2);
+        return -1;
+    }
+    return c;
+}
+
+__device__ __forceinline__ void put_entry(const BinCtx& bc, int c, int pos, float re, float im,
+                                          float w, int key) {
+    if (c >= 0) {
+        bc.bins[(size_t)c * BSX_CHUNK + (pos & (BSX_CHUNK - 1))] =
+            make_float4(re, im, w, __int_as_float(key & 4095));
+    } else {
+        float* rec = bc.acc + (size_t)(unsigned)key * 3;
+        unsafeAtomicAdd(rec + 0, re);
+        unsafeAtomicAdd(rec + 1, im);
+        unsafeAtomicAdd(rec + 2, w);
+    }
+}
+
+// Resolves the (at most two) chunks of a reserved range [base, base + cnt):
+// allocations first, then waits, so that no allocator is held up behind a
+// wait of its own wavefront.
+__device__ __forceinline__ void range_chunks(const BinCtx& bc, int b, int base, int cnt,
+                                             bool any, int& c0, int& c1) {
+    const int i0 = base / BSX_CHUNK, i1 = (base + cnt - 1) / BSX_CHUNK;
+    if (any) {
+        if ((base & (BSX_CHUNK - 1)) == 0) chunk_alloc(bc, b, i0);
+        if (i1 != i0) chunk_alloc(bc, b, i1);
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (any) {
+        c0 = chunk_wait(bc, b, i0);
+        c1 = i1 != i0 ? chunk_wait(bc, b, i1) : c0;
+    }
+}
 
 __device__ __forceinline__ void stage_entry(const BinCtx& bc, int key, float re, float im,
                                             float w) {
@@ -228,11 +309,15 @@ __device__ __forceinline__ void stage_entry(const BinCtx& bc, int key, float re,
             hs = (hs + 1) & (BSX_STAGE_HASH - 1);
         }
     }
-    if (slot < 0) {
-        // list or table full: reserve this entry on its own
+    const bool alone = slot < 0;   // list or table full: reserve this entry on its own
+    int pos = 0, c0 = -1, c1 = -1;
+    if (alone) {
         if (i < BSX_STAGE_CAP) bc.st.r[i] = -1;
-        const int s = atomicAdd(bc.ctr + bin * BSX_SLOTS + bc.slot, 1);
-        if (s < bc.cap) bc.bins[s] = make_float4(re, im, w, __int_as_float((int)((unsigned)key & 4095u)));
+        pos = atomicAdd(bc.fill + bin, 1);
+    }
+    range_chunks(bc, bin, pos, 1, alone, c0, c1);
+    if (alone) {
+        put_entry(bc, c0, pos, re, im, w, key);
         return;
     }
     const int rank = atomicAdd(&bc.st.hcnt[slot], 1);
@@ -240,24 +325,36 @@ __device__ __forceinline__ void stage_entry(const BinCtx& bc, int key, float re,
     bc.st.r[i] = (slot << 20) | rank;
 }
 
-// Workgroup-uniform: reserve every staged bin, write the staged entries,
-// clear the list and the table.
+// Workgroup-uniform: reserve every staged bin range, write the staged
+// entries, clear the list and the table.
 __device__ __forceinline__ void stage_flush(const BinCtx& bc) {
     __syncthreads();
     const int tid = threadIdx.x;
+    int b = -1, cnt = 0, base = 0, c0 = -1, c1 = -1;
     if (tid < BSX_STAGE_HASH) {
-        const int b = bc.st.hkey[tid];
-        if (b >= 0) bc.st.hbase[tid] = atomicAdd(bc.ctr + b * BSX_SLOTS + bc.slot, bc.st.hcnt[tid]);
+        b = bc.st.hkey[tid];
+        if (b >= 0) {
+            cnt = bc.st.hcnt[tid];
+            base = atomicAdd(bc.fill + b, cnt);
+        }
+    }
+    range_chunks(bc, b, base, cnt, b >= 0, c0, c1);
+    if (b >= 0) {
+        bc.st.hbase[tid] = base;
+        bc.st.hc0[tid] = c0;
+        bc.st.hc1[tid] = c1;
     }
     __syncthreads();
     const int n = min(*bc.st.count, BSX_STAGE_CAP);
     for (int i = tid; i < n; i += blockDim.x) {
         const int pr = bc.st.r[i];
         if (pr < 0) continue;
+        const int hs = pr >> 20;
         const float4 v = bc.st.e[i];
-        const int sl = bc.st.hbase[pr >> 20] + (pr & 0xFFFFF);
-        if (sl < bc.cap)
-            bc.bins[sl] = make_float4(v.x, v.y, v.z, __int_as_float(__float_as_int(v.w) & 4095));
+        const int hb = bc.st.hbase[hs];
+        const int pos = hb + (pr & 0xFFFFF);
+        const int c = pos / BSX_CHUNK == hb / BSX_CHUNK ? bc.st.hc0[hs] : bc.st.hc1[hs];
+        put_entry(bc, c, pos, v.x, v.y, v.z, __float_as_int(v.w));
     }
     __syncthreads();
     if (tid < BSX_STAGE_HASH) {
@@ -287,24 +384,20 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
             const int nk = __shfl_down(e.key, d, 64);
             const int nkk = __shfl_down(e.kk, d, 64);
             const int ninc = __shfl_down(e.incl, d, 64);
-            float r0 = 0.f, r1 = 0.f, r2 = 0.f, r3 = 0.f, r4 = 0.f;
-            if (MODE != MODE_COUNT) {
-                r0 = __shfl_down(e.re, d, 64);
-                r1 = __shfl_down(e.im, d, 64);
-                r2 = __shfl_down(e.w, d, 64);
-                if (FOM) {
-                    r3 = __shfl_down(e.p, d, 64);
-                    r4 = __shfl_down(e.w2, d, 64);
-                }
+            float r3 = 0.f, r4 = 0.f;
+            const float r0 = __shfl_down(e.re, d, 64);
+            const float r1 = __shfl_down(e.im, d, 64);
+            const float r2 = __shfl_down(e.w, d, 64);
+            if (FOM) {
+                r3 = __shfl_down(e.p, d, 64);
+                r4 = __shfl_down(e.w2, d, 64);
             }
             if (lg + d < MG && nk == e.key && abs(nkk - e.kk) == d) {
                 e.incl |= ninc;
-                if (MODE != MODE_COUNT) {
-                    e.re += r0;
-                    e.im += r1;
-                    e.w += r2;
-                    if (FOM) { e.p += r3; e.w2 += r4; }
-                }
+                e.re += r0;
+                e.im += r1;
+                e.w += r2;
+                if (FOM) { e.p += r3; e.w2 += r4; }
             }
         }
         const int pk = __shfl_up(e.key, 1, 64);
@@ -324,43 +417,39 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
         }
         return;
     }
-    const int bin = (int)((unsigned)e.key >> 12);
-    const int lane = __lane_id();
-    unsigned long long active = __ballot(out);
-    if (MODE == MODE_COUNT) {
-        // fire-and-forget: one counter increment per (wavefront, bin)
-        while (active) {
-            const int leader = __builtin_ctzll(active);
-            const int b0 = __shfl(bin, leader, 64);
-            const unsigned long long m = __ballot(out && bin == b0);
-            if (lane == leader) atomicAdd(bc.ctr + b0 * BSX_SLOTS + bc.slot, __popcll(m));
-            active &= ~m;
-        }
-        return;
-    }
-    if (MODE == MODE_FILL && bc.st.on) {
+    if (bc.st.on) {
         if (out) stage_entry(bc, e.key, e.re, e.im, e.w);
         return;
     }
-    // fill: issue every reservation of this wavefront before waiting for any
-    int mybase = 0, myleader = lane, myrank = 0;
+    // per wavefront: one reservation per bin, all issued before any chunk
+    // lookup waits on them
+    const int bin = (int)((unsigned)e.key >> 12);
+    const int lane = __lane_id();
+    unsigned long long active = __ballot(out);
+    int mybase = 0, myleader = lane, myrank = 0, mycnt = 0;
     while (active) {
         const int leader = __builtin_ctzll(active);
         const int b0 = __shfl(bin, leader, 64);
         const bool mine = out && bin == b0;
         const unsigned long long m = __ballot(mine);
-        if (lane == leader) mybase = atomicAdd(bc.ctr + b0 * BSX_SLOTS + bc.slot, __popcll(m));
+        if (lane == leader) {
+            mycnt = __popcll(m);
+            mybase = atomicAdd(bc.fill + b0, mycnt);
+        }
         if (mine) {
             myleader = leader;
             myrank = __popcll(m & ((1ull << lane) - 1ull));
         }
         active &= ~m;
     }
+    int myc0 = -1, myc1 = -1;   // this lane leads its bin's group: resolve its chunks
+    range_chunks(bc, bin, mybase, mycnt, mycnt > 0, myc0, myc1);
     const int base = __shfl(mybase, myleader, 64);
+    const int c0 = __shfl(myc0, myleader, 64);
+    const int c1 = __shfl(myc1, myleader, 64);
     if (out) {
-        const int slot = base + myrank;
-        if (slot < bc.cap)
-            bc.bins[slot] = make_float4(e.re, e.im, e.w, __int_as_float((int)((unsigned)e.key & 4095u)));
+        const int pos = base + myrank;
+        put_entry(bc, pos / BSX_CHUNK == base / BSX_CHUNK ? c0 : c1, pos, e.re, e.im, e.w, e.key);
     }
 }
 
@@ -411,7 +500,6 @@ __device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev&
     const double d = (K.maxs2 - *s2) * K.mscale;
     if (d < 1) ww *= sqrt(d);
     *w = ww;
-    if (MODE == MODE_COUNT) return true;
     float sn, cs;
     sincos_red((double)h * P.phx + (double)k * P.phy, &sn, &cs);
     float vr = F.x * cs - F.y * sn, vi = F.x * sn + F.y * cs;
@@ -530,8 +618,6 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
 }
 
 // Column tasks: task = (particle, x) with x in [0, FT/2]; G tasks per WG.
-// MODE_COUNT runs the geometry only (no transform) with the same lane layout
-// as MODE_FILL, so both passes make identical emission decisions.
 template <int FT, bool FOM, int MODE>
 __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
                                               const float* __restrict__ bkg,
@@ -543,13 +629,13 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     constexpr int NT = G * T;
     constexpr int HX = FT / 2 + 1;
     constexpr int MG = T < 16 ? T : 16;   // run-merge group: lanes of one column
-    constexpr int LDSN = MODE == MODE_COUNT ? 1 : G * FT + FT;
+    constexpr int LDSN = G * FT + FT;
     constexpr int SCAP = MODE == MODE_FILL && !FOM ? BSX_STAGE_CAP : 1;
     constexpr int SHASH = MODE == MODE_FILL && !FOM ? BSX_STAGE_HASH : 1;
     __shared__ float2 lds[LDSN];
     __shared__ float4 st_e[SCAP];
     __shared__ int st_r[SCAP];
-    __shared__ int st_hkey[SHASH], st_hcnt[SHASH], st_hbase[SHASH];
+    __shared__ int st_hkey[SHASH], st_hcnt[SHASH], st_hbase[SHASH], st_hc0[SHASH], st_hc1[SHASH];
     __shared__ int st_count;
     const int tid = threadIdx.x;
     const int g = tid / T, t = tid - g * T;
@@ -557,8 +643,8 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     const int p = (int)(task / HX);
     const int x = (int)(task - (long)p * HX);
     const bool valid = p < nb;
-    float2* line = lds + (MODE == MODE_COUNT ? 0 : g * FT);
-    if (MODE != MODE_COUNT) {
+    float2* line = lds + g * FT;
+    {
         float2* tw = lds + G * FT;
         for (int i = tid; i < FT; i += NT) tw[i] = twg[i];
         const float2* col = inter + ((size_t)p * HX + x) * box;
@@ -566,11 +652,11 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         __syncthreads();
         FftLine<FT, -1, T>::run(line, tw, t);
     }
-    bc.slot = (int)(blockIdx.x % BSX_SLOTS);
     // staging: 2 rounds of at most 2 emissions per thread fit the list
     const bool staged = MODE == MODE_FILL && !FOM && K.interp < 2 && !K.ewald && NT == 256 &&
                         2 * 2 * NT <= BSX_STAGE_CAP;
-    bc.st = Stage{st_e, st_r, st_hkey, st_hcnt, st_hbase, &st_count, staged ? 1 : 0};
+    bc.st = Stage{st_e, st_r, st_hkey, st_hcnt, st_hbase, st_hc0, st_hc1, &st_count,
+                  staged ? 1 : 0};
     if (staged) {
         if (tid < BSX_STAGE_HASH) {
             st_hkey[tid] = -1;
@@ -589,13 +675,10 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
 #pragma unroll 1
     for (int r = 0; r < 8; ++r) {
         const int y = t + r * T;
-        float2 F = make_float2(0.f, 0.f);
-        if (MODE != MODE_COUNT) {
-            F = line[y];
-            F.x *= inv;
-            F.y *= inv;
-            if (x == 0 && y == 0) F.x += b * FT;
-        }
+        float2 F = line[y];
+        F.x *= inv;
+        F.y *= inv;
+        if (x == 0 && y == 0) F.x += b * FT;
         const long k = (y <= (FT - 1) / 2) ? y : y - FT;
         // Friedel mate of the column: array position (FT-x, FT-y), value conj
         const int ym = (FT - y) & (FT - 1);
@@ -612,66 +695,72 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
 #define BSX_PIECE 16384
 #endif
 
-// Exclusive scans (one workgroup) of the per-(bin, slot) counts: entry
-// offsets offsets[0..nbins*SLOTS] (cursors = offsets), and per-bin piece
-// offsets poff[0..nbins]; counts cleared for the next batch.
-__global__ __launch_bounds__(1024) void k_scan(int* __restrict__ counts, int nbins,
-                                               int* __restrict__ offsets,
-                                               int* __restrict__ cursors,
-                                               int* __restrict__ poff,
-                                               unsigned long long* __restrict__ total) {
-    __shared__ int part[1024], ppart[1024];
+// Per-bin piece offsets (one workgroup): poff[b] = sum over bins below b of
+// ceil(binned / PIECE), poff[nbins] = all pieces; `total` accumulates the
+// entries binned since the last reset. Positions past the chunk table
+// (maxc chunks) went to the atomic fallback and are not binned.
+__global__ __launch_bounds__(1024) void k_piece_scan(const int* __restrict__ fill, int nbins,
+                                                     int maxc, int* __restrict__ poff,
+                                                     unsigned long long* __restrict__ total) {
+    __shared__ int ppart[1024];
+    __shared__ unsigned long long epart[1024];
     const int tid = threadIdx.x;
     const int per = (nbins + 1023) / 1024;   // bins per thread
     const int b0 = tid * per, b1 = min(nbins, b0 + per);
-    int sum = 0, psum = 0;
+    const int cap = maxc * BSX_CHUNK;
+    int psum = 0;
+    unsigned long long esum = 0;
     for (int i = b0; i < b1; ++i) {
-        int c = 0;
-        for (int q = 0; q < BSX_SLOTS; ++q) c += counts[i * BSX_SLOTS + q];
-        sum += c;
-        psum += (c + BSX_PIECE - 1) / BSX_PIECE;
+        const int f = min(fill[i], cap);
+        psum += (f + BSX_PIECE - 1) / BSX_PIECE;
+        esum += (unsigned)f;
     }
-    part[tid] = sum;
     ppart[tid] = psum;
+    epart[tid] = esum;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
-        const int v = tid >= o ? part[tid - o] : 0;
         const int pv = tid >= o ? ppart[tid - o] : 0;
         __syncthreads();
-        part[tid] += v;
         ppart[tid] += pv;
         __syncthreads();
     }
-    int run = part[tid] - sum, prun = ppart[tid] - psum;
+    int prun = ppart[tid] - psum;
     for (int i = b0; i < b1; ++i) {
-        int c = 0;
-        for (int q = 0; q < BSX_SLOTS; ++q) {
-            const int j = i * BSX_SLOTS + q;
-            const int cq = counts[j];
-            offsets[j] = run + c;
-            cursors[j] = run + c;
-            c += cq;
-            counts[j] = 0;
-        }
         poff[i] = prun;
-        run += c;
-        prun += (c + BSX_PIECE - 1) / BSX_PIECE;
+        prun += (min(fill[i], cap) + BSX_PIECE - 1) / BSX_PIECE;
     }
-    if (tid == 1023) {
-        offsets[nbins * BSX_SLOTS] = part[1023];
-        poff[nbins] = ppart[1023];
-        *total += (unsigned long long)part[1023];   // entries binned since the last reset
+    for (int o = 512; o > 0; o >>= 1) {
+        if (tid < o) epart[tid] += epart[tid + o];
+        __syncthreads();
     }
+    if (tid == 1023) poff[nbins] = ppart[1023];
+    if (tid == 0) *total += epart[0];
+}
+
+// Clears the bin counters and the used chunk-table entries for the next
+// batch (kernels, not memsets, so that the batch can be captured in a graph).
+__global__ void k_bin_reset(int* __restrict__ fill, int* __restrict__ chunks, int maxc, int nbins,
+                            int* __restrict__ pool) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) pool[0] = 0;
+    if (b >= nbins) return;
+    const int n = min(maxc, (fill[b] + BSX_CHUNK - 1) / BSX_CHUNK);
+    int* c = chunks + (size_t)b * maxc;
+    for (int i = 0; i < n; ++i) c[i] = -1;
+    fill[b] = 0;
 }
 
 // One workgroup per piece of a (half, tile) bin: sum its entries in LDS, then
 // add the tile partial to the volume — a plain read-modify-write when the bin
 // is a single piece, float atomics per touched record for multi-piece bins.
 // Each thread keeps 8 entry loads in flight before its LDS adds.
-__global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ offsets,
+__global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill,
+                                                    const int* __restrict__ chunks, int maxc,
                                                     const int* __restrict__ poff, int nbins,
-                                                    const float4* __restrict__ bins, int cap,
+                                                    const float4* __restrict__ bins,
                                                     float* __restrict__ acc) {
+    constexpr int PC = BSX_PIECE / BSX_CHUNK;   // chunks per piece
+    static_assert(BSX_PIECE % BSX_CHUNK == 0, "pieces are whole chunks");
     const int piece = blockIdx.x;
     if (piece >= poff[nbins]) return;
     int lo = 0, hi = nbins - 1;   // last bin with poff[bin] <= piece
@@ -683,10 +772,15 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ offs
     const int bin = lo;
     const int j = piece - poff[bin];
     const bool single = poff[bin + 1] - poff[bin] == 1;
-    const int beg = offsets[bin * BSX_SLOTS] + j * BSX_PIECE;
-    const int end = min(min(offsets[(bin + 1) * BSX_SLOTS], beg + BSX_PIECE), cap);
+    const int beg = j * BSX_PIECE;
+    const int end = min(min(fill[bin], maxc * BSX_CHUNK), beg + BSX_PIECE);
     if (beg >= end) return;
     __shared__ float sre[4096], sim[4096], sw[4096];
+    __shared__ int pc[PC];
+    if (threadIdx.x < PC) {
+        const int ci = beg / BSX_CHUNK + threadIdx.x;
+        pc[threadIdx.x] = (ci < maxc && ci * BSX_CHUNK < end) ? chunks[(size_t)bin * maxc + ci] : -1;
+    }
     float* g = acc + (size_t)bin * 4096 * 3;
     for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
         sre[i] = 0.f;
@@ -701,12 +795,14 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ offs
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int e = e0 + u * stride;
-            v[u] = e < end ? bins[e] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+            const int c = e < end ? pc[(e - beg) / BSX_CHUNK] : -1;
+            v[u] = c >= 0 ? bins[(size_t)c * BSX_CHUNK + (e & (BSX_CHUNK - 1))]
+                          : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int li = __float_as_int(v[u].w);
-            if (li >= 0) {
+            if (li >= 0 && li < 4096) {
                 atomicAdd(&sre[li], v[u].x);
                 atomicAdd(&sim[li], v[u].y);
                 atomicAdd(&sw[li], v[u].z);
@@ -889,9 +985,11 @@ struct bsx_recon {
     PartDev* recs = nullptr;
     int* nonfinite = nullptr;
     int64_t recs_cap = 0;
-    int* counts = nullptr;         // per (half, tile) bin
-    int* offsets = nullptr;
-    int* cursors = nullptr;
+    int* fill = nullptr;           // per (half, tile) bin: entries reserved
+    int* chunks = nullptr;         // per bin: pool chunk of each bin chunk
+    int* pool = nullptr;           // next free chunk, fallback flags
+    int npool = 0;
+    int maxc = 0;
     int* poff = nullptr;           // piece offsets per bin
     unsigned long long* entries = nullptr;   // bin entries of the current insert
     float4* bins = nullptr;
@@ -899,7 +997,7 @@ struct bsx_recon {
     int nbins = 0;
     int batch = 0;
     int64_t inserted[2] = {0, 0};
-    double ktime[7] = {0, 0, 0, 0, 0, 0, 0};   // background rows count scan fill tile direct
+    double ktime[7] = {0, 0, 0, 0, 0, 0, 0};   // background rows fallback scan fill tile direct
     double last_entries = 0;
     int64_t last_launches = 0;
     std::vector<hipEvent_t> evs;
@@ -923,7 +1021,8 @@ struct bsx_recon {
                                         // the legacy default stream, which cannot capture)
 };
 
-enum { KT_BG = 0, KT_ROWS, KT_COUNT, KT_SCAN, KT_FILL, KT_TILE, KT_DIRECT, KT_END };
+// KT_FALLBACK holds the atomic-fallback flags of the insert, not a time
+enum { KT_BG = 0, KT_ROWS, KT_FALLBACK, KT_SCAN, KT_FILL, KT_TILE, KT_DIRECT, KT_END };
 
 // Timing marks: an event after each kernel, labelled with the kernel it closes.
 static int tmark(bsx_recon* h, hipStream_t s, int label) {
@@ -992,27 +1091,25 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
     const PartDev* rec = h->recs + p0;
     const float* bk = h->bkg + p0;
     if (h->binned) {
-        BinCtx bc{h->counts, nullptr, (int)h->bins_cap};
-        hipLaunchKernelGGL((k_cols<FT, false, MODE_COUNT>), grid, dim3(NT), 0, s, h->inter, bk,
-                           rec, nb, box, h->K, h->acc, tw, bc);
-        BSX_HIP(hipGetLastError());
-        if ((r = tmark(h, s, KT_COUNT))) return r;
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, h->counts, h->nbins, h->offsets,
-                           h->cursors, h->poff, h->entries);
-        BSX_HIP(hipGetLastError());
-        if ((r = tmark(h, s, KT_SCAN))) return r;
-        BinCtx bf{h->cursors, h->bins, (int)h->bins_cap};
+        BinCtx bf{h->fill, h->chunks, h->pool, h->bins, h->acc, h->npool, h->maxc, Stage{}};
         hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL>), grid, dim3(NT), 0, s, h->inter, bk,
                            rec, nb, box, h->K, h->acc, tw, bf);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_FILL))) return r;
-        const unsigned maxpieces = (unsigned)(h->nbins + h->bins_cap / BSX_PIECE + 1);
-        hipLaunchKernelGGL(k_tile_accum, dim3(maxpieces), dim3(512), 0, s, h->offsets, h->poff,
-                           h->nbins, h->bins, (int)h->bins_cap, h->acc);
+        hipLaunchKernelGGL(k_piece_scan, dim3(1), dim3(1024), 0, s, h->fill, h->nbins, h->maxc,
+                           h->poff, h->entries);
+        BSX_HIP(hipGetLastError());
+        if ((r = tmark(h, s, KT_SCAN))) return r;
+        const unsigned maxpieces = (unsigned)(h->nbins + (size_t)h->npool * BSX_CHUNK / BSX_PIECE + 1);
+        hipLaunchKernelGGL(k_tile_accum, dim3(maxpieces), dim3(512), 0, s, h->fill, h->chunks,
+                           h->maxc, h->poff, h->nbins, h->bins, h->acc);
+        BSX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_bin_reset, dim3((h->nbins + 255) / 256), dim3(256), 0, s, h->fill,
+                           h->chunks, h->maxc, h->nbins, h->pool);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_TILE))) return r;
     } else {
-        BinCtx bc{nullptr, nullptr, 0};
+        BinCtx bc{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, Stage{}};
         if (h->K.fom)
             hipLaunchKernelGGL((k_cols<FT, true, MODE_DIRECT>), grid, dim3(NT), 0, s, h->inter,
                                bk, rec, nb, box, h->K, h->acc, tw, bc);
@@ -1140,9 +1237,18 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     h->bins_cap = (size_t)batch * ft2 * mult;
     BSX_REQUIRE(!h->binned || h->bins_cap < (1ull << 31), BSX_ERR_ARG,
                 "batch %d too large for 32-bit bin offsets", batch);
-    const size_t bins_bytes = h->binned ? h->bins_cap * sizeof(float4) : 0;
+    // pool: every entry of a batch plus a partly filled chunk per bin; chunk
+    // table: 256 chunks (1 M entries) per bin and batch share of 2048 entries
+    // per particle and bin, whichever is more (beyond it entries take the
+    // atomic fallback)
+    const size_t cap_chunks = (h->bins_cap + BSX_CHUNK - 1) / BSX_CHUNK;
+    h->npool = (int)(cap_chunks + std::min<size_t>(h->nbins, cap_chunks) + 1);
+    h->maxc = (int)std::min<size_t>(cap_chunks + 1,
+                                    std::max<size_t>(256, ((size_t)batch * 2048 * mult +
+                                                           BSX_CHUNK - 1) / BSX_CHUNK + 1));
+    const size_t bins_bytes = h->binned ? (size_t)h->npool * BSX_CHUNK * sizeof(float4) : 0;
     const size_t bin_idx_bytes =
-        h->binned ? ((3 * BSX_SLOTS + 1) * (size_t)h->nbins + 2) * sizeof(int) : 0;
+        h->binned ? ((size_t)h->nbins * (h->maxc + 2) + 3) * sizeof(int) + 8 : 0;
     h->bytes = h->acc_elems * sizeof(float) + inter_bytes + bins_bytes + bin_idx_bytes;
     if (p->mem_cap > 0 && (int64_t)h->bytes > p->mem_cap) {
         set_error("device memory %zu bytes exceeds the cap %lld", h->bytes, (long long)p->mem_cap);
@@ -1154,9 +1260,9 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
               hipMalloc(&h->nonfinite, sizeof(int)) == hipSuccess;
     if (ok && h->binned)
         ok = hipMalloc(&h->bins, bins_bytes) == hipSuccess &&
-             hipMalloc(&h->counts, sizeof(int) * h->nbins * BSX_SLOTS) == hipSuccess &&
-             hipMalloc(&h->offsets, sizeof(int) * (h->nbins * BSX_SLOTS + 1)) == hipSuccess &&
-             hipMalloc(&h->cursors, sizeof(int) * h->nbins * BSX_SLOTS) == hipSuccess &&
+             hipMalloc(&h->fill, sizeof(int) * h->nbins) == hipSuccess &&
+             hipMalloc(&h->chunks, sizeof(int) * (size_t)h->nbins * h->maxc) == hipSuccess &&
+             hipMalloc(&h->pool, sizeof(int) * 2) == hipSuccess &&
              hipMalloc(&h->poff, sizeof(int) * (h->nbins + 1)) == hipSuccess &&
              hipMalloc(&h->entries, sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
@@ -1164,7 +1270,11 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
         bsx_recon_destroy(h);
         return BSX_ERR_NOMEM;
     }
-    if (h->binned) BSX_HIP(hipMemset(h->counts, 0, sizeof(int) * h->nbins * BSX_SLOTS));
+    if (h->binned) {
+        BSX_HIP(hipMemset(h->fill, 0, sizeof(int) * h->nbins));
+        BSX_HIP(hipMemset(h->chunks, 0xFF, sizeof(int) * (size_t)h->nbins * h->maxc));
+        BSX_HIP(hipMemset(h->pool, 0, sizeof(int) * 2));
+    }
     BSX_HIP(hipMemset(h->acc, 0, h->acc_elems * sizeof(float)));
     BSX_HIP(hipDeviceSynchronize());
     *out = h;
@@ -1179,9 +1289,9 @@ int bsx_recon_destroy(bsx_recon* h) {
     if (h->recs) (void)hipFree(h->recs);
     if (h->nonfinite) (void)hipFree(h->nonfinite);
     if (h->bins) (void)hipFree(h->bins);
-    if (h->counts) (void)hipFree(h->counts);
-    if (h->offsets) (void)hipFree(h->offsets);
-    if (h->cursors) (void)hipFree(h->cursors);
+    if (h->fill) (void)hipFree(h->fill);
+    if (h->chunks) (void)hipFree(h->chunks);
+    if (h->pool) (void)hipFree(h->pool);
     if (h->poff) (void)hipFree(h->poff);
     if (h->entries) (void)hipFree(h->entries);
     for (auto e : h->evs) (void)hipEventDestroy(e);
@@ -1199,6 +1309,14 @@ int bsx_recon_destroy(bsx_recon* h) {
 int bsx_recon_reset(bsx_recon* h, void* stream) {
     BSX_REQUIRE(h, BSX_ERR_ARG, "null handle");
     BSX_HIP(hipMemsetAsync(h->acc, 0, h->acc_elems * sizeof(float), (hipStream_t)stream));
+    if (h->binned) {
+        // bin state is cleared after every batch; cleared again here in case an
+        // earlier insert stopped between its fill and its reset
+        BSX_HIP(hipMemsetAsync(h->fill, 0, sizeof(int) * h->nbins, (hipStream_t)stream));
+        BSX_HIP(hipMemsetAsync(h->chunks, 0xFF, sizeof(int) * (size_t)h->nbins * h->maxc,
+                               (hipStream_t)stream));
+        BSX_HIP(hipMemsetAsync(h->pool, 0, sizeof(int) * 2, (hipStream_t)stream));
+    }
     h->inserted[0] = h->inserted[1] = 0;
     return BSX_OK;
 }
@@ -1376,8 +1494,13 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
             h->ktime[h->evlab[i]] += ms;
         }
         unsigned long long ne = 0;
-        if (h->binned)
+        if (h->binned) {
             BSX_HIP(hipMemcpy(&ne, h->entries, sizeof(ne), hipMemcpyDeviceToHost));
+            int flags = 0;
+            BSX_HIP(hipMemcpy(&flags, h->pool + 1, sizeof(int), hipMemcpyDeviceToHost));
+            BSX_HIP(hipMemset(h->pool + 1, 0, sizeof(int)));
+            h->ktime[KT_FALLBACK] = flags;
+        }
         h->last_entries = (double)ne;
     }
     return BSX_OK;
@@ -1440,7 +1563,7 @@ int bsx_recon_last_timing(const bsx_recon* h, double* insert_ms, int64_t* launch
                           double* stage_ms) {
     BSX_REQUIRE(h, BSX_ERR_ARG, "null handle");
     const double* t = h->ktime;
-    if (insert_ms) *insert_ms = t[KT_COUNT] + t[KT_SCAN] + t[KT_FILL] + t[KT_TILE] + t[KT_DIRECT];
+    if (insert_ms) *insert_ms = t[KT_SCAN] + t[KT_FILL] + t[KT_TILE] + t[KT_DIRECT];
     if (launches) *launches = h->last_launches;
     if (stage_ms) *stage_ms = t[KT_BG] + t[KT_ROWS];
     return BSX_OK;

@@ -134,7 +134,7 @@ class Reconstruction:
         call("bsx_recon_last_timing", self.h, C.byref(ms), C.byref(nl), C.byref(st))
         return ms.value, nl.value, st.value
 
-    KERNELS = ("background", "rows", "count", "scan", "fill", "tile", "direct", "entries")
+    KERNELS = ("background", "rows", "fallback", "scan", "fill", "tile", "direct", "entries")
 
     def kernel_times(self) -> dict:
         """Per-kernel milliseconds of the previous insert (BSX_FLAG_TIMING) and the
