@@ -50,7 +50,9 @@ struct alignas(16) PartDev {
 
 struct ReconConst {
     int N, ft, box, stride, interp, ctf_mode, fom, nhalves;   // ctf_mode 0 none 1 flip 2 apply 3 wiener
-    int merge, ewald, NT, pad_;                               // NT: 16^3 tiles per edge
+    int merge, ewald, NT, pairs;                              // NT: 16^3 tiles per edge
+    // pairs: binned entries stand for a sample and its Friedel mate (value
+    // at v, conjugate at -v), folded into the volume by k_fold
     double mins2, maxs2, mscale, shi2, wiener;
 };
 
@@ -177,6 +179,7 @@ enum { MODE_DIRECT = 0, MODE_FILL = 2 };
 // passes agree exactly).
 struct Emit {
     int key, kk, incl;
+    int bin;         // MODE_FILL: 1 = binned entry, 0 = added to the volume directly
     float re, im, w, p, w2;
 };
 
@@ -384,6 +387,7 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
             const int nk = __shfl_down(e.key, d, 64);
             const int nkk = __shfl_down(e.kk, d, 64);
             const int ninc = __shfl_down(e.incl, d, 64);
+            const int nbin = __shfl_down(e.bin, d, 64);
             float r3 = 0.f, r4 = 0.f;
             const float r0 = __shfl_down(e.re, d, 64);
             const float r1 = __shfl_down(e.im, d, 64);
@@ -392,7 +396,7 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
                 r3 = __shfl_down(e.p, d, 64);
                 r4 = __shfl_down(e.w2, d, 64);
             }
-            if (lg + d < MG && nk == e.key && abs(nkk - e.kk) == d) {
+            if (lg + d < MG && nk == e.key && abs(nkk - e.kk) == d && nbin == e.bin) {
                 e.incl |= ninc;
                 e.re += r0;
                 e.im += r1;
@@ -402,11 +406,12 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
         }
         const int pk = __shfl_up(e.key, 1, 64);
         const int pkk = __shfl_up(e.kk, 1, 64);
-        head = !(lg != 0 && pk == e.key && abs(e.kk - pkk) == 1);
+        const int pb = __shfl_up(e.bin, 1, 64);
+        head = !(lg != 0 && pk == e.key && abs(e.kk - pkk) == 1 && pb == e.bin);
     }
-    const bool out = head && e.incl;
-    if (MODE == MODE_DIRECT) {
-        if (!out) return;
+    const bool out0 = head && e.incl;
+    if (MODE == MODE_DIRECT || !e.bin) {
+        if (!out0) return;
         float* rec = acc + (size_t)(unsigned)e.key * stride;
         unsafeAtomicAdd(rec + 0, e.re);
         unsafeAtomicAdd(rec + 1, e.im);
@@ -417,6 +422,7 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
         }
         return;
     }
+    const bool out = out0;   // lanes still here hold binned entries
     if (bc.st.on) {
         if (out) stage_entry(bc, e.key, e.re, e.im, e.w);
         return;
@@ -455,11 +461,12 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
 
 template <bool FOM>
 __device__ __forceinline__ Emit make_emit(unsigned key, int kk, bool incl, float fr, float fi,
-                                          double w) {
+                                          double w, int bin) {
     Emit e;
     e.key = (int)key;
     e.kk = kk;
     e.incl = incl ? 1 : 0;
+    e.bin = bin;
     if (!incl) w = 0;
     e.re = (float)(fr * w);
     e.im = (float)(fi * w);
@@ -536,7 +543,7 @@ template <int MG, bool FOM, int MODE>
 __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const ReconConst& K,
                                             const double m[3], bool inband, double w, float fr,
                                             float fi, int kk, bool merge, unsigned keybase,
-                                            const BinCtx& bc) {
+                                            const BinCtx& bc, int bin = 1) {
     const long N = K.N;
     if (K.interp < 2) {
         const double fx = floor(m[0] + 0.5), fy = floor(m[1] + 0.5), fz = floor(m[2] + 0.5);
@@ -548,7 +555,7 @@ __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const Recon
         const long cx = wrapi((long)fx, N), cy = wrapi((long)fy, N), cz = wrapi((long)fz, N);
         emit<MG, FOM, MODE>(acc, K.stride, merge,
                             make_emit<FOM>(keybase + (unsigned)rec_index(cx, cy, cz, K.NT), kk,
-                                           inband && ww > 0, fr, fi, ww),
+                                           inband && ww > 0, fr, fi, ww, bin),
                             bc);
     } else {
         const double fx = floor(m[0]), fy = floor(m[1]), fz = floor(m[2]);
@@ -561,7 +568,7 @@ __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const Recon
             const double ww = (xx ? dx : 1 - dx) * (yy ? dy : 1 - dy) * (zz ? dz : 1 - dz) * w;
             emit<MG, FOM, MODE>(acc, K.stride, merge,
                                 make_emit<FOM>(keybase + (unsigned)rec_index(jx, jy, jz, K.NT), kk,
-                                               inband, fr, fi, ww),
+                                               inband, fr, fi, ww, bin),
                                 bc);
         }
     }
@@ -581,6 +588,13 @@ __device__ __forceinline__ void coords(const ReconConst& K, const PartDev& P, do
 // conjugate: computed once, emitted twice. Every lane of a wavefront runs the
 // same emits (excluded samples carry incl = 0), so shuffles and ballots stay
 // uniform.
+//
+// With K.pairs (binned nearest-neighbour insertion without Ewald) a sample
+// whose mate rounds to exactly the opposite voxel (-v for v; all but exact
+// .5 ties) is emitted once, as a pair entry at v that k_fold later also adds,
+// conjugated, at -v. Samples without such a mate (the x = 0 and x = FT/2
+// columns, the Nyquist row, ties) add to the volume directly, and the mate
+// of a tie is emitted on its own, as before.
 template <int MG, bool FOM, int MODE>
 __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const ReconConst& K,
                                             const PartDev& P, bool active, bool mirror, long h,
@@ -592,17 +606,10 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
     double m[3];
     const double mz0 = P.ew * s2;
     coords(K, P, s0, s1, mz0, m);
-    interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
-    if (K.ewald) {
+    if (K.ewald) {   // all lanes: two emissions per frequency, mates computed in full
+        interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
         coords(K, P, s0, s1, -mz0, m);
         interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
-    }
-    // the mate
-    if (!K.ewald && km == -k) {
-        const double mm[3] = {-m[0], -m[1], -m[2]};
-        interp_emit<MG, FOM, MODE>(acc, K, mm, inband && mirror, w, fr, -fi, (int)km, merge,
-                                   keybase, bc);
-    } else {
         double t0, t1, t2, tw;
         float gr, gi;
         const bool ib = sample_value<MODE>(K, P, mirror, -h, km, make_float2(F.x, -F.y), &t0,
@@ -610,11 +617,40 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
         const double mz = P.ew * t2;
         coords(K, P, t0, t1, mz, m);
         interp_emit<MG, FOM, MODE>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
-        if (K.ewald) {
-            coords(K, P, t0, t1, -mz, m);
-            interp_emit<MG, FOM, MODE>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
-        }
+        coords(K, P, t0, t1, -mz, m);
+        interp_emit<MG, FOM, MODE>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
+        return;
     }
+    // the mate, per lane: -m and the conjugate away from the Nyquist row,
+    // else computed in full; emitted below by all lanes together
+    double mm[3];
+    double tw = w;
+    float gr = fr, gi = -fi;
+    bool ib = inband && mirror;
+    if (km == -k) {
+        mm[0] = -m[0];
+        mm[1] = -m[1];
+        mm[2] = -m[2];
+    } else {
+        double t0, t1, t2;
+        ib = sample_value<MODE>(K, P, mirror, -h, km, make_float2(F.x, -F.y), &t0, &t1, &t2,
+                                &tw, &gr, &gi);
+        coords(K, P, t0, t1, 0.0, mm);
+    }
+    if (MODE == MODE_FILL && K.pairs) {
+        bool sym = km == -k;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) sym = sym && floor(-m[q] + 0.5) == -floor(m[q] + 0.5);
+        const bool pair = mirror && sym;
+        interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc,
+                                   pair ? 1 : 0);
+        const bool mate = ib && !sym;
+        if (__ballot(mate) == 0) return;   // wavefront-uniform
+        interp_emit<MG, FOM, MODE>(acc, K, mm, mate, tw, gr, gi, (int)km, merge, keybase, bc, 0);
+        return;
+    }
+    interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
+    interp_emit<MG, FOM, MODE>(acc, K, mm, ib, tw, gr, gi, (int)km, merge, keybase, bc);
 }
 
 // Column tasks: task = (particle, x) with x in [0, FT/2]; G tasks per WG.
@@ -960,6 +996,53 @@ __global__ __launch_bounds__(256) void k_friedel(const float2* __restrict__ v, i
         for (int q = 0; q < 3; ++q) atomicAdd(sums + q, sh[q][0]);
 }
 
+// Folds the binned accumulator X into the volume records S (3 floats per
+// record, both in the tiled layout, `nrec` records per half) and clears X.
+// pairs = 0: X holds plain sums, S += X. pairs = 1: an X record at v stands
+// for the same sums at v and their conjugates at -v (mod N), so
+// S(v) += X(v) + conj(X(-v)) and the weight W(v) += Xw(v) + Xw(-v). One
+// thread per voxel pair {v, -v}; linear order decides which voxel of the
+// pair handles it.
+__global__ void k_fold(float* __restrict__ S, float* __restrict__ X, long long n3, int N, int NT,
+                       long long nrec, int nhalves, int pairs) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n3 * nhalves;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int half = (int)(i / n3);
+        const long long j = i - (long long)half * n3;
+        const long x = j % N, y = (j / N) % N, z = j / ((long long)N * N);
+        float* s = S + (size_t)half * nrec * 3;
+        float* xa = X + (size_t)half * nrec * 3;
+        const size_t rv = (size_t)rec_index(x, y, z, NT) * 3;
+        if (!pairs) {
+            s[rv] += xa[rv];
+            s[rv + 1] += xa[rv + 1];
+            s[rv + 2] += xa[rv + 2];
+            xa[rv] = xa[rv + 1] = xa[rv + 2] = 0.f;
+            continue;
+        }
+        const long mx = x ? N - x : 0, my = y ? N - y : 0, mz = z ? N - z : 0;
+        const long long jm = mx + (long long)N * (my + (long long)N * mz);
+        if (jm < j) continue;
+        const size_t ru = (size_t)rec_index(mx, my, mz, NT) * 3;
+        const float vr = xa[rv], vi = xa[rv + 1], vw = xa[rv + 2];
+        if (jm == j) {
+            s[rv] += 2.f * vr;
+            s[rv + 2] += 2.f * vw;
+            xa[rv] = xa[rv + 1] = xa[rv + 2] = 0.f;
+            continue;
+        }
+        const float ur = xa[ru], ui = xa[ru + 1], uw = xa[ru + 2];
+        s[rv] += vr + ur;
+        s[rv + 1] += vi - ui;
+        s[rv + 2] += vw + uw;
+        s[ru] += ur + vr;
+        s[ru + 1] += ui - vi;
+        s[ru + 2] += uw + vw;
+        xa[rv] = xa[rv + 1] = xa[rv + 2] = 0.f;
+        xa[ru] = xa[ru + 1] = xa[ru + 2] = 0.f;
+    }
+}
+
 // Raw accumulator planes of one half set in (z, y, x) order.
 __global__ void k_planes(const float* __restrict__ acc, long long n3, int N, int NT, int stride,
                          int fom, float* __restrict__ out) {
@@ -979,6 +1062,7 @@ struct bsx_recon {
     int dev;
     bool binned = true;            // tile-binned insertion (else direct atomics)
     float* acc = nullptr;
+    float* xacc = nullptr;         // binned accumulator, folded into acc by k_fold
     size_t acc_elems = 0;
     float2* inter = nullptr;
     float* bkg = nullptr;
@@ -1091,7 +1175,7 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
     const PartDev* rec = h->recs + p0;
     const float* bk = h->bkg + p0;
     if (h->binned) {
-        BinCtx bf{h->fill, h->chunks, h->pool, h->bins, h->acc, h->npool, h->maxc, Stage{}};
+        BinCtx bf{h->fill, h->chunks, h->pool, h->bins, h->xacc, h->npool, h->maxc, Stage{}};
         hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL>), grid, dim3(NT), 0, s, h->inter, bk,
                            rec, nb, box, h->K, h->acc, tw, bf);
         BSX_HIP(hipGetLastError());
@@ -1102,7 +1186,7 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
         if ((r = tmark(h, s, KT_SCAN))) return r;
         const unsigned maxpieces = (unsigned)(h->nbins + (size_t)h->npool * BSX_CHUNK / BSX_PIECE + 1);
         hipLaunchKernelGGL(k_tile_accum, dim3(maxpieces), dim3(512), 0, s, h->fill, h->chunks,
-                           h->maxc, h->poff, h->nbins, h->bins, h->acc);
+                           h->maxc, h->poff, h->nbins, h->bins, h->xacc);
         BSX_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_bin_reset, dim3((h->nbins + 255) / 256), dim3(256), 0, s, h->fill,
                            h->chunks, h->maxc, h->nbins, h->pool);
@@ -1217,6 +1301,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     K.shi2 = 1.0 / (h->hi_res * h->hi_res);          // img_ctf_calculate hires
     K.ewald = (p->flags & BSX_FLAG_EWALD) ? 1 : 0;
     K.merge = 0;
+    K.pairs = h->binned && !K.ewald && p->interp < 2 ? 1 : 0;
     const size_t ntile = (size_t)K.NT * K.NT * K.NT;
     const size_t nrec = ntile * 4096;   // records incl. tile padding
     h->acc_elems = nrec * K.stride * p->nhalves;
@@ -1249,7 +1334,8 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     const size_t bins_bytes = h->binned ? (size_t)h->npool * BSX_CHUNK * sizeof(float4) : 0;
     const size_t bin_idx_bytes =
         h->binned ? ((size_t)h->nbins * (h->maxc + 2) + 3) * sizeof(int) + 8 : 0;
-    h->bytes = h->acc_elems * sizeof(float) + inter_bytes + bins_bytes + bin_idx_bytes;
+    h->bytes = h->acc_elems * sizeof(float) * (h->binned ? 2 : 1) + inter_bytes + bins_bytes +
+               bin_idx_bytes;
     if (p->mem_cap > 0 && (int64_t)h->bytes > p->mem_cap) {
         set_error("device memory %zu bytes exceeds the cap %lld", h->bytes, (long long)p->mem_cap);
         delete h;
@@ -1260,6 +1346,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
               hipMalloc(&h->nonfinite, sizeof(int)) == hipSuccess;
     if (ok && h->binned)
         ok = hipMalloc(&h->bins, bins_bytes) == hipSuccess &&
+             hipMalloc(&h->xacc, h->acc_elems * sizeof(float)) == hipSuccess &&
              hipMalloc(&h->fill, sizeof(int) * h->nbins) == hipSuccess &&
              hipMalloc(&h->chunks, sizeof(int) * (size_t)h->nbins * h->maxc) == hipSuccess &&
              hipMalloc(&h->pool, sizeof(int) * 2) == hipSuccess &&
@@ -1271,6 +1358,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
         return BSX_ERR_NOMEM;
     }
     if (h->binned) {
+        BSX_HIP(hipMemset(h->xacc, 0, h->acc_elems * sizeof(float)));
         BSX_HIP(hipMemset(h->fill, 0, sizeof(int) * h->nbins));
         BSX_HIP(hipMemset(h->chunks, 0xFF, sizeof(int) * (size_t)h->nbins * h->maxc));
         BSX_HIP(hipMemset(h->pool, 0, sizeof(int) * 2));
@@ -1284,6 +1372,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
 int bsx_recon_destroy(bsx_recon* h) {
     if (!h) return BSX_OK;
     if (h->acc) (void)hipFree(h->acc);
+    if (h->xacc) (void)hipFree(h->xacc);
     if (h->inter) (void)hipFree(h->inter);
     if (h->bkg) (void)hipFree(h->bkg);
     if (h->recs) (void)hipFree(h->recs);
@@ -1310,6 +1399,7 @@ int bsx_recon_reset(bsx_recon* h, void* stream) {
     BSX_REQUIRE(h, BSX_ERR_ARG, "null handle");
     BSX_HIP(hipMemsetAsync(h->acc, 0, h->acc_elems * sizeof(float), (hipStream_t)stream));
     if (h->binned) {
+        BSX_HIP(hipMemsetAsync(h->xacc, 0, h->acc_elems * sizeof(float), (hipStream_t)stream));
         // bin state is cleared after every batch; cleared again here in case an
         // earlier insert stopped between its fill and its reset
         BSX_HIP(hipMemsetAsync(h->fill, 0, sizeof(int) * h->nbins, (hipStream_t)stream));
@@ -1576,9 +1666,23 @@ int bsx_recon_kernel_times(const bsx_recon* h, double* ms, int n) {
     return BSX_OK;
 }
 
+// Brings the binned accumulator into the volume records (stream-ordered after
+// the inserts); a no-op pass when nothing was binned since the last fold.
+static int fold(bsx_recon* h, hipStream_t s) {
+    if (!h->binned) return BSX_OK;
+    const long long n3 = (long long)h->K.N * h->K.N * h->K.N;
+    const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
+    hipLaunchKernelGGL(k_fold, dim3(2048), dim3(256), 0, s, h->acc, h->xacc, n3, h->K.N, h->K.NT,
+                       nrec, h->p.nhalves, h->K.pairs);
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
 int bsx_recon_accumulators(bsx_recon* h, int half, float* planes_dev, void* stream) {
     BSX_REQUIRE(h && planes_dev, BSX_ERR_ARG, "null argument");
     BSX_REQUIRE(half >= 0 && half < h->p.nhalves, BSX_ERR_ARG, "half %d out of range", half);
+    const int rf = fold(h, (hipStream_t)stream);
+    if (rf) return rf;
     const long long n3 = (long long)h->K.N * h->K.N * h->K.N;
     const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
     hipLaunchKernelGGL(k_planes, dim3(1024), dim3(256), 0, (hipStream_t)stream,
@@ -1603,6 +1707,8 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
     }
     hipStream_t s = (hipStream_t)stream;
     BSX_HIP(hipSetDevice(h->dev));
+    const int rf = fold(h, s);
+    if (rf) return rf;
     const int N = h->K.N;
     const long long n3 = (long long)N * N * N;
     float2* vol = (float2*)ft_dev;
