@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "bsx_common.h"
+#include "bsx_fft_dev.h"
 
 namespace bsx {
 int fft_axis_ex(float2* data, float* real_out, int nx, int ny, int nz, int64_t batch, int axis,
@@ -71,23 +72,18 @@ struct SPart {
     int pad_;
 };
 
-// One workgroup per particle. spec: the particle's N x N forward transform
-// (sign -1, x 1/N). Writes the disk values F (phase-shifted to the origin),
-// the CTF values, and the BF16 coarse row (F*CTF normalised per ring, x 1/(R-1)).
-__global__ __launch_bounds__(256) void k_part_stage(
-    const float2* __restrict__ spec, int N, const SPart* __restrict__ sp,
-    const int2* __restrict__ hk, const int* __restrict__ ring_start, int R, int Nd, int Kp,
-    int64_t p0, float2* __restrict__ fp, float* __restrict__ ctf, uint16_t* __restrict__ arow) {
-    extern __shared__ float2 sv[];          // Nd values F*CTF
-    __shared__ float rn[65];
-    const int b = blockIdx.x;
-    const SPart P = sp[b];
-    const float2* S = spec + (size_t)b * N * N;
-    const int64_t p = p0 + b;
+// Disk values of one particle from its spectrum values sv[i] = F(h_i, k_i)
+// (sign -1, x 1/N) in LDS: phase shift to the origin, CTF, then the BF16
+// coarse row (F*CTF normalised per ring, x 1/(R-1)). sv is overwritten with
+// F*CTF. Called by the whole workgroup.
+__device__ __forceinline__ void stage_disk(float2* sv, float* rn, const SPart& P,
+                                           const int2* __restrict__ hk,
+                                           const int* __restrict__ ring_start, int R, int Nd,
+                                           int Kp, int64_t p, float2* __restrict__ fp,
+                                           float* __restrict__ ctf, uint16_t* __restrict__ arow) {
     for (int i = threadIdx.x; i < Nd; i += blockDim.x) {
         const int h = hk[i].x, k = hk[i].y;
-        const int xx = h < 0 ? h + N : h, yy = k < 0 ? k + N : k;
-        const float2 F = S[(size_t)yy * N + xx];
+        const float2 F = sv[i];
         double sn, cs;
         sincos((double)h * P.phx + (double)k * P.phy, &sn, &cs);
         const float vr = (float)(F.x * cs - F.y * sn), vi = (float)(F.x * sn + F.y * cs);
@@ -117,6 +113,92 @@ __global__ __launch_bounds__(256) void k_part_stage(
         }
         *(uint32_t*)(A + 2 * q) = w;
     }
+}
+
+// One workgroup per particle. spec: the particle's N x N forward transform
+// (sign -1, x 1/N). Used where the fused kernel below does not fit.
+__global__ __launch_bounds__(256) void k_part_stage(
+    const float2* __restrict__ spec, int N, const SPart* __restrict__ sp,
+    const int2* __restrict__ hk, const int* __restrict__ ring_start, int R, int Nd, int Kp,
+    int64_t p0, float2* __restrict__ fp, float* __restrict__ ctf, uint16_t* __restrict__ arow) {
+    extern __shared__ float2 sv[];          // Nd values
+    __shared__ float rn[65];
+    const int b = blockIdx.x;
+    const float2* S = spec + (size_t)b * N * N;
+    for (int i = threadIdx.x; i < Nd; i += blockDim.x) {
+        const int h = hk[i].x, k = hk[i].y;
+        const int xx = h < 0 ? h + N : h, yy = k < 0 ? k + N : k;
+        sv[i] = S[(size_t)yy * N + xx];
+    }
+    __syncthreads();
+    stage_disk(sv, rn, sp[b], hk, ring_start, R, Nd, Kp, p0 + b, fp, ctf, arow);
+}
+
+// Fused staging, one workgroup per particle: the image is read once and only
+// the low frequencies the disk needs are transformed.
+//  1. rows, two real rows per complex line (z = a + i b): FftLine<N> over the
+//     lines, then X_a(h) = (Z(h) + conj Z(-h)) / 2 and
+//     X_b(h) = (Z(h) - conj Z(-h)) / 2i for h = 0..R-1 into C[h][y];
+//  2. columns: FftLine<N> over C[h] for h = 0..R-1, in place;
+//  3. disk values F(h, k) = C[h][k] / N for h >= 0 and conj(C[-h][-k]) / N
+//     for h < 0 (real image: F(-h, -k) = conj F(h, k)), then stage_disk.
+// Dynamic LDS (float2): G lines x N | N twiddles | R x N columns | Nd values.
+template <int N>
+__global__ __launch_bounds__(256) void k_stage_fused(
+    const float* __restrict__ imgs, const float2* __restrict__ twg, const SPart* __restrict__ sp,
+    const int2* __restrict__ hk, const int* __restrict__ ring_start, int R, int Nd, int Kp,
+    int64_t p0, float2* __restrict__ fp, float* __restrict__ ctf, uint16_t* __restrict__ arow) {
+    constexpr int T = FftThreads<N>::value;
+    constexpr int G = 256 / T;
+    extern __shared__ float2 lds[];
+    __shared__ float rn[65];
+    float2* lines = lds;
+    float2* tw = lines + G * N;
+    float2* C = tw + N;
+    float2* sv = C + (size_t)R * N;
+    const int tid = threadIdx.x;
+    const int g = tid / T, t = tid - g * T;
+    const int b = blockIdx.x;
+    const float* img = imgs + (size_t)b * N * N;
+    for (int i = tid; i < N; i += blockDim.x) tw[i] = twg[i];
+    float2* line = lines + g * N;
+    for (int l0 = 0; l0 < N / 2; l0 += G) {
+        const int l = l0 + g;
+        const bool ok = g < G && l < N / 2;
+        if (ok) {
+            const float* ra = img + (size_t)(2 * l) * N;
+            for (int x = t; x < N; x += T) line[x] = make_float2(ra[x], ra[x + N]);
+        }
+        __syncthreads();
+        FftLine<N, -1, T>::run(line, tw, t);
+        if (ok) {
+            for (int h = t; h < R; h += T) {
+                const float2 zp = line[h], zm = line[(N - h) % N];
+                C[(size_t)h * N + 2 * l] = make_float2(0.5f * (zp.x + zm.x), 0.5f * (zp.y - zm.y));
+                C[(size_t)h * N + 2 * l + 1] = make_float2(0.5f * (zp.y + zm.y), 0.5f * (zm.x - zp.x));
+            }
+        }
+        __syncthreads();
+    }
+    for (int c0 = 0; c0 < R; c0 += G) {
+        const int h = c0 + g;
+        float2* col = (g < G && h < R) ? C + (size_t)h * N : line;
+        FftLine<N, -1, T>::run(col, tw, t);
+    }
+    const float inv = 1.0f / N;
+    for (int i = tid; i < Nd; i += blockDim.x) {
+        const int h = hk[i].x, k = hk[i].y;
+        float2 F;
+        if (h >= 0) {
+            F = C[(size_t)h * N + k % N];
+        } else {
+            F = C[(size_t)(-h) * N + (N - k) % N];
+            F.y = -F.y;
+        }
+        sv[i] = make_float2(F.x * inv, F.y * inv);
+    }
+    __syncthreads();
+    stage_disk(sv, rn, sp[b], hk, ring_start, R, Nd, Kp, p0 + b, fp, ctf, arow);
 }
 
 // ---------------------------------------------------------------- library
@@ -866,6 +948,55 @@ inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
 }  // namespace
 
+namespace {
+
+// Dynamic LDS of k_stage_fused for box N, disk radius R and Nd points; 0 when
+// the box has no fused kernel or the columns do not fit.
+size_t fused_stage_lds(int N, int R, int Nd) {
+    if (N != 32 && N != 64 && N != 128 && N != 256 && N != 512) return 0;
+    if (R < 1 || R > N / 2) return 0;
+    const int G = 256 / (N / 8);
+    const size_t bytes = sizeof(float2) * ((size_t)G * N + N + (size_t)R * N + Nd);
+    return bytes <= 120 * 1024 ? bytes : 0;
+}
+
+template <int N>
+int launch_fused(unsigned nb, size_t lds, hipStream_t s, const float* imgs, const float2* tw,
+                 const SPart* sp, const int2* hk, const int* ring, int R, int Nd, int Kp,
+                 float2* fp, float* ctf, uint16_t* arow) {
+    static bool attr = false;
+    if (!attr) {
+        BSX_HIP(hipFuncSetAttribute((const void*)k_stage_fused<N>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 120 * 1024));
+        attr = true;
+    }
+    constexpr unsigned MAXB = 1u << 22;   // workgroups per launch
+    for (unsigned p0 = 0; p0 < nb; p0 += MAXB) {
+        const unsigned nl = std::min(MAXB, nb - p0);
+        hipLaunchKernelGGL(k_stage_fused<N>, dim3(nl), dim3(256), lds, s,
+                           imgs + (size_t)p0 * N * N, tw, sp + p0, hk, ring, R, Nd, Kp,
+                           (int64_t)p0, fp, ctf, arow);
+        BSX_HIP(hipGetLastError());
+    }
+    return BSX_OK;
+}
+
+int launch_stage_fused(int N, unsigned nb, size_t lds, hipStream_t s, const float* imgs,
+                       const float2* tw, const SPart* sp, const int2* hk, const int* ring, int R,
+                       int Nd, int Kp, float2* fp, float* ctf, uint16_t* arow) {
+    switch (N) {
+        case 32: return launch_fused<32>(nb, lds, s, imgs, tw, sp, hk, ring, R, Nd, Kp, fp, ctf, arow);
+        case 64: return launch_fused<64>(nb, lds, s, imgs, tw, sp, hk, ring, R, Nd, Kp, fp, ctf, arow);
+        case 128: return launch_fused<128>(nb, lds, s, imgs, tw, sp, hk, ring, R, Nd, Kp, fp, ctf, arow);
+        case 256: return launch_fused<256>(nb, lds, s, imgs, tw, sp, hk, ring, R, Nd, Kp, fp, ctf, arow);
+        case 512: return launch_fused<512>(nb, lds, s, imgs, tw, sp, hk, ring, R, Nd, Kp, fp, ctf, arow);
+    }
+    set_error("no fused staging kernel for box %d", N);
+    return BSX_ERR_ARG;
+}
+
+}  // namespace
+
 extern "C" {
 
 int bsx_search_create(bsx_search** out, const bsx_search_params* prm) {
@@ -1026,8 +1157,23 @@ int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* p
     if (rc) return rc;
     BSX_HIP(hipMemsetAsync(h->d_pbf, 0, sizeof(uint16_t) * ppad * Kp, s));
     h->nparts = n;
-    // transform in chunks of at most 256 MB of spectra
+    if (n == 0) return BSX_OK;
     const int64_t per = (int64_t)N * N;
+    // fused path: one kernel reads each image once
+    const size_t fused_lds = fused_stage_lds(N, h->R, Nd);
+    if (fused_lds) {
+        rc = grow(h, &h->d_sp, &h->sp_cap, n, "particle records");
+        if (rc) return rc;
+        BSX_HIP(hipMemcpyAsync(h->d_sp, sp.data(), sizeof(SPart) * n, hipMemcpyHostToDevice, s));
+        const float2* tw = twiddles(N, s);
+        BSX_REQUIRE(tw, BSX_ERR_DEVICE, "twiddles(%d) failed", N);
+        rc = launch_stage_fused(N, (unsigned)n, fused_lds, s, imgs_dev, tw, h->d_sp, h->d_hk,
+                                h->d_ring, h->R, Nd, Kp, h->d_fp, h->d_ctf, h->d_pbf);
+        if (rc) return rc;
+        BSX_HIP(hipStreamSynchronize(s));   // the host records are released on return
+        return BSX_OK;
+    }
+    // otherwise: full spectra, in chunks of at most 256 MB
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n, (256ll << 20) / (8 * per)));
     rc = grow(h, &h->d_spec, &h->spec_cap, chunk * per, "spectra");
     if (!rc) rc = grow(h, &h->d_sp, &h->sp_cap, std::max<int64_t>(chunk, 1), "particle records");
