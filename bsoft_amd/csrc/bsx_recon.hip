@@ -161,6 +161,20 @@ __host__ __device__ __forceinline__ long rec_index(long x, long y, long z, long 
            ((y & 15) << 4) | (x & 15);
 }
 
+// 32-bit forms for the insertion hot path (coordinates stay within a few
+// box sizes; record indices below 2^31 per half).
+__device__ __forceinline__ unsigned rec_index32(int x, int y, int z, int NT) {
+    return (unsigned)(((((z >> 4) * NT + (y >> 4)) * NT + (x >> 4)) << 12) | ((z & 15) << 8) |
+                      ((y & 15) << 4) | (x & 15));
+}
+__device__ __forceinline__ int wrap32(int c, int n) {
+    if (c >= 0 && c < n) return c;
+    if (c < 0 && c >= -n) return c + n;
+    if (c >= n && c < 2 * n) return c - n;
+    c %= n;
+    return c < 0 ? c + n : c;
+}
+
 // Periodic wrap (index_wrap, Bimage.h:418-426); the fast path covers the
 // in-range and single-step cases that the default geometry produces.
 __device__ __forceinline__ long wrapi(long c, long n) {
@@ -370,8 +384,50 @@ __device__ __forceinline__ void stage_flush(const BinCtx& bc) {
     __syncthreads();
 }
 
-// Merges runs of equal keys inside aligned groups of MG lanes (segmented
-// Hillis-Steele suffix sums); lanes l and l+d merge only if their keys are
+// Row shifts of a wavefront (DPP, rows of 16 lanes): dpp_down<D> gives lane
+// i the value of lane i + D of its row, dpp_up1 that of lane i - 1; lanes
+// whose source lies outside the row read 0 (the merge conditions exclude
+// them).
+template <int D>
+__device__ __forceinline__ int dpp_down(int v) {
+    static_assert(D >= 1 && D <= 15, "row shift");
+    return __builtin_amdgcn_update_dpp(0, v, 0x100 + D, 0xF, 0xF, true);
+}
+template <int D>
+__device__ __forceinline__ float dpp_downf(float v) {
+    return __int_as_float(dpp_down<D>(__float_as_int(v)));
+}
+__device__ __forceinline__ int dpp_up1(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
+}
+
+// One step of the segmented suffix sum below: lane l takes lane l + D's sums
+// when both hold the same key and bin flag and their sample rows differ by D.
+// pk packs (row << 2) | (bin << 1) | incl.
+template <int D, int MG, bool FOM>
+__device__ __forceinline__ void merge_step(Emit& e, int& pk, int lg) {
+    if (D >= MG) return;
+    const int nk = dpp_down<D>(e.key);
+    const int np = dpp_down<D>(pk);
+    const float r0 = dpp_downf<D>(e.re);
+    const float r1 = dpp_downf<D>(e.im);
+    const float r2 = dpp_downf<D>(e.w);
+    float r3 = 0.f, r4 = 0.f;
+    if (FOM) {
+        r3 = dpp_downf<D>(e.p);
+        r4 = dpp_downf<D>(e.w2);
+    }
+    if (lg + D < MG && nk == e.key && ((np ^ pk) & 2) == 0 && abs((np >> 2) - e.kk) == D) {
+        pk |= np & 1;
+        e.re += r0;
+        e.im += r1;
+        e.w += r2;
+        if (FOM) { e.p += r3; e.w2 += r4; }
+    }
+}
+
+// Merges runs of equal keys inside aligned groups of MG <= 16 lanes (segmented
+// Hillis-Steele suffix sums over DPP row shifts); lanes l and l+d merge only if their keys are
 // equal and their sample rows differ by exactly d, so the samples between lie
 // on one straight segment and equal keys are contiguous (a periodic revisit
 // would need a segment longer than N-1 voxels, excluded on the host). The
@@ -384,32 +440,17 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
     bool head = true;
     if (MG > 1 && merge) {
         const int lg = __lane_id() & (MG - 1);
-#pragma unroll
-        for (int d = 1; d < MG; d <<= 1) {
-            const int nk = __shfl_down(e.key, d, 64);
-            const int nkk = __shfl_down(e.kk, d, 64);
-            const int ninc = __shfl_down(e.incl, d, 64);
-            const int nbin = __shfl_down(e.bin, d, 64);
-            float r3 = 0.f, r4 = 0.f;
-            const float r0 = __shfl_down(e.re, d, 64);
-            const float r1 = __shfl_down(e.im, d, 64);
-            const float r2 = __shfl_down(e.w, d, 64);
-            if (FOM) {
-                r3 = __shfl_down(e.p, d, 64);
-                r4 = __shfl_down(e.w2, d, 64);
-            }
-            if (lg + d < MG && nk == e.key && abs(nkk - e.kk) == d && nbin == e.bin) {
-                e.incl |= ninc;
-                e.re += r0;
-                e.im += r1;
-                e.w += r2;
-                if (FOM) { e.p += r3; e.w2 += r4; }
-            }
-        }
-        const int pk = __shfl_up(e.key, 1, 64);
-        const int pkk = __shfl_up(e.kk, 1, 64);
-        const int pb = __shfl_up(e.bin, 1, 64);
-        head = !(lg != 0 && pk == e.key && abs(e.kk - pkk) == 1 && pb == e.bin);
+        // sample row, bin flag and inclusion in one word for the shifts
+        int pk = (e.kk * 4) | (e.bin << 1) | e.incl;
+        merge_step<1, MG, FOM>(e, pk, lg);
+        merge_step<2, MG, FOM>(e, pk, lg);
+        merge_step<4, MG, FOM>(e, pk, lg);
+        merge_step<8, MG, FOM>(e, pk, lg);
+        e.incl = pk & 1;
+        const int prev_key = dpp_up1(e.key);
+        const int prev_pk = dpp_up1(pk);
+        head = !(lg != 0 && prev_key == e.key && ((prev_pk ^ pk) & 2) == 0 &&
+                 abs(e.kk - (prev_pk >> 2)) == 1);
     }
     const bool out0 = head && e.incl;
     if (MODE == MODE_DIRECT || !e.bin) {
@@ -517,19 +558,22 @@ __device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev&
         const double cs2 = fx * fx + fy * fy;
         float c = 0.f;
         if (cs2 <= K.shi2) {
-            double c2a = 1, s2a = 0;
-            if (cs2 > 0) {
-                c2a = (fx * fx - fy * fy) / cs2;
-                s2a = 2 * fx * fy / cs2;
-            }
+            // chi = c0 + c2m2 s^2 sin 2a + c20 s^2 + c22 s^2 cos 2a + c40 s^4 with
+            // s^2 cos 2a = fx^2 - fy^2 and s^2 sin 2a = 2 fx fy (no division)
             double dphi = P.c0;
-            dphi += P.c2m2 * cs2 * s2a;
+            dphi += P.c2m2 * (2 * fx * fy);
             dphi += P.c20 * cs2;
-            dphi += P.c22 * cs2 * c2a;
+            dphi += P.c22 * (fx * fx - fy * fy);
             dphi += P.c40 * cs2 * cs2;
-            c = sin_red(dphi);
-            if (K.ctf_mode == 1) c = (c < 0.f) ? -1.f : 1.f;
-            else if (K.ctf_mode == 3) c = (float)((K.wiener + 1) * c / ((double)c * c + K.wiener));
+            if (K.ctf_mode == 1) {
+                // phase flip needs the sign only: sin chi < 0 iff the reduced
+                // angle lies in (-pi, 0)
+                const double r = dphi - 6.283185307179586 * rint(dphi * 0.15915494309189535);
+                c = (r < 0) ? -1.f : 1.f;
+            } else {
+                c = sin_red(dphi);
+                if (K.ctf_mode == 3) c = (float)((K.wiener + 1) * c / ((double)c * c + K.wiener));
+            }
         }
         vr *= c;
         vi *= c;
@@ -546,7 +590,6 @@ __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const Recon
                                             const double m[3], bool inband, double w, float fr,
                                             float fi, int kk, bool merge, unsigned keybase,
                                             const BinCtx& bc, int bin = 1) {
-    const long N = K.N;
     if (K.interp < 2) {
         const double fx = floor(m[0] + 0.5), fy = floor(m[1] + 0.5), fz = floor(m[2] + 0.5);
         double ww = w;
@@ -554,22 +597,22 @@ __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const Recon
             const double dx = m[0] - fx, dy = m[1] - fy, dz = m[2] - fz;
             ww *= 1 - sqrt(dx * dx + dy * dy + dz * dz);
         }
-        const long cx = wrapi((long)fx, N), cy = wrapi((long)fy, N), cz = wrapi((long)fz, N);
+        const int cx = wrap32((int)fx, K.N), cy = wrap32((int)fy, K.N), cz = wrap32((int)fz, K.N);
         emit<MG, FOM, MODE>(acc, K.stride, merge,
-                            make_emit<FOM>(keybase + (unsigned)rec_index(cx, cy, cz, K.NT), kk,
+                            make_emit<FOM>(keybase + rec_index32(cx, cy, cz, K.NT), kk,
                                            inband && ww > 0, fr, fi, ww, bin),
                             bc);
     } else {
         const double fx = floor(m[0]), fy = floor(m[1]), fz = floor(m[2]);
         const double dx = m[0] - fx, dy = m[1] - fy, dz = m[2] - fz;
-        const long cx = (long)fx, cy = (long)fy, cz = (long)fz;
+        const int cx = (int)fx, cy = (int)fy, cz = (int)fz;
 #pragma unroll
         for (int corner = 0; corner < 8; ++corner) {
             const int xx = corner & 1, yy = (corner >> 1) & 1, zz = corner >> 2;
-            const long jx = wrapi(cx + xx, N), jy = wrapi(cy + yy, N), jz = wrapi(cz + zz, N);
+            const int jx = wrap32(cx + xx, K.N), jy = wrap32(cy + yy, K.N), jz = wrap32(cz + zz, K.N);
             const double ww = (xx ? dx : 1 - dx) * (yy ? dy : 1 - dy) * (zz ? dz : 1 - dz) * w;
             emit<MG, FOM, MODE>(acc, K.stride, merge,
-                                make_emit<FOM>(keybase + (unsigned)rec_index(jx, jy, jz, K.NT), kk,
+                                make_emit<FOM>(keybase + rec_index32(jx, jy, jz, K.NT), kk,
                                                inband, fr, fi, ww, bin),
                                 bc);
         }
