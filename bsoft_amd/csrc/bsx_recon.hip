@@ -778,11 +778,13 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
 #endif
 
 // Per-bin piece offsets (one workgroup): poff[b] = sum over bins below b of
-// ceil(binned / PIECE), poff[nbins] = all pieces; `total` accumulates the
-// entries binned since the last reset. Positions past the chunk table
-// (maxc chunks) went to the atomic fallback and are not binned.
+// ceil(binned / PIECE), poff[nbins] = all pieces, pbin[piece] = its bin (at
+// most maxpieces pieces are listed); `total` accumulates the entries binned
+// since the last reset. Positions past the chunk table (maxc chunks) went to
+// the atomic fallback and are not binned.
 __global__ __launch_bounds__(1024) void k_piece_scan(const int* __restrict__ fill, int nbins,
                                                      int maxc, int* __restrict__ poff,
+                                                     int* __restrict__ pbin, int maxpieces,
                                                      unsigned long long* __restrict__ total) {
     __shared__ int ppart[1024];
     __shared__ unsigned long long epart[1024];
@@ -809,7 +811,9 @@ __global__ __launch_bounds__(1024) void k_piece_scan(const int* __restrict__ fil
     int prun = ppart[tid] - psum;
     for (int i = b0; i < b1; ++i) {
         poff[i] = prun;
-        prun += (min(fill[i], cap) + BSX_PIECE - 1) / BSX_PIECE;
+        const int np = (min(fill[i], cap) + BSX_PIECE - 1) / BSX_PIECE;
+        for (int q = 0; q < np && prun + q < maxpieces; ++q) pbin[prun + q] = i;
+        prun += np;
     }
     for (int o = 512; o > 0; o >>= 1) {
         if (tid < o) epart[tid] += epart[tid + o];
@@ -838,20 +842,15 @@ __global__ void k_bin_reset(int* __restrict__ fill, int* __restrict__ chunks, in
 // Each thread keeps 8 entry loads in flight before its LDS adds.
 __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill,
                                                     const int* __restrict__ chunks, int maxc,
-                                                    const int* __restrict__ poff, int nbins,
+                                                    const int* __restrict__ poff,
+                                                    const int* __restrict__ pbin, int nbins,
                                                     const float4* __restrict__ bins,
                                                     float* __restrict__ acc) {
     constexpr int PC = BSX_PIECE / BSX_CHUNK;   // chunks per piece
     static_assert(BSX_PIECE % BSX_CHUNK == 0, "pieces are whole chunks");
     const int piece = blockIdx.x;
     if (piece >= poff[nbins]) return;
-    int lo = 0, hi = nbins - 1;   // last bin with poff[bin] <= piece
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (poff[mid] <= piece) lo = mid;
-        else hi = mid - 1;
-    }
-    const int bin = lo;
+    const int bin = pbin[piece];
     const int j = piece - poff[bin];
     const bool single = poff[bin + 1] - poff[bin] == 1;
     const int beg = j * BSX_PIECE;
@@ -1121,6 +1120,7 @@ struct bsx_recon {
     int npool = 0;
     int maxc = 0;
     int* poff = nullptr;           // piece offsets per bin
+    int* pbin = nullptr;           // bin of each piece
     unsigned long long* entries = nullptr;   // bin entries of the current insert
     float4* bins = nullptr;
     size_t bins_cap = 0;
@@ -1226,13 +1226,13 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
                            rec, nb, box, h->K, h->acc, tw, bf);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_FILL))) return r;
+        const unsigned maxpieces = (unsigned)(h->nbins + (size_t)h->npool * BSX_CHUNK / BSX_PIECE + 1);
         hipLaunchKernelGGL(k_piece_scan, dim3(1), dim3(1024), 0, s, h->fill, h->nbins, h->maxc,
-                           h->poff, h->entries);
+                           h->poff, h->pbin, (int)maxpieces, h->entries);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_SCAN))) return r;
-        const unsigned maxpieces = (unsigned)(h->nbins + (size_t)h->npool * BSX_CHUNK / BSX_PIECE + 1);
         hipLaunchKernelGGL(k_tile_accum, dim3(maxpieces), dim3(512), 0, s, h->fill, h->chunks,
-                           h->maxc, h->poff, h->nbins, h->bins, h->xacc);
+                           h->maxc, h->poff, h->pbin, h->nbins, h->bins, h->xacc);
         BSX_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_bin_reset, dim3((h->nbins + 255) / 256), dim3(256), 0, s, h->fill,
                            h->chunks, h->maxc, h->nbins, h->pool);
@@ -1379,7 +1379,9 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
                                                            BSX_CHUNK - 1) / BSX_CHUNK + 1));
     const size_t bins_bytes = h->binned ? (size_t)h->npool * BSX_CHUNK * sizeof(float4) : 0;
     const size_t bin_idx_bytes =
-        h->binned ? ((size_t)h->nbins * (h->maxc + 2) + 3) * sizeof(int) + 8 : 0;
+        h->binned ? ((size_t)h->nbins * (h->maxc + 3) + (size_t)h->npool * BSX_CHUNK / BSX_PIECE +
+                     4) * sizeof(int) + 8
+                  : 0;
     h->bytes = h->acc_elems * sizeof(float) * (h->binned ? 2 : 1) + inter_bytes + bins_bytes +
                bin_idx_bytes;
     if (p->mem_cap > 0 && (int64_t)h->bytes > p->mem_cap) {
@@ -1397,6 +1399,9 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
              hipMalloc(&h->chunks, sizeof(int) * (size_t)h->nbins * h->maxc) == hipSuccess &&
              hipMalloc(&h->pool, sizeof(int) * 2) == hipSuccess &&
              hipMalloc(&h->poff, sizeof(int) * (h->nbins + 1)) == hipSuccess &&
+             hipMalloc(&h->pbin, sizeof(int) * ((size_t)h->nbins +
+                                                (size_t)h->npool * BSX_CHUNK / BSX_PIECE + 1)) ==
+                 hipSuccess &&
              hipMalloc(&h->entries, sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
         set_error("device allocation of %zu bytes failed", h->bytes);
@@ -1428,6 +1433,7 @@ int bsx_recon_destroy(bsx_recon* h) {
     if (h->chunks) (void)hipFree(h->chunks);
     if (h->pool) (void)hipFree(h->pool);
     if (h->poff) (void)hipFree(h->poff);
+    if (h->pbin) (void)hipFree(h->pbin);
     if (h->entries) (void)hipFree(h->entries);
     for (auto e : h->evs) (void)hipEventDestroy(e);
     if (h->copied) {
