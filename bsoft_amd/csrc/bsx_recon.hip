@@ -227,7 +227,7 @@ struct Stage {
     int* hbase;      // reserved base per hash slot
     int* hc0;        // pool chunk of the range start
     int* hc1;        // pool chunk of the range end
-    int* count;      // entries in the list
+    int slot0;       // list slot of thread 0 in this round (one entry per thread and round)
     int on;
 };
 
@@ -313,27 +313,23 @@ __device__ __forceinline__ void range_chunks(const BinCtx& bc, int b, int base, 
     }
 }
 
-// One entry into the workgroup's list (per-lane LDS atomics: measured faster
-// here than grouping the lanes of a wavefront by bin first).
+// One entry into the workgroup's list at the thread's slot of this round
+// (per-lane LDS atomics for the bin table: measured faster here than
+// grouping the lanes of a wavefront by bin first).
 __device__ __forceinline__ void stage_entry(const BinCtx& bc, int key, float re, float im,
                                            float w) {
     const int bin = (int)((unsigned)key >> 12);
-    const int i = atomicAdd(bc.st.count, 1);
+    const int i = bc.st.slot0 + (int)threadIdx.x;
     int slot = -1;
-    if (i < BSX_STAGE_CAP) {
-        int hs = (int)(((unsigned)bin * 2654435761u) >> 25) & (BSX_STAGE_HASH - 1);
-        for (int t = 0; t < BSX_STAGE_HASH; ++t) {
-            const int prev = atomicCAS(&bc.st.hkey[hs], -1, bin);
-            if (prev == -1 || prev == bin) { slot = hs; break; }
-            hs = (hs + 1) & (BSX_STAGE_HASH - 1);
-        }
+    int hs = (int)(((unsigned)bin * 2654435761u) >> 25) & (BSX_STAGE_HASH - 1);
+    for (int t = 0; t < BSX_STAGE_HASH; ++t) {
+        const int prev = atomicCAS(&bc.st.hkey[hs], -1, bin);
+        if (prev == -1 || prev == bin) { slot = hs; break; }
+        hs = (hs + 1) & (BSX_STAGE_HASH - 1);
     }
-    const bool alone = slot < 0;   // list or table full: reserve this entry on its own
+    const bool alone = slot < 0;   // table full: reserve this entry on its own
     int pos = 0, c0 = -1, c1 = -1;
-    if (alone) {
-        if (i < BSX_STAGE_CAP) bc.st.r[i] = -1;
-        pos = atomicAdd(bc.fill + bin, 1);
-    }
+    if (alone) pos = atomicAdd(bc.fill + bin, 1);
     range_chunks(bc, bin, pos, 1, alone, c0, c1);
     if (alone) {
         put_entry(bc, c0, pos, re, im, w, key);
@@ -345,7 +341,7 @@ __device__ __forceinline__ void stage_entry(const BinCtx& bc, int key, float re,
 }
 
 // Workgroup-uniform: reserve every staged bin range, write the staged
-// entries, clear the list and the table.
+// entries, clear the list (r = -1: empty slot) and the table.
 __device__ __forceinline__ void stage_flush(const BinCtx& bc) {
     __syncthreads();
     const int tid = threadIdx.x;
@@ -364,10 +360,10 @@ __device__ __forceinline__ void stage_flush(const BinCtx& bc) {
         bc.st.hc1[tid] = c1;
     }
     __syncthreads();
-    const int n = min(*bc.st.count, BSX_STAGE_CAP);
-    for (int i = tid; i < n; i += blockDim.x) {
+    for (int i = tid; i < BSX_STAGE_CAP; i += blockDim.x) {
         const int pr = bc.st.r[i];
         if (pr < 0) continue;
+        bc.st.r[i] = -1;
         const int hs = pr >> 20;
         const float4 v = bc.st.e[i];
         const int hb = bc.st.hbase[hs];
@@ -380,7 +376,6 @@ __device__ __forceinline__ void stage_flush(const BinCtx& bc) {
         bc.st.hkey[tid] = -1;
         bc.st.hcnt[tid] = 0;
     }
-    if (tid == 0) *bc.st.count = 0;
     __syncthreads();
 }
 
@@ -717,7 +712,6 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     __shared__ float4 st_e[SCAP];
     __shared__ int st_r[SCAP];
     __shared__ int st_hkey[SHASH], st_hcnt[SHASH], st_hbase[SHASH], st_hc0[SHASH], st_hc1[SHASH];
-    __shared__ int st_count;
     const int tid = threadIdx.x;
     const int g = tid / T, t = tid - g * T;
     const long task = (long)blockIdx.x * G + g;
@@ -737,14 +731,13 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     // round, so 4 rounds fit the list)
     const bool staged = MODE == MODE_FILL && !FOM && K.pairs && NT == 256 &&
                         4 * NT <= BSX_STAGE_CAP;
-    bc.st = Stage{st_e, st_r, st_hkey, st_hcnt, st_hbase, st_hc0, st_hc1, &st_count,
-                  staged ? 1 : 0};
+    bc.st = Stage{st_e, st_r, st_hkey, st_hcnt, st_hbase, st_hc0, st_hc1, 0, staged ? 1 : 0};
     if (staged) {
         if (tid < BSX_STAGE_HASH) {
             st_hkey[tid] = -1;
             st_hcnt[tid] = 0;
         }
-        if (tid == 0) st_count = 0;
+        for (int i = tid; i < BSX_STAGE_CAP; i += NT) st_r[i] = -1;
         __syncthreads();
     }
     const PartDev& P = recs[valid ? p : 0];
@@ -765,6 +758,7 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         // Friedel mate of the column: array position (FT-x, FT-y), value conj
         const int ym = (FT - y) & (FT - 1);
         const long km = (ym <= (FT - 1) / 2) ? ym : ym - FT;
+        bc.st.slot0 = (r & 3) * NT;
         insert_pair<MG, FOM, MODE>(acc, K, P, valid, mirror, h, k, km, F, merge, keybase, bc);
         if (MODE == MODE_FILL && staged && (r & 3) == 3) stage_flush(bc);
     }
