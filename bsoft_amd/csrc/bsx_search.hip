@@ -707,43 +707,107 @@ __global__ __launch_bounds__(256) void k_rescore(const float2* __restrict__ fp,
     }
 }
 
-// One workgroup per particle: the kernel table is loaded once and each wave
-// takes candidate rotations w, w+4, ...: the section at the disk points is
-// interpolated on the fly into the wave's LDS slice, then the wave forms the
-// FRC with the same fixed-order ring reduction as k_rescore.
+// One workgroup per particle: FRC (the k_rescore formula) of the particle
+// against central sections at ncand arbitrary rotations, each section value
+// interpolated from the map transform with the FSI kernel on the fly.
+// Lanes are (disk point, candidate) pairs: for ncand <= 32 a wavefront takes
+// two points with one candidate per lane, so the lanes of one point gather
+// from the same small neighbourhood of the map and a gather instruction
+// touches few cache lines. Each lane keeps its candidate's ring sums in
+// registers and folds them into LDS when the ring changes; candidates beyond
+// 32 (64) run in further blocks.
+__device__ __forceinline__ void lds_fadd(float* p, float x) {
+    unsigned* u = reinterpret_cast<unsigned*>(p);
+    unsigned old = *reinterpret_cast<volatile unsigned*>(u);
+    while (true) {
+        const unsigned assumed = old;
+        old = atomicCAS(u, assumed, __float_as_uint(__uint_as_float(assumed) + x));
+        if (old == assumed) break;
+    }
+}
+
 template <int WD>
 __global__ __launch_bounds__(256) void k_score_mats(
     const float2* __restrict__ ft, int N, const double* __restrict__ mats, int ncand, double inv,
     double rs, const float* __restrict__ ktab, const int2* __restrict__ hk,
     const int* __restrict__ ring_start, int R, int Nd, const float2* __restrict__ fp,
-    const float* __restrict__ ctf, int slices, float* __restrict__ frc) {
+    const float* __restrict__ ctf, float* __restrict__ frc) {
     __shared__ float K[(FSI_NDIV + 1) * WD];
-    extern __shared__ float2 sv[];            // one Nd slice per working wave
+    __shared__ int rs_l[65];
+    __shared__ float dp[64];
+    __shared__ float anum[64 * 64], adr[64 * 64];   // [candidate in block][ring]
     for (int i = threadIdx.x; i < (FSI_NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
-    __syncthreads();
+    for (int i = threadIdx.x; i <= R; i += blockDim.x) rs_l[i] = ring_start[i];
     const int64_t p = blockIdx.x;
+    const float2* P = fp + p * Nd;
+    const float* Cf = ctf + p * Nd;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (wave >= slices) return;
-    float2* my = sv + (size_t)wave * Nd;
-    for (int c = wave; c < ncand; c += slices) {
-        const double* M = mats + 9 * (p * ncand + c);
-        double m0[3], m1[3];
-        for (int q = 0; q < 3; ++q) { m0[q] = M[3 * q]; m1[q] = M[3 * q + 1]; }
-        for (int i = lane; i < Nd; i += 64) {
-            const double ivx = (double)hk[i].x * inv, ivy = (double)hk[i].y * inv;
-            double m[3];
-            for (int q = 0; q < 3; ++q) m[q] = (ivx * m0[q] + ivy * m1[q] + 0.0 * M[3 * q + 2]) * rs;
-            double vr, vi;
-            fsi_sample<WD>(ft, N, m, K, vr, vi);
-            my[i] = make_float2((float)vr, (float)vi);
+    // particle power per ring
+    for (int j = wave; j < R; j += 4) {
+        float s = 0.f;
+        for (int i = ring_start[j] + lane; i < ring_start[j + 1]; i += 64) {
+            const float2 a = P[i];
+            s += a.x * a.x + a.y * a.y;
         }
-        // the wave's own LDS writes are visible to its lanes after this fence
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        const float f = frc_wave(fp + p * Nd, ctf + p * Nd, my, ring_start, R, lane);
-        if (lane == 0) frc[p * ncand + c] = f;
-        __builtin_amdgcn_wave_barrier();
+        s = wave_sum(s);
+        if (lane == 0) dp[j] = s;
+    }
+    const int cs = ncand <= 32 ? 32 : 64;   // lanes per point
+    const int ppw = 64 / cs;                // points per wavefront step
+    const int slot = lane / cs, cl = lane - slot * cs;
+    for (int cb = 0; cb < ncand; cb += cs) {
+        for (int i = threadIdx.x; i < cs * R; i += blockDim.x) {
+            anum[i] = 0.f;
+            adr[i] = 0.f;
+        }
+        __syncthreads();
+        const int c = cb + cl;
+        const bool act = c < ncand;
+        double m0[3] = {0, 0, 0}, m1[3] = {0, 0, 0};
+        if (act) {
+            const double* M = mats + 9 * (p * ncand + c);
+            for (int q = 0; q < 3; ++q) { m0[q] = M[3 * q]; m1[q] = M[3 * q + 1]; }
+        }
+        int cur = -1, j = 0;
+        float num = 0.f, dr = 0.f;
+        for (int i = wave * ppw + slot; i < Nd; i += 4 * ppw) {
+            while (j < R - 1 && rs_l[j + 1] <= i) ++j;
+            if (j != cur) {
+                if (act && cur >= 0) {
+                    lds_fadd(&anum[cl * R + cur], num);
+                    lds_fadd(&adr[cl * R + cur], dr);
+                }
+                cur = j;
+                num = 0.f;
+                dr = 0.f;
+            }
+            if (act) {
+                const double ivx = (double)hk[i].x * inv, ivy = (double)hk[i].y * inv;
+                double m[3];
+                for (int q = 0; q < 3; ++q) m[q] = (ivx * m0[q] + ivy * m1[q]) * rs;
+                double vr, vi;
+                fsi_sample<WD>(ft, N, m, K, vr, vi);
+                const float2 a = P[i];
+                const float cf = Cf[i];
+                const float rr = (float)vr * cf, ri = (float)vi * cf;
+                num += a.x * rr + a.y * ri;
+                dr += rr * rr + ri * ri;
+            }
+        }
+        if (act && cur >= 0) {
+            lds_fadd(&anum[cl * R + cur], num);
+            lds_fadd(&adr[cl * R + cur], dr);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < cs && cb + t < ncand; t += blockDim.x) {
+            float f = 0.f;
+            for (int jj = 0; jj < R; ++jj) {
+                const float den = dp[jj] * adr[t * R + jj];
+                if (den > 0.f) f += anum[t * R + jj] / sqrtf(den);
+            }
+            frc[p * ncand + cb + t] = f / (float)R;
+        }
+        __syncthreads();
     }
 }
 
@@ -1399,13 +1463,10 @@ int bsx_search_score_mats(bsx_search* h, const float* ft_dev, const double* mats
     const int N = h->p.box;
     const double inv = 1.0 / (N * h->p.sampling), rs = N * h->p.sampling;
     const unsigned g = (unsigned)h->nparts;
-    // LDS slices of Nd values, one per working wave, within 64 KB
-    const int slices = std::max(1, std::min(4, 8192 / h->Nd));
-    const size_t shm = sizeof(float2) * h->Nd * slices;
     switch (h->p.kernel_width) {
-#define BSX_SM(W) case W: hipLaunchKernelGGL((k_score_mats<W>), dim3(g), dim3(256), shm, s, \
+#define BSX_SM(W) case W: hipLaunchKernelGGL((k_score_mats<W>), dim3(g), dim3(256), 0, s, \
         (const float2*)ft_dev, N, mats_dev, ncand, inv, rs, h->d_ktab, h->d_hk, h->d_ring, h->R, \
-        h->Nd, h->d_fp, h->d_ctf, slices, frc_dev); break;
+        h->Nd, h->d_fp, h->d_ctf, frc_dev); break;
         BSX_SM(1) BSX_SM(2) BSX_SM(3) BSX_SM(4) BSX_SM(5)
         BSX_SM(6) BSX_SM(7) BSX_SM(8) BSX_SM(9) BSX_SM(10)
 #undef BSX_SM

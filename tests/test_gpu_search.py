@@ -131,9 +131,11 @@ def test_rescore_and_score_mats_match_reference_frc(setup64):
     torch.cuda.synchronize()
     frc, best, bf, fm = frc.cpu().numpy(), best.cpu().numpy(), bf.cpu().numpy(), fm.cpu().numpy()
     assert frc[3, 5] == -np.inf
-    # on-the-fly sections reproduce the library values exactly
+    # on-the-fly sections use the library's interpolation arithmetic; the FRC
+    # ring sums run in another order (per-lane partial sums), so the two
+    # agree to FP32 summation rounding: 2e-6 absolute on FRC values <= 1
     ok = cand >= 0
-    assert np.array_equal(frc[ok], fm[ok])
+    np.testing.assert_allclose(fm[ok], frc[ok], rtol=0, atol=2e-6)
     o = Oracle(64)
     hk, rs = s.points()
     parts = synth.particles_list(ps.parts)
