@@ -39,6 +39,8 @@ struct alignas(16) PartDev {
     double R[9];        // View::matrix (row-major d[i][j])
     double invscale;    // (1/mag) / (ft * pixel_size)      fspace_pack_2D invscale
     double phx, phy;    // 2 pi ori / ft                      phase_shift_to_origin
+    double rot[4];      // cos, sin of T phy and (T - ft) phy, T = ft / 8: the phase
+                        // steps between the rounds of a column thread
     double c0, c20, c22, c2m2, c40;   // aberration weights   ctf.h:321-335
     double fscale;      // 1 / (ft * pixel_size)              img_ctf_calculate freq_scale
     double ew;          // Ewald offset lambda/2, 0 if off
@@ -529,10 +531,13 @@ __device__ __forceinline__ float sin_red(double a) {
 // Band, taper, phase shift and CTF of one sample (fspace_pack_2D :147-157,
 // phase_shift_to_origin, img_ctf_calculate): returns whether the sample is in
 // the resolution band, its weight and (unless counting) its value.
+// ph: the phase factor exp(i (h phx + k phy)) when the caller tracks it
+// (nullptr: evaluated here).
 template <int MODE>
 __device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev& P, bool active,
                                              long h, long k, float2 F, double* s0, double* s1,
-                                             double* s2, double* w, float* fr, float* fi) {
+                                             double* s2, double* w, float* fr, float* fi,
+                                             const float2* ph = nullptr) {
     *s0 = (double)h * P.invscale;
     *s1 = (double)k * P.invscale;
     *s2 = *s0 * *s0 + *s1 * *s1;
@@ -546,7 +551,12 @@ __device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev&
     if (d < 1) ww *= sqrt(d);
     *w = ww;
     float sn, cs;
-    sincos_red((double)h * P.phx + (double)k * P.phy, &sn, &cs);
+    if (ph) {
+        cs = ph->x;
+        sn = ph->y;
+    } else {
+        sincos_red((double)h * P.phx + (double)k * P.phy, &sn, &cs);
+    }
     float vr = F.x * cs - F.y * sn, vi = F.x * sn + F.y * cs;
     if (K.ctf_mode) {
         const double fx = (double)h * P.fscale, fy = (double)k * P.fscale;
@@ -639,10 +649,11 @@ template <int MG, bool FOM, int MODE>
 __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const ReconConst& K,
                                             const PartDev& P, bool active, bool mirror, long h,
                                             long k, long km, float2 F, bool merge,
-                                            unsigned keybase, const BinCtx& bc) {
+                                            unsigned keybase, const BinCtx& bc, float2 ph) {
     double s0, s1, s2, w;
     float fr, fi;
-    const bool inband = sample_value<MODE>(K, P, active, h, k, F, &s0, &s1, &s2, &w, &fr, &fi);
+    const bool inband =
+        sample_value<MODE>(K, P, active, h, k, F, &s0, &s1, &s2, &w, &fr, &fi, &ph);
     double m[3];
     const double mz0 = P.ew * s2;
     coords(K, P, s0, s1, mz0, m);
@@ -747,6 +758,15 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     const long h = (x <= (FT - 1) / 2) ? x : x - FT;
     const bool mirror = valid && x >= 1 && x <= FT / 2 - 1;
     const bool merge = K.merge;
+    // phase factor of this thread's sample, advanced by exact FP64 rotations
+    // (k grows by T per round and wraps by -FT after round 3, as y passes FT/2)
+    double er, ei;
+    {
+        float sn, cs;
+        sincos_red((double)h * P.phx + (double)t * P.phy, &sn, &cs);
+        er = cs;
+        ei = sn;
+    }
 #pragma unroll 1
     for (int r = 0; r < 8; ++r) {
         const int y = t + r * T;
@@ -759,7 +779,14 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         const int ym = (FT - y) & (FT - 1);
         const long km = (ym <= (FT - 1) / 2) ? ym : ym - FT;
         bc.st.slot0 = (r & 3) * NT;
-        insert_pair<MG, FOM, MODE>(acc, K, P, valid, mirror, h, k, km, F, merge, keybase, bc);
+        insert_pair<MG, FOM, MODE>(acc, K, P, valid, mirror, h, k, km, F, merge, keybase, bc,
+                                   make_float2((float)er, (float)ei));
+        {
+            const double c = r == 3 ? P.rot[2] : P.rot[0], sd = r == 3 ? P.rot[3] : P.rot[1];
+            const double nr = er * c - ei * sd;
+            ei = er * sd + ei * c;
+            er = nr;
+        }
         if (MODE == MODE_FILL && staged && (r & 3) == 3) stage_flush(bc);
     }
 }
@@ -1191,6 +1218,11 @@ static void make_rec(const bsx_recon* h, const bsx_particle& q, PartDev& r) {
     r.invscale = scale / (ft * q.pixel_size);
     r.phx = q.ori[0] / ft * 2 * M_PI;
     r.phy = q.ori[1] / ft * 2 * M_PI;
+    const int T = ft / 8;
+    r.rot[0] = cos(T * r.phy);
+    r.rot[1] = sin(T * r.phy);
+    r.rot[2] = cos((T - ft) * r.phy);
+    r.rot[3] = sin((T - ft) * r.phy);
     double wl = lambda_of(q.volt);
     double t1 = M_PI_2 * wl * wl * wl, t2 = -M_PI * wl, dev = fabs(q.def_dev);
     r.c0 = -q.amp_shift;
