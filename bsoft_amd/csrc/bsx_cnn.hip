@@ -543,10 +543,14 @@ struct bsx_cnn {
     float* d_gact[5] = {};
     float* d_loss = nullptr;
     long step = 0;
-    // MFMA inference: permuted BF16 weights of layers 2-4 and the FC, NHWC
-    // BF16 activations of layers 1-4
+    // MFMA inference: permuted BF16 weights of layers 2-4 and the FC; NHWC
+    // BF16 activations of layers 1-4 and FP32 logits, sized for chunks larger
+    // than the training batch (a 512-particle chunk leaves most of the chip
+    // idle in layers 3-4 and the FC)
     uint16_t* d_wi[5] = {};
-    uint16_t* d_ia[5] = {};
+    int icap = 0;
+    uint16_t* d_ib[5] = {};
+    float* d_ilog = nullptr;
 };
 
 namespace {
@@ -559,8 +563,9 @@ void free_cnn(bsx_cnn* h) {
         if (h->d_act[i]) (void)hipFree(h->d_act[i]);
         if (h->d_gact[i]) (void)hipFree(h->d_gact[i]);
         if (h->d_wi[i]) (void)hipFree(h->d_wi[i]);
-        if (h->d_ia[i]) (void)hipFree(h->d_ia[i]);
+        if (h->d_ib[i]) (void)hipFree(h->d_ib[i]);
     }
+    if (h->d_ilog) (void)hipFree(h->d_ilog);
 }
 
 long act_size(const bsx_cnn* h, int l, int B) {
@@ -709,8 +714,6 @@ int bsx_cnn_create(bsx_cnn** out, int32_t in_size, const int32_t* channels, int3
         const long n = act_size(h, l, max_batch);
         if ((e = hipMalloc(&h->d_act[l], sizeof(float) * n)) != hipSuccess) return fail(e, "activations");
         if ((e = hipMalloc(&h->d_gact[l], sizeof(float) * n)) != hipSuccess) return fail(e, "activations");
-        if (l >= 1 && (e = hipMalloc(&h->d_ia[l], sizeof(uint16_t) * n)) != hipSuccess)
-            return fail(e, "inference activations");
     }
     for (int l = 1; l < 4; ++l)
         if ((e = hipMalloc(&h->d_wi[l], sizeof(uint16_t) * h->ch[l + 1] * h->ch[l] * 9)) != hipSuccess)
@@ -825,8 +828,34 @@ int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int
         h->pb_valid = true;
     }
     const long in = (long)h->S * h->S;
-    for (int64_t b0 = 0; b0 < n; b0 += h->maxb) {
-        const int nb = (int)std::min<int64_t>(h->maxb, n - b0);
+    // MFMA path: chunks of up to 4096 particles in the inference buffers
+    int chunk = h->maxb;
+    if (mfma) {
+        chunk = (int)std::min<int64_t>(std::max<int64_t>(n, 1), 4096);
+        chunk = std::max(chunk, std::min(h->maxb, 4096));
+        if (chunk > h->icap) {
+            for (int l = 1; l < 5; ++l) {
+                if (h->d_ib[l]) (void)hipFree(h->d_ib[l]);
+                h->d_ib[l] = nullptr;
+            }
+            if (h->d_ilog) (void)hipFree(h->d_ilog);
+            h->d_ilog = nullptr;
+            h->icap = 0;
+            for (int l = 1; l < 5; ++l)
+                if (hipMalloc(&h->d_ib[l], sizeof(uint16_t) * act_size(h, l, chunk)) != hipSuccess) {
+                    set_error("bsx_cnn_infer_topk: inference activations (%d particles)", chunk);
+                    return BSX_ERR_NOMEM;
+                }
+            if (hipMalloc(&h->d_ilog, sizeof(float) * (size_t)chunk * h->nclass) != hipSuccess) {
+                set_error("bsx_cnn_infer_topk: logits (%d particles)", chunk);
+                return BSX_ERR_NOMEM;
+            }
+            h->icap = chunk;
+        }
+        chunk = h->icap;
+    }
+    for (int64_t b0 = 0; b0 < n; b0 += chunk) {
+        const int nb = (int)std::min<int64_t>(chunk, n - b0);
         if (!mfma) {
             int rc = forward<uint16_t>(h, x_dev + b0 * in, nb, h->d_pb, s);
             if (rc) return rc;
@@ -834,25 +863,25 @@ int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int
             const uint16_t* P = h->d_pb;
             const long pos1 = (long)nb * in;
             hipLaunchKernelGGL(k_conv1_nhwc, dim3(nblk(pos1)), dim3(256), 0, s, x_dev + b0 * in,
-                               P + h->woff[0], P + h->boff[0], nb, h->S, h->ch[1], h->d_ia[1]);
+                               P + h->woff[0], P + h->boff[0], nb, h->S, h->ch[1], h->d_ib[1]);
             BSX_HIP(hipGetLastError());
             for (int l = 2; l <= 4; ++l) {
                 const long M = (long)nb * h->hw[l] * h->hw[l];
                 dim3 g((unsigned)((M + IM - 1) / IM), (unsigned)((h->ch[l] + IN - 1) / IN));
-                hipLaunchKernelGGL((k_conv_mfma<false>), g, dim3(256), 0, s, h->d_ia[l - 1], nb,
+                hipLaunchKernelGGL((k_conv_mfma<false>), g, dim3(256), 0, s, h->d_ib[l - 1], nb,
                                    h->hw[l - 1], h->hw[l - 1], h->ch[l - 1], 3, 2, h->hw[l], h->hw[l],
                                    h->d_wi[l - 1], P + h->boff[l - 1], h->ch[l], (long)h->ch[l],
-                                   (void*)h->d_ia[l]);
+                                   (void*)h->d_ib[l]);
                 BSX_HIP(hipGetLastError());
             }
             dim3 g((unsigned)((nb + IM - 1) / IM), (unsigned)((h->nclass + IN - 1) / IN));
-            hipLaunchKernelGGL((k_conv_mfma<true>), g, dim3(256), 0, s, h->d_ia[4], nb, 1, 1,
+            hipLaunchKernelGGL((k_conv_mfma<true>), g, dim3(256), 0, s, h->d_ib[4], nb, 1, 1,
                                (int)h->feat, 1, 1, 1, 1, h->d_wi[4], P + h->boff[4], h->nclass,
-                               (long)h->nclass, (void*)h->d_act[0]);
+                               (long)h->nclass, (void*)h->d_ilog);
             BSX_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_row_topk, dim3(nb), dim3(256), 0, s, h->d_act[0], h->nclass, k,
-                           cls_dev + b0 * k, prob_dev + b0 * k);
+        hipLaunchKernelGGL(k_row_topk, dim3(nb), dim3(256), 0, s, mfma ? h->d_ilog : h->d_act[0],
+                           h->nclass, k, cls_dev + b0 * k, prob_dev + b0 * k);
         BSX_HIP(hipGetLastError());
     }
     return BSX_OK;
