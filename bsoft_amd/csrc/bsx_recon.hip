@@ -67,17 +67,27 @@ __global__ __launch_bounds__(256) void k_background(const float* __restrict__ im
     const float* img = imgs + (size_t)recs[p].src * box * box;
     const double ox = recs[p].hox, oy = recs[p].hoy;
     const long hx = box / 2, hy = box / 2;
+    // squared normalised distances per column and per row (the same FP64
+    // operations as a per-pixel evaluation, computed once)
+    __shared__ double cx2[1024], ry2[1024];
+    for (int i = threadIdx.x; i < box; i += blockDim.x) {
+        double x2 = (i - ox) / hx;
+        cx2[i] = x2 * x2;
+        double y2 = (i - oy) / hy;
+        ry2[i] = y2 * y2;
+    }
+    __syncthreads();
     double sum = 0;
     int cnt = 0, bad = 0;
+    int yy = (int)threadIdx.x / box, xx = (int)threadIdx.x - yy * box;
+    const int step_y = (int)blockDim.x / box, step_x = (int)blockDim.x - step_y * box;
     for (int i = threadIdx.x; i < box * box; i += blockDim.x) {
-        const int yy = i / box, xx = i - yy * box;
         const float v = img[i];
         if (!isfinite(v)) bad = 1;
-        double y2 = (yy - oy) / hy;
-        y2 *= y2;
-        double x2 = (xx - ox) / hx;
-        x2 *= x2;
-        if (x2 + y2 - 1 > 0) { sum += v; ++cnt; }
+        if (cx2[xx] + ry2[yy] - 1 > 0) { sum += v; ++cnt; }
+        xx += step_x;
+        yy += step_y;
+        if (xx >= box) { xx -= box; ++yy; }
     }
     __shared__ double ssum[256];
     __shared__ int scnt[256], sbad[256];

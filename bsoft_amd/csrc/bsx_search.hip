@@ -726,9 +726,72 @@ __device__ __forceinline__ void lds_fadd(float* p, float x) {
     }
 }
 
+// Copy of the map transform around the origin: cube[(c S + b) S + a] =
+// ft[wrap(c - C)][wrap(b - C)][wrap(a - C)], C = S / 2, with the periodic
+// wrap of fsi_sample, so a sample whose taps all lie within |i| <= C - 1 reads
+// the same voxels from the cube without wrapping.
+__global__ void k_center_cube(const float2* __restrict__ ft, int N, int S,
+                              float2* __restrict__ cube) {
+    const int64_t n = (int64_t)S * S * S;
+    const int C = S / 2;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(i % S), b = (int)((i / S) % S), c = (int)(i / ((int64_t)S * S));
+        const int x = wrap_index(a - C, N), y = wrap_index(b - C, N), z = wrap_index(c - C, N);
+        cube[i] = ft[((int64_t)z * N + y) * N + x];
+    }
+}
+
+// fsi_sample on the centred cube: the same taps, weights and summation order,
+// with indices offset by C instead of wrapped. Returns false (nothing read)
+// when a tap would leave the cube, e.g. for a rotation matrix that is not
+// orthonormal; the caller then samples the full map.
+template <int WD>
+__device__ inline bool fsi_sample_cube(const float2* __restrict__ cube, int S, const double m[3],
+                                       const float* K, double& vr, double& vi) {
+    constexpr int HW = (WD - 1) / 2;
+    const int C = S / 2;
+    const double fx = floor(m[0]), fy = floor(m[1]), fz = floor(m[2]);
+    const int ikx = ((int)((m[0] - fx) * FSI_NDIV + 0.5)) * WD;
+    const int iky = ((int)((m[1] - fy) * FSI_NDIV + 0.5)) * WD;
+    const int ikz = ((int)((m[2] - fz) * FSI_NDIV + 0.5)) * WD;
+    const double lo = (double)(HW - C), hi = (double)(S - WD + HW - C);
+    if (!(fx >= lo && fx <= hi && fy >= lo && fy <= hi && fz >= lo && fz <= hi)) return false;
+    const int ix = (int)fx - HW + C, iy = (int)fy - HW + C, iz = (int)fz - HW + C;
+    float kx[WD], ky[WD];
+#pragma unroll
+    for (int q = 0; q < WD; ++q) {
+        kx[q] = K[ikx + q];
+        ky[q] = K[iky + q];
+    }
+    vr = 0;
+    vi = 0;
+#pragma unroll 1
+    for (int zz = 0; zz < WD; ++zz) {
+        const float kzz = K[ikz + zz];
+        float pr = 0.f, pi = 0.f;
+#pragma unroll
+        for (int yy2 = 0; yy2 < WD; ++yy2) {
+            const float w = ky[yy2] * kzz;
+            const float2* row = cube + ((int64_t)(iz + zz) * S + (iy + yy2)) * S + ix;
+#pragma unroll
+            for (int q = 0; q < WD; ++q) {
+                const float kw = kx[q] * w;
+                const float2 a = row[q];
+                pr = fmaf(a.x, kw, pr);
+                pi = fmaf(a.y, kw, pi);
+            }
+        }
+        vr += pr;
+        vi += pi;
+    }
+    return true;
+}
+
 template <int WD>
 __global__ __launch_bounds__(256) void k_score_mats(
-    const float2* __restrict__ ft, int N, const double* __restrict__ mats, int ncand, double inv,
+    const float2* __restrict__ ft, int N, const float2* __restrict__ cube, int S,
+    const double* __restrict__ mats, int ncand, double inv,
     double rs, const float* __restrict__ ktab, const int2* __restrict__ hk,
     const int* __restrict__ ring_start, int R, int Nd, const float2* __restrict__ fp,
     const float* __restrict__ ctf, float* __restrict__ frc) {
@@ -786,7 +849,8 @@ __global__ __launch_bounds__(256) void k_score_mats(
                 double m[3];
                 for (int q = 0; q < 3; ++q) m[q] = (ivx * m0[q] + ivy * m1[q]) * rs;
                 double vr, vi;
-                fsi_sample<WD>(ft, N, m, K, vr, vi);
+                if (!cube || !fsi_sample_cube<WD>(cube, S, m, K, vr, vi))
+                    fsi_sample<WD>(ft, N, m, K, vr, vi);
                 const float2 a = P[i];
                 const float cf = Cf[i];
                 const float rr = (float)vr * cf, ri = (float)vi * cf;
@@ -948,6 +1012,8 @@ struct bsx_search {
     // scratch
     float2* d_spec = nullptr;
     int64_t spec_cap = 0;
+    float2* d_cube = nullptr;      // centred low-frequency cube of the map (refinement)
+    int64_t cube_cap = 0;
     SPart* d_sp = nullptr;
     int64_t sp_cap = 0;
     double* d_libmats = nullptr;     // nviews x 9, kept for the chosen views
@@ -1125,7 +1191,7 @@ int bsx_search_destroy(bsx_search* h) {
                     h->d_ctf,  h->d_pbf,     h->d_spec,   h->d_sp,
                     h->d_ktab, h->d_pscore,  h->d_pidx,   h->d_libmats, h->d_lmats,
                     h->d_lfrc, h->d_sub,     h->d_members, h->d_mcount, h->d_counts,
-                    h->d_cursors, h->d_pairs, h->d_items, h->d_zrow};
+                    h->d_cursors, h->d_pairs, h->d_items, h->d_zrow, h->d_cube};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     delete h;
@@ -1463,10 +1529,22 @@ int bsx_search_score_mats(bsx_search* h, const float* ft_dev, const double* mats
     const int N = h->p.box;
     const double inv = 1.0 / (N * h->p.sampling), rs = N * h->p.sampling;
     const unsigned g = (unsigned)h->nparts;
+    // sections at the disk points only reach |coordinate| < R: gather from a
+    // centred cube of the map around the origin when it is smaller than the map
+    const int S = 2 * (h->R + h->p.kernel_width + 2);
+    const float2* cube = nullptr;
+    if (S < N) {
+        int rc = grow(h, &h->d_cube, &h->cube_cap, (int64_t)S * S * S, "centred cube");
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_center_cube, dim3(1024), dim3(256), 0, s, (const float2*)ft_dev, N,
+                           S, h->d_cube);
+        BSX_HIP(hipGetLastError());
+        cube = h->d_cube;
+    }
     switch (h->p.kernel_width) {
 #define BSX_SM(W) case W: hipLaunchKernelGGL((k_score_mats<W>), dim3(g), dim3(256), 0, s, \
-        (const float2*)ft_dev, N, mats_dev, ncand, inv, rs, h->d_ktab, h->d_hk, h->d_ring, h->R, \
-        h->Nd, h->d_fp, h->d_ctf, frc_dev); break;
+        (const float2*)ft_dev, N, cube, S, mats_dev, ncand, inv, rs, h->d_ktab, h->d_hk, h->d_ring, \
+        h->R, h->Nd, h->d_fp, h->d_ctf, frc_dev); break;
         BSX_SM(1) BSX_SM(2) BSX_SM(3) BSX_SM(4) BSX_SM(5)
         BSX_SM(6) BSX_SM(7) BSX_SM(8) BSX_SM(9) BSX_SM(10)
 #undef BSX_SM
