@@ -120,6 +120,9 @@ SIGNATURES = {
     "bsx_recon_autotune": ([_vp, _vp, C.POINTER(Particle), C.c_int64, C.POINTER(C.c_int32), _dp,
                             _vp], C.c_int),
     "bsx_recon_graph_launches": ([_vp, C.POINTER(C.c_int64)], C.c_int),
+    "bsx_recon_set_symmetry": ([_vp, C.c_char_p], C.c_int),
+    "bsx_symmetry_views": ([C.c_char_p, _dp, C.c_int64, _dp, C.c_int64, C.POINTER(C.c_int64)],
+                           C.c_int),
     "bsx_recon_kernel_times": ([_vp, _dp, C.c_int], C.c_int),
     "bsx_fft": ([_vp, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, _vp], C.c_int),
     "bsx_map_transform": ([_vp, C.c_int, _dp, _vp, _vp], C.c_int),

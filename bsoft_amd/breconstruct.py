@@ -12,6 +12,8 @@ Options (a subset of breconstruct's, same meaning):
   -pad n                 padding factor of the particle transforms (default 2)
   -CTF flip|apply|wiener CTF treatment; -wiener w (default 0.2 for wiener)
   -interpolate nearest|weighted|trilinear
+  -symmetry label        point group (C<n>, D<n>, T, O, I, ...; default C1):
+                         each particle is inserted at all its related views
   -bootstrap             weight particles by their selection number
   -halfmaps              also write the two half maps (<name>_1, <name>_2)
   -fom                   also write the FOM map (<name>_fom)
@@ -36,6 +38,7 @@ def parse_args(argv):
     ap.add_argument("-wiener", type=float, default=0.0)
     ap.add_argument("-interpolate", default="nearest", choices=["nearest", "weighted", "trilinear"])
     ap.add_argument("-bootstrap", action="store_true")
+    ap.add_argument("-symmetry", default="C1")
     ap.add_argument("-halfmaps", action="store_true")
     ap.add_argument("-fom", action="store_true")
     ap.add_argument("project")
@@ -80,6 +83,7 @@ def main(argv=None) -> int:
     rec = Reconstruction(size, box, ft, sampling, hi_res=a.resolution, interp=interp,
                          ctf_action=ctf, wiener=a.wiener, flags=flags,
                          nhalves=2 if a.halfmaps else 1)
+    rec.set_symmetry(a.symmetry)
     rec.insert(torch.as_tensor(imgs).cuda().contiguous(), parts)
     stem, ext = os.path.splitext(a.reconstruction)
     outs = [(0, 3 if a.halfmaps else 1, a.reconstruction)]

@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <cstring>
 #include <vector>
+#include <cctype>
+#include <string>
 
 #include "bsx_fft_dev.h"
 #include "bsx_common.h"
@@ -1194,6 +1196,8 @@ struct bsx_recon {
     int64_t graph_launches = 0;
     hipStream_t cap_stream = nullptr;   // capture happens here (the caller's stream may be
                                         // the legacy default stream, which cannot capture)
+    std::vector<int> sym_order;         // point-group operator orders (empty: C1)
+    std::vector<double> sym_axis;       // their unit axes, 3 per operator
 };
 
 // KT_FALLBACK holds the atomic-fallback flags of the insert, not a time
@@ -1249,6 +1253,165 @@ static void make_rec(const bsx_recon* h, const bsx_particle& q, PartDev& r) {
     if (pad_ratio > 1) { pad_ratio *= pad_ratio * 0.6; w /= pad_ratio; }
     r.wp = (float)w;
     r.half = h->p.nhalves > 1 ? (q.half & 1) : 0;
+}
+
+// ------------------------------------------------------------- symmetry
+// Point-group operators (axis, order) of Bsymmetry::Bsymmetry
+// (src/util/symmetry.cpp:42-122, labels cleaned as Bsymmetry::clean_symstring,
+// include/symmetry.h:73-99); helical symmetry is not a point group and is
+// rejected.
+struct SymOp {
+    double ax[3];
+    int order;
+};
+
+static int parse_symmetry(const char* label, std::vector<SymOp>& ops) {
+    ops.clear();
+    std::string l;
+    for (const char* c = label ? label : ""; *c; ++c)
+        if (!isspace((unsigned char)*c)) l += *c;
+    if (l.empty()) l = "C1";
+    if (isdigit((unsigned char)l[0])) {
+        const int j = atoi(l.c_str());
+        if (l.find("532") != std::string::npos && l.find("90") != std::string::npos) l = "I90";
+        else if (l.find("532") != std::string::npos) l = "I";
+        else if (l.find("432") != std::string::npos) l = "O";
+        else if (l.find("23") != std::string::npos) l = "T";
+        else if (j > 0) l = "C" + std::to_string(j);
+        else l = "C1";
+    }
+    l[0] = (char)toupper((unsigned char)l[0]);
+    auto has = [&](const char* t) { return l.find(t) != std::string::npos; };
+    auto add = [&](double x, double y, double z, int order) {
+        const double n = sqrt(x * x + y * y + z * z);
+        ops.push_back(SymOp{{x / n, y / n, z / n}, order});
+    };
+    const double G = 1.61803398874989484820458683436563811772;   // golden ratio
+    if (l[0] == 'C') {
+        const int n = (l == "Cs" || l == "CS") ? 1 : atoi(l.c_str() + 1);
+        BSX_REQUIRE(n >= 1 && n <= 1000, BSX_ERR_ARG, "symmetry %s: bad order", l.c_str());
+        add(0, 0, 1, n);
+    } else if (l[0] == 'D') {
+        const int n = atoi(l.c_str() + 1);
+        BSX_REQUIRE(n >= 1 && n <= 1000, BSX_ERR_ARG, "symmetry %s: bad order", l.c_str());
+        add(0, 0, 1, n);
+        add(1, 0, 0, 2);
+    } else if (l[0] == 'T') {
+        if (!has("-3")) add(1, 1, 1, 3);
+        add(0, 0, 1, 2);
+        add(1, 0, 0, 2);
+    } else if (l[0] == 'O') {
+        if (!has("-3")) add(1, 1, 1, 3);
+        if (has("-4")) {
+            add(1, -1, 0, 2);
+        } else if (has("-2")) {
+            add(0, 0, 1, 4);
+        } else {
+            add(0, 0, 1, 4);
+            add(1, 0, 0, 2);
+        }
+    } else if (l[0] == 'I') {
+        if (!has("-3")) add(1, 1, 1, 3);
+        if (has("-5")) add(1, 0, 0, 2);
+        else if (has("-2")) add(1, -1, 0, 2);
+        else add(1, 1.0 / G, G, 2);
+        if (!has("-5")) add(1.0 / G, 1, 0, 5);
+        if (!has("-2")) add(0, 0, 1, 2);
+        if (has("90"))   // axes rotated by 90 degrees about z
+            for (SymOp& o : ops) {
+                const double x = o.ax[0], y = o.ax[1];
+                o.ax[0] = -y;
+                o.ax[1] = x;
+            }
+    } else {
+        set_error("symmetry %s: not a point group (C, D, T, O, I)", l.c_str());
+        return BSX_ERR_ARG;
+    }
+    return BSX_OK;
+}
+
+// View::normalize / check (include/View.h:32-36, 110-116): unit view vector,
+// tiny components zeroed, angle in [-pi, pi].
+static void view_normalize(double v[4]) {
+    const double size = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (size < 1e-30) { v[0] = v[1] = 0; v[2] = 1; }
+    else { v[0] /= size; v[1] /= size; v[2] /= size; }
+    for (int i = 0; i < 3; ++i)
+        if (fabs(v[i]) < 1e-30) v[i] = 0;
+    v[3] = remainder(v[3], 2 * M_PI);
+    if (fabs(v[3]) < 1e-30) v[3] = 0;
+}
+
+// Quaternion::check (include/Quaternion.h:28-32)
+static void quat_check(double q[4]) {
+    for (int i = 0; i < 4; ++i)
+        if (fabs(q[i]) < 1e-30) q[i] = 0;
+}
+
+// View::quaternion (include/View.h:131-153)
+static void view_quaternion(double v[4], double q[4]) {
+    view_normalize(v);
+    const double ang = v[3] / 2.0L, ca = cos(ang), sa = sin(ang);
+    const double z1 = v[2] + 1;
+    q[0] = ca; q[1] = 0; q[2] = 0; q[3] = sa;
+    if (v[2] < 1.0L - 1e-30) {
+        if (v[2] <= 1e-30 - 1.0L) {
+            q[0] = 0; q[1] = sa; q[2] = ca; q[3] = 0;
+        } else {
+            const double f1 = sqrt(z1 / 2.0), f2 = sqrt(1.0 / (2 * z1));
+            q[0] = f1 * ca;
+            q[1] = f2 * (v[0] * sa - v[1] * ca);
+            q[2] = f2 * (v[0] * ca + v[1] * sa);
+            q[3] = f1 * sa;
+        }
+    }
+    quat_check(q);
+}
+
+// View(Quaternion&) (include/View.h:52-63)
+static void quaternion_view(const double q[4], double v[4]) {
+    double ang = atan2(q[3], q[0]);
+    const double ca = cos(ang), sa = sin(ang);
+    const double q2 = q[0] * q[0] + q[3] * q[3];
+    const double f2 = 2 * sqrt(q2);
+    if (q2 < 1e-30) ang = atan2(q[1], q[2]);
+    v[0] = f2 * (q[2] * ca + q[1] * sa);
+    v[1] = f2 * (q[2] * sa - q[1] * ca);
+    v[2] = 2 * q2 - 1;
+    v[3] = 2 * ang;
+    view_normalize(v);
+}
+
+// symmetry_get_all_views (src/util/symmetry.cpp:277-319): the view first,
+// then for each operator and each multiple j of its angle the rotated copies
+// of all views so far (Quaternion(axis, angle) * view quaternion).
+static void symmetry_views(const std::vector<SymOp>& ops, const double view[4],
+                           std::vector<double>& out) {
+    out.assign(view, view + 4);
+    view_normalize(out.data());
+    size_t nview = 1;
+    for (const SymOp& o : ops) {
+        for (int j = 1; j < o.order; ++j) {
+            const double angle = j * (2 * M_PI) * 1.0L / o.order;
+            const double a = angle / 2.0L;
+            double q[4] = {cos(a), o.ax[0] * sin(a), o.ax[1] * sin(a), o.ax[2] * sin(a)};
+            quat_check(q);
+            for (size_t k = 0; k < nview; ++k) {
+                double v[4] = {out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]};
+                double p[4];
+                view_quaternion(v, p);
+                const double r[4] = {
+                    q[0] * p[0] - (q[1] * p[1] + q[2] * p[2] + q[3] * p[3]),
+                    q[0] * p[1] + q[1] * p[0] + q[2] * p[3] - q[3] * p[2],
+                    q[0] * p[2] + q[2] * p[0] + q[3] * p[1] - q[1] * p[3],
+                    q[0] * p[3] + q[3] * p[0] + q[1] * p[2] - q[2] * p[1]};
+                double nv[4];
+                quaternion_view(r, nv);
+                out.insert(out.end(), nv, nv + 4);
+            }
+        }
+        nview *= (size_t)o.order;
+    }
 }
 
 template <int FT>
@@ -1539,6 +1702,29 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
         BSX_REQUIRE(h->p.nhalves == 1 || parts[i].half == 0 || parts[i].half == 1, BSX_ERR_ARG,
                     "particle %lld: half must be 0 or 1", (long long)i);
     }
+    // point-group symmetry: one record per symmetry-related view, all reading
+    // the particle's image (fspace_pack_2D over symmetry_get_all_views)
+    const int64_t n_parts = n;
+    std::vector<bsx_particle> expanded;
+    std::vector<int64_t> image_of;
+    if (!h->sym_order.empty()) {
+        std::vector<SymOp> ops;
+        for (size_t o = 0; o < h->sym_order.size(); ++o)
+            ops.push_back(SymOp{{h->sym_axis[3 * o], h->sym_axis[3 * o + 1], h->sym_axis[3 * o + 2]},
+                                h->sym_order[o]});
+        std::vector<double> sv;
+        for (int64_t i = 0; i < n_parts; ++i) {
+            symmetry_views(ops, parts[i].view, sv);
+            for (size_t k = 0; k < sv.size() / 4; ++k) {
+                bsx_particle q = parts[i];
+                for (int c = 0; c < 4; ++c) q.view[c] = sv[4 * k + c];
+                expanded.push_back(q);
+                image_of.push_back(i);
+            }
+        }
+        parts = expanded.data();
+        n = (int64_t)expanded.size();
+    }
     if (n > h->recs_cap) {
         if (h->recs) (void)hipFree(h->recs);
         if (h->bkg) (void)hipFree(h->bkg);
@@ -1596,7 +1782,7 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
     double max_step = 0;
     for (int64_t i = 0; i < n; ++i) {
         make_rec(h, parts[perm[i]], h->hrecs[i]);
-        h->hrecs[i].src = perm[i];
+        h->hrecs[i].src = image_of.empty() ? perm[i] : image_of[perm[i]];
         max_step = std::max(max_step, h->hrecs[i].invscale * h->K.mscale);
     }
     // run merge: equal voxels are contiguous within a merge group of up to 16
@@ -1675,7 +1861,9 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
             h->g_merge = h->K.merge;
         }
     }
-    for (int64_t i = 0; i < n; ++i) h->inserted[h->p.nhalves > 1 ? (parts[i].half & 1) : 0]++;
+    for (int64_t i = 0; i < n; ++i)
+        if (image_of.empty() || i == 0 || image_of[i] != image_of[i - 1])
+            h->inserted[h->p.nhalves > 1 ? (parts[i].half & 1) : 0]++;
     h->last_launches = nbatches;
     if (h->timing) {
         BSX_HIP(hipStreamSynchronize(s));
@@ -1743,6 +1931,52 @@ int bsx_recon_autotune(bsx_recon* h, const float* imgs_dev, const bsx_particle* 
     if (chosen_batch) *chosen_batch = best;
     if (best_ms) *best_ms = bms;
     return r2;
+}
+
+int bsx_recon_set_symmetry(bsx_recon* h, const char* sym) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_recon_set_symmetry: null handle");
+    std::vector<SymOp> ops;
+    const int rc = parse_symmetry(sym, ops);
+    if (rc) return rc;
+    h->sym_order.clear();
+    h->sym_axis.clear();
+    size_t total = 1;
+    for (const SymOp& o : ops) total *= (size_t)o.order;
+    if (total <= 1) return BSX_OK;   // C1
+    for (const SymOp& o : ops) {
+        h->sym_order.push_back(o.order);
+        h->sym_axis.insert(h->sym_axis.end(), o.ax, o.ax + 3);
+    }
+    // the graph of an earlier insert holds the record count of that insert
+    h->g_seen = 0;
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+    h->g_n = -1;
+    return BSX_OK;
+}
+
+int bsx_symmetry_views(const char* sym, const double* views, int64_t n, double* out, int64_t cap,
+                       int64_t* count) {
+    BSX_REQUIRE(count && (n == 0 || views) && n >= 0, BSX_ERR_ARG,
+                "bsx_symmetry_views: bad arguments");
+    std::vector<SymOp> ops;
+    const int rc = parse_symmetry(sym, ops);
+    if (rc) return rc;
+    int64_t order = 1;
+    for (const SymOp& o : ops) order *= o.order;
+    *count = n * order;
+    if (!out) return BSX_OK;
+    BSX_REQUIRE(cap >= n * order, BSX_ERR_ARG, "bsx_symmetry_views: room for %lld views, need %lld",
+                (long long)cap, (long long)(n * order));
+    std::vector<double> sv;
+    for (int64_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 4; ++c)
+            BSX_REQUIRE(std::isfinite(views[4 * i + c]), BSX_ERR_ARG,
+                        "bsx_symmetry_views: view %lld is not finite", (long long)i);
+        symmetry_views(ops, views + 4 * i, sv);
+        std::copy(sv.begin(), sv.end(), out + 4 * i * order);
+    }
+    return BSX_OK;
 }
 
 int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n) {

@@ -120,6 +120,11 @@ class Reconstruction:
         self._batch = b.value
         return b.value
 
+    def set_symmetry(self, sym: str | None):
+        """Point-group symmetry of later inserts (fspace_pack_2D with a
+        Bsymmetry): each particle is added at all its symmetry-related views."""
+        call("bsx_recon_set_symmetry", self.h, (sym or "C1").encode())
+
     def batch(self) -> int:
         return getattr(self, "_batch", self.params.batch)
 
@@ -180,6 +185,19 @@ def particle_reconstruct(imgs: torch.Tensor, parts: np.ndarray, size: int, sampl
                          batch=batch)
     rec.insert(imgs, parts)
     return rec
+
+
+def symmetry_views(sym: str, views: np.ndarray) -> np.ndarray:
+    """symmetry_get_all_views (src/util/symmetry.cpp:277-319) of each
+    (x, y, z, angle) row: (n, order, 4), the view itself first."""
+    v = np.ascontiguousarray(views, dtype=np.float64).reshape(-1, 4)
+    cnt = C.c_int64()
+    call("bsx_symmetry_views", sym.encode(), v.ctypes.data_as(C.POINTER(C.c_double)), len(v),
+         None, 0, C.byref(cnt))
+    out = np.zeros((max(cnt.value, 0), 4))
+    call("bsx_symmetry_views", sym.encode(), v.ctypes.data_as(C.POINTER(C.c_double)), len(v),
+         out.ctypes.data_as(C.POINTER(C.c_double)), len(out), C.byref(cnt))
+    return out.reshape(len(v), -1, 4)
 
 
 def img_reconstruction_sum_weigh(rec: Reconstruction, imap: int = 0, nmaps: int = 1,

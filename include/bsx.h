@@ -149,9 +149,6 @@ int bsx_recon_accumulators(bsx_recon* h, int half, float* planes_dev, void* stre
 /* Device bytes held by a handle. */
 int bsx_recon_device_bytes(const bsx_recon* h, int64_t* bytes);
 
-/* Kernel timings of the previous bsx_recon_insert (needs BSX_FLAG_TIMING):
- * insertion kernels (count, scan, fill, tile accumulate or direct), staging
- * kernels (background, row transforms) in ms, and the number of batches. */
 /* Batch-size autotuner (no counterpart in the reference): times the insertion
  * of n sample particles at the allocated batch and its halvings (down to 1/8,
  * at least 32) and keeps the fastest for later inserts. Requires an empty
@@ -161,8 +158,19 @@ int bsx_recon_autotune(bsx_recon* h, const float* imgs_dev, const bsx_particle* 
 /* Number of inserts served by graph replay so far (BSX_FLAG_GRAPH). */
 int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n);
 
+/* Kernel timings of the previous bsx_recon_insert (needs BSX_FLAG_TIMING):
+ * insertion kernels (scan, fill, tile accumulate or direct) and staging
+ * kernels (background, row transforms) in ms, and the number of batches. */
 int bsx_recon_last_timing(const bsx_recon* h, double* insert_ms, int64_t* launches,
                           double* stage_ms);
+
+/* Point-group symmetry of the insertion (fspace_pack_2D with a Bsymmetry,
+ * Bimage_reconstruct.cpp:248-266): every later insert adds each particle at
+ * all its symmetry-related views (symmetry_get_all_views,
+ * src/util/symmetry.cpp:277-319). sym: "C1", "C<n>", "D<n>", "T", "O", "I",
+ * with the reference's variants T-3, O-3, O-2, O-4, I-3, I-2, I-5, I90
+ * (Bsymmetry::Bsymmetry, symmetry.cpp:42-122); NULL or "C1" switches it off. */
+int bsx_recon_set_symmetry(bsx_recon* h, const char* sym);
 /* Per-kernel totals of the previous insert (ms): background, rows, count,
  * scan, fill, tile accumulate, direct insertion; an 8th value receives the
  * number of bin entries written (n <= 8 values). */
@@ -370,6 +378,12 @@ int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int
  * (angle_step 0: none). views: cap x 4 (x, y, z, angle); *count = total. */
 int bsx_views_c1(double theta_step, double phi_step, int32_t flag, double angle_step,
                  double* views, int64_t cap, int64_t* count);
+
+/* symmetry_get_all_views (src/util/symmetry.cpp:277-319) for each of n views
+ * (x, y, z, angle): out holds, view after view, the order(sym) related views
+ * (the view itself first); cap = room in out (views); *count = n * order. */
+int bsx_symmetry_views(const char* sym, const double* views, int64_t n, double* out,
+                       int64_t cap, int64_t* count);
 
 #ifdef __cplusplus
 }

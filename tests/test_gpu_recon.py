@@ -14,7 +14,7 @@ import torch
 
 from bsoft_amd import abi, synth
 from bsoft_amd.abi import BsxError, ReconParams
-from bsoft_amd.recon import Reconstruction, part_ft_size
+from bsoft_amd.recon import Reconstruction, part_ft_size, symmetry_views
 from oracle.oracle import Oracle, OracleRecon
 
 pytestmark = pytest.mark.gpu
@@ -279,3 +279,31 @@ def test_breconstruct_cli_on_star_project(tmp_path):
     assert np.abs(m.data[0] - ref).max() <= 1e-5 * np.abs(ref).max()
     h1 = formats.read_mrc(str(tmp_path / "map_1.mrc")).data[0]
     assert np.abs(h1 - rec.finish(1, 3).map.cpu().numpy()).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("sym", ["C3", "D2", "O"])
+def test_symmetry_equals_expanded_particles(sym):
+    """Point-group symmetry (fspace_pack_2D over symmetry_get_all_views) adds
+    each particle at all its related views: the same sums as inserting the
+    explicitly expanded particle list, up to FP32 summation order; the
+    particle count stays that of the input."""
+    say(f"symmetry {sym}")
+    n = 12
+    bm, ps = dataset(64, 10, n, 64, seed=12)
+    d = ps.images.cuda()
+    r = Reconstruction(64, 64, 128, 1.5, ctf_action=1)
+    r.set_symmetry(sym)
+    r.insert(d, ps.parts)
+    sv = symmetry_views(sym, ps.parts["view"])
+    order = sv.shape[1]
+    ex = np.repeat(ps.parts, order)
+    ex["view"] = sv.reshape(-1, 4)
+    r2 = Reconstruction(64, 64, 128, 1.5, ctf_action=1)
+    r2.insert(d.repeat_interleave(order, dim=0).contiguous(), ex)
+    a, b = r.accumulators(0), r2.accumulators(0)
+    for q in range(3):
+        e = ((a[q] - b[q]).abs().max() / b[q].abs().max()).item()
+        assert e < 2e-5, (sym, q, e)
+    g = r.finish()
+    assert g.particles == n
+    assert g.weight_sum == pytest.approx(r2.finish().weight_sum, rel=1e-6)
