@@ -125,6 +125,51 @@ __global__ __launch_bounds__(256) void k_fsc(const float2* __restrict__ a,
         if (sh[i] != 0) atomicAdd(&sums[i], sh[i]);
 }
 
+// FP64 shell sums of Bimage::fsc_dpr (Bimage_resolution.cpp:93-181) for two
+// real maps transformed separately (the same F1, F2 as unpacking the packed
+// transform): over x < N/2, linear interpolation between shells i and i+1
+// (both must exist), sums[c * maxrad + i] for c = |F1|^2, |F2|^2,
+// Re F1 conj F2, |F1|^2 + |F2|^2, (|F1|^2 + |F2|^2) dphi^2.
+__global__ __launch_bounds__(256) void k_fsc_dpr(const float2* __restrict__ a,
+                                                 const float2* __restrict__ b, int N,
+                                                 double inv_ratio, long maxrad,
+                                                 double* __restrict__ sums) {
+    extern __shared__ double sh[];   // 5 * maxrad
+    for (long i = threadIdx.x; i < 5 * maxrad; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    const int hx = N / 2;
+    const long long nv = (long long)hx * N * N;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long xx = t % hx, yz = t / hx, yy = yz % N, zz = yz / N;
+        const double h = xx > (N - 1) / 2 ? (double)(xx - N) : (double)xx;
+        const double k = yy > (N - 1) / 2 ? (double)(yy - N) : (double)yy;
+        const double l = zz > (N - 1) / 2 ? (double)(zz - N) : (double)zz;
+        const double radius = inv_ratio * sqrt(h * h + k * k + l * l);
+        const long ir = (long)radius;
+        if (ir + 1 >= maxrad) continue;
+        const double f = radius - ir, f1 = 1 - f;
+        const long long i = (zz * N + yy) * N + xx;
+        const float2 u = a[i], v = b[i];
+        const double I1 = (double)u.x * u.x + (double)u.y * u.y;
+        const double I2 = (double)v.x * v.x + (double)v.y * v.y;
+        double dphi = atan2((double)u.y, (double)u.x) - atan2((double)v.y, (double)v.x);
+        dphi = remainder(dphi, 2 * M_PI);
+        dphi *= dphi;
+        const double re = (double)u.x * v.x + (double)u.y * v.y;
+        const double is = I1 + I2;
+        const double val[5] = {I1, I2, re, is, is * dphi};
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            atomicAdd(&sh[c * maxrad + ir], f1 * val[c]);
+            atomicAdd(&sh[c * maxrad + ir + 1], f * val[c]);
+        }
+    }
+    __syncthreads();
+    for (long i = threadIdx.x; i < 5 * maxrad; i += blockDim.x)
+        if (sh[i] != 0) atomicAdd(&sums[i], sh[i]);
+}
+
 // ------------------------------------------------------------ FRC scoring
 // One workgroup per (particle, reference) pair. Ring sums in LDS (FP64).
 __global__ __launch_bounds__(256) void k_frc_pairs(const float2* __restrict__ part,
@@ -278,6 +323,71 @@ int bsx_fsc(const float* a_dev, const float* b_dev, int n, double sampling, doub
     }
     if (maxrad > 0) fsc[0] = 1;
     *nshell = (int)maxrad;
+    return BSX_OK;
+}
+
+int bsx_fsc_dpr(const float* map1_dev, const float* map2_dev, int n, double sampling,
+                double hi_res, double sampling_ratio, double* curves, int* nrows, void* stream) {
+    BSX_REQUIRE(map1_dev && map2_dev && curves && nrows, BSX_ERR_ARG, "bsx_fsc_dpr: null argument");
+    BSX_REQUIRE(n >= 2 && n % 2 == 0 && sampling > 0, BSX_ERR_ARG, "bsx_fsc_dpr: bad size/sampling");
+    if (sampling_ratio <= 0) sampling_ratio = 1;
+    hipStream_t s = (hipStream_t)stream;
+    hi_res = check_res(hi_res, n, sampling, 3);
+    const double rad_scale = n * sampling / sampling_ratio;
+    const long maxrad = (long)(2 + rad_scale / hi_res);
+    BSX_REQUIRE(maxrad <= *nrows, BSX_ERR_ARG, "bsx_fsc_dpr: %ld rows exceed capacity %d", maxrad,
+                *nrows);
+    const long long n3 = (long long)n * n * n;
+    float2* v = nullptr;
+    double* d = nullptr;
+    BSX_HIP(hipMalloc(&v, sizeof(float2) * 2 * n3));
+    if (hipMalloc(&d, sizeof(double) * 5 * maxrad) != hipSuccess) {
+        (void)hipFree(v);
+        set_error("bsx_fsc_dpr: allocation failed");
+        return BSX_ERR_NOMEM;
+    }
+    int r = BSX_OK;
+    for (int m = 0; m < 2 && r == BSX_OK; ++m) {
+        hipLaunchKernelGGL(k_real_to_complex, dim3(2048), dim3(256), 0, s, m ? map2_dev : map1_dev,
+                           v + m * n3, n3);
+        for (int axis = 0; axis < 3 && r == BSX_OK; ++axis)
+            r = fft_axis_ex(v + m * n3, nullptr, n, n, n, 1, axis, -1, 1.0f, s);
+    }
+    if (r == BSX_OK) {
+        (void)hipMemsetAsync(d, 0, sizeof(double) * 5 * maxrad, s);
+        hipLaunchKernelGGL(k_fsc_dpr, dim3(1024), dim3(256), sizeof(double) * 5 * maxrad, s, v,
+                           v + n3, n, 1.0 / sampling_ratio, maxrad, d);
+        if (hipGetLastError() != hipSuccess) r = BSX_ERR_DEVICE;
+    }
+    std::vector<double> h(5 * maxrad);
+    if (r == BSX_OK &&
+        (hipMemcpyAsync(h.data(), d, sizeof(double) * 5 * maxrad, hipMemcpyDeviceToHost, s) !=
+             hipSuccess ||
+         hipStreamSynchronize(s) != hipSuccess))
+        r = BSX_ERR_DEVICE;
+    (void)hipFree(v);
+    (void)hipFree(d);
+    if (r) {
+        set_error("bsx_fsc_dpr: device error");
+        return r;
+    }
+    // curves: (s, FSC, DPR in degrees) per row (Bimage_resolution.cpp:225-239)
+    for (long i = 0; i < maxrad; ++i) {
+        double fsc = h[2 * maxrad + i], dpr = 0;
+        if (i == 0) {
+            fsc = 1;
+        } else {
+            const double f12 = h[i] * h[maxrad + i];
+            if (f12 > 1e-30) fsc /= sqrt(f12);
+            fsc = std::min(1.0, std::max(-1.0, fsc));
+            const double fs = h[3 * maxrad + i];
+            dpr = fs > 1e-30 ? sqrt(h[4 * maxrad + i] / fs) * 180.0 / M_PI : 0;
+        }
+        curves[3 * i] = i / rad_scale;
+        curves[3 * i + 1] = fsc;
+        curves[3 * i + 2] = dpr;
+    }
+    *nrows = (int)maxrad;
     return BSX_OK;
 }
 

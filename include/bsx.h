@@ -208,6 +208,15 @@ int bsx_extract(const float* ft_dev, int n, double sampling, const double* mats,
 int bsx_fsc(const float* a_dev, const float* b_dev, int n, double sampling, double hi_res,
             double* fsc, int* nshell, void* stream);
 
+/* Bimage::fsc_dpr after resolution_prepare (src/img/Bimage_resolution.cpp:
+ * 17-40, 93-240; bresolve src/bresolve.cpp:136-150) for two real n^3 maps on
+ * the device: FSC and differential phase residual per shell out to
+ * hi_res (checked as Bimage::check_resolution), shells every
+ * 1/(n sampling / sampling_ratio). curves: rows of (s [1/A], FSC, DPR [deg]);
+ * *nrows = capacity in, rows written out. */
+int bsx_fsc_dpr(const float* map1_dev, const float* map2_dev, int n, double sampling,
+                double hi_res, double sampling_ratio, double* curves, int* nrows, void* stream);
+
 /* ---- projection-matching score (img_recip_space_fom, fom_type 0,
  *      src/mg/mg_refine.cpp:660-781): ring-weighted FRC of particle transforms
  *      against CTF-free reference sections, FP32 on the device.
