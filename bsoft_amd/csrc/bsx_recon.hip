@@ -443,7 +443,7 @@ __device__ __forceinline__ void merge_step(Emit& e, int& pk, int lg) {
 // first lane of each run then either adds to the volume with atomics
 // (MODE_DIRECT) or reserves a bin slot: lanes of a wavefront are grouped by
 // bin with ballots and one integer atomic per (wavefront, bin).
-template <int MG, bool FOM, int MODE>
+template <int MG, bool FOM, int MODE, bool FAST = false>
 __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool merge, Emit e,
                                      const BinCtx& bc) {
     bool head = true;
@@ -475,7 +475,7 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
         return;
     }
     const bool out = out0;   // lanes still here hold binned entries
-    if (bc.st.on) {
+    if (FAST || bc.st.on) {
         if (out) stage_entry(bc, e.key, e.re, e.im, e.w);
         return;
     }
@@ -602,20 +602,20 @@ __device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev&
 
 // Interpolation of one sample at voxel coordinates m (fspace_2D_interpolate,
 // Bimage_reconstruct.cpp:25-88): nearest / weighted nearest / trilinear.
-template <int MG, bool FOM, int MODE>
+template <int MG, bool FOM, int MODE, bool FAST = false>
 __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const ReconConst& K,
                                             const double m[3], bool inband, double w, float fr,
                                             float fi, int kk, bool merge, unsigned keybase,
                                             const BinCtx& bc, int bin = 1) {
-    if (K.interp < 2) {
+    if (FAST || K.interp < 2) {
         const double fx = floor(m[0] + 0.5), fy = floor(m[1] + 0.5), fz = floor(m[2] + 0.5);
         double ww = w;
-        if (K.interp == 1) {
+        if (!FAST && K.interp == 1) {
             const double dx = m[0] - fx, dy = m[1] - fy, dz = m[2] - fz;
             ww *= 1 - sqrt(dx * dx + dy * dy + dz * dz);
         }
         const int cx = wrap32((int)fx, K.N), cy = wrap32((int)fy, K.N), cz = wrap32((int)fz, K.N);
-        emit<MG, FOM, MODE>(acc, K.stride, merge,
+        emit<MG, FOM, MODE, FAST>(acc, K.stride, merge,
                             make_emit<FOM>(keybase + rec_index32(cx, cy, cz, K.NT), kk,
                                            inband && ww > 0, fr, fi, ww, bin),
                             bc);
@@ -628,7 +628,7 @@ __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const Recon
             const int xx = corner & 1, yy = (corner >> 1) & 1, zz = corner >> 2;
             const int jx = wrap32(cx + xx, K.N), jy = wrap32(cy + yy, K.N), jz = wrap32(cz + zz, K.N);
             const double ww = (xx ? dx : 1 - dx) * (yy ? dy : 1 - dy) * (zz ? dz : 1 - dz) * w;
-            emit<MG, FOM, MODE>(acc, K.stride, merge,
+            emit<MG, FOM, MODE, FAST>(acc, K.stride, merge,
                                 make_emit<FOM>(keybase + rec_index32(jx, jy, jz, K.NT), kk,
                                                inband, fr, fi, ww, bin),
                                 bc);
@@ -657,7 +657,7 @@ __device__ __forceinline__ void coords(const ReconConst& K, const PartDev& P, do
 // conjugated, at -v. Samples without such a mate (the x = 0 and x = FT/2
 // columns, the Nyquist row, ties) add to the volume directly, and the mate
 // of a tie is emitted on its own, as before.
-template <int MG, bool FOM, int MODE>
+template <int MG, bool FOM, int MODE, bool FAST = false>
 __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const ReconConst& K,
                                             const PartDev& P, bool active, bool mirror, long h,
                                             long k, long km, float2 F, bool merge,
@@ -669,19 +669,19 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
     double m[3];
     const double mz0 = P.ew * s2;
     coords(K, P, s0, s1, mz0, m);
-    if (K.ewald) {   // all lanes: two emissions per frequency, mates computed in full
-        interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
+    if (!FAST && K.ewald) {   // all lanes: two emissions per frequency, mates computed in full
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
         coords(K, P, s0, s1, -mz0, m);
-        interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
         double t0, t1, t2, tw;
         float gr, gi;
         const bool ib = sample_value<MODE>(K, P, mirror, -h, km, make_float2(F.x, -F.y), &t0,
                                            &t1, &t2, &tw, &gr, &gi);
         const double mz = P.ew * t2;
         coords(K, P, t0, t1, mz, m);
-        interp_emit<MG, FOM, MODE>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
         coords(K, P, t0, t1, -mz, m);
-        interp_emit<MG, FOM, MODE>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
         return;
     }
     // the mate, per lane: -m and the conjugate away from the Nyquist row,
@@ -700,24 +700,24 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
                                 &tw, &gr, &gi);
         coords(K, P, t0, t1, 0.0, mm);
     }
-    if (MODE == MODE_FILL && K.pairs) {
+    if (FAST || (MODE == MODE_FILL && K.pairs)) {
         bool sym = km == -k;
 #pragma unroll
         for (int q = 0; q < 3; ++q) sym = sym && floor(-m[q] + 0.5) == -floor(m[q] + 0.5);
         const bool pair = mirror && sym;
-        interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc,
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc,
                                    pair ? 1 : 0);
         const bool mate = ib && !sym;
         if (__ballot(mate) == 0) return;   // wavefront-uniform
-        interp_emit<MG, FOM, MODE>(acc, K, mm, mate, tw, gr, gi, (int)km, merge, keybase, bc, 0);
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, mm, mate, tw, gr, gi, (int)km, merge, keybase, bc, 0);
         return;
     }
-    interp_emit<MG, FOM, MODE>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
-    interp_emit<MG, FOM, MODE>(acc, K, mm, ib, tw, gr, gi, (int)km, merge, keybase, bc);
+    interp_emit<MG, FOM, MODE, FAST>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
+    interp_emit<MG, FOM, MODE, FAST>(acc, K, mm, ib, tw, gr, gi, (int)km, merge, keybase, bc);
 }
 
 // Column tasks: task = (particle, x) with x in [0, FT/2]; G tasks per WG.
-template <int FT, bool FOM, int MODE>
+template <int FT, bool FOM, int MODE, bool FAST = false>
 __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
                                               const float* __restrict__ bkg,
                                               const PartDev* __restrict__ recs, int nb, int box,
@@ -752,8 +752,8 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     }
     // staging (pair entries: at most one binned emission per thread and
     // round, so 4 rounds fit the list)
-    const bool staged = MODE == MODE_FILL && !FOM && K.pairs && NT == 256 &&
-                        4 * NT <= BSX_STAGE_CAP;
+    const bool staged = FAST || (MODE == MODE_FILL && !FOM && K.pairs && NT == 256 &&
+                                 4 * NT <= BSX_STAGE_CAP);
     bc.st = Stage{st_e, st_r, st_hkey, st_hcnt, st_hbase, st_hc0, st_hc1, 0, staged ? 1 : 0};
     if (staged) {
         if (tid < BSX_STAGE_HASH) {
@@ -791,7 +791,7 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         const int ym = (FT - y) & (FT - 1);
         const long km = (ym <= (FT - 1) / 2) ? ym : ym - FT;
         bc.st.slot0 = (r & 3) * NT;
-        insert_pair<MG, FOM, MODE>(acc, K, P, valid, mirror, h, k, km, F, merge, keybase, bc,
+        insert_pair<MG, FOM, MODE, FAST>(acc, K, P, valid, mirror, h, k, km, F, merge, keybase, bc,
                                    make_float2((float)er, (float)ei));
         {
             const double c = r == 3 ? P.rot[2] : P.rot[0], sd = r == 3 ? P.rot[3] : P.rot[1];
@@ -1435,8 +1435,13 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
     const float* bk = h->bkg + p0;
     if (h->binned) {
         BinCtx bf{h->fill, h->chunks, h->pool, h->bins, h->xacc, h->npool, h->maxc, Stage{}};
-        hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL>), grid, dim3(NT), 0, s, h->inter, bk,
-                           rec, nb, box, h->K, h->acc, tw, bf);
+        // nearest-neighbour pair entries with LDS staging: the specialised kernel
+        if (h->K.pairs && h->K.interp == 0 && NT == 256)
+            hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL, true>), grid, dim3(NT), 0, s, h->inter,
+                               bk, rec, nb, box, h->K, h->acc, tw, bf);
+        else
+            hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL>), grid, dim3(NT), 0, s, h->inter, bk,
+                               rec, nb, box, h->K, h->acc, tw, bf);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_FILL))) return r;
         const unsigned maxpieces = (unsigned)(h->nbins + (size_t)h->npool * BSX_CHUNK / BSX_PIECE + 1);
