@@ -232,10 +232,13 @@ struct Emit {
 #define BSX_STAGE_CAP 1024     // entries per flush: 4 rounds x 1 binned emission x 256 threads
 #define BSX_STAGE_HASH 128     // bins per flush (more fall back to direct reservations)
 static_assert(BSX_STAGE_CAP <= BSX_CHUNK, "a flushed bin range must span at most two chunks");
+static_assert(BSX_STAGE_CAP <= 1024 && BSX_STAGE_HASH <= 256, "list ranks take 10 bits, slots 8");
 
 struct Stage {
-    float4* e;       // (re, im, w, key bits)
-    int* r;          // (hash slot << 20) | rank in the bin, -1 = written directly
+    float* re;       // entry sums, structure of arrays (12 B per entry)
+    float* im;
+    float* w;
+    int* r;          // (hash slot << 22) | (rank in the bin << 12) | record in tile; -1 = empty
     int* hkey;       // bin per hash slot, -1 = free
     int* hcnt;       // entries per hash slot
     int* hbase;      // reserved base per hash slot
@@ -350,8 +353,10 @@ __device__ __forceinline__ void stage_entry(const BinCtx& bc, int key, float re,
         return;
     }
     const int rank = atomicAdd(&bc.st.hcnt[slot], 1);
-    bc.st.e[i] = make_float4(re, im, w, __int_as_float(key));
-    bc.st.r[i] = (slot << 20) | rank;
+    bc.st.re[i] = re;
+    bc.st.im[i] = im;
+    bc.st.w[i] = w;
+    bc.st.r[i] = (slot << 22) | (rank << 12) | (key & 4095);
 }
 
 // Workgroup-uniform: reserve every staged bin range, write the staged
@@ -378,12 +383,13 @@ __device__ __forceinline__ void stage_flush(const BinCtx& bc) {
         const int pr = bc.st.r[i];
         if (pr < 0) continue;
         bc.st.r[i] = -1;
-        const int hs = pr >> 20;
-        const float4 v = bc.st.e[i];
+        const int hs = pr >> 22;
         const int hb = bc.st.hbase[hs];
-        const int pos = hb + (pr & 0xFFFFF);
+        const int pos = hb + ((pr >> 12) & 1023);
         const int c = pos / BSX_CHUNK == hb / BSX_CHUNK ? bc.st.hc0[hs] : bc.st.hc1[hs];
-        put_entry(bc, c, pos, v.x, v.y, v.z, __float_as_int(v.w));
+        // full record key (bin from the hash slot) for the atomic fallback
+        const int key = (int)(((unsigned)bc.st.hkey[hs] << 12) | (unsigned)(pr & 4095));
+        put_entry(bc, c, pos, bc.st.re[i], bc.st.im[i], bc.st.w[i], key);
     }
     __syncthreads();
     if (tid < BSX_STAGE_HASH) {
@@ -732,7 +738,7 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     constexpr int SCAP = MODE == MODE_FILL && !FOM ? BSX_STAGE_CAP : 1;
     constexpr int SHASH = MODE == MODE_FILL && !FOM ? BSX_STAGE_HASH : 1;
     __shared__ float2 lds[LDSN];
-    __shared__ float4 st_e[SCAP];
+    __shared__ float st_re[SCAP], st_im[SCAP], st_w[SCAP];
     __shared__ int st_r[SCAP];
     __shared__ int st_hkey[SHASH], st_hcnt[SHASH], st_hbase[SHASH], st_hc0[SHASH], st_hc1[SHASH];
     const int tid = threadIdx.x;
@@ -754,7 +760,8 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     // round, so 4 rounds fit the list)
     const bool staged = FAST || (MODE == MODE_FILL && !FOM && K.pairs && NT == 256 &&
                                  4 * NT <= BSX_STAGE_CAP);
-    bc.st = Stage{st_e, st_r, st_hkey, st_hcnt, st_hbase, st_hc0, st_hc1, 0, staged ? 1 : 0};
+    bc.st = Stage{st_re, st_im, st_w, st_r, st_hkey, st_hcnt, st_hbase, st_hc0, st_hc1, 0,
+                  staged ? 1 : 0};
     if (staged) {
         if (tid < BSX_STAGE_HASH) {
             st_hkey[tid] = -1;
