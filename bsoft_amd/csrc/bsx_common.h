@@ -5,6 +5,8 @@
 
 #include <cmath>
 #include <vector>
+#include <thread>
+#include <algorithm>
 
 #include "bsx_internal.h"
 
@@ -19,6 +21,28 @@ inline double lambda_of(double av) {
     return PLANCK * 1e10 /
            sqrt(2.0 * EMASS * ECHARGE * av *
                 (1.0 + ECHARGE * av / (2.0 * EMASS * LIGHTSPEED * LIGHTSPEED)));
+}
+
+// Host loop over [0, n) split into contiguous ranges on up to 16 threads
+// (f(begin, end, thread)); runs inline below `grain` items. Used for the
+// per-particle record preparation, which is independent per item.
+template <class F>
+inline void host_parallel(int64_t n, int64_t grain, F f) {
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = std::max(1u, std::min(nt, 16u));
+    if (n < grain || nt == 1) {
+        f((int64_t)0, n, 0u);
+        return;
+    }
+    nt = (unsigned)std::min<int64_t>(nt, (n + grain - 1) / grain);
+    std::vector<std::thread> pool;
+    const int64_t per = (n + nt - 1) / nt;
+    for (unsigned t = 1; t < nt; ++t) {
+        const int64_t b = t * per, e = std::min(n, b + per);
+        if (b < e) pool.emplace_back(f, b, e, t);
+    }
+    f((int64_t)0, std::min(n, per), 0u);
+    for (auto& th : pool) th.join();
 }
 
 // View::quaternion + Matrix3::from_quaternion (include/View.h:131-157,

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <vector>
 #include <cctype>
+#include <climits>
 #include <string>
 
 #include "bsx_fft_dev.h"
@@ -1701,18 +1702,35 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
     BSX_REQUIRE(n > 0 && imgs_dev && parts, BSX_ERR_ARG, "bsx_recon_insert: bad arguments");
     hipStream_t s = (hipStream_t)stream;
     BSX_HIP(hipSetDevice(h->dev));
-    for (int64_t i = 0; i < n; ++i) {
-        BSX_REQUIRE(parts[i].pixel_size > 0, BSX_ERR_ARG, "particle %lld: pixel size not set",
-                    (long long)i);
-        const bsx_particle& q = parts[i];
-        const double vals[] = {q.view[0], q.view[1], q.view[2], q.view[3], q.ori[0], q.ori[1],
-                               q.hdr_ori[0], q.hdr_ori[1], q.mag, q.volt, q.cs, q.amp_shift,
-                               q.def_avg, q.def_dev, q.ast_angle, q.sel};
-        for (double v : vals)
-            BSX_REQUIRE(std::isfinite(v), BSX_ERR_ARG, "particle %lld: non-finite parameter",
-                        (long long)i);
-        BSX_REQUIRE(h->p.nhalves == 1 || parts[i].half == 0 || parts[i].half == 1, BSX_ERR_ARG,
-                    "particle %lld: half must be 0 or 1", (long long)i);
+    {
+        // validation on host threads; the lowest failing particle is reported
+        std::vector<int64_t> bad(16, INT64_MAX);
+        std::vector<int> why(16, 0);
+        host_parallel(n, 8192, [&](int64_t b, int64_t e, unsigned t) {
+            for (int64_t i = b; i < e; ++i) {
+                const bsx_particle& q = parts[i];
+                int w = 0;
+                if (!(q.pixel_size > 0)) w = 1;
+                const double vals[] = {q.view[0], q.view[1], q.view[2], q.view[3], q.ori[0],
+                                       q.ori[1], q.hdr_ori[0], q.hdr_ori[1], q.mag, q.volt, q.cs,
+                                       q.amp_shift, q.def_avg, q.def_dev, q.ast_angle, q.sel};
+                for (double v : vals)
+                    if (!w && !std::isfinite(v)) w = 2;
+                if (!w && h->p.nhalves > 1 && q.half != 0 && q.half != 1) w = 3;
+                if (w) {
+                    bad[t] = i;
+                    why[t] = w;
+                    return;
+                }
+            }
+        });
+        int64_t first = INT64_MAX;
+        int w = 0;
+        for (int t = 0; t < 16; ++t)
+            if (bad[t] < first) { first = bad[t]; w = why[t]; }
+        BSX_REQUIRE(w != 1, BSX_ERR_ARG, "particle %lld: pixel size not set", (long long)first);
+        BSX_REQUIRE(w != 2, BSX_ERR_ARG, "particle %lld: non-finite parameter", (long long)first);
+        BSX_REQUIRE(w != 3, BSX_ERR_ARG, "particle %lld: half must be 0 or 1", (long long)first);
     }
     // point-group symmetry: one record per symmetry-related view, all reading
     // the particle's image (fspace_pack_2D over symmetry_get_all_views)
@@ -1792,10 +1810,18 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
                          [&](int64_t a, int64_t b) { return key[a] < key[b]; });
     }
     double max_step = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        make_rec(h, parts[perm[i]], h->hrecs[i]);
-        h->hrecs[i].src = image_of.empty() ? perm[i] : image_of[perm[i]];
-        max_step = std::max(max_step, h->hrecs[i].invscale * h->K.mscale);
+    {
+        std::vector<double> tmax(16, 0.0);
+        host_parallel(n, 8192, [&](int64_t b, int64_t e, unsigned t) {
+            double m = 0;
+            for (int64_t i = b; i < e; ++i) {
+                make_rec(h, parts[perm[i]], h->hrecs[i]);
+                h->hrecs[i].src = image_of.empty() ? perm[i] : image_of[perm[i]];
+                m = std::max(m, h->hrecs[i].invscale * h->K.mscale);
+            }
+            tmax[t] = m;
+        });
+        for (double m : tmax) max_step = std::max(max_step, m);
     }
     // run merge: equal voxels are contiguous within a merge group of up to 16
     // consecutive samples when 15 steps stay shorter than N - 1 voxels

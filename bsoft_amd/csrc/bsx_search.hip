@@ -1681,11 +1681,12 @@ int bsx_view_matrices(const double* views, int64_t n, double* mats) {
 // conversion) followed by View(Quaternion&) (include/View.h:52-63).
 int bsx_matrix_views(const double* mats, int64_t n, double* views) {
     BSX_REQUIRE((mats && views) || n == 0, BSX_ERR_ARG, "bsx_matrix_views: null argument");
-    for (int64_t i = 0; i < n; ++i) {
+    for (int64_t i = 0; i < 9 * n; ++i)
+        BSX_REQUIRE(std::isfinite(mats[i]), BSX_ERR_ARG, "bsx_matrix_views: matrix %lld is not finite",
+                    (long long)(i / 9));
+    host_parallel(n, 8192, [&](int64_t b, int64_t e, unsigned) {
+      for (int64_t i = b; i < e; ++i) {
         const double* d = mats + 9 * i;
-        for (int j = 0; j < 9; ++j)
-            BSX_REQUIRE(std::isfinite(d[j]), BSX_ERR_ARG, "bsx_matrix_views: matrix %lld is not finite",
-                        (long long)i);
         double q[4];
         const double tr = 1 + d[0] + d[4] + d[8];
         if (tr > 1) {
@@ -1715,7 +1716,8 @@ int bsx_matrix_views(const double* mats, int64_t n, double* views) {
         views[4 * i + 1] = v1;
         views[4 * i + 2] = v2;
         views[4 * i + 3] = remainder(2 * ang, 2 * M_PI);
-    }
+      }
+    });
     return BSX_OK;
 }
 
