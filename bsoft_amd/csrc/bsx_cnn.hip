@@ -274,6 +274,7 @@ __global__ __launch_bounds__(256) void k_softmax_ce(const float* __restrict__ lo
 
 // One workgroup per row: the k largest logits (k <= 32) as classes and
 // softmax probabilities, best first (ties: lower class).
+template <int T>   // per-thread list length, >= k
 __global__ __launch_bounds__(256) void k_row_topk(const float* __restrict__ logits, int C, int k,
                                                   int* __restrict__ cls, float* __restrict__ prob) {
     __shared__ float sv[256];
@@ -281,7 +282,6 @@ __global__ __launch_bounds__(256) void k_row_topk(const float* __restrict__ logi
     __shared__ float red[256];
     const long b = blockIdx.x;
     const float* l = logits + b * C;
-    constexpr int T = 32;
     float best[T];
     int bi[T];
 #pragma unroll
@@ -384,14 +384,19 @@ __global__ __launch_bounds__(256) void k_conv1_nhwc(const float* __restrict__ x,
                                                     const uint16_t* __restrict__ W,
                                                     const uint16_t* __restrict__ bias, int B, int S,
                                                     int C1, uint16_t* __restrict__ y) {
-    __shared__ float w[1024 * 10];   // C1 <= 1024 weights x 9 taps + biases
+    // thread = (output position, group of 8 channels): a wavefront's 16-byte
+    // stores of 8 BF16 channels are contiguous in the NHWC output
+    extern __shared__ float w[];   // C1 x 9 weights, then C1 biases
     for (int i = threadIdx.x; i < C1 * 9; i += blockDim.x) w[i] = bf16_to_f(W[i]);
     for (int i = threadIdx.x; i < C1; i += blockDim.x) w[C1 * 9 + i] = bf16_to_f(bias[i]);
     __syncthreads();
+    const int ng = C1 / 8;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)B * S * S) return;
-    const int xo = (int)(t % S), yo = (int)((t / S) % S);
-    const long b = t / ((long)S * S);
+    if (t >= (long)B * S * S * ng) return;
+    const int g = (int)(t % ng);
+    const long pos = t / ng;
+    const int xo = (int)(pos % S), yo = (int)((pos / S) % S);
+    const long b = pos / ((long)S * S);
     float v[9];
 #pragma unroll
     for (int ky = 0; ky < 3; ++ky)
@@ -400,17 +405,20 @@ __global__ __launch_bounds__(256) void k_conv1_nhwc(const float* __restrict__ x,
             const int yi = yo - 1 + ky, xi = xo - 1 + kx;
             v[ky * 3 + kx] = (yi >= 0 && yi < S && xi >= 0 && xi < S) ? x[(b * S + yi) * S + xi] : 0.f;
         }
-    uint16_t* o = y + t * C1;
-    for (int c = 0; c < C1; c += 2) {
+    uint32_t packed[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int c = g * 8 + 2 * j;
         float a0 = w[C1 * 9 + c], a1 = w[C1 * 9 + c + 1];
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
             a0 = fmaf(w[c * 9 + q], v[q], a0);
             a1 = fmaf(w[(c + 1) * 9 + q], v[q], a1);
         }
-        *(uint32_t*)(o + c) = (uint32_t)f_to_bf16(fmaxf(a0, 0.f)) |
-                              ((uint32_t)f_to_bf16(fmaxf(a1, 0.f)) << 16);
+        packed[j] = (uint32_t)f_to_bf16(fmaxf(a0, 0.f)) |
+                    ((uint32_t)f_to_bf16(fmaxf(a1, 0.f)) << 16);
     }
+    *(uint4*)(y + pos * C1 + g * 8) = make_uint4(packed[0], packed[1], packed[2], packed[3]);
 }
 
 // Implicit-GEMM convolution (ks = 3: pad 1; ks = 1: a plain GEMM over
@@ -861,9 +869,10 @@ int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int
             if (rc) return rc;
         } else {
             const uint16_t* P = h->d_pb;
-            const long pos1 = (long)nb * in;
-            hipLaunchKernelGGL(k_conv1_nhwc, dim3(nblk(pos1)), dim3(256), 0, s, x_dev + b0 * in,
-                               P + h->woff[0], P + h->boff[0], nb, h->S, h->ch[1], h->d_ib[1]);
+            const long pos1 = (long)nb * in * (h->ch[1] / 8);
+            hipLaunchKernelGGL(k_conv1_nhwc, dim3(nblk(pos1)), dim3(256),
+                               sizeof(float) * 10 * h->ch[1], s, x_dev + b0 * in, P + h->woff[0],
+                               P + h->boff[0], nb, h->S, h->ch[1], h->d_ib[1]);
             BSX_HIP(hipGetLastError());
             for (int l = 2; l <= 4; ++l) {
                 const long M = (long)nb * h->hw[l] * h->hw[l];
@@ -880,8 +889,14 @@ int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int
                                (long)h->nclass, (void*)h->d_ilog);
             BSX_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_row_topk, dim3(nb), dim3(256), 0, s, mfma ? h->d_ilog : h->d_act[0],
-                           h->nclass, k, cls_dev + b0 * k, prob_dev + b0 * k);
+        if (k <= 16)
+            hipLaunchKernelGGL(k_row_topk<16>, dim3(nb), dim3(256), 0, s,
+                               mfma ? h->d_ilog : h->d_act[0], h->nclass, k, cls_dev + b0 * k,
+                               prob_dev + b0 * k);
+        else
+            hipLaunchKernelGGL(k_row_topk<32>, dim3(nb), dim3(256), 0, s,
+                               mfma ? h->d_ilog : h->d_act[0], h->nclass, k, cls_dev + b0 * k,
+                               prob_dev + b0 * k);
         BSX_HIP(hipGetLastError());
     }
     return BSX_OK;
