@@ -87,6 +87,8 @@ def main() -> int:
     ap.add_argument("--recon-only", action="store_true")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed-size run: boxes 128/192/256 in a 50/30/20 ratio")
+    ap.add_argument("--exhaustive-sample", type=int, default=1024,
+                    help="particles in the timed FP32 exhaustive-search sample")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for PMC traffic scaling)")
     args = ap.parse_args()
@@ -267,8 +269,40 @@ def main() -> int:
         "graph_replays": recp.graph_launches(),
     })
     result["roofline"] = pipeline_roofline(out, s, P, args, recon_roof)
+    result["fp32_exhaustive_baseline"], result["speedup_learned_vs_fp32_exhaustive"] = \
+        fp32_exhaustive_baseline(s, ps, P / T_rec, result["value"], args)
     finish_result(args, result, ps, rec, ft, dev)
     return 0
+
+
+def fp32_exhaustive_baseline(s, ps, recon_rate, learned_rate, args):
+    """The plain GPU exhaustive search of the reference's method: the FP32
+    FRC (img_recip_space_fom, k_rescore) of each particle against every
+    library view, timed on a sample of particles that fills the GPU, then
+    combined with the reconstruction time per particle."""
+    import torch
+    n = min(args.exhaustive_sample, len(ps.parts))
+    s.stage(ps.images[:n].contiguous(), ps.parts[:n])
+    nv = s.nviews
+    blk = 4096                                             # candidates per rescore call
+    views = torch.arange(nv, dtype=torch.int32, device=s.device)
+    s.rescore(views[:256].expand(n, 256).contiguous())     # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in range(0, nv, blk):
+        s.rescore(views[j:j + blk].expand(n, min(blk, nv - j)).contiguous())
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    search_rate = n / t
+    combined = 1.0 / (1.0 / search_rate + 1.0 / recon_rate)
+    log(f"[bench] FP32 exhaustive FRC search: {n} particles x {nv} views in {t:.2f} s "
+        f"({search_rate:.0f} particles/s; with reconstruction {combined:.0f}/s)")
+    return ({"particles": n, "views": nv, "search_s": round(t, 3),
+             "search_particles_per_s": round(search_rate, 1),
+             "search_plus_recon_particles_per_s": round(combined, 1),
+             "method": "k_rescore FP32 FRC (reference img_recip_space_fom formula) of every particle "
+                       "against all library views; reconstruction at the measured known-view rate"},
+            round(learned_rate / combined, 1))
 
 
 def run_mixed(args) -> int:
