@@ -193,7 +193,7 @@ def main() -> int:
     # uses the autotuned batch size
     recp = Reconstruction(N, box, ft, pix, ctf_action=abi.BSX_CTF_FLIP, flags=abi.BSX_FLAG_GRAPH)
     tb = recp.autotune(ps.images[:8192].contiguous(), ps.parts[:8192])
-    setup["autotuned_batch"] = tb
+    setup["autotuned_batch"], setup["autotuned_piece"] = recp.tuning()
 
     def pipeline(kind, timer):
         s.stage(ps.images, ps.parts)

@@ -811,12 +811,13 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     }
 }
 
-// Bins are processed in pieces of at most PIECE entries so that the few hot
-// bins around the origin (every plane passes through the low-frequency core)
-// are spread over many workgroups.
-#ifndef BSX_PIECE
-#define BSX_PIECE 16384
-#endif
+// Bins are processed in pieces of at most `piece` entries (a whole number of
+// chunks, 8192 to 32768; 16384 by default, chosen by bsx_recon_autotune) so
+// that the few hot bins around the origin (every plane passes through the
+// low-frequency core) are spread over many workgroups.
+#define BSX_PIECE_DEFAULT 16384
+#define BSX_PIECE_MIN 8192
+#define BSX_PIECE_MAX 32768
 
 // Per-bin piece offsets (one workgroup): poff[b] = sum over bins below b of
 // ceil(binned / PIECE), poff[nbins] = all pieces, pbin[piece] = its bin (at
@@ -824,7 +825,7 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
 // since the last reset. Positions past the chunk table (maxc chunks) went to
 // the atomic fallback and are not binned.
 __global__ __launch_bounds__(1024) void k_piece_scan(const int* __restrict__ fill, int nbins,
-                                                     int maxc, int* __restrict__ poff,
+                                                     int maxc, int piece, int* __restrict__ poff,
                                                      int* __restrict__ pbin, int maxpieces,
                                                      unsigned long long* __restrict__ total) {
     __shared__ int ppart[1024];
@@ -837,7 +838,7 @@ __global__ __launch_bounds__(1024) void k_piece_scan(const int* __restrict__ fil
     unsigned long long esum = 0;
     for (int i = b0; i < b1; ++i) {
         const int f = min(fill[i], cap);
-        psum += (f + BSX_PIECE - 1) / BSX_PIECE;
+        psum += (f + piece - 1) / piece;
         esum += (unsigned)f;
     }
     ppart[tid] = psum;
@@ -852,7 +853,7 @@ __global__ __launch_bounds__(1024) void k_piece_scan(const int* __restrict__ fil
     int prun = ppart[tid] - psum;
     for (int i = b0; i < b1; ++i) {
         poff[i] = prun;
-        const int np = (min(fill[i], cap) + BSX_PIECE - 1) / BSX_PIECE;
+        const int np = (min(fill[i], cap) + piece - 1) / piece;
         for (int q = 0; q < np && prun + q < maxpieces; ++q) pbin[prun + q] = i;
         prun += np;
     }
@@ -897,23 +898,23 @@ __device__ __forceinline__ void lds_fadd(float* p, float x) {
 // Each thread keeps 8 entry loads in flight before its LDS adds.
 __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill,
                                                     const int* __restrict__ chunks, int maxc,
-                                                    const int* __restrict__ poff,
+                                                    int piece_len, const int* __restrict__ poff,
                                                     const int* __restrict__ pbin, int nbins,
                                                     const float4* __restrict__ bins,
                                                     float* __restrict__ acc) {
-    constexpr int PC = BSX_PIECE / BSX_CHUNK;   // chunks per piece
-    static_assert(BSX_PIECE % BSX_CHUNK == 0, "pieces are whole chunks");
+    constexpr int PC = BSX_PIECE_MAX / BSX_CHUNK;   // chunks per piece, at most
+    const int pcn = piece_len / BSX_CHUNK;
     const int piece = blockIdx.x;
     if (piece >= poff[nbins]) return;
     const int bin = pbin[piece];
     const int j = piece - poff[bin];
     const bool single = poff[bin + 1] - poff[bin] == 1;
-    const int beg = j * BSX_PIECE;
-    const int end = min(min(fill[bin], maxc * BSX_CHUNK), beg + BSX_PIECE);
+    const int beg = j * piece_len;
+    const int end = min(min(fill[bin], maxc * BSX_CHUNK), beg + piece_len);
     if (beg >= end) return;
     __shared__ float sre[4096], sim[4096], sw[4096];
     __shared__ int pc[PC];
-    if (threadIdx.x < PC) {
+    if (threadIdx.x < PC && (int)threadIdx.x < pcn) {
         const int ci = beg / BSX_CHUNK + threadIdx.x;
         pc[threadIdx.x] = (ci < maxc && ci * BSX_CHUNK < end) ? chunks[(size_t)bin * maxc + ci] : -1;
     }
@@ -1181,6 +1182,8 @@ struct bsx_recon {
     size_t bins_cap = 0;
     int nbins = 0;
     int batch = 0;
+    int batch_cap = 0;               // batch the buffers were sized for
+    int piece = BSX_PIECE_DEFAULT;   // tile-pass piece length (entries)
     int64_t inserted[2] = {0, 0};
     double ktime[7] = {0, 0, 0, 0, 0, 0, 0};   // background rows fallback scan fill tile direct
     double last_entries = 0;
@@ -1200,6 +1203,7 @@ struct bsx_recon {
     int64_t g_n = -1;
     int g_batch = 0;
     int g_merge = -1;              // kernel constants baked into the graph
+    int g_piece = 0;
     int g_seen = 0;                // identical calls seen (capture on the second)
     int64_t graph_launches = 0;
     hipStream_t cap_stream = nullptr;   // capture happens here (the caller's stream may be
@@ -1452,13 +1456,13 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
                                rec, nb, box, h->K, h->acc, tw, bf);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_FILL))) return r;
-        const unsigned maxpieces = (unsigned)(h->nbins + (size_t)h->npool * BSX_CHUNK / BSX_PIECE + 1);
+        const unsigned maxpieces = (unsigned)(h->nbins + (size_t)h->npool * BSX_CHUNK / h->piece + 1);
         hipLaunchKernelGGL(k_piece_scan, dim3(1), dim3(1024), 0, s, h->fill, h->nbins, h->maxc,
-                           h->poff, h->pbin, (int)maxpieces, h->entries);
+                           h->piece, h->poff, h->pbin, (int)maxpieces, h->entries);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_SCAN))) return r;
         hipLaunchKernelGGL(k_tile_accum, dim3(maxpieces), dim3(512), 0, s, h->fill, h->chunks,
-                           h->maxc, h->poff, h->pbin, h->nbins, h->bins, h->xacc);
+                           h->maxc, h->piece, h->poff, h->pbin, h->nbins, h->bins, h->xacc);
         BSX_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_bin_reset, dim3((h->nbins + 255) / 256), dim3(256), 0, s, h->fill,
                            h->chunks, h->maxc, h->nbins, h->pool);
@@ -1590,6 +1594,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
         batch = (int)std::max<size_t>(1, std::min<size_t>(4096, b));
     }
     h->batch = batch;
+    h->batch_cap = batch;
     const size_t inter_bytes = (size_t)batch * inter_pp;
     h->bins_cap = (size_t)batch * ft2 * mult;
     BSX_REQUIRE(!h->binned || h->bins_cap < (1ull << 31), BSX_ERR_ARG,
@@ -1605,7 +1610,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
                                                            BSX_CHUNK - 1) / BSX_CHUNK + 1));
     const size_t bins_bytes = h->binned ? (size_t)h->npool * BSX_CHUNK * sizeof(float4) : 0;
     const size_t bin_idx_bytes =
-        h->binned ? ((size_t)h->nbins * (h->maxc + 3) + (size_t)h->npool * BSX_CHUNK / BSX_PIECE +
+        h->binned ? ((size_t)h->nbins * (h->maxc + 3) + (size_t)h->npool * BSX_CHUNK / BSX_PIECE_MIN +
                      4) * sizeof(int) + 8
                   : 0;
     h->bytes = h->acc_elems * sizeof(float) * (h->binned ? 2 : 1) + inter_bytes + bins_bytes +
@@ -1626,7 +1631,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
              hipMalloc(&h->pool, sizeof(int) * 2) == hipSuccess &&
              hipMalloc(&h->poff, sizeof(int) * (h->nbins + 1)) == hipSuccess &&
              hipMalloc(&h->pbin, sizeof(int) * ((size_t)h->nbins +
-                                                (size_t)h->npool * BSX_CHUNK / BSX_PIECE + 1)) ==
+                                                (size_t)h->npool * BSX_CHUNK / BSX_PIECE_MIN + 1)) ==
                  hipSuccess &&
              hipMalloc(&h->entries, sizeof(unsigned long long)) == hipSuccess;
     if (!ok) {
@@ -1861,7 +1866,7 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
     };
     const bool graph = (h->p.flags & BSX_FLAG_GRAPH) && !h->timing;
     const bool same = imgs_dev == h->g_imgs && n == h->g_n && h->batch == h->g_batch &&
-                      h->K.merge == h->g_merge;
+                      h->K.merge == h->g_merge && h->piece == h->g_piece;
     if (graph && same && h->gexec) {
         BSX_HIP(hipGraphLaunch(h->gexec, s));
         h->graph_launches++;
@@ -1897,6 +1902,7 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
             h->g_n = n;
             h->g_batch = h->batch;
             h->g_merge = h->K.merge;
+            h->g_piece = h->piece;
         }
     }
     for (int64_t i = 0; i < n; ++i)
@@ -1933,7 +1939,7 @@ int bsx_recon_autotune(bsx_recon* h, const float* imgs_dev, const bsx_particle* 
     BSX_REQUIRE(h->inserted[0] == 0 && h->inserted[1] == 0, BSX_ERR_STATE,
                 "bsx_recon_autotune: call it before inserting particles");
     hipStream_t s = (hipStream_t)stream;
-    const int cap = h->batch;
+    const int cap = h->batch_cap;
     const int saved_flags = h->p.flags;
     h->p.flags &= ~(BSX_FLAG_TIMING | BSX_FLAG_GRAPH);
     hipEvent_t e0, e1;
@@ -1959,6 +1965,31 @@ int bsx_recon_autotune(bsx_recon* h, const float* imgs_dev, const bsx_particle* 
         }
         if (ms < bms) { bms = ms; best = b; }
     }
+    // then the piece length of the tile pass at that batch
+    h->batch = best;
+    int best_piece = h->piece;
+    if (h->binned) {
+        for (int pl = BSX_PIECE_MIN; pl <= BSX_PIECE_MAX && rc == BSX_OK; pl *= 2) {
+            if (pl == h->piece) continue;   // measured above
+            const int saved = h->piece;
+            h->piece = pl;
+            rc = bsx_recon_insert(h, imgs_dev, parts, n, stream);
+            if (rc) break;
+            float ms = 0;
+            if (hipEventRecord(e0, s) != hipSuccess) { rc = BSX_ERR_DEVICE; break; }
+            rc = bsx_recon_insert(h, imgs_dev, parts, n, stream);
+            if (rc) break;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                set_error("bsx_recon_autotune: timing failed");
+                rc = BSX_ERR_DEVICE;
+                break;
+            }
+            if (ms < bms) { bms = ms; best_piece = pl; }
+            h->piece = saved;
+        }
+        h->piece = best_piece;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     h->p.flags = saved_flags;
@@ -1969,6 +2000,26 @@ int bsx_recon_autotune(bsx_recon* h, const float* imgs_dev, const bsx_particle* 
     if (chosen_batch) *chosen_batch = best;
     if (best_ms) *best_ms = bms;
     return r2;
+}
+
+int bsx_recon_tuning(const bsx_recon* h, int32_t* batch, int32_t* piece) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_recon_tuning: null handle");
+    if (batch) *batch = h->batch;
+    if (piece) *piece = h->piece;
+    return BSX_OK;
+}
+
+int bsx_recon_set_tuning(bsx_recon* h, int32_t batch, int32_t piece) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_recon_set_tuning: null handle");
+    BSX_REQUIRE(batch <= h->batch_cap, BSX_ERR_ARG,
+                "bsx_recon_set_tuning: batch %d above the allocated %d", batch, h->batch_cap);
+    BSX_REQUIRE(piece <= 0 || (piece >= BSX_PIECE_MIN && piece <= BSX_PIECE_MAX &&
+                               piece % BSX_CHUNK == 0),
+                BSX_ERR_ARG, "bsx_recon_set_tuning: piece %d not a multiple of %d in [%d, %d]",
+                piece, BSX_CHUNK, BSX_PIECE_MIN, BSX_PIECE_MAX);
+    if (batch > 0) h->batch = batch;
+    if (piece > 0) h->piece = piece;
+    return BSX_OK;
 }
 
 int bsx_recon_set_symmetry(bsx_recon* h, const char* sym) {

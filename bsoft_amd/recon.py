@@ -126,7 +126,17 @@ class Reconstruction:
         call("bsx_recon_set_symmetry", self.h, (sym or "C1").encode())
 
     def batch(self) -> int:
-        return getattr(self, "_batch", self.params.batch)
+        return self.tuning()[0]
+
+    def set_tuning(self, batch: int = 0, piece: int = 0):
+        """Pin the insertion batch and/or the tile-pass piece length (0 keeps)."""
+        call("bsx_recon_set_tuning", self.h, int(batch), int(piece))
+
+    def tuning(self):
+        """(batch, tile-pass piece length) in use (bsx_recon_tuning)."""
+        b, p = C.c_int32(), C.c_int32()
+        call("bsx_recon_tuning", self.h, C.byref(b), C.byref(p))
+        return b.value, p.value
 
     def graph_launches(self) -> int:
         n = C.c_int64()

@@ -149,12 +149,19 @@ int bsx_recon_accumulators(bsx_recon* h, int half, float* planes_dev, void* stre
 /* Device bytes held by a handle. */
 int bsx_recon_device_bytes(const bsx_recon* h, int64_t* bytes);
 
-/* Batch-size autotuner (no counterpart in the reference): times the insertion
+/* Insertion autotuner (no counterpart in the reference): times the insertion
  * of n sample particles at the allocated batch and its halvings (down to 1/8,
- * at least 32) and keeps the fastest for later inserts. Requires an empty
- * handle and leaves it empty (accumulators reset). */
+ * at least 32), keeps the fastest, then times the tile-pass piece lengths
+ * 8192, 16384 and 32768 at that batch and keeps the fastest. Requires an
+ * empty handle and leaves it empty (accumulators reset). */
 int bsx_recon_autotune(bsx_recon* h, const float* imgs_dev, const bsx_particle* parts, int64_t n,
                        int32_t* chosen_batch, double* best_ms, void* stream);
+/* Current insertion batch (particles per launch sequence) and tile-pass
+ * piece length (bin entries per workgroup), as set by bsx_recon_autotune. */
+int bsx_recon_tuning(const bsx_recon* h, int32_t* batch, int32_t* piece);
+/* Sets them by hand (0 keeps a value): batch up to the one the handle was
+ * created for, piece a multiple of 4096 in [8192, 32768]. */
+int bsx_recon_set_tuning(bsx_recon* h, int32_t batch, int32_t piece);
 /* Number of inserts served by graph replay so far (BSX_FLAG_GRAPH). */
 int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n);
 

@@ -240,10 +240,20 @@ def test_graph_replay_and_autotune():
          C.byref(cb), C.byref(ms), None)
     say(f"  autotune chose batch {cb.value} ({ms.value:.3f} ms)")
     assert cb.value in (32, 64, 128, 256)
+    batch, piece = t.tuning()
+    say(f"  tuning: batch {batch}, piece {piece}")
+    assert batch == cb.value and piece in (8192, 16384, 32768)
     assert t.accumulators(0).abs().max().item() == 0
     t.insert(d, ps.parts)
     a = t.accumulators(0)
     assert (a[:3] - a0[:3]).abs().max() <= 2e-5 * a0[:3].abs().max()
+    # every piece length gives the same sums
+    for pl in (8192, 32768):
+        t.reset()
+        t.set_tuning(piece=pl)
+        t.insert(d, ps.parts)
+        a = t.accumulators(0)
+        assert (a[:3] - a0[:3]).abs().max() <= 2e-5 * a0[:3].abs().max(), pl
 
 
 def test_breconstruct_cli_on_star_project(tmp_path):
