@@ -68,8 +68,11 @@ struct SPart {
     double phx, phy;     // 2 pi ori / N
     double fscale;       // 1 / (N * pixel_size)
     CtfCoef c;
+    double ox, oy;       // particle origin used for the phase shift (pixels)
+    double def_avg;      // defocus the CTF values were computed with
+    double c20u;         // c20 per unit defocus (-pi lambda)
     int apply;           // CTF applied (flag and defocus >= 1, mg_refine.cpp:687-688)
-    int pad_;
+    int flag;            // CTF flag set (and a voltage given)
 };
 
 // Disk values of one particle from its spectrum values sv[i] = F(h_i, k_i)
@@ -875,6 +878,145 @@ __global__ __launch_bounds__(256) void k_score_mats(
     }
 }
 
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// img_recip_space_fom (fom_type 0; src/mg/mg_refine.cpp:660-781) of the
+// particle moved to origin (ox + dx, oy + dy), i.e. its staged values times
+// exp(2 pi i (h dx + k dy) / N) (ux = 2 pi dx / N), against the section S.
+// With redef the CTF is evaluated at defocus def and applied only when the
+// CTF flag is set and def >= 1 (mg_refine.cpp:687-688); otherwise the staged
+// CTF is used. Products in FP32 as the reference's Complex<float> data, ring
+// sums in FP64. Called by a whole wavefront.
+__device__ inline double frc_fit(const float2* __restrict__ P, const float* __restrict__ Cf,
+                                 const float2* __restrict__ S, const int2* __restrict__ hk,
+                                 const int* __restrict__ ring_start, int R, int lane, double ux,
+                                 double uy, const SPart& sp, bool redef, double def) {
+    CtfCoef cc = sp.c;
+    cc.c20 = sp.c20u * def;
+    const bool ctf_on = sp.flag && def >= 1;
+    double f = 0;
+    for (int j = 0; j < R; ++j) {
+        double num = 0, dp = 0, dr = 0;
+        for (int i = ring_start[j] + lane; i < ring_start[j + 1]; i += 64) {
+            const int h = hk[i].x, k = hk[i].y;
+            double sn, cs;
+            sincos((double)h * ux + (double)k * uy, &sn, &cs);
+            const float2 a0 = P[i];
+            const float ar = (float)(a0.x * cs - a0.y * sn), ai = (float)(a0.x * sn + a0.y * cs);
+            float c = Cf[i];
+            if (redef) c = ctf_on ? (float)sin(ctf_chi(cc, h * sp.fscale, k * sp.fscale)) : 1.f;
+            const float rr = S[i].x * c, ri = S[i].y * c;
+            num += (double)(ar * rr) + (double)(ai * ri);
+            dp += (double)(ar * ar) + (double)(ai * ai);
+            dr += (double)(rr * rr) + (double)(ri * ri);
+        }
+        num = wave_sum_d(num);
+        dp = wave_sum_d(dp);
+        dr = wave_sum_d(dr);
+        const double den = dp * dr;
+        if (den > 0) f += num / sqrt(den);
+    }
+    return f / R;
+}
+
+// img_fit_in_recip_space_grid (src/mg/mg_refine.cpp:862-946), one workgroup
+// per particle at its current orientation. The central section at the disk
+// points is interpolated once into LDS. A defocus scan from def - 5 def_std
+// to def + 5 def_std in steps of def_std runs when def_std > 0 and either the
+// CTF flag is set or the defocus is nonzero (the reference's apply_ctf); then
+// 3 x 3 origin grids around the starting origin (the reference's floating
+// loops, which never move the centre), the step halved when the best is the
+// centre or the FOM stops changing, until the step falls below accuracy.
+// The result is the best of the last grid: origin (pixels), defocus, FOM.
+template <int WD>
+__global__ __launch_bounds__(256) void k_fit_grid(
+    const float2* __restrict__ ft, int N, const double* __restrict__ mats, double inv, double rs,
+    const float* __restrict__ ktab, const int2* __restrict__ hk, const int* __restrict__ ring_start,
+    int R, int Nd, const float2* __restrict__ fp, const float* __restrict__ ctf,
+    const SPart* __restrict__ sps, double step0, double accuracy, double def_std,
+    double* __restrict__ origin, double* __restrict__ defocus, float* __restrict__ fom) {
+    __shared__ float K[(FSI_NDIV + 1) * WD];
+    __shared__ double cf[16];
+    extern __shared__ float2 sec[];   // Nd section values
+    for (int i = threadIdx.x; i < (FSI_NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
+    __syncthreads();
+    const int64_t p = blockIdx.x;
+    const SPart sp = sps[p];
+    const double* M = mats + 9 * p;
+    for (int i = threadIdx.x; i < Nd; i += blockDim.x) {
+        const double ivx = (double)hk[i].x * inv, ivy = (double)hk[i].y * inv;
+        double m[3];
+        for (int q = 0; q < 3; ++q) m[q] = (ivx * M[3 * q] + ivy * M[3 * q + 1]) * rs;
+        double vr, vi;
+        fsi_sample<WD>(ft, N, m, K, vr, vi);
+        sec[i] = make_float2((float)vr, (float)vi);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const float2* P = fp + p * Nd;
+    const float* Cf = ctf + p * Nd;
+    const double tpn = 2 * M_PI / N;
+    double bestdef = sp.def_avg, bestfom = -1;
+    const bool scan = def_std > 0 && (sp.flag || sp.def_avg != 0);
+    if (scan) {
+        const double d0 = bestdef - 5 * def_std, d1 = bestdef + 5 * def_std;
+        double dv[16];
+        int nb = 0;
+        for (double d = d0; d <= d1 && nb < 16; d += def_std) dv[nb++] = d;
+        for (int c0 = 0; c0 < nb; c0 += 4) {
+            const int c = c0 + wave;
+            if (c < nb) {
+                const double f = frc_fit(P, Cf, sec, hk, ring_start, R, lane, 0, 0, sp, true, dv[c]);
+                if (lane == 0) cf[c] = f;
+            }
+        }
+        __syncthreads();
+        for (int c = 0; c < nb; ++c)
+            if (bestfom < cf[c]) { bestfom = cf[c]; bestdef = dv[c]; }
+        __syncthreads();
+    }
+    // Each pass either halves the step or repeats the same grid, whose FOMs
+    // are then equal, so the step halves at least every second pass; the
+    // caller bounds step / accuracy, and the pass count is capped besides.
+    double step = step0, bx = sp.ox, by = sp.oy;
+    for (int pass = 0; step >= accuracy && pass < 128; ++pass) {
+        const double prevfom = bestfom;
+        bestfom = -1;
+        double gx[9], gy[9];
+        int n = 0;
+        for (double y = sp.oy - step; y <= sp.oy + step && n < 9; y += step)
+            for (double x = sp.ox - step; x <= sp.ox + step && n < 9; x += step) {
+                gx[n] = x;
+                gy[n] = y;
+                ++n;
+            }
+        for (int c0 = 0; c0 < n; c0 += 4) {
+            const int c = c0 + wave;
+            if (c < n) {
+                const double f = frc_fit(P, Cf, sec, hk, ring_start, R, lane, (gx[c] - sp.ox) * tpn,
+                                         (gy[c] - sp.oy) * tpn, sp, scan, bestdef);
+                if (lane == 0) cf[c] = f;
+            }
+        }
+        __syncthreads();
+        int ibest = 0;
+        for (int c = 0; c < n; ++c)
+            if (bestfom < cf[c]) { bestfom = cf[c]; ibest = c; bx = gx[c]; by = gy[c]; }
+        __syncthreads();
+        if (ibest == n / 2 || fabs(prevfom - bestfom) < 1e-6) step /= 2;
+    }
+    if (threadIdx.x == 0) {
+        origin[2 * p] = bx;
+        origin[2 * p + 1] = by;
+        if (defocus) defocus[p] = bestdef;
+        fom[p] = (float)bestfom;
+    }
+}
+
 // Local refinement grid: candidate c of particle p is R0 * Rot(delta) with
 // delta = step * ((c%3)-1, (c/3%3)-1, (c/9)-1) a rotation vector (Rodrigues).
 __global__ void k_local_mats(const double* __restrict__ r0, int64_t n, double step,
@@ -1276,8 +1418,12 @@ int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* p
         r.phy = q.ori[1] / N * 2 * M_PI;
         r.fscale = 1.0 / (N * q.pixel_size);
         r.c = ctf_coef(q);
-        r.apply = (h->p.flags & BSX_SEARCH_CTF) && q.def_avg >= 1 && q.volt > 0;
-        r.pad_ = 0;
+        r.ox = q.ori[0];
+        r.oy = q.ori[1];
+        r.def_avg = q.def_avg;
+        r.c20u = -M_PI * lambda_of(q.volt);
+        r.flag = (h->p.flags & BSX_SEARCH_CTF) && q.volt > 0;
+        r.apply = r.flag && q.def_avg >= 1;
     }
     hipStream_t s = (hipStream_t)stream;
     const int64_t ppad = round_up(std::max<int64_t>(n, 1), GM);
@@ -1306,11 +1452,12 @@ int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* p
     // otherwise: full spectra, in chunks of at most 256 MB
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n, (256ll << 20) / (8 * per)));
     rc = grow(h, &h->d_spec, &h->spec_cap, chunk * per, "spectra");
-    if (!rc) rc = grow(h, &h->d_sp, &h->sp_cap, std::max<int64_t>(chunk, 1), "particle records");
+    if (!rc) rc = grow(h, &h->d_sp, &h->sp_cap, n, "particle records");
     if (rc) return rc;
+    // all records stay on the device: the origin/defocus fit reads them later
+    BSX_HIP(hipMemcpyAsync(h->d_sp, sp.data(), sizeof(SPart) * n, hipMemcpyHostToDevice, s));
     for (int64_t p0 = 0; p0 < n; p0 += chunk) {
         const int64_t nb = std::min(chunk, n - p0);
-        BSX_HIP(hipMemcpyAsync(h->d_sp, sp.data() + p0, sizeof(SPart) * nb, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_real_to_complex_b, dim3(2048), dim3(256), 0, s, imgs_dev + p0 * per,
                            h->d_spec, (long long)(nb * per));
         BSX_HIP(hipGetLastError());
@@ -1318,7 +1465,7 @@ int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* p
         if (!rc) rc = fft_axis_ex(h->d_spec, nullptr, N, N, 1, nb, 1, -1, 1.0f / N, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_part_stage, dim3((unsigned)nb), dim3(256), sizeof(float2) * Nd, s,
-                           h->d_spec, N, h->d_sp, h->d_hk, h->d_ring, h->R, Nd, Kp, p0, h->d_fp,
+                           h->d_spec, N, h->d_sp + p0, h->d_hk, h->d_ring, h->R, Nd, Kp, p0, h->d_fp,
                            h->d_ctf, h->d_pbf);
         BSX_HIP(hipGetLastError());
         BSX_HIP(hipStreamSynchronize(s));   // records and spectra are reused
@@ -1548,6 +1695,38 @@ int bsx_search_score_mats(bsx_search* h, const float* ft_dev, const double* mats
         BSX_SM(1) BSX_SM(2) BSX_SM(3) BSX_SM(4) BSX_SM(5)
         BSX_SM(6) BSX_SM(7) BSX_SM(8) BSX_SM(9) BSX_SM(10)
 #undef BSX_SM
+    }
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_search_fit_origin(bsx_search* h, const float* ft_dev, const double* mats_dev, double step,
+                          double accuracy, double def_std, double* origin_dev, double* defocus_dev,
+                          float* fom_dev, void* stream) {
+    BSX_REQUIRE(h && ft_dev && mats_dev && origin_dev && fom_dev, BSX_ERR_ARG,
+                "bsx_search_fit_origin: null argument");
+    BSX_REQUIRE(step > 0 && accuracy > 0 && std::isfinite(step) && std::isfinite(accuracy) &&
+                    step / accuracy < 1e6 && def_std >= 0 && std::isfinite(def_std),
+                BSX_ERR_ARG, "bsx_search_fit_origin: bad step %g / accuracy %g / def_std %g", step,
+                accuracy, def_std);
+    if (h->nparts == 0) return BSX_OK;
+    BSX_REQUIRE(h->sp_cap >= h->nparts, BSX_ERR_STATE,
+                "bsx_search_fit_origin: particle records missing (stage with the fused path)");
+    hipStream_t s = (hipStream_t)stream;
+    const int N = h->p.box;
+    const double inv = 1.0 / (N * h->p.sampling), rs = N * h->p.sampling;
+    const size_t shm = sizeof(float2) * h->Nd;
+    switch (h->p.kernel_width) {
+#define BSX_FG(W) case W: \
+        BSX_HIP(hipFuncSetAttribute((const void*)k_fit_grid<W>, \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)); \
+        hipLaunchKernelGGL((k_fit_grid<W>), dim3((unsigned)h->nparts), dim3(256), \
+        shm, s, (const float2*)ft_dev, N, mats_dev, inv, rs, h->d_ktab, h->d_hk, h->d_ring, h->R, \
+        h->Nd, h->d_fp, h->d_ctf, h->d_sp, step, accuracy, def_std, origin_dev, defocus_dev, \
+        fom_dev); break;
+        BSX_FG(1) BSX_FG(2) BSX_FG(3) BSX_FG(4) BSX_FG(5)
+        BSX_FG(6) BSX_FG(7) BSX_FG(8) BSX_FG(9) BSX_FG(10)
+#undef BSX_FG
     }
     BSX_HIP(hipGetLastError());
     return BSX_OK;

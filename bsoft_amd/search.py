@@ -210,6 +210,21 @@ class Search:
              st.ctypes.data_as(C.POINTER(C.c_double)), len(st), _stream())
         return mats, frc
 
+    def fit_origin(self, mats: torch.Tensor, step: float = 1.0, accuracy: float = 0.25,
+                   def_std: float = 0.0, ft: torch.Tensor | None = None):
+        """img_fit_in_recip_space_grid at rotations mats (n, 3, 3): the best
+        origin (n, 2) float64 pixels, defocus (n,) float64 and FOM (n,)."""
+        ft = self._ft if ft is None else ft
+        mats = mats.to(device=self.device, dtype=torch.float64).contiguous()
+        assert mats.shape[0] == self.nparts
+        ori = torch.empty((self.nparts, 2), dtype=torch.float64, device=self.device)
+        dfc = torch.empty(self.nparts, dtype=torch.float64, device=self.device)
+        fom = torch.empty(self.nparts, dtype=torch.float32, device=self.device)
+        call("bsx_search_fit_origin", self.h, _dev_ptr(ft), _dev_ptr(mats), float(step),
+             float(accuracy), float(def_std), _dev_ptr(ori), _dev_ptr(dfc), _dev_ptr(fom),
+             _stream())
+        return ori, dfc, fom
+
     def particle_data(self):
         v = torch.empty((self.nparts, self.npoints), dtype=torch.complex64, device=self.device)
         c = torch.empty((self.nparts, self.npoints), dtype=torch.float32, device=self.device)

@@ -336,6 +336,23 @@ int bsx_search_view_mats(const bsx_search* h, const int32_t* views_dev, int64_t 
 int bsx_search_refine(bsx_search* h, const float* ft_dev, double* mats_dev, float* frc_dev,
                       const double* steps, int32_t nsteps, void* stream);
 
+/* Origin and defocus fit of every staged particle at its orientation, as
+ * img_fit_in_recip_space_grid (src/mg/mg_refine.cpp:862-946) does: the
+ * section at mats_dev (n x 9 FP64, device) is taken once; when def_std > 0
+ * and either the search's CTF flag is set or the particle's defocus is
+ * nonzero, the defocus is scanned from def - 5 def_std to def + 5 def_std in
+ * steps of def_std (the CTF applied where the flag is set and the defocus is
+ * >= 1); then 3 x 3 grids of origins around the staged origin are scored,
+ * the step halved when the centre wins or the FOM stops changing, while
+ * step >= accuracy. FOM: img_recip_space_fom, fom_type 0, rings 0..R-1, FP64
+ * ring sums. Outputs, per particle: origin_dev n x 2 FP64 (pixels, the best
+ * of the last grid), defocus_dev n FP64 (optional; the staged defocus when
+ * not scanned), fom_dev n. def_std == 0 skips the scan (the reference's scan
+ * loop does not end for it). Asynchronous. */
+int bsx_search_fit_origin(bsx_search* h, const float* ft_dev, const double* mats_dev, double step,
+                          double accuracy, double def_std, double* origin_dev, double* defocus_dev,
+                          float* fom_dev, void* stream);
+
 /* Rotation matrices back to views: Matrix3::quaternion (include/Matrix3.h:
  * 129-166) then View(Quaternion&) (include/View.h:52-63). Host only. */
 int bsx_matrix_views(const double* mats, int64_t n, double* views);

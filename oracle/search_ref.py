@@ -9,6 +9,7 @@ Follows:
       sqrt(sum |F_p|^2 sum |F_ref CTF|^2), mean over rings 0..max_rad-1
   phase_shift_to_origin             src/img/Bimage_complex.cpp:490-543
   CTFparam::calculate = sin(chi)    include/ctf.h:321-365 (via the C oracle)
+  img_fit_in_recip_space_grid       src/mg/mg_refine.cpp:862-946 (fit_grid)
 The half-plane form used by the kernels equals the full-plane sum because
 the transforms of real images are Hermitian (checked in tests/test_oracle.py).
 """
@@ -105,3 +106,63 @@ def frc_full(pft: np.ndarray, ref_ctf: np.ndarray, n: int, sampling: float, radi
     N * sampling / R so that rounding cannot admit ring R."""
     f, _ = oracle.recip_space_fom(pft, ref_ctf, sampling, n * sampling / (radius - 1e-6))
     return f
+
+
+def _ctf_at(part: Particle, hk: np.ndarray, n: int, def_avg: float, ctf_flag: bool, oracle):
+    """CTF values at the disk points with the defocus replaced; ones unless
+    the CTF flag is set and def >= 1 (img_recip_space_fom, mg_refine.cpp:687-688)."""
+    if not (ctf_flag and def_avg >= 1 and part.volt > 0):
+        return np.ones(len(hk))
+    q = Particle.from_buffer_copy(part)
+    q.def_avg = def_avg
+    fs = 1.0 / (n * part.pixel_size)
+    return np.array([oracle.ctf_value(q, float(a) * fs, float(b) * fs) for a, b in hk])
+
+
+def fit_grid(v0: np.ndarray, part: Particle, svals: np.ndarray, hk: np.ndarray,
+             ring_start: np.ndarray, n: int, oracle, step: float, accuracy: float,
+             def_std: float, ctf_flag: bool = True):
+    """img_fit_in_recip_space_grid on the disk points. v0: the particle's
+    transform (sign -1, x 1/N) at hk before any phase shift; svals: the
+    section. Defocus scan over def +- 5 def_std when def_std > 0 and either
+    the CTF flag is set or the defocus is nonzero (def_std == 0 with the flag
+    set would not end in the reference and is skipped); then 3 x 3 origin
+    grids around the starting origin, the step halved when the centre wins
+    or |prevfom - bestfom| < 1e-6, while step >= accuracy.
+    Returns (origin (x, y), defocus, fom, number of FOM evaluations)."""
+    h, k = hk[:, 0].astype(np.float64), hk[:, 1].astype(np.float64)
+
+    def fom(ox, oy, c):
+        v = v0 * np.exp(2j * np.pi * (h * ox + k * oy) / n)
+        return frc_half(v, c, svals, ring_start)
+
+    ox, oy = float(part.ori[0]), float(part.ori[1])
+    bestdef, bestfom, number = float(part.def_avg), -1.0, 0
+    c = _ctf_at(part, hk, n, bestdef, ctf_flag, oracle)
+    if def_std > 0 and (ctf_flag or part.def_avg != 0):
+        d, d1 = bestdef - 5 * def_std, bestdef + 5 * def_std
+        while d <= d1:
+            f = fom(ox, oy, _ctf_at(part, hk, n, d, ctf_flag, oracle))
+            number += 1
+            if bestfom < f:
+                bestfom, bestdef = f, d
+            d += def_std
+        c = _ctf_at(part, hk, n, bestdef, ctf_flag, oracle)
+    best = (ox, oy)
+    while step >= accuracy:
+        prevfom, bestfom = bestfom, -1.0
+        i = ibest = 0
+        y = oy - step
+        while y <= oy + step:
+            x = ox - step
+            while x <= ox + step:
+                f = fom(x, y, c)
+                number += 1
+                if bestfom < f:
+                    bestfom, ibest, best = f, i, (x, y)
+                x += step
+                i += 1
+            y += step
+        if ibest == i // 2 or abs(prevfom - bestfom) < 1e-6:
+            step /= 2
+    return best, bestdef, bestfom, number

@@ -150,3 +150,53 @@ def test_matrix_view_round_trip():
     np.testing.assert_allclose(search.view_matrices(back), m, atol=1e-12)
     ok = np.abs(v[:, 2]) < 0.999
     np.testing.assert_allclose(back[ok], v[ok], atol=1e-9)
+
+
+def _fit_case(oracle64, n=32, R=12, centre=(16.0, 15.0), d_true=1.5e4, ctf=True):
+    """A section G (transform of a random image) and the particle disk values
+    G * CTF(d_true) moved to `centre`: the FOM is exactly 1 at (centre, d_true)."""
+    rng = np.random.default_rng(8)
+    hk, rs = sr.disk_points(R)
+    G = np.fft.fft2(rng.normal(size=(n, n))) / n
+    sv = G[hk[:, 1] % n, hk[:, 0] % n]
+    c = sr._ctf_at(_part(def_avg=d_true), hk, n, d_true, ctf, oracle64)
+    h, k = hk[:, 0], hk[:, 1]
+    v0 = sv * c * np.exp(-2j * np.pi * (h * centre[0] + k * centre[1]) / n)
+    return hk, rs, sv, v0
+
+
+def test_fit_grid_recovers_origin_and_defocus(oracle64):
+    """Start half a pixel off in x and y and two defocus steps off: the scan
+    lands on the true defocus and the 0.5-pixel grid on the true origin."""
+    n, cx, cy, dt, sd = 32, 16.0, 15.0, 1.5e4, 700.0
+    hk, rs, sv, v0 = _fit_case(oracle64, n, 12, (cx, cy), dt)
+    part = _part(def_avg=dt + 2 * sd, ori=(cx + 0.5, cy - 0.5))
+    (ox, oy), d, f, num = sr.fit_grid(v0, part, sv, hk, rs, n, oracle64, 1.0, 0.5, sd)
+    assert d == pytest.approx(dt, rel=1e-12)
+    assert (ox, oy) == (pytest.approx(cx, abs=1e-12), pytest.approx(cy, abs=1e-12))
+    assert f == pytest.approx(1.0, abs=1e-9)
+    assert num >= 11 + 2 * 9
+
+
+def test_fit_grid_centre_never_moves(oracle64):
+    """The grids stay centred on the starting origin and the result is the
+    best of the last grid: from centre + (1, 0) with steps 1 then 0.5 the fit
+    returns centre + (0.5, 0), after 4 grids (each step's grid twice: the best
+    is not the centre, then the repeat leaves the FOM unchanged)."""
+    n, cx, cy, dt = 32, 16.0, 15.0, 1.5e4
+    hk, rs, sv, v0 = _fit_case(oracle64, n, 12, (cx, cy), dt)
+    part = _part(def_avg=dt, ori=(cx + 1, cy))
+    (ox, oy), d, f, num = sr.fit_grid(v0, part, sv, hk, rs, n, oracle64, 1.0, 0.5, 0.0)
+    assert (ox, oy) == (cx + 0.5, cy) and d == dt and num == 4 * 9
+    assert f < 1 - 1e-3
+
+
+def test_fit_grid_defocus_scan_without_ctf(oracle64):
+    """def_std > 0 and a nonzero defocus start the scan even with the CTF
+    flag off; the FOM does not change with the defocus, so the first value,
+    def - 5 def_std, is kept (strict comparison)."""
+    n, cx, cy, dt, sd = 32, 16.0, 15.0, 1.5e4, 500.0
+    hk, rs, sv, v0 = _fit_case(oracle64, n, 12, (cx, cy), dt, ctf=False)
+    part = _part(def_avg=dt, ori=(cx, cy))
+    _, d, _, num = sr.fit_grid(v0, part, sv, hk, rs, n, oracle64, 1.0, 1.0, sd, ctf_flag=False)
+    assert d == dt - 5 * sd and num == 11 + 9
