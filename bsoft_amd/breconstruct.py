@@ -12,8 +12,11 @@ Options (a subset of breconstruct's, same meaning):
   -pad n                 padding factor of the particle transforms (default 2)
   -CTF flip|apply|wiener CTF treatment; -wiener w (default 0.2 for wiener)
   -interpolate nearest|weighted|trilinear
-  -symmetry label        point group (C<n>, D<n>, T, O, I, ...; default C1):
-                         each particle is inserted at all its related views
+  -symmetry label        point group (C<n>, D<n>, T, O, I, ...; default C1)
+  -mode 0|1|2            symmetry mode: 0 insert each particle at all its
+                         related views (default), 1 insert once and
+                         symmetrize the map in real space, 2 insert once at a
+                         randomly picked related view (-seed n, default 1)
   -bootstrap             weight particles by their selection number
   -halfmaps              also write the two half maps (<name>_1, <name>_2)
   -fom                   also write the FOM map (<name>_fom)
@@ -39,6 +42,8 @@ def parse_args(argv):
     ap.add_argument("-interpolate", default="nearest", choices=["nearest", "weighted", "trilinear"])
     ap.add_argument("-bootstrap", action="store_true")
     ap.add_argument("-symmetry", default="C1")
+    ap.add_argument("-mode", type=int, default=0, choices=[0, 1, 2])
+    ap.add_argument("-seed", type=int, default=1)
     ap.add_argument("-halfmaps", action="store_true")
     ap.add_argument("-fom", action="store_true")
     ap.add_argument("project")
@@ -50,7 +55,7 @@ def main(argv=None) -> int:
     import torch
     from . import abi
     from .formats import load_particle_images, project_particles, write_mrc
-    from .recon import Reconstruction, part_ft_size
+    from .recon import Reconstruction, part_ft_size, random_symmetric_views, symmetrize
 
     parts, srcs = project_particles(a.project)
     keep = parts["sel"] > 0                      # project_selected_partlist
@@ -83,7 +88,13 @@ def main(argv=None) -> int:
     rec = Reconstruction(size, box, ft, sampling, hi_res=a.resolution, interp=interp,
                          ctf_action=ctf, wiener=a.wiener, flags=flags,
                          nhalves=2 if a.halfmaps else 1)
-    rec.set_symmetry(a.symmetry)
+    # symmetry modes of particle_reconstruct (mg_reconstruct.cpp:175, 215-218)
+    # and breconstruct.cpp:335
+    if a.mode == 0:
+        rec.set_symmetry(a.symmetry)
+    elif a.mode == 2:
+        parts["view"] = random_symmetric_views(a.symmetry, parts["view"],
+                                               np.random.default_rng(a.seed))
     rec.insert(torch.as_tensor(imgs).cuda().contiguous(), parts)
     stem, ext = os.path.splitext(a.reconstruction)
     outs = [(0, 3 if a.halfmaps else 1, a.reconstruction)]
@@ -92,7 +103,8 @@ def main(argv=None) -> int:
     for imap, nmaps, path in outs:
         r = rec.finish(imap, nmaps, want_fom=a.fom and imap == 0)
         org = (size // 2,) * 3
-        write_mrc(path, r.map.cpu().numpy(), (sampling,) * 3, org)
+        vol = symmetrize(r.map, a.symmetry, org) if a.mode == 1 else r.map
+        write_mrc(path, vol.cpu().numpy(), (sampling,) * 3, org)
         if r.fom is not None:
             write_mrc(f"{stem}_fom{ext or '.mrc'}", r.fom.cpu().numpy(), (sampling,) * 3, org)
         print(f"breconstruct: {path}: {r.particles} particles, coverage {r.coverage}, "

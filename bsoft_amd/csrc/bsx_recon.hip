@@ -1426,6 +1426,144 @@ static void symmetry_views(const std::vector<SymOp>& ops, const double view[4],
     }
 }
 
+// ------------------------------------------------ real-space symmetrization
+// Bimage::symmetrize (src/img/Bimage_symmetry.cpp:31-100) of a real N^3 map:
+// for each operator (axis, order m) the map becomes the sum of itself and
+// its m - 1 rotations about the origin, each rotation
+// Bimage::transform (src/img/Bimage_transform.cpp:138-214) with
+// Bimage::interpolate (src/img/Bimage.cpp:1436-1484) and the background as
+// the fill; after the last operator the map is multiplied by 1 / order.
+
+// Rotation matrix of Matrix3(axis, angle): Quaternion::from_axis_angle
+// (include/Quaternion.h:34-41, check() zeroes components below 1e-30), then
+// Matrix3::from_quaternion and check() (include/Matrix3.h:21-42).
+static void axis_angle_matrix(const double ax[3], double angle, double M[9]) {
+    const double n = sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+    const double a = angle / 2;
+    double q[4] = {cos(a), ax[0] / n * sin(a), ax[1] / n * sin(a), ax[2] / n * sin(a)};
+    for (int i = 1; i < 4; ++i)
+        if (fabs(q[i]) < 1e-30) q[i] = 0;
+    M[0] = q[0] * q[0] + q[1] * q[1] - q[2] * q[2] - q[3] * q[3];
+    M[1] = 2 * q[1] * q[2] - 2 * q[0] * q[3];
+    M[2] = 2 * q[1] * q[3] + 2 * q[0] * q[2];
+    M[3] = 2 * q[1] * q[2] + 2 * q[0] * q[3];
+    M[4] = q[0] * q[0] - q[1] * q[1] + q[2] * q[2] - q[3] * q[3];
+    M[5] = 2 * q[2] * q[3] - 2 * q[0] * q[1];
+    M[6] = 2 * q[1] * q[3] - 2 * q[0] * q[2];
+    M[7] = 2 * q[2] * q[3] + 2 * q[0] * q[1];
+    M[8] = q[0] * q[0] - q[1] * q[1] - q[2] * q[2] + q[3] * q[3];
+    for (int i = 0; i < 9; ++i) {
+        if (fabs(M[i]) < 1e-12) M[i] = 0;
+        if (M[i] > 1) M[i] = 1;
+        if (M[i] < -1) M[i] = -1;
+    }
+}
+
+// Bimage::interpolate: trilinear over the (up to) 2 x 2 x 2 neighbours inside
+// the map, weights toggled as the reference's loops toggle them (an edge
+// neighbour missing in one axis leaves that axis's weight alternating),
+// normalised by their sum; outside the map: fill.
+__device__ inline double interp_fill(const float* __restrict__ a, int n, double xx, double yy,
+                                     double zz, double fill) {
+    if (xx < 0 || xx >= n || yy < 0 || yy >= n || zz < 0 || zz >= n) return fill;
+    const long ix = (long)xx, iy = (long)yy, iz = (long)zz;
+    const int nx = (n < ix + 2) ? 1 : 2, ny = (n < iy + 2) ? 1 : 2, nz = (n < iz + 2) ? 1 : 2;
+    double fx = xx - ix, fy = yy - iy, fz = zz - iz;
+    double value = 0, ws = 0;
+    for (int zk = 0; zk < nz; ++zk) {
+        fz = 1.0 - fz;
+        for (int yk = 0; yk < ny; ++yk) {
+            fy = 1.0 - fy;
+            const double wyz = fy * fz;
+            const size_t row = ((size_t)(iz + zk) * n + (iy + yk)) * n;
+            for (int xk = 0; xk < nx; ++xk) {
+                fx = 1.0 - fx;
+                const double w = fx * wyz;
+                ws += w;
+                value += (double)a[row + ix + xk] * w;
+            }
+        }
+    }
+    return ws != 0 ? value / ws : fill;
+}
+
+// Background of Bimage::calculate_background (flag 0,
+// src/img/Bimage_background.cpp:27-80): mean of the voxels outside the
+// ellipsoid ((x - o) / (N/2))^2 + ... = 1. Per-block FP64 partial sums, then
+// one block adds them in a fixed order.
+__global__ __launch_bounds__(256) void k_bkg_partial(const float* __restrict__ a, int n, double o0,
+                                                     double o1, double o2,
+                                                     double* __restrict__ part) {
+    __shared__ double ss[256], sc[256];
+    const int64_t total = (int64_t)n * n * n;
+    const double h = (double)(n / 2);
+    double s = 0, c = 0;
+    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < total;
+         v += (int64_t)gridDim.x * 256) {
+        const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+        double x2 = (x - o0) / h, y2 = (y - o1) / h, z2 = (z - o2) / h;
+        x2 *= x2;
+        y2 *= y2;
+        z2 *= z2;
+        if (x2 + y2 + z2 - 1 > 0) {
+            s += a[v];
+            c += 1;
+        }
+    }
+    ss[threadIdx.x] = s;
+    sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            ss[threadIdx.x] += ss[threadIdx.x + o];
+            sc[threadIdx.x] += sc[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ss[0];
+        part[2 * blockIdx.x + 1] = sc[0];
+    }
+}
+
+__global__ void k_bkg_final(const double* __restrict__ part, int nblocks, double* __restrict__ bkg) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0, c = 0;
+    for (int b = 0; b < nblocks; ++b) {
+        s += part[2 * b];
+        c += part[2 * b + 1];
+    }
+    bkg[0] = c > 0 ? s / c : 0;
+}
+
+// One operator: out = in + sum over j of in rotated by mats[j] (m - 1
+// matrices, the image rotated by R samples the input at R^T (x - o) + o),
+// each term rounded to FP32 and added in FP32 as Bimage::sum does; the last
+// operator also multiplies by `scale` (1 / order).
+__global__ __launch_bounds__(256) void k_sym_op(const float* __restrict__ in,
+                                                float* __restrict__ out, int n, double o0,
+                                                double o1, double o2,
+                                                const double* __restrict__ mats, int nm,
+                                                const double* __restrict__ bkg, double scale) {
+    const int64_t total = (int64_t)n * n * n;
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= total) return;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+    const double u0 = x - o0, u1 = y - o1, u2 = z - o2;
+    const double fill = bkg[0];
+    float acc = in[v];
+    for (int j = 0; j < nm; ++j) {
+        const double* M = mats + 9 * j;
+        const double a0 = M[0] * u0 + M[3] * u1 + M[6] * u2 + o0;
+        const double a1 = M[1] * u0 + M[4] * u1 + M[7] * u2 + o1;
+        const double a2 = M[2] * u0 + M[5] * u1 + M[8] * u2 + o2;
+        const float pj = (float)interp_fill(in, n, a0, a1, a2, fill);
+        acc = (float)((double)acc + (double)pj);
+    }
+    if (scale != 1) acc = (float)((double)acc * scale);
+    out[v] = acc;
+}
+
 template <int FT>
 static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hipStream_t s) {
     constexpr int T = FftThreads<FT>::value;
@@ -2065,6 +2203,80 @@ int bsx_symmetry_views(const char* sym, const double* views, int64_t n, double* 
         symmetry_views(ops, views + 4 * i, sv);
         std::copy(sv.begin(), sv.end(), out + 4 * i * order);
     }
+    return BSX_OK;
+}
+
+int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double* origin,
+                   const char* sym, void* stream) {
+    BSX_REQUIRE(in_dev && out_dev && origin && n >= 1 && n <= 2048, BSX_ERR_ARG,
+                "bsx_symmetrize: bad arguments");
+    BSX_REQUIRE(in_dev != out_dev, BSX_ERR_ARG, "bsx_symmetrize: input and output must differ");
+    for (int c = 0; c < 3; ++c)
+        BSX_REQUIRE(std::isfinite(origin[c]), BSX_ERR_ARG, "bsx_symmetrize: origin not finite");
+    std::vector<SymOp> all, ops;
+    const int rc = parse_symmetry(sym, all);
+    if (rc) return rc;
+    int64_t order = 1;
+    for (const SymOp& o : all) {
+        order *= o.order;
+        if (o.order > 1) ops.push_back(o);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t total = (int64_t)n * n * n;
+    if (ops.empty()) {   // C1 and Cs: Bimage::symmetrize leaves the map unchanged
+        BSX_HIP(hipMemcpyAsync(out_dev, in_dev, sizeof(float) * total, hipMemcpyDeviceToDevice, s));
+        return BSX_OK;
+    }
+    // matrices of every operator, j = 1 .. m - 1 (angle j 2 pi / m)
+    std::vector<double> mats;
+    std::vector<int> first;
+    for (const SymOp& o : ops) {
+        first.push_back((int)(mats.size() / 9));
+        for (int j = 1; j < o.order; ++j) {
+            double M[9];
+            axis_angle_matrix(o.ax, j * M_PI * 2.0 / o.order, M);
+            mats.insert(mats.end(), M, M + 9);
+        }
+    }
+    const int nblk = 1024;
+    float* tmp = nullptr;
+    double* dm = nullptr;
+    double* part = nullptr;
+    auto release = [&]() {
+        if (tmp) (void)hipFree(tmp);
+        if (dm) (void)hipFree(dm);
+        if (part) (void)hipFree(part);
+    };
+    hipError_t e = hipMalloc((void**)&dm, sizeof(double) * mats.size());
+    if (e == hipSuccess) e = hipMalloc((void**)&part, sizeof(double) * (2 * nblk + 1));
+    if (e == hipSuccess && ops.size() > 1) e = hipMalloc((void**)&tmp, sizeof(float) * total);
+    if (e != hipSuccess) {
+        release();
+        BSX_REQUIRE(false, BSX_ERR_DEVICE, "bsx_symmetrize: device allocation failed (%s)",
+                    hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(dm, mats.data(), sizeof(double) * mats.size(), hipMemcpyHostToDevice, s);
+    double* bkg = part + 2 * nblk;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_bkg_partial, dim3(nblk), dim3(256), 0, s, in_dev, n, origin[0],
+                           origin[1], origin[2], part);
+        hipLaunchKernelGGL(k_bkg_final, dim3(1), dim3(64), 0, s, part, nblk, bkg);
+        e = hipGetLastError();
+    }
+    const int K = (int)ops.size();
+    const float* src = in_dev;
+    for (int k = 0; k < K && e == hipSuccess; ++k) {
+        float* dst = ((K - 1 - k) % 2 == 0) ? out_dev : tmp;
+        const double scale = k == K - 1 ? 1.0 / (double)order : 1.0;
+        hipLaunchKernelGGL(k_sym_op, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src,
+                           dst, n, origin[0], origin[1], origin[2], dm + 9 * first[k],
+                           ops[k].order - 1, (const double*)bkg, scale);
+        e = hipGetLastError();
+        src = dst;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);   // the buffers are released below
+    release();
+    BSX_HIP(e);
     return BSX_OK;
 }
 

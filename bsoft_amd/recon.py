@@ -210,6 +210,32 @@ def symmetry_views(sym: str, views: np.ndarray) -> np.ndarray:
     return out.reshape(len(v), -1, 4)
 
 
+def symmetrize(vol: torch.Tensor, sym: str, origin=None, stream=None) -> torch.Tensor:
+    """Bimage::symmetrize (src/img/Bimage_symmetry.cpp:31-100) of a real
+    N^3 map on the device, about `origin` (voxels, x y z; default N/2):
+    breconstruct's symmetry mode 1, applied after the inverse transform."""
+    n = int(vol.shape[-1])
+    if tuple(vol.shape) != (n, n, n) or vol.dtype != torch.float32:
+        raise ValueError("a cubic float32 map is required")
+    vol = vol.contiguous()
+    out = torch.empty_like(vol)
+    o = np.asarray(origin if origin is not None else (n // 2,) * 3, dtype=np.float64)
+    call("bsx_symmetrize", _ptr(vol), _ptr(out), n, o.ctypes.data_as(C.POINTER(C.c_double)),
+         sym.encode(), _stream(stream))
+    return out
+
+
+def random_symmetric_views(sym: str, views: np.ndarray, rng: np.random.Generator) -> np.ndarray:
+    """random_symmetric_view (src/util/symmetry.cpp:816-833) for each row:
+    one of the order(sym) related views, index floor(order * u * 0.99999)
+    with u uniform in [0, 1) (breconstruct's symmetry mode 2). The
+    reference draws u from random(); this draws it from `rng`."""
+    allv = symmetry_views(sym, views)
+    n, order = allv.shape[:2]
+    idx = (order * rng.random(n) * 0.99999).astype(np.int64)
+    return allv[np.arange(n), idx]
+
+
 def img_reconstruction_sum_weigh(rec: Reconstruction, imap: int = 0, nmaps: int = 1,
                                  **kw) -> FinishResult:
     """img_reconstruction_sum_weigh (mg_reconstruct.cpp:266-361) followed by

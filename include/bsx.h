@@ -412,6 +412,17 @@ int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int
 int bsx_views_c1(double theta_step, double phi_step, int32_t flag, double angle_step,
                  double* views, int64_t cap, int64_t* count);
 
+/* Real-space symmetrization of a reconstructed map, breconstruct's symmetry
+ * mode 1 (src/breconstruct.cpp:335, Bimage::symmetrize with the default
+ * reference view, src/img/Bimage_symmetry.cpp:31-100): for each operator
+ * (axis, order m) of sym the map becomes itself plus its m - 1 rotations
+ * about origin (voxels, x y z), trilinear as Bimage::interpolate with the
+ * map's background (mean outside the inscribed ellipsoid) as the fill; the
+ * result is multiplied by 1 / order. in_dev, out_dev: distinct N^3 FP32 maps
+ * (z, y, x order) on the device. C1 copies. Synchronises the stream. */
+int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double* origin,
+                   const char* sym, void* stream);
+
 /* symmetry_get_all_views (src/util/symmetry.cpp:277-319) for each of n views
  * (x, y, z, angle): out holds, view after view, the order(sym) related views
  * (the view itself first); cap = room in out (views); *count = n * order. */

@@ -10,10 +10,20 @@ Follows:
   View::quaternion                include/View.h:131-153
   View(Quaternion&)               include/View.h:52-63
   Quaternion(axis, angle), *      include/Quaternion.h:34-41, 117-124
+  Bimage::symmetrize              src/img/Bimage_symmetry.cpp:31-100 (symmetrize)
+  Bimage::transform               src/img/Bimage_transform.cpp:138-214
+  Bimage::interpolate             src/img/Bimage.cpp:1436-1484
+  Bimage::calculate_background    src/img/Bimage_background.cpp:27-80
+  Matrix3(axis, angle)            include/Matrix3.h:21-42, 73-76
+The real-space symmetrization is pinned in tests/test_symmetry.py by exact
+group averages where the rotations permute the grid (C2, C4, D2, D4, T, O)
+and by linear maps, whose trilinear interpolation is exact (C3, C5, I).
 """
 from __future__ import annotations
 
 import math
+
+import numpy as np
 
 GOLDEN = 1.61803398874989484820458683436563811772
 
@@ -137,3 +147,96 @@ def all_views(label: str, view):
                 out.append(_view(_qmul(q, _quat(out[k]))))
         nview *= order
     return out
+
+
+def axis_angle_matrix(axis, angle):
+    """Matrix3(axis, angle): unit quaternion (components below 1e-30 zeroed),
+    rotation matrix, entries below 1e-12 zeroed and clamped to [-1, 1]."""
+    ax = np.asarray(axis, dtype=np.float64)
+    ax = ax / math.sqrt(float(ax @ ax))
+    a = angle / 2
+    q = [math.cos(a)] + [float(c) * math.sin(a) for c in ax]
+    q = [q[0]] + [0.0 if abs(c) < 1e-30 else c for c in q[1:]]
+    M = np.array([
+        [q[0] * q[0] + q[1] * q[1] - q[2] * q[2] - q[3] * q[3],
+         2 * q[1] * q[2] - 2 * q[0] * q[3], 2 * q[1] * q[3] + 2 * q[0] * q[2]],
+        [2 * q[1] * q[2] + 2 * q[0] * q[3],
+         q[0] * q[0] - q[1] * q[1] + q[2] * q[2] - q[3] * q[3], 2 * q[2] * q[3] - 2 * q[0] * q[1]],
+        [2 * q[1] * q[3] - 2 * q[0] * q[2], 2 * q[2] * q[3] + 2 * q[0] * q[1],
+         q[0] * q[0] - q[1] * q[1] - q[2] * q[2] + q[3] * q[3]]])
+    M[np.abs(M) < 1e-12] = 0
+    return np.clip(M, -1, 1)
+
+
+def background(vol: np.ndarray, origin) -> float:
+    """Mean of the voxels outside the ellipsoid with semi-axes N/2 (integer)
+    about origin (x, y, z)."""
+    n = vol.shape[0]
+    h = float(n // 2)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    r = ((x - origin[0]) / h) ** 2 + ((y - origin[1]) / h) ** 2 + ((z - origin[2]) / h) ** 2 - 1
+    out = r > 0
+    return float(vol[out].astype(np.float64).mean()) if out.any() else 0.0
+
+
+def interpolate(vol: np.ndarray, xx, yy, zz, fill: float) -> np.ndarray:
+    """Bimage::interpolate at arrays of positions: the (up to) 2 x 2 x 2
+    neighbours inside the map, each axis weight toggled once per pass of its
+    loop as the reference's loops do, normalised by the weight sum."""
+    n = vol.shape[0]
+    flat = vol.astype(np.float64).ravel()
+    inside = (xx >= 0) & (xx < n) & (yy >= 0) & (yy < n) & (zz >= 0) & (zz < n)
+    xs, ys, zs = (np.where(inside, c, 0.0) for c in (xx, yy, zz))
+    ix, iy, iz = xs.astype(np.int64), ys.astype(np.int64), zs.astype(np.int64)
+    nx, ny, nz = (np.where(n < i + 2, 1, 2) for i in (ix, iy, iz))
+    fx, fy, fz = xs - ix, ys - iy, zs - iz
+    value = np.zeros_like(xs)
+    ws = np.zeros_like(xs)
+    for zk in range(2):
+        zon = zk < nz
+        fz = np.where(zon, 1.0 - fz, fz)
+        for yk in range(2):
+            yon = zon & (yk < ny)
+            fy = np.where(yon, 1.0 - fy, fy)
+            wyz = fy * fz
+            for xk in range(2):
+                on = yon & (xk < nx)
+                fx = np.where(on, 1.0 - fx, fx)
+                w = fx * wyz
+                idx = ((iz + zk) * n + (iy + yk)) * n + (ix + xk)
+                v = flat[np.where(on, idx, 0)]
+                ws = np.where(on, ws + w, ws)
+                value = np.where(on, value + v * w, value)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        res = np.where(ws != 0, value / np.where(ws != 0, ws, 1), fill)
+    return np.where(inside, res, fill)
+
+
+def symmetrize(vol: np.ndarray, label: str, origin) -> np.ndarray:
+    """Bimage::symmetrize(sym, View(), 1) of a real N^3 map (z, y, x): per
+    operator the map plus its order - 1 rotations about origin (each rotated
+    map samples R^T (x - o) + o, rounded to FP32, added in FP32), with the
+    background of the input as the fill; finally x 1/order in FP32."""
+    ops = operators(label)
+    order = 1
+    for _, m in ops:
+        order *= m
+    cur = np.asarray(vol, dtype=np.float32).copy()
+    if all(m == 1 for _, m in ops):
+        return cur
+    n = cur.shape[0]
+    o = [float(c) for c in origin]
+    fill = background(cur, o)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    u = [x - o[0], y - o[1], z - o[2]]
+    for axis, m in ops:
+        if m == 1:
+            continue
+        acc = cur.copy()
+        for j in range(1, m):
+            M = axis_angle_matrix(axis, j * math.pi * 2.0 / m)
+            old = [M[0, i] * u[0] + M[1, i] * u[1] + M[2, i] * u[2] + o[i] for i in range(3)]
+            pj = interpolate(cur, old[0], old[1], old[2], fill).astype(np.float32)
+            acc = (acc.astype(np.float64) + pj.astype(np.float64)).astype(np.float32)
+        cur = acc
+    return (cur.astype(np.float64) * (1.0 / order)).astype(np.float32)

@@ -256,11 +256,8 @@ def test_graph_replay_and_autotune():
         assert (a[:3] - a0[:3]).abs().max() <= 2e-5 * a0[:3].abs().max(), pl
 
 
-def test_breconstruct_cli_on_star_project(tmp_path):
-    """breconstruct over a STAR project and an MRC stack gives the map of the
-    direct API call on the same particles, written as MRC with the sampling
-    and origin of the reference's output."""
-    from bsoft_amd import breconstruct, formats
+def _star_project(tmp_path):
+    from bsoft_amd import formats
     bm, ps = dataset(32, 10, 60, 32, seed=11)
     n = len(ps.parts)
     formats.write_mrc(str(tmp_path / "p.mrcs"), ps.images.numpy(), (1.5, 1.5, 1.0), (16, 16, 0),
@@ -276,6 +273,16 @@ def test_breconstruct_cli_on_star_project(tmp_path):
                  "particle.select"], rows)])
     star = str(tmp_path / "proj.star")
     formats.write_star(star, [b])
+    return star, ps
+
+
+def test_breconstruct_cli_on_star_project(tmp_path):
+    """breconstruct over a STAR project and an MRC stack gives the map of the
+    direct API call on the same particles, written as MRC with the sampling
+    and origin of the reference's output."""
+    from bsoft_amd import breconstruct, formats
+    star, ps = _star_project(tmp_path)
+    n = len(ps.parts)
     out = str(tmp_path / "map.mrc")
     assert breconstruct.main(["-reconstruction", out, "-CTF", "flip", "-halfmaps", star]) == 0
     m = formats.read_mrc(out)
@@ -289,6 +296,32 @@ def test_breconstruct_cli_on_star_project(tmp_path):
     assert np.abs(m.data[0] - ref).max() <= 1e-5 * np.abs(ref).max()
     h1 = formats.read_mrc(str(tmp_path / "map_1.mrc")).data[0]
     assert np.abs(h1 - rec.finish(1, 3).map.cpu().numpy()).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_breconstruct_symmetry_modes(tmp_path, mode):
+    """-mode 1: the particles inserted once, then the map symmetrized in real
+    space (the FP64 restatement of Bimage::symmetrize on the direct API's
+    map); -mode 2: each particle inserted once at the related view that
+    random_symmetric_views draws with the same seed."""
+    from bsoft_amd import breconstruct, formats
+    from bsoft_amd.recon import random_symmetric_views
+    from oracle import sym_ref
+    say(f"breconstruct -mode {mode}")
+    star, ps = _star_project(tmp_path)
+    out = str(tmp_path / "map.mrc")
+    assert breconstruct.main(["-reconstruction", out, "-CTF", "flip", "-symmetry", "C3",
+                              "-mode", str(mode), "-seed", "5", star]) == 0
+    got = formats.read_mrc(out).data[0]
+    rec = Reconstruction(32, 32, 64, 1.5, ctf_action=1)
+    parts = ps.parts.copy()
+    if mode == 2:
+        parts["view"] = random_symmetric_views("C3", parts["view"], np.random.default_rng(5))
+    rec.insert(ps.images.cuda().contiguous(), parts)
+    ref = rec.finish(0, 1).map.cpu().numpy()
+    if mode == 1:
+        ref = sym_ref.symmetrize(ref, "C3", (16, 16, 16))
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
 @pytest.mark.parametrize("sym", ["C3", "D2", "O"])

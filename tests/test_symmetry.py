@@ -86,3 +86,113 @@ def test_bad_labels():
     for bad in ("H10,20", "X5", "C0"):
         with pytest.raises(abi.BsxError):
             symmetry_views(bad, np.array([[0, 0, 1, 0.0]]))
+
+
+# ------------------------------------------------ real-space symmetrization
+def _ball_map(n, seed, radius):
+    """Random map, zero outside `radius` of the centre (so the background and
+    the fill are 0 and no rotation reaches a nonzero voxel outside the map)."""
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(n, n, n)).astype(np.float32)
+    z, y, x = np.meshgrid(*(np.arange(n) - n // 2,) * 3, indexing="ij")
+    v[x * x + y * y + z * z > radius * radius] = 0
+    return v
+
+
+def _closure(gens):
+    G = [np.eye(3, dtype=np.int64)]
+    grew = True
+    while grew:
+        grew = False
+        for a in list(G):
+            for g in gens:
+                p = g @ a
+                if not any(np.array_equal(p, b) for b in G):
+                    G.append(p)
+                    grew = True
+    return G
+
+
+_Z4 = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1]])
+_Z2 = _Z4 @ _Z4
+_X2 = np.diag([1, -1, -1])
+_C3 = np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0]])      # 3-fold about (1, 1, 1)
+GRID_GROUPS = {"C2": [_Z2], "C4": [_Z4], "D2": [_Z2, _X2], "D4": [_Z4, _X2],
+               "T": [_C3, _Z2, _X2], "O": [_C3, _Z4, _X2]}
+
+
+@pytest.mark.parametrize("label", sorted(GRID_GROUPS))
+def test_symmetrize_oracle_is_the_group_average(label):
+    """Rotations that permute the grid: the symmetrized map is the exact
+    average of f(g (x - o) + o) over the group's elements."""
+    n = 16
+    f = _ball_map(n, 4, 6)
+    o = n // 2
+    G = _closure(GRID_GROUPS[label])
+    assert len(G) == {"C2": 2, "C4": 4, "D2": 4, "D4": 8, "T": 12, "O": 24}[label]
+    out = sym_ref.symmetrize(f, label, (o, o, o))
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    u = np.stack([x - o, y - o, z - o])
+    ref = np.zeros((n, n, n))
+    for g in G:
+        w = np.einsum("ij,jxyz->ixyz", g, u) + o
+        ok = np.all((w >= 0) & (w < n), axis=0)
+        ref += np.where(ok, f[np.clip(w[2], 0, n - 1), np.clip(w[1], 0, n - 1),
+                              np.clip(w[0], 0, n - 1)], 0.0)
+    ref /= len(G)
+    np.testing.assert_allclose(out, ref, rtol=0, atol=1e-6 * np.abs(f).max())
+
+
+@pytest.mark.parametrize("label", ["C3", "C5", "I"])
+def test_symmetrize_oracle_linear_maps(label):
+    """Trilinear interpolation reproduces a linear map exactly, so inside a
+    ball that every rotation keeps in the map the result is b + a . P (x - o),
+    P the average of the group's rotations: the projection on z for C_n, 0
+    for I."""
+    n, o = 24, 12
+    a = np.array([0.3, -0.2, 0.5])
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    f = (1.5 + a[0] * (x - o) + a[1] * (y - o) + a[2] * (z - o)).astype(np.float32)
+    out = sym_ref.symmetrize(f, label, (o, o, o))
+    r2 = (x - o) ** 2 + (y - o) ** 2 + (z - o) ** 2
+    inner = r2 <= 8 * 8
+    ref = 1.5 + (a[2] * (z - o) if label[0] == "C" else 0.0)
+    np.testing.assert_allclose(out[inner], np.broadcast_to(ref, out.shape)[inner], atol=2e-5)
+
+
+def test_background_is_the_outside_mean():
+    n = 10
+    v = np.zeros((n, n, n), np.float32)
+    v[0, 0, 0] = 8.0                 # a corner: outside the inscribed sphere
+    v[5, 5, 5] = 100.0               # the centre: inside
+    z, y, x = np.meshgrid(*(np.arange(n) - 5,) * 3, indexing="ij")
+    nout = int(np.sum((x / 5.0) ** 2 + (y / 5.0) ** 2 + (z / 5.0) ** 2 > 1))
+    assert sym_ref.background(v, (5, 5, 5)) == pytest.approx(8.0 / nout)
+
+
+def test_random_symmetric_views():
+    """Mode 2: each particle gets one of its related views; over many draws
+    every related view is picked."""
+    from bsoft_amd.recon import random_symmetric_views
+    base = np.array([[0.2, 0.3, 0.9, 0.5]] * 400)
+    allv = symmetry_views("D3", base[:1])[0]
+    pick = random_symmetric_views("D3", base, np.random.default_rng(0))
+    idx = [int(np.argmin(np.abs(allv - p).sum(axis=1))) for p in pick]
+    for p, i in zip(pick, idx):
+        np.testing.assert_allclose(p, allv[i], atol=1e-12)
+    assert set(idx) == set(range(6))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("label", ["C1", "C3", "D5", "T", "O", "I", "I90"])
+def test_gpu_symmetrize_matches_oracle(label):
+    import torch
+    from bsoft_amd.recon import symmetrize
+    n = 24
+    rng = np.random.default_rng(7)
+    f = rng.normal(size=(n, n, n)).astype(np.float32) + 0.25   # nonzero background
+    o = (n // 2, n // 2 - 1, n // 2 + 0.5)
+    print(f"symmetrize {label} n={n} origin={o}", flush=True)
+    out = symmetrize(torch.as_tensor(f).cuda(), label, o).cpu().numpy()
+    ref = sym_ref.symmetrize(f, label, o)
+    np.testing.assert_allclose(out, ref, rtol=0, atol=1e-6 * np.abs(f).max())
