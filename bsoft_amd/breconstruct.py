@@ -20,6 +20,8 @@ Options (a subset of breconstruct's, same meaning):
   -bootstrap             weight particles by their selection number
   -halfmaps              also write the two half maps (<name>_1, <name>_2)
   -fom                   also write the FOM map (<name>_fom)
+  -verbose               print the Fourier shell statistics of the full map
+                         (fspace_reconstruction_stats)
 """
 from __future__ import annotations
 
@@ -46,6 +48,7 @@ def parse_args(argv):
     ap.add_argument("-seed", type=int, default=1)
     ap.add_argument("-halfmaps", action="store_true")
     ap.add_argument("-fom", action="store_true")
+    ap.add_argument("-verbose", action="store_true")
     ap.add_argument("project")
     return ap.parse_args(argv)
 
@@ -83,7 +86,8 @@ def main(argv=None) -> int:
     ctf = {"none": abi.BSX_CTF_NONE, "flip": abi.BSX_CTF_FLIP, "apply": abi.BSX_CTF_APPLY,
            "wiener": abi.BSX_CTF_WIENER}[a.CTF]
     interp = {"nearest": 0, "weighted": 1, "trilinear": 2}[a.interpolate]
-    flags = (abi.BSX_FLAG_BOOTSTRAP if a.bootstrap else 0) | (abi.BSX_FLAG_FOM if a.fom else 0)
+    flags = ((abi.BSX_FLAG_BOOTSTRAP if a.bootstrap else 0)
+             | (abi.BSX_FLAG_FOM if a.fom or a.verbose else 0))
     ft = part_ft_size(max(box, size), 1.0, a.pad)
     rec = Reconstruction(size, box, ft, sampling, hi_res=a.resolution, interp=interp,
                          ctf_action=ctf, wiener=a.wiener, flags=flags,
@@ -101,6 +105,14 @@ def main(argv=None) -> int:
     if a.halfmaps:
         outs += [(1, 3, f"{stem}_1{ext or '.mrc'}"), (2, 3, f"{stem}_2{ext or '.mrc'}")]
     for imap, nmaps, path in outs:
+        if a.verbose and imap == 0:        # img_reconstruction_sum_weigh :346-347
+            rows, (r_snr, r_fsc) = rec.stats(imap, nmaps)
+            print("Radius\ts(1/A)\tRes(A)\tWeight\tPower\tSignal\tNoise\tSNR\tFSC")
+            for r in rows:
+                print(f"{int(r[0])}\t{r[1]:.4g}\t{r[2]:.2f}\t{r[3]:.2f}\t{r[4]:.6g}\t"
+                      f"{r[5]:.6g}\t{r[6]:.6g}\t{r[7]:.4g}\t{r[8]:.4g}")
+            print(f"\nResolution estimates:\nSNR(0.5):                       {r_snr:g} A")
+            print(f"FSC(0.3):                       {r_fsc:g} A\n")
         r = rec.finish(imap, nmaps, want_fom=a.fom and imap == 0)
         org = (size // 2,) * 3
         vol = symmetrize(r.map, a.symmetry, org) if a.mode == 1 else r.map

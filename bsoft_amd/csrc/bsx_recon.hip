@@ -1064,6 +1064,110 @@ __global__ __launch_bounds__(256) void k_weigh(const float* __restrict__ acc, lo
     }
 }
 
+// Shell sums of fspace_reconstruction_stats (Bimage_reconstruct.cpp:526-599).
+// Per voxel the values fspace_reconstruction_weigh (:486-515) leaves in the
+// four accumulator images (F / sum w, the weighed power, sum w, and
+// w - sum w^2 / w; empty voxels 0, 0, 1e-30, 1e-30; power[0] = 1), each held
+// in FP32 as the reference's images are; then
+//  - |F|^2 over the shells of fspace_radial_power (Bimage_fspace.cpp:
+//    848-925): radius = rad_scale |s|, split between the two nearest shells
+//    (sums 0: counts, 1: |F|^2), nr1 shells;
+//  - the power, weight and weight2 over the shells of Bimage::radial
+//    (Bimage_polar.cpp:79-170, origin 0, wrap): radius |(dx, dy, dz)|
+//    (sums 2: counts, 3: power, 4: weight, 5: weight2), nr2 <= nr1 shells.
+// FP64 sums in LDS per block; block partials to part[block][6][nr1].
+__global__ __launch_bounds__(256) void k_recon_stats(const float* __restrict__ acc, long long n3,
+                                                     int N, int NT, long long nrec, int stride,
+                                                     int hmask, int nhalves, double rad_scale,
+                                                     double freq_scale, int nr1, int nr2,
+                                                     double* __restrict__ part) {
+    extern __shared__ double sh[];
+    for (int i = threadIdx.x; i < 6 * nr1; i += blockDim.x) sh[i] = 0;
+    __syncthreads();
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+         i += (long long)gridDim.x * blockDim.x) {
+        double fr = 0, fi = 0, W = 0, P = 0, W2 = 0;
+        const long long xx = i % N, yy = (i / N) % N, zz = i / ((long long)N * N);
+        const long long ri = rec_index(xx, yy, zz, NT);
+        for (int hh = 0; hh < nhalves; ++hh) {
+            if (!((hmask >> hh) & 1)) continue;
+            const float* r = acc + ((size_t)hh * nrec + ri) * stride;
+            fr += r[0];
+            fi += r[1];
+            W += r[2];
+            P += r[3];
+            W2 += r[4];
+        }
+        float ofr = 0.f, ofi = 0.f;
+        double w2 = 1e-30;
+        if (W > 1e-30) {
+            w2 = W - W2 / W;
+            ofr = (float)(fr / W);
+            ofi = (float)(fi / W);
+            if (w2 > 1) P /= w2;
+            else if (W > 1) P /= W;
+            if (P < 0) P = 0;
+        } else {
+            P = 0;
+            W = 1e-30;
+        }
+        if (i == 0) P = 1;
+        const double Pf = (float)P, Wf = (float)W, W2f = (float)w2;
+        const double pw = (double)ofr * ofr + (double)ofi * ofi;
+        // fspace_radial_power: wrap above (N - 1) / 2
+        const long hx = xx > (N - 1) / 2 ? xx - N : xx;
+        const long hy = yy > (N - 1) / 2 ? yy - N : yy;
+        const long hz = zz > (N - 1) / 2 ? zz - N : zz;
+        double rx = hx * freq_scale, ry = hy * freq_scale, rz = hz * freq_scale;
+        rx *= rx;
+        ry *= ry;
+        rz *= rz;
+        const double radius = rad_scale * sqrt(rx + ry + rz);
+        const long ir = (long)radius;
+        if (ir < nr1) {
+            const double f = radius - ir, f1 = 1 - f;
+            atomicAdd(&sh[ir], f1);
+            atomicAdd(&sh[nr1 + ir], f1 * pw);
+            if (ir + 1 < nr1) {
+                atomicAdd(&sh[ir + 1], f);
+                atomicAdd(&sh[nr1 + ir + 1], f * pw);
+            }
+        }
+        // Bimage::radial, origin 0 with wrap: d = x for x <= N/2, else x - N
+        const long dx = xx > N / 2 ? xx - N : xx;
+        const long dy = yy > N / 2 ? yy - N : yy;
+        const long dz = zz > N / 2 ? zz - N : zz;
+        const double r2 = sqrt((double)(dx * dx) + (double)(dy * dy) + (double)(dz * dz));
+        const long jr = (long)r2;
+        if (jr < nr2) {
+            const double f = r2 - jr, f1 = 1 - f;
+            atomicAdd(&sh[2 * nr1 + jr], f1);
+            atomicAdd(&sh[3 * nr1 + jr], f1 * Pf);
+            atomicAdd(&sh[4 * nr1 + jr], f1 * Wf);
+            atomicAdd(&sh[5 * nr1 + jr], f1 * W2f);
+            if (jr + 1 < nr2) {
+                atomicAdd(&sh[2 * nr1 + jr + 1], f);
+                atomicAdd(&sh[3 * nr1 + jr + 1], f * Pf);
+                atomicAdd(&sh[4 * nr1 + jr + 1], f * Wf);
+                atomicAdd(&sh[5 * nr1 + jr + 1], f * W2f);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 6 * nr1; i += blockDim.x)
+        part[(size_t)blockIdx.x * 6 * nr1 + i] = sh[i];
+}
+
+// Sum of the block partials in block order: one thread per (sum, shell).
+__global__ void k_stats_sum(const double* __restrict__ part, int nblocks, int m,
+                            double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double s = 0;
+    for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * m + i];
+    out[i] = s;
+}
+
 // Friedel residual sums (Bimage::friedel_check, Bimage_fspace.cpp:2024-2074).
 __global__ __launch_bounds__(256) void k_friedel(const float2* __restrict__ v, int N,
                                                  double* __restrict__ sums) {
@@ -2326,6 +2430,105 @@ int bsx_recon_accumulators(bsx_recon* h, int half, float* planes_dev, void* stre
                        h->acc + (size_t)half * nrec * h->K.stride, n3, h->K.N, h->K.NT,
                        h->K.stride, h->K.fom, planes_dev);
     BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_recon_stats(bsx_recon* h, int imap, int nmaps, double resolution, double* rows,
+                    int32_t cap, int32_t* nrows, double* estimates, void* stream) {
+    BSX_REQUIRE(h && nrows, BSX_ERR_ARG, "bsx_recon_stats: null argument");
+    BSX_REQUIRE(h->K.fom, BSX_ERR_STATE, "bsx_recon_stats needs BSX_FLAG_FOM");
+    BSX_REQUIRE(nmaps >= 1 && nmaps <= 3 && imap >= 0 && imap < nmaps, BSX_ERR_ARG,
+                "imap %d / nmaps %d invalid", imap, nmaps);
+    BSX_REQUIRE(std::isfinite(resolution), BSX_ERR_ARG, "bsx_recon_stats: resolution not finite");
+    int hmask = 3;
+    if (nmaps == 2) hmask = 1 << (imap % 2);
+    if (nmaps == 3 && imap % 3) hmask = 1 << (imap % 3 - 1);
+    if (h->p.nhalves == 1) {
+        BSX_REQUIRE(hmask == 3, BSX_ERR_STATE, "half maps need nhalves = 2");
+        hmask = 1;
+    }
+    const int N = h->K.N;
+    const double rs = N * h->p.sampling;   // real size = rad_scale (sampling ratio 1)
+    // fspace_maximum_radius (Bimage_fspace.cpp:26-38) of the resolution as
+    // given (the table's rows) and as check_resolution clamps it (the radial
+    // power spectrum); Bimage::radial keeps at most N/2 shells
+    auto maxrad = [&](double res) -> long { return res > 0.1 ? (long)(rs / res) : (long)(N / 2.0); };
+    const long nr_stats = maxrad(resolution);
+    const long nr1 = std::max<long>(1, maxrad(check_resolution(resolution, N, h->p.sampling)));
+    long nr2 = nr_stats < 1 ? N / 2 : std::min<long>(nr_stats, N / 2);
+    nr2 = std::min(nr2, nr1);
+    const size_t shm = sizeof(double) * 6 * nr1;
+    BSX_REQUIRE(shm <= 64 * 1024, BSX_ERR_ARG, "bsx_recon_stats: %ld shells exceed the LDS table",
+                nr1);
+    hipStream_t s = (hipStream_t)stream;
+    BSX_HIP(hipSetDevice(h->dev));
+    const int rf = fold(h, s);
+    if (rf) return rf;
+    const int nblk = 1024;
+    const int m = (int)(6 * nr1);
+    double* part = nullptr;
+    BSX_HIP(hipMalloc((void**)&part, sizeof(double) * ((size_t)nblk * m + m)));
+    double* dsum = part + (size_t)nblk * m;
+    const long long n3 = (long long)N * N * N;
+    const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
+    hipLaunchKernelGGL(k_recon_stats, dim3(nblk), dim3(256), shm, s, h->acc, n3, N, h->K.NT, nrec,
+                       h->K.stride, hmask, h->p.nhalves, rs, 1.0 / rs, (int)nr1, (int)nr2, part);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_stats_sum, dim3((m + 255) / 256), dim3(256), 0, s, part, nblk, m, dsum);
+        e = hipGetLastError();
+    }
+    std::vector<double> S((size_t)m);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(S.data(), dsum, sizeof(double) * m, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(part);
+    BSX_HIP(e);
+    // radial averages: sum / count where the count is nonzero, else 0
+    auto avg = [&](int q, int c, long i) {
+        const double n = S[(size_t)c * nr1 + i];
+        return n != 0 ? S[(size_t)q * nr1 + i] / n : 0.0;
+    };
+    // the table (fspace_reconstruction_stats :547-593); rows beyond the
+    // shorter radial arrays are not defined there and are left out
+    const long nrow = std::max<long>(0, std::min(nr_stats, nr2) - 1);
+    *nrows = (int32_t)nrow;
+    double fscp = 1, snrp = 10, res_fsc = 0, res_snr = 0;
+    const double fsc_cut = 0.3, snr_cut = 0.5;
+    for (long i = 1; i <= nrow; ++i) {
+        const double sfreq = i / rs, res = rs / i;
+        const double prad = avg(1, 0, i), pfom = avg(3, 2, i);
+        const double w = avg(4, 2, i);
+        double fsc = 0, snr = 0, signal = 0, noise = 0;
+        if (w > 1e-3) {
+            const double w2 = avg(5, 2, i);
+            noise = (pfom - prad) / w2;
+            signal = (w * prad - pfom) / w2;
+            if (noise > 0) snr = signal / noise;
+            fsc = snr / (snr + 1);
+            if (fsc < 0) fsc = 0;
+        }
+        if (fscp >= fsc_cut && fsc < fsc_cut) res_fsc = 1 / (sfreq - (fsc_cut - fsc) / ((fscp - fsc) * rs));
+        if (snrp >= snr_cut && snr < snr_cut) res_snr = 1 / (sfreq - (snr_cut - snr) / ((snrp - snr) * rs));
+        fscp = fsc;
+        snrp = snr;
+        if (rows && i <= cap) {
+            double* r = rows + 9 * (i - 1);
+            r[0] = (double)i;
+            r[1] = sfreq;
+            r[2] = res;
+            r[3] = w;
+            r[4] = prad;
+            r[5] = signal;
+            r[6] = noise;
+            r[7] = snr;
+            r[8] = fsc;
+        }
+    }
+    if (estimates) {
+        estimates[0] = res_snr;
+        estimates[1] = res_fsc;
+    }
     return BSX_OK;
 }
 

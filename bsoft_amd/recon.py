@@ -181,6 +181,20 @@ class Reconstruction:
              _stream(stream))
         return FinishResult(m, fom, ft, st.coverage, st.friedel, st.particles, st.weight_sum)
 
+    def stats(self, imap: int = 0, nmaps: int = 1, resolution: float | None = None, stream=None):
+        """fspace_reconstruction_stats: (rows (n, 9): radius, s, resolution,
+        weight, power, signal, noise, SNR, FSC; (SNR(0.5), FSC(0.3)) in A).
+        Needs the FOM sums (BSX_FLAG_FOM)."""
+        res = self.params.hi_res if resolution is None else resolution
+        cap = self.size
+        rows = np.zeros((cap, 9))
+        nrow = C.c_int32()
+        est = np.zeros(2)
+        call("bsx_recon_stats", self.h, imap, nmaps, float(res),
+             rows.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(nrow),
+             est.ctypes.data_as(C.POINTER(C.c_double)), _stream(stream))
+        return rows[: nrow.value], (float(est[0]), float(est[1]))
+
 
 def particle_reconstruct(imgs: torch.Tensor, parts: np.ndarray, size: int, sampling: float,
                          hi_res: float = 0.0, interp_type: int = 0, ctf_action: int = 0,

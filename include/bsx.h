@@ -165,6 +165,21 @@ int bsx_recon_set_tuning(bsx_recon* h, int32_t batch, int32_t piece);
 /* Number of inserts served by graph replay so far (BSX_FLAG_GRAPH). */
 int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n);
 
+/* Fourier shell statistics of a weighed reconstruction, the table that
+ * fspace_reconstruction_stats (src/img/Bimage_reconstruct.cpp:526-599) prints
+ * from img_reconstruction_sum_weigh (src/mg/mg_reconstruct.cpp:346-347):
+ * radial averages of |F|^2 (fspace_radial_power), of the weighed power, the
+ * weight and w - sum w^2 / w (Bimage::radial), then per shell i >= 1 the
+ * signal, noise, SNR and FSC = SNR / (SNR + 1). Needs BSX_FLAG_FOM. imap /
+ * nmaps select the half sets as bsx_recon_finish does. resolution: the
+ * limit as passed to img_reconstruction_sum_weigh (<= 0.1: N/2 shells).
+ * rows (cap x 9, optional): radius, s (1/A), resolution (A), weight, power,
+ * signal, noise, SNR, FSC; *nrows = shells defined (those below N/2 and
+ * the resolution limit). estimates (2, optional): the SNR(0.5) and FSC(0.3)
+ * resolutions (A, 0 when not crossed). Synchronises the stream. */
+int bsx_recon_stats(bsx_recon* h, int imap, int nmaps, double resolution, double* rows,
+                    int32_t cap, int32_t* nrows, double* estimates, void* stream);
+
 /* Kernel timings of the previous bsx_recon_insert (needs BSX_FLAG_TIMING):
  * insertion kernels (scan, fill, tile accumulate or direct) and staging
  * kernels (background, row transforms) in ms, and the number of batches. */

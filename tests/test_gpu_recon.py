@@ -276,7 +276,7 @@ def _star_project(tmp_path):
     return star, ps
 
 
-def test_breconstruct_cli_on_star_project(tmp_path):
+def test_breconstruct_cli_on_star_project(tmp_path, capsys):
     """breconstruct over a STAR project and an MRC stack gives the map of the
     direct API call on the same particles, written as MRC with the sampling
     and origin of the reference's output."""
@@ -284,7 +284,10 @@ def test_breconstruct_cli_on_star_project(tmp_path):
     star, ps = _star_project(tmp_path)
     n = len(ps.parts)
     out = str(tmp_path / "map.mrc")
-    assert breconstruct.main(["-reconstruction", out, "-CTF", "flip", "-halfmaps", star]) == 0
+    assert breconstruct.main(["-reconstruction", out, "-CTF", "flip", "-halfmaps", "-verbose",
+                              star]) == 0
+    printed = capsys.readouterr().out
+    assert "SNR(0.5):" in printed and "FSC(0.3):" in printed and "Radius\ts(1/A)" in printed
     m = formats.read_mrc(out)
     assert m.data.shape == (1, 32, 32, 32) and m.sampling == pytest.approx((1.5,) * 3)
     assert m.origin == pytest.approx((16, 16, 16))

@@ -108,3 +108,81 @@ def test_bresolve_cli(tmp_path, capsys):
     c = np.loadtxt(tsv)
     rs, rf, rd = resolution_ref.fsc_dpr(a, b, 1.5)
     np.testing.assert_allclose(c[:, 1], rf, atol=2e-5)
+
+
+# -------------------------------------- reconstruction shell statistics
+def _identical_acc(n, K, seed):
+    """Accumulators of K identical unit-weight contributions S per voxel."""
+    rng = np.random.default_rng(seed)
+    S = rng.normal(size=(n, n, n)) + 1j * rng.normal(size=(n, n, n))
+    if np.isscalar(K):
+        K = np.full((n, n, n), float(K))
+    return np.stack([K * S.real, K * S.imag, K, K * np.abs(S) ** 2, K]), S
+
+
+def test_stats_closed_form_identical_contributions():
+    """K >= 3 identical contributions: F = S, power = K |S|^2 / (K - 1), weight K,
+    weight2 = K - 1, so noise = <|S|^2> / (K - 1)^2, signal =
+    K (K - 2) <|S|^2> / (K - 1)^2 and SNR = K (K - 2) in every shell."""
+    n, K, samp = 16, 5, 2.0
+    acc, S = _identical_acc(n, K, 3)
+    rows, (r_snr, r_fsc) = resolution_ref.reconstruction_stats(acc, samp, samp)
+    assert len(rows) == n // 2 - 1                       # shells 1 .. N/2 - 1
+    np.testing.assert_allclose(rows[:, 0], np.arange(1, n // 2))
+    np.testing.assert_allclose(rows[:, 1], rows[:, 0] / (n * samp))
+    np.testing.assert_allclose(rows[:, 3], K)            # radial average of a constant
+    np.testing.assert_allclose(rows[:, 7], K * (K - 2), rtol=1e-5)
+    np.testing.assert_allclose(rows[:, 8], K * (K - 2) / (K * (K - 2) + 1), rtol=1e-6)
+    np.testing.assert_allclose(rows[:, 6] * (K - 1) ** 2, rows[:, 4], rtol=1e-5)
+    assert r_snr == 0 and r_fsc == 0                     # never below the cuts
+
+
+def test_stats_resolution_limit_and_crossing():
+    """Three unit-weight contributions S + n_k with sum n_k = 0 and
+    sum |n_k|^2 = a |S|^2: power (3 + a) |S|^2 / 2, so SNR = (3 - a) / (1 + a):
+    3 inside radius 4.5 (a = 0), 1/7 outside (a = 2.5); the FSC(0.3) and
+    SNR(0.5) crossings fall between shells 4 and 6. A resolution limit
+    shortens the table to rs / res - 1 rows."""
+    n, samp = 16, 1.5
+    idx = np.arange(n)
+    d = np.where(idx > n // 2, idx - n, idx)
+    z, y, x = np.meshgrid(d, d, d, indexing="ij")
+    a = np.where(np.sqrt(x * x + y * y + z * z) < 4.5, 0.0, 2.5)
+    acc, S = _identical_acc(n, 3, 4)
+    acc[3] = (3 + a) * np.abs(S) ** 2
+    rows, _ = resolution_ref.reconstruction_stats(acc, samp, n * samp / 6)
+    assert len(rows) == 5
+    rows, (r_snr, r_fsc) = resolution_ref.reconstruction_stats(acc, samp, samp)
+    np.testing.assert_allclose(rows[:3, 7], 3, rtol=1e-5)
+    np.testing.assert_allclose(rows[5:, 7], 1 / 7, rtol=1e-5)
+    rs = n * samp
+    assert rs / 6 < r_fsc < rs / 4 and rs / 6 < r_snr < rs / 4
+
+
+@pytest.mark.gpu
+def test_gpu_recon_stats_matches_oracle():
+    """bsx_recon_stats on a reconstruction with the FOM sums against the
+    restatement evaluated on the same accumulators (both halves)."""
+    import torch
+    from bsoft_amd import abi, synth
+    from bsoft_amd.recon import Reconstruction
+    n = 32
+    rng = np.random.default_rng(9)
+    bm = synth.make_blob_map(n, 12, rng)
+    ps = synth.make_particles(bm, 300, n, rng, snr=0.5, seed=9)
+    parts = ps.parts.copy()
+    parts["half"] = np.arange(len(parts)) % 2
+    rec = Reconstruction(n, n, 64, 1.5, ctf_action=abi.BSX_CTF_FLIP, flags=abi.BSX_FLAG_FOM,
+                         nhalves=2)
+    rec.insert(ps.images.cuda().contiguous(), parts)
+    acc = (rec.accumulators(0).cpu().numpy().astype(np.float64)
+           + rec.accumulators(1).cpu().numpy().astype(np.float64))
+    for res in (1.5, 4.0):
+        print(f"recon stats n={n} resolution={res}", flush=True)
+        rows, est = rec.stats(0, 3, res)
+        ref, rest = resolution_ref.reconstruction_stats(acc, 1.5, res)
+        assert rows.shape == ref.shape and len(rows) > 3
+        for c in range(9):
+            np.testing.assert_allclose(rows[:, c], ref[:, c], rtol=1e-6,
+                                       atol=1e-9 * max(1e-30, np.abs(ref[:, c]).max()))
+        np.testing.assert_allclose(est, rest, rtol=1e-6)
