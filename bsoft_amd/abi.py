@@ -23,6 +23,7 @@ BSX_CTF_FLIP = 1
 BSX_CTF_APPLY = 2
 BSX_CTF_WIENER = 4
 
+BSX_FLAG_2D = 2
 BSX_FLAG_BOOTSTRAP = 4
 BSX_FLAG_EWALD = 8
 BSX_FLAG_FOM = 16

@@ -18,6 +18,7 @@ Options (a subset of breconstruct's, same meaning):
                          symmetrize the map in real space, 2 insert once at a
                          randomly picked related view (-seed n, default 1)
   -bootstrap             weight particles by their selection number
+  -TwoD                  a 2D reconstruction (one plane, size x size x 1)
   -halfmaps              also write the two half maps (<name>_1, <name>_2)
   -fom                   also write the FOM map (<name>_fom)
   -verbose               print the Fourier shell statistics of the full map
@@ -43,6 +44,7 @@ def parse_args(argv):
     ap.add_argument("-wiener", type=float, default=0.0)
     ap.add_argument("-interpolate", default="nearest", choices=["nearest", "weighted", "trilinear"])
     ap.add_argument("-bootstrap", action="store_true")
+    ap.add_argument("-TwoD", action="store_true")
     ap.add_argument("-symmetry", default="C1")
     ap.add_argument("-mode", type=int, default=0, choices=[0, 1, 2])
     ap.add_argument("-seed", type=int, default=1)
@@ -87,7 +89,8 @@ def main(argv=None) -> int:
            "wiener": abi.BSX_CTF_WIENER}[a.CTF]
     interp = {"nearest": 0, "weighted": 1, "trilinear": 2}[a.interpolate]
     flags = ((abi.BSX_FLAG_BOOTSTRAP if a.bootstrap else 0)
-             | (abi.BSX_FLAG_FOM if a.fom or a.verbose else 0))
+             | (abi.BSX_FLAG_FOM if a.fom or (a.verbose and not a.TwoD) else 0)
+             | (abi.BSX_FLAG_2D if a.TwoD else 0))
     ft = part_ft_size(max(box, size), 1.0, a.pad)
     rec = Reconstruction(size, box, ft, sampling, hi_res=a.resolution, interp=interp,
                          ctf_action=ctf, wiener=a.wiener, flags=flags,
@@ -105,7 +108,7 @@ def main(argv=None) -> int:
     if a.halfmaps:
         outs += [(1, 3, f"{stem}_1{ext or '.mrc'}"), (2, 3, f"{stem}_2{ext or '.mrc'}")]
     for imap, nmaps, path in outs:
-        if a.verbose and imap == 0:        # img_reconstruction_sum_weigh :346-347
+        if a.verbose and imap == 0 and not a.TwoD:   # img_reconstruction_sum_weigh :346-347
             rows, (r_snr, r_fsc) = rec.stats(imap, nmaps)
             print("Radius\ts(1/A)\tRes(A)\tWeight\tPower\tSignal\tNoise\tSNR\tFSC")
             for r in rows:
@@ -114,7 +117,7 @@ def main(argv=None) -> int:
             print(f"\nResolution estimates:\nSNR(0.5):                       {r_snr:g} A")
             print(f"FSC(0.3):                       {r_fsc:g} A\n")
         r = rec.finish(imap, nmaps, want_fom=a.fom and imap == 0)
-        org = (size // 2,) * 3
+        org = (size // 2, size // 2, 0 if a.TwoD else size // 2)
         vol = symmetrize(r.map, a.symmetry, org) if a.mode == 1 else r.map
         write_mrc(path, vol.cpu().numpy(), (sampling,) * 3, org)
         if r.fom is not None:

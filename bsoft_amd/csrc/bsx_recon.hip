@@ -59,6 +59,11 @@ struct ReconConst {
     // pairs: binned entries stand for a sample and its Friedel mate (value
     // at v, conjugate at -v), folded into the volume by k_fold
     double mins2, maxs2, mscale, shi2, wiener;
+    // z extent: a 2D map (breconstruct -TwoD) has one plane (nz = 1) and z
+    // real size 1 (particle_reconstruct sets the z sampling to 1,
+    // mg_reconstruct.cpp:117), which scales m_z and the resolution taper
+    double mscz;
+    int nz;
 };
 
 // ------------------------------------------------------------------- K1
@@ -566,7 +571,7 @@ __device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev&
     *fi = 0.f;
     if (!inband) return false;
     double ww = P.wp;
-    const double d = (K.maxs2 - *s2) * K.mscale;
+    const double d = (K.maxs2 - *s2) * K.mscz;
     if (d < 1) ww *= sqrt(d);
     *w = ww;
     float sn, cs;
@@ -621,7 +626,7 @@ __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const Recon
             const double dx = m[0] - fx, dy = m[1] - fy, dz = m[2] - fz;
             ww *= 1 - sqrt(dx * dx + dy * dy + dz * dz);
         }
-        const int cx = wrap32((int)fx, K.N), cy = wrap32((int)fy, K.N), cz = wrap32((int)fz, K.N);
+        const int cx = wrap32((int)fx, K.N), cy = wrap32((int)fy, K.N), cz = wrap32((int)fz, K.nz);
         emit<MG, FOM, MODE, FAST>(acc, K.stride, merge,
                             make_emit<FOM>(keybase + rec_index32(cx, cy, cz, K.NT), kk,
                                            inband && ww > 0, fr, fi, ww, bin),
@@ -633,7 +638,7 @@ __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const Recon
 #pragma unroll
         for (int corner = 0; corner < 8; ++corner) {
             const int xx = corner & 1, yy = (corner >> 1) & 1, zz = corner >> 2;
-            const int jx = wrap32(cx + xx, K.N), jy = wrap32(cy + yy, K.N), jz = wrap32(cz + zz, K.N);
+            const int jx = wrap32(cx + xx, K.N), jy = wrap32(cy + yy, K.N), jz = wrap32(cz + zz, K.nz);
             const double ww = (xx ? dx : 1 - dx) * (yy ? dy : 1 - dy) * (zz ? dz : 1 - dz) * w;
             emit<MG, FOM, MODE, FAST>(acc, K.stride, merge,
                                 make_emit<FOM>(keybase + rec_index32(jx, jy, jz, K.NT), kk,
@@ -647,7 +652,7 @@ __device__ __forceinline__ void coords(const ReconConst& K, const PartDev& P, do
                                        double sz, double m[3]) {
     m[0] = (s0 * P.R[0] + s1 * P.R[1] + sz * P.R[2]) * K.mscale;
     m[1] = (s0 * P.R[3] + s1 * P.R[4] + sz * P.R[5]) * K.mscale;
-    m[2] = (s0 * P.R[6] + s1 * P.R[7] + sz * P.R[8]) * K.mscale;
+    m[2] = (s0 * P.R[6] + s1 * P.R[7] + sz * P.R[8]) * K.mscz;
 }
 
 // One sample (h, k) with raw value F and the Friedel mate of its array
@@ -1169,15 +1174,15 @@ __global__ void k_stats_sum(const double* __restrict__ part, int nblocks, int m,
 }
 
 // Friedel residual sums (Bimage::friedel_check, Bimage_fspace.cpp:2024-2074).
-__global__ __launch_bounds__(256) void k_friedel(const float2* __restrict__ v, int N,
+__global__ __launch_bounds__(256) void k_friedel(const float2* __restrict__ v, int N, int nz,
                                                  double* __restrict__ sums) {
     const long long nx2 = (N + 1) / 2;
-    const long long total = nx2 * N * N;
+    const long long total = nx2 * N * nz;
     double s = 0, rr = 0, ri = 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (long long)gridDim.x * blockDim.x) {
         const long long xx = i % nx2, yy = (i / nx2) % N, zz = i / (nx2 * N);
-        const long long xf = xx ? N - xx : 0, yf = yy ? N - yy : 0, zf = zz ? N - zz : 0;
+        const long long xf = xx ? N - xx : 0, yf = yy ? N - yy : 0, zf = zz ? nz - zz : 0;
         const float2 a = v[(zz * N + yy) * N + xx];
         const float2 b = v[(zf * N + yf) * N + xf];
         const double w = ((double)a.x * a.x + (double)a.y * a.y + (double)b.x * b.x + (double)b.y * b.y) / 2;
@@ -1334,10 +1339,11 @@ static int tmark(bsx_recon* h, hipStream_t s, int label) {
     return BSX_OK;
 }
 
-static double check_resolution(double res, int n, double sampling) {
-    // Bimage::check_resolution (Bimage.cpp:633-648), cubic 3D map
+static double check_resolution(double res, int n, double sampling, int ndim = 3) {
+    // Bimage::check_resolution (Bimage.cpp:633-648): a square or cubic map
+    // with ndim axes longer than one voxel
     if (res > n * sampling) return n * sampling;
-    double rm = 2 / sqrt(3.0 / (sampling * sampling));
+    double rm = 2 / sqrt(ndim / (sampling * sampling));
     return res < rm ? rm : res;
 }
 
@@ -1796,7 +1802,8 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     bsx_recon* h = new bsx_recon();
     h->p = *p;
     BSX_HIP(hipGetDevice(&h->dev));
-    h->hi_res = check_resolution(p->hi_res > 0 ? p->hi_res : p->sampling, p->size, p->sampling);
+    h->hi_res = check_resolution(p->hi_res > 0 ? p->hi_res : p->sampling, p->size, p->sampling,
+                                 (p->flags & BSX_FLAG_2D) ? 2 : 3);
     ReconConst& K = h->K;
     K.N = p->size;
     K.ft = p->ft_size;
@@ -1816,6 +1823,8 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     K.mins2 = p->lo_res > 0 ? 1.0 / (p->lo_res * p->lo_res) : 0;
     K.maxs2 = 1.0 / (hi * hi);
     K.mscale = p->size * p->sampling;
+    K.mscz = (p->flags & BSX_FLAG_2D) ? 1.0 : K.mscale;
+    K.nz = (p->flags & BSX_FLAG_2D) ? 1 : p->size;
     K.shi2 = 1.0 / (h->hi_res * h->hi_res);          // img_ctf_calculate hires
     K.ewald = (p->flags & BSX_FLAG_EWALD) ? 1 : 0;
     K.merge = 0;
@@ -2437,6 +2446,7 @@ int bsx_recon_stats(bsx_recon* h, int imap, int nmaps, double resolution, double
                     int32_t cap, int32_t* nrows, double* estimates, void* stream) {
     BSX_REQUIRE(h && nrows, BSX_ERR_ARG, "bsx_recon_stats: null argument");
     BSX_REQUIRE(h->K.fom, BSX_ERR_STATE, "bsx_recon_stats needs BSX_FLAG_FOM");
+    BSX_REQUIRE(h->K.nz == h->K.N, BSX_ERR_STATE, "bsx_recon_stats: 3D maps only");
     BSX_REQUIRE(nmaps >= 1 && nmaps <= 3 && imap >= 0 && imap < nmaps, BSX_ERR_ARG,
                 "imap %d / nmaps %d invalid", imap, nmaps);
     BSX_REQUIRE(std::isfinite(resolution), BSX_ERR_ARG, "bsx_recon_stats: resolution not finite");
@@ -2549,8 +2559,8 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
     BSX_HIP(hipSetDevice(h->dev));
     const int rf = fold(h, s);
     if (rf) return rf;
-    const int N = h->K.N;
-    const long long n3 = (long long)N * N * N;
+    const int N = h->K.N, nz = h->K.nz;
+    const long long n3 = (long long)N * N * nz;   // a 2D map: one plane
     float2* vol = (float2*)ft_dev;
     float2* tmp = nullptr;
     if (!vol) {
@@ -2566,7 +2576,7 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
     hipLaunchKernelGGL(k_weigh, dim3(2048), dim3(256), 0, s, h->acc, n3, N, h->K.NT, nrec,
                        h->K.stride, hmask, h->p.nhalves, h->K.fom, vol, fom_dev, dcov, dsum);
     BSX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_friedel, dim3(1024), dim3(256), 0, s, vol, N, dsum + 1);
+    hipLaunchKernelGGL(k_friedel, dim3(1024), dim3(256), 0, s, vol, N, nz, dsum + 1);
     BSX_HIP(hipGetLastError());
     unsigned long long hcov = 0;
     double hsum[4] = {0, 0, 0, 0};
@@ -2583,9 +2593,13 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
             work = wtmp;
         }
         const float sc = (float)(1.0 / sqrt((double)n3));
-        r = fft_axis_ex(work, nullptr, N, N, N, 1, 0, +1, 1.0f, s);
-        if (r == BSX_OK) r = fft_axis_ex(work, nullptr, N, N, N, 1, 1, +1, 1.0f, s);
-        if (r == BSX_OK) r = fft_axis_ex(work, map_dev, N, N, N, 1, 2, +1, sc, s);
+        r = fft_axis_ex(work, nullptr, N, N, nz, 1, 0, +1, 1.0f, s);
+        if (nz == 1) {
+            if (r == BSX_OK) r = fft_axis_ex(work, map_dev, N, N, 1, 1, 1, +1, sc, s);
+        } else {
+            if (r == BSX_OK) r = fft_axis_ex(work, nullptr, N, N, N, 1, 1, +1, 1.0f, s);
+            if (r == BSX_OK) r = fft_axis_ex(work, map_dev, N, N, N, 1, 2, +1, sc, s);
+        }
         BSX_HIP(hipStreamSynchronize(s));
         if (wtmp) BSX_HIP(hipFree(wtmp));
     }

@@ -173,9 +173,10 @@ class Reconstruction:
     def finish(self, imap: int = 0, nmaps: int = 1, want_fom: bool = False,
                want_ft: bool = False, want_map: bool = True, stream=None) -> FinishResult:
         n = self.size
-        m = torch.empty((n, n, n), device="cuda", dtype=torch.float32) if want_map else None
-        fom = torch.empty((n, n, n), device="cuda", dtype=torch.float32) if want_fom else None
-        ft = torch.empty((n, n, n), device="cuda", dtype=torch.complex64) if want_ft else None
+        shp = (1 if self.params.flags & abi.BSX_FLAG_2D else n, n, n)   # 2D: one plane
+        m = torch.empty(shp, device="cuda", dtype=torch.float32) if want_map else None
+        fom = torch.empty(shp, device="cuda", dtype=torch.float32) if want_fom else None
+        ft = torch.empty(shp, device="cuda", dtype=torch.complex64) if want_ft else None
         st = Stats()
         call("bsx_recon_finish", self.h, imap, nmaps, _ptr(m), _ptr(fom), _ptr(ft), C.byref(st),
              _stream(stream))

@@ -36,6 +36,8 @@ extern "C" {
 
 /* ---- reconstruction flags: src/mg/mg_reconstruct.cpp:83 (1 rescale,
  *      2 2D, 4 bootstrap, 8 Ewald) plus library-specific bits ---- */
+#define BSX_FLAG_2D          2   /* 2D map (breconstruct -TwoD): one z plane,
+                                    map / FOM / transform outputs N x N     */
 #define BSX_FLAG_BOOTSTRAP   4
 #define BSX_FLAG_EWALD       8
 #define BSX_FLAG_FOM        16   /* also accumulate sum(w|F|^2) and sum(w^2) */

@@ -309,7 +309,8 @@ int or_stage_particle(const bsx_recon_params* rp, const bsx_particle* pp, const 
 {
     long box = rp->box, ft = rp->ft_size;
     if (box < 2 || ft < box) return -1;
-    double hi = or_check_resolution(rp->hi_res, rp->size, rp->sampling, 3);
+    /* a 2D map (flags & 2) has one z plane: its check_resolution counts two axes */
+    double hi = or_check_resolution(rp->hi_res, rp->size, rp->sampling, (rp->flags & 2) ? 2 : 3);
     double bkg = or_background(img, box, pp->hdr_ori);
     double w = 1;
     if (rp->flags & BSX_FLAG_BOOTSTRAP) w = pp->sel;
@@ -366,7 +367,7 @@ typedef struct {
 struct or_recon {
     bsx_recon_params p;
     double hi_res;
-    long N, N3;
+    long N, N3, Nz;   /* Nz = 1 for a 2D map (breconstruct -TwoD), else N */
     int T;
     or_acc* acc;
     long next;   /* running particle index (round-robin partition) */
@@ -385,7 +386,7 @@ static void acc_add(or_acc* a, long j, double cre, double cim, double w)
 }
 
 /* Bimage::fspace_2D_interpolate, Bimage_reconstruct.cpp:25-88 */
-static void interp_2d(or_acc* a, long N, double cre, double cim, const double m[3],
+static void interp_2d(or_acc* a, long N, long Nz, double cre, double cim, const double m[3],
                       double part_weight, int interp)
 {
     if (interp < 2) {
@@ -398,8 +399,9 @@ static void interp_2d(or_acc* a, long N, double cre, double cim, const double m[
         }
         if (w > 0) {
             for (int i = 0; i < 3; i++) {                 /* index_wrap, Bimage.h:418 */
-                while (c[i] < 0) c[i] += N;
-                while (c[i] >= N) c[i] -= N;
+                const long L = i == 2 ? Nz : N;
+                while (c[i] < 0) c[i] += L;
+                while (c[i] >= L) c[i] -= L;
             }
             acc_add(a, (c[2] * N + c[1]) * N + c[0], cre, cim, w);
         }
@@ -409,7 +411,8 @@ static void interp_2d(or_acc* a, long N, double cre, double cim, const double m[
         for (int i = 0; i < 3; i++) { c[i] = (long)floor(m[i]); f[i] = m[i] - c[i]; }
         for (int zz = 0; zz < 2; zz++) {
             long jz = c[2] + zz;
-            if (jz < 0) jz += N; else if (jz >= N) jz -= N;
+            while (jz < 0) jz += Nz;
+            while (jz >= Nz) jz -= Nz;
             double wz = zz ? f[2] : 1 - f[2];
             for (int yy = 0; yy < 2; yy++) {
                 long jy = c[1] + yy;
@@ -440,6 +443,9 @@ static void pack_2d(or_acc* a, const or_recon* r, const orr* F, const double R[9
     double scale = pp->mag > 0 ? 1.0 / pp->mag : 1.0;
     double invscale = scale / (ft * pp->pixel_size);
     double mscale = N * r->p.sampling;
+    /* real size along z: N sampling in 3D, 1 x 1 in 2D (particle_reconstruct
+     * sets the z sampling to 1, mg_reconstruct.cpp:117) */
+    double mscz = (r->p.flags & 2) ? 1.0 : mscale;
     double ew = (r->p.flags & BSX_FLAG_EWALD) ? or_lambda(pp->volt) / 2 : 0;
     long hx = (ft - 1) / 2;
     double s[3] = {0, 0, 0}, m[3];
@@ -454,17 +460,19 @@ static void pack_2d(or_acc* a, const or_recon* r, const orr* F, const double R[9
             double s2 = s[0] * s[0] + s[1] * s[1];
             if (s2 >= mins2 && s2 <= maxs2) {
                 double w = part_weight;
-                double d = (maxs2 - s2) * mscale;
+                double d = (maxs2 - s2) * mscz;
                 if (d < 1) w *= sqrt(d);
                 if (ew) s[2] = ew * s2;
                 for (int k = 0; k < 3; k++)
-                    m[k] = (s[0] * R[3 * k] + s[1] * R[3 * k + 1] + s[2] * R[3 * k + 2]) * mscale;
-                interp_2d(a, N, F[2 * i], F[2 * i + 1], m, w, r->p.interp);
+                    m[k] = (s[0] * R[3 * k] + s[1] * R[3 * k + 1] + s[2] * R[3 * k + 2]) *
+                           (k == 2 ? mscz : mscale);
+                interp_2d(a, N, r->Nz, F[2 * i], F[2 * i + 1], m, w, r->p.interp);
                 if (ew) {
                     s[2] = -s[2];
                     for (int k = 0; k < 3; k++)
-                        m[k] = (s[0] * R[3 * k] + s[1] * R[3 * k + 1] + s[2] * R[3 * k + 2]) * mscale;
-                    interp_2d(a, N, F[2 * i], F[2 * i + 1], m, w, r->p.interp);
+                        m[k] = (s[0] * R[3 * k] + s[1] * R[3 * k + 1] + s[2] * R[3 * k + 2]) *
+                               (k == 2 ? mscz : mscale);
+                    interp_2d(a, N, r->Nz, F[2 * i], F[2 * i + 1], m, w, r->p.interp);
                 }
             }
         }
@@ -477,9 +485,10 @@ or_recon* or_recon_new(const bsx_recon_params* rp, int npartial)
     or_recon* r = (or_recon*)calloc(1, sizeof(or_recon));
     r->p = *rp;
     r->N = rp->size;
-    r->N3 = r->N * r->N * r->N;
+    r->Nz = (rp->flags & 2) ? 1 : rp->size;
+    r->N3 = r->N * r->N * r->Nz;
     r->T = npartial;
-    r->hi_res = or_check_resolution(rp->hi_res, rp->size, rp->sampling, 3);
+    r->hi_res = or_check_resolution(rp->hi_res, rp->size, rp->sampling, (rp->flags & 2) ? 2 : 3);
     r->acc = (or_acc*)calloc(npartial, sizeof(or_acc));
     for (int t = 0; t < npartial; t++) {
         r->acc[t].F = (orr*)calloc(2 * r->N3, sizeof(orr));
@@ -586,7 +595,7 @@ double or_friedel(const orr* d, long x, long y, long z)
 int or_recon_finish(or_recon* r, int imap, int nmaps, orr* map_out, orr* fom_out,
                     orr* ft_out, bsx_stats* st)
 {
-    long N = r->N, N3 = r->N3;
+    long N = r->N, N3 = r->N3, Nz = r->Nz;
     int T = r->T;
     long s0 = 0, inc = 1;
     if (nmaps == 2) { s0 = imap % 2; inc = 2; }
@@ -628,7 +637,7 @@ int or_recon_finish(or_recon* r, int imap, int nmaps, orr* map_out, orr* fom_out
         }
     }
     P[0] = 1;
-    double fr = or_friedel(F, N, N, N);
+    double fr = or_friedel(F, N, N, Nz);
     /* SNR figure of merit */
     for (long i = 0; i < N3; i++) {
         double signal = (double)F[2 * i] * F[2 * i] + (double)F[2 * i + 1] * F[2 * i + 1];
@@ -642,8 +651,8 @@ int or_recon_finish(or_recon* r, int imap, int nmaps, orr* map_out, orr* fom_out
     P[0] = 1;
     /* phase shift to centre: shift = size/2, t = shift/size * (-2pi) */
     double t = -2 * M_PI * (double)(N / 2) / N;
-    for (long zz = 0, i = 0; zz < N; zz++) {
-        long l = wrap_freq(zz, N);
+    for (long zz = 0, i = 0; zz < Nz; zz++) {
+        long l = wrap_freq(zz, Nz);
         for (long yy = 0; yy < N; yy++) {
             long k = wrap_freq(yy, N);
             for (long xx = 0; xx < N; xx++, i++) {
@@ -658,7 +667,7 @@ int or_recon_finish(or_recon* r, int imap, int nmaps, orr* map_out, orr* fom_out
     }
     if (ft_out) memcpy(ft_out, F, sizeof(orr) * 2 * N3);
     if (fom_out) memcpy(fom_out, P, sizeof(orr) * N3);
-    if (or_fft(F, N, N, N, 1, +1, 1)) { free(F); free(P); free(W); free(W2); return -1; }
+    if (or_fft(F, N, N, Nz, 1, +1, 1)) { free(F); free(P); free(W); free(W2); return -1; }
     if (map_out)
         for (long i = 0; i < N3; i++) map_out[i] = F[2 * i];
     if (st) {

@@ -215,17 +215,21 @@ class OracleRecon:
         r = self.o.L.or_recon_insert(self.h, arr, self.o._p(imgs), len(parts), int(nthreads))
         assert r == 0
 
+    def _shape(self):
+        n = self.rp.size
+        return (1 if self.rp.flags & 2 else n, n, n)     # flags & 2: a 2D map
+
     def partial_acc(self, t: int) -> np.ndarray:
-        n3 = self.rp.size ** 3
-        out = np.zeros(5 * n3, dtype=self.o.dt)
+        shp = self._shape()
+        out = np.zeros(5 * int(np.prod(shp)), dtype=self.o.dt)
         assert self.o.L.or_recon_partial_acc(self.h, int(t), self.o._p(out)) == 0
-        return out.reshape(5, self.rp.size, self.rp.size, self.rp.size)
+        return out.reshape(5, *shp)
 
     def finish(self, imap=0, nmaps=1, want_ft=False, want_fom=False):
-        n = self.rp.size
-        m = np.zeros((n, n, n), dtype=self.o.dt)
-        fom = np.zeros((n, n, n), dtype=self.o.dt) if want_fom else None
-        ft = np.zeros((n, n, n), dtype=self.o.cdt) if want_ft else None
+        shp = self._shape()
+        m = np.zeros(shp, dtype=self.o.dt)
+        fom = np.zeros(shp, dtype=self.o.dt) if want_fom else None
+        ft = np.zeros(shp, dtype=self.o.cdt) if want_ft else None
         st = Stats()
         r = self.o.L.or_recon_finish(self.h, int(imap), int(nmaps), self.o._p(m),
                                      self.o._p(fom) if fom is not None else None,

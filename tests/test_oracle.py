@@ -274,6 +274,32 @@ def test_full_chain_closed_form(oracle64):
     assert st.friedel < 1e-12
 
 
+def test_two_d_chain_closed_form(oracle64):
+    """A 2D map (flags & 2: one z plane, z real size 1): one particle at the
+    identity view with ft = N. Every in-band sample lands on its own voxel of
+    the single plane with weight sqrt(maxs2 - s^2) (the taper with z real
+    size 1 is below one everywhere); the finished transform is the particle's
+    DFT / N inside the band and the map its band-limited image."""
+    N, samp, hi = 16, 1.5, 4.0
+    rng = np.random.default_rng(6)
+    img = rng.normal(size=(N, N)).astype(np.float32)
+    r = OracleRecon(RP(size=N, box=N, ft_size=N, sampling=samp, hi_res=hi, flags=2), 1)
+    r.insert([P(view=(0, 0, 1, 0), ori=(N // 2, N // 2), hdr_ori=(N // 2, N // 2), pix=samp)],
+             img[None])
+    acc = r.partial_acc(0)
+    assert acc.shape == (5, 1, N, N)
+    h = np.fft.fftfreq(N, 1 / N)
+    s2 = (h[None, :] ** 2 + h[:, None] ** 2) / (N * samp) ** 2
+    band = s2 < 1 / hi ** 2
+    np.testing.assert_allclose(acc[2, 0], np.where(band, np.sqrt(np.maximum(1 / hi ** 2 - s2, 0)), 0),
+                               atol=1e-14)
+    m, _, ft, st = r.finish(want_ft=True)
+    F = np.fft.fft2(img.astype(np.float64))
+    np.testing.assert_allclose(ft[0], np.where(band, F / N, 0), atol=1e-12)
+    np.testing.assert_allclose(m[0], np.fft.ifft2(np.where(band, F, 0)).real, atol=1e-12)
+    assert st.coverage == int(band.sum())
+
+
 def test_weight_sum_counts_samples(oracle64):
     """Sum of weights = inserted samples x particle weight, for every
     interpolation (trilinear weights sum to one) and independent of the CTF."""
