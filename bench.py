@@ -422,7 +422,8 @@ def insertion_roofline(kt_sum, launches, nparticles, box, ft, N):
     dom_ms = max(cand[dom], 1e-9)
     achieved = kernel_bytes[dom] / (dom_ms * 1e-3) / 1e9
     ins_ms = sum(kt_sum.get(k, 0.0) for k in ("scan", "fill", "tile", "direct"))
-    kname = {"fill": f"k_cols<{ft}, false, 2>", "tile": "k_tile_accum",
+    # the fill of nearest-neighbour insertion runs the staged specialisation
+    kname = {"fill": f"k_cols<{ft}, false, 2, true>", "tile": "k_tile_accum",
              "direct": f"k_cols<{ft}, false, 0>"}[dom]
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -437,7 +438,7 @@ def insertion_roofline(kt_sum, launches, nparticles, box, ft, N):
 
 def pipeline_roofline(out, s, P, args, recon_roof):
     """The learned step's dominant kernel is the insertion's fill pass
-    (k_cols<512,false,2>): its HBM roofline, timed with HIP events in the
+    (k_cols<512,false,2,true>): its HBM roofline, timed with HIP events in the
     known-view reconstruction (same kernel, same particles), is `roofline`.
     The GEMM (MFMA-bound: 2 * particles * views * K flops) and the
     refinement's gathers are reported beside it."""
