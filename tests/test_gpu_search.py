@@ -309,3 +309,42 @@ def test_fit_origin_matches_oracle(ctf):
         assert ori[i, 0] == pytest.approx(ox, abs=1e-9) and ori[i, 1] == pytest.approx(oy, abs=1e-9)
         assert dfc[i] == pytest.approx(d, rel=1e-12)
         assert fom[i] == pytest.approx(f, abs=2e-5)
+
+
+def test_chunked_staging_keeps_every_record():
+    """A box without the fused staging kernel (48 = 3 x 16) and more
+    particles than one 256-MB spectrum chunk holds (14563 at 48^2): the disk
+    values of a particle in the second chunk match the oracle, and the
+    origin/defocus fit, which reads every particle's record, matches the FP64
+    restatement for particles in both chunks."""
+    n, R, npart = 48, 12, 15000
+    rng = np.random.default_rng(47)
+    bm = synth.make_blob_map(n, 12, rng)
+    ft = map_transform(bm.volume().cuda())
+    ps = synth.make_particles(bm, npart, n, rng, snr=2.0, seed=47, device="cuda")
+    parts = ps.parts.copy()
+    parts["ori"] += rng.uniform(-1, 1, size=(npart, 2))
+    say(f"chunked staging box={n} particles={npart}")
+    s = search.Search(n, R, 1.5)
+    s.stage(ps.images.contiguous(), parts)
+    truth = synth.view_to_matrix(parts["view"])
+    ori, dfc, fom = s.fit_origin(torch.as_tensor(truth), 1.0, 0.5, 400.0, ft=ft)
+    v, c, _ = s.particle_data()
+    torch.cuda.synchronize()
+    o = Oracle(64)
+    hk, rs = s.points()
+    plist = synth.particles_list(parts)
+    ftc = ft.cpu().numpy()
+    for i in (3, npart - 2):
+        img = ps.images[i].cpu().numpy()
+        ov, oc = sr.particle_disk(img, plist[i], hk, o)
+        assert np.abs(v[i].cpu().numpy() - ov).max() <= 1e-5 * np.abs(ov).max()
+        F = np.fft.fft2(img.astype(np.float64)) / n
+        osec = o.central_section(ftc, 1.5, truth[i], 1.5, 8, 2, 0)
+        (ox, oy), d, f, _ = sr.fit_grid(F[hk[:, 1] % n, hk[:, 0] % n], plist[i],
+                                        osec[hk[:, 1] % n, hk[:, 0] % n], hk, rs, n, o,
+                                        1.0, 0.5, 400.0)
+        assert ori[i, 0].item() == pytest.approx(ox, abs=1e-9)
+        assert ori[i, 1].item() == pytest.approx(oy, abs=1e-9)
+        assert dfc[i].item() == pytest.approx(d, rel=1e-12)
+        assert fom[i].item() == pytest.approx(f, abs=2e-5)
