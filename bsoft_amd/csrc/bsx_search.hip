@@ -1703,7 +1703,8 @@ int bsx_search_score_mats(bsx_search* h, const float* ft_dev, const double* mats
 int bsx_search_fit_origin(bsx_search* h, const float* ft_dev, const double* mats_dev, double step,
                           double accuracy, double def_std, double* origin_dev, double* defocus_dev,
                           float* fom_dev, void* stream) {
-    BSX_REQUIRE(h && ft_dev && mats_dev && origin_dev && fom_dev, BSX_ERR_ARG,
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_search_fit_origin: null handle");
+    BSX_REQUIRE(h->nparts == 0 || (ft_dev && mats_dev && origin_dev && fom_dev), BSX_ERR_ARG,
                 "bsx_search_fit_origin: null argument");
     BSX_REQUIRE(step > 0 && accuracy > 0 && std::isfinite(step) && std::isfinite(accuracy) &&
                     step / accuracy < 1e6 && def_std >= 0 && std::isfinite(def_std),

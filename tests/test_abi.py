@@ -53,3 +53,21 @@ def test_errors_are_status_codes():
     p = abi.ReconParams(size=64, box=64, ft_size=100, sampling=1.5, nhalves=1)
     assert L.bsx_recon_create(C.byref(h), C.byref(p)) == abi.BSX_ERR_ARG
     assert b"ft_size" in L.bsx_last_error()
+
+
+def test_argument_errors_of_map_and_fit_entry_points():
+    """Argument checks that fail before any device call: status codes and
+    messages, no GPU needed."""
+    L = abi.lib()
+    buf = C.c_void_p(0x1000)       # never dereferenced: the checks fail first
+    org = (C.c_double * 3)(8, 8, 8)
+    assert L.bsx_symmetrize(buf, buf, 16, org, b"C3", None) == abi.BSX_ERR_ARG
+    assert b"differ" in L.bsx_last_error()
+    assert L.bsx_symmetrize(None, buf, 16, org, b"C3", None) == abi.BSX_ERR_ARG
+    bad = (C.c_double * 3)(float("nan"), 8, 8)
+    assert L.bsx_symmetrize(buf, C.c_void_p(0x2000), 16, bad, b"C3", None) == abi.BSX_ERR_ARG
+    assert L.bsx_symmetrize(buf, C.c_void_p(0x2000), 16, org, b"X5", None) == abi.BSX_ERR_ARG
+    n = C.c_int32()
+    assert L.bsx_recon_stats(None, 0, 1, 2.0, None, 0, C.byref(n), None, None) == abi.BSX_ERR_ARG
+    assert L.bsx_search_fit_origin(None, buf, buf, 1.0, 0.25, 0.0, buf, None, buf,
+                                   None) == abi.BSX_ERR_ARG

@@ -348,3 +348,26 @@ def test_chunked_staging_keeps_every_record():
         assert ori[i, 1].item() == pytest.approx(oy, abs=1e-9)
         assert dfc[i].item() == pytest.approx(d, rel=1e-12)
         assert fom[i].item() == pytest.approx(f, abs=2e-5)
+
+
+def test_fit_origin_edge_cases():
+    """No staged particles: nothing to do. A step or accuracy that is not
+    positive, or a step / accuracy ratio of 1e6 or more, is an argument error."""
+    from bsoft_amd.abi import BsxError
+    n, R = 32, 8
+    rng = np.random.default_rng(3)
+    bm = synth.make_blob_map(n, 6, rng)
+    ft = map_transform(bm.volume().cuda())
+    s = search.Search(n, R, 1.5)
+    s.stage(torch.empty((0, n, n), device="cuda"), synth.make_particles(bm, 1, n, rng).parts[:0])
+    ori, dfc, fom = s.fit_origin(torch.empty((0, 3, 3), dtype=torch.float64), ft=ft)
+    assert ori.shape == (0, 2) and fom.shape == (0,)
+    ps = synth.make_particles(bm, 4, n, rng, snr=None)
+    s.stage(ps.images.cuda().contiguous(), ps.parts)
+    mats = torch.as_tensor(synth.view_to_matrix(ps.parts["view"]))
+    for step, acc in ((0.0, 0.25), (1.0, 0.0), (1.0, 1e-7)):
+        with pytest.raises(BsxError):
+            s.fit_origin(mats, step, acc, ft=ft)
+    ori, _, _ = s.fit_origin(mats, 1.0, 1.0, ft=ft)
+    torch.cuda.synchronize()
+    assert np.all(np.abs(ori.cpu().numpy() - n // 2) <= 1.0)
