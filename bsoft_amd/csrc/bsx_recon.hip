@@ -897,6 +897,22 @@ __device__ __forceinline__ void lds_fadd(float* p, float x) {
     }
 }
 
+// Two float adds to one 8-byte LDS word by a 64-bit compare-and-swap loop:
+// the real and imaginary sums of a record move together.
+__device__ __forceinline__ void lds_fadd2(float2* p, float x, float y) {
+    unsigned long long* u = reinterpret_cast<unsigned long long*>(p);
+    unsigned long long old = *reinterpret_cast<volatile unsigned long long*>(u);
+    while (true) {
+        const unsigned long long assumed = old;
+        const float ax = __uint_as_float((unsigned)assumed) + x;
+        const float ay = __uint_as_float((unsigned)(assumed >> 32)) + y;
+        const unsigned long long next =
+            (unsigned long long)__float_as_uint(ax) | ((unsigned long long)__float_as_uint(ay) << 32);
+        old = atomicCAS(u, assumed, next);
+        if (old == assumed) break;
+    }
+}
+
 // One workgroup per piece of a (half, tile) bin: sum its entries in LDS, then
 // add the tile partial to the volume — a plain read-modify-write when the bin
 // is a single piece, float atomics per touched record for multi-piece bins.
@@ -917,7 +933,8 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
     const int beg = j * piece_len;
     const int end = min(min(fill[bin], maxc * BSX_CHUNK), beg + piece_len);
     if (beg >= end) return;
-    __shared__ float sre[4096], sim[4096], sw[4096];
+    __shared__ float2 sri[4096];   // (sum re, sum im) per record
+    __shared__ float sw[4096];
     __shared__ int pc[PC];
     if (threadIdx.x < PC && (int)threadIdx.x < pcn) {
         const int ci = beg / BSX_CHUNK + threadIdx.x;
@@ -925,8 +942,7 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
     }
     float* g = acc + (size_t)bin * 4096 * 3;
     for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
-        sre[i] = 0.f;
-        sim[i] = 0.f;
+        sri[i] = make_float2(0.f, 0.f);
         sw[i] = 0.f;
     }
     __syncthreads();
@@ -945,8 +961,7 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
         for (int u = 0; u < U; ++u) {
             const int li = __float_as_int(v[u].w);
             if (li >= 0 && li < 4096) {
-                lds_fadd(&sre[li], v[u].x);
-                lds_fadd(&sim[li], v[u].y);
+                lds_fadd2(&sri[li], v[u].x, v[u].y);
                 lds_fadd(&sw[li], v[u].z);
             }
         }
@@ -966,7 +981,7 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int li = (f + q) / 3, c = (f + q) - li * 3;
-                add[q] = c == 0 ? sre[li] : c == 1 ? sim[li] : sw[li];
+                add[q] = c == 0 ? sri[li].x : c == 1 ? sri[li].y : sw[li];
             }
             cur[u].x += add[0];
             cur[u].y += add[1];
@@ -976,9 +991,10 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
         }
     } else {
         for (int li = threadIdx.x; li < 4096; li += blockDim.x) {
-            if (sw[li] != 0.f || sre[li] != 0.f || sim[li] != 0.f) {
-                unsafeAtomicAdd(g + 3 * li + 0, sre[li]);
-                unsafeAtomicAdd(g + 3 * li + 1, sim[li]);
+            const float2 ri = sri[li];
+            if (sw[li] != 0.f || ri.x != 0.f || ri.y != 0.f) {
+                unsafeAtomicAdd(g + 3 * li + 0, ri.x);
+                unsafeAtomicAdd(g + 3 * li + 1, ri.y);
                 unsafeAtomicAdd(g + 3 * li + 2, sw[li]);
             }
         }
