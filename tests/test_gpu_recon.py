@@ -393,3 +393,4 @@ def test_two_d_reconstruction(interp, tilted):
     assert err <= 1e-4 and fe <= 1e-4
     assert g.coverage == st.coverage and g.particles == st.particles
     assert abs(g.friedel - st.friedel) <= 1e-3 * st.friedel + 1e-5 * np.abs(ftc).max()
+
