@@ -22,7 +22,9 @@ Options (a subset of breconstruct's, same meaning):
   -halfmaps              also write the two half maps (<name>_1, <name>_2)
   -fom                   also write the FOM map (<name>_fom)
   -verbose               print the Fourier shell statistics of the full map
-                         (fspace_reconstruction_stats)
+                         (fspace_reconstruction_stats); this accumulates the
+                         power and squared-weight sums too, through the
+                         slower direct-atomic insertion
 """
 from __future__ import annotations
 
