@@ -137,6 +137,8 @@ SIGNATURES = {
     "bsx_fsc": ([_vp, _vp, C.c_int, C.c_double, C.c_double, _dp, C.POINTER(C.c_int), _vp], C.c_int),
     "bsx_fsc_dpr": ([_vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
                      C.POINTER(C.c_int), _vp], C.c_int),
+    "bsx_fsc_dpr_2d": ([_vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
+                        C.POINTER(C.c_int), _vp], C.c_int),
     "bsx_score_frc": ([_vp, C.c_int64, _vp, C.c_int64, C.c_int, C.c_double, C.c_double,
                        C.c_double, C.POINTER(C.c_int32), C.c_int64, _dp, _vp], C.c_int),
     "bsx_search_create": ([C.POINTER(_vp), C.POINTER(SearchParams)], C.c_int),

@@ -26,16 +26,24 @@ import numpy as np
 
 
 def fsc_dpr(map1, map2, sampling: float, hi_res: float = 0.0, sampling_ratio: float = 1.0):
-    """Bimage::fsc_dpr of two real cubic maps (torch tensors on the device):
-    arrays s (1/A), FSC and DPR (degrees), one row per shell."""
+    """Bimage::fsc_dpr of two real cubic maps or two square images (torch
+    tensors on the device; an image may carry a leading axis of 1): arrays
+    s (1/A), FSC and DPR (degrees), one row per shell."""
     from .abi import call
     from .recon import _ptr, _stream
     n = int(map1.shape[-1])
-    if tuple(map1.shape) != (n, n, n) or tuple(map2.shape) != (n, n, n):
-        raise ValueError("two cubic maps of the same size are required")
+    if tuple(map1.shape) != tuple(map2.shape):
+        raise ValueError("two maps of the same size are required")
+    shp = tuple(map1.shape)
+    if shp == (n, n, n) and n > 1:
+        fn = "bsx_fsc_dpr"
+    elif shp in ((n, n), (1, n, n)):
+        fn = "bsx_fsc_dpr_2d"
+    else:
+        raise ValueError("cubic maps or square images are required")
     cap = C.c_int(4 * n + 8)
     out = np.zeros((cap.value, 3))
-    call("bsx_fsc_dpr", _ptr(map1.contiguous()), _ptr(map2.contiguous()), n, float(sampling),
+    call(fn, _ptr(map1.contiguous()), _ptr(map2.contiguous()), n, float(sampling),
          float(hi_res), float(sampling_ratio), out.ctypes.data_as(C.POINTER(C.c_double)),
          C.byref(cap), _stream(None))
     out = out[: cap.value]
@@ -110,7 +118,8 @@ def main(argv=None) -> int:
     for c in _floats(a.dprcut):
         if c:
             print(f"DPR({c:g}): {1 / cut(s, dpr, c, 1):g} A")
-    print(f"Information volume radius: {information_volume_radius(s, fsc):g} A")
+    print(f"Information volume radius: "
+          f"{information_volume_radius(s, fsc, three_d=v1.shape[0] > 1):g} A")
     if a.tsv:
         np.savetxt(a.tsv, np.stack([s, fsc, dpr], axis=1), delimiter="\t")
     return 0

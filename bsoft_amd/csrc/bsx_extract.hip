@@ -131,20 +131,20 @@ __global__ __launch_bounds__(256) void k_fsc(const float2* __restrict__ a,
 // (both must exist), sums[c * maxrad + i] for c = |F1|^2, |F2|^2,
 // Re F1 conj F2, |F1|^2 + |F2|^2, (|F1|^2 + |F2|^2) dphi^2.
 __global__ __launch_bounds__(256) void k_fsc_dpr(const float2* __restrict__ a,
-                                                 const float2* __restrict__ b, int N,
+                                                 const float2* __restrict__ b, int N, int nz,
                                                  double inv_ratio, long maxrad,
                                                  double* __restrict__ sums) {
     extern __shared__ double sh[];   // 5 * maxrad
     for (long i = threadIdx.x; i < 5 * maxrad; i += blockDim.x) sh[i] = 0;
     __syncthreads();
     const int hx = N / 2;
-    const long long nv = (long long)hx * N * N;
+    const long long nv = (long long)hx * N * nz;   // nz = 1: a 2D image
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < nv;
          t += (long long)gridDim.x * blockDim.x) {
         const long long xx = t % hx, yz = t / hx, yy = yz % N, zz = yz / N;
         const double h = xx > (N - 1) / 2 ? (double)(xx - N) : (double)xx;
         const double k = yy > (N - 1) / 2 ? (double)(yy - N) : (double)yy;
-        const double l = zz > (N - 1) / 2 ? (double)(zz - N) : (double)zz;
+        const double l = zz > (nz - 1) / 2 ? (double)(zz - nz) : (double)zz;
         const double radius = inv_ratio * sqrt(h * h + k * k + l * l);
         const long ir = (long)radius;
         if (ir + 1 >= maxrad) continue;
@@ -326,18 +326,20 @@ int bsx_fsc(const float* a_dev, const float* b_dev, int n, double sampling, doub
     return BSX_OK;
 }
 
-int bsx_fsc_dpr(const float* map1_dev, const float* map2_dev, int n, double sampling,
-                double hi_res, double sampling_ratio, double* curves, int* nrows, void* stream) {
+// n x n x nz maps, nz = n (3D) or 1 (2D images).
+static int fsc_dpr_impl(const float* map1_dev, const float* map2_dev, int n, int nz,
+                        double sampling, double hi_res, double sampling_ratio, double* curves,
+                        int* nrows, void* stream) {
     BSX_REQUIRE(map1_dev && map2_dev && curves && nrows, BSX_ERR_ARG, "bsx_fsc_dpr: null argument");
     BSX_REQUIRE(n >= 2 && n % 2 == 0 && sampling > 0, BSX_ERR_ARG, "bsx_fsc_dpr: bad size/sampling");
     if (sampling_ratio <= 0) sampling_ratio = 1;
     hipStream_t s = (hipStream_t)stream;
-    hi_res = check_res(hi_res, n, sampling, 3);
+    hi_res = check_res(hi_res, n, sampling, nz > 1 ? 3 : 2);
     const double rad_scale = n * sampling / sampling_ratio;
     const long maxrad = (long)(2 + rad_scale / hi_res);
     BSX_REQUIRE(maxrad <= *nrows, BSX_ERR_ARG, "bsx_fsc_dpr: %ld rows exceed capacity %d", maxrad,
                 *nrows);
-    const long long n3 = (long long)n * n * n;
+    const long long n3 = (long long)n * n * nz;
     float2* v = nullptr;
     double* d = nullptr;
     BSX_HIP(hipMalloc(&v, sizeof(float2) * 2 * n3));
@@ -350,13 +352,13 @@ int bsx_fsc_dpr(const float* map1_dev, const float* map2_dev, int n, double samp
     for (int m = 0; m < 2 && r == BSX_OK; ++m) {
         hipLaunchKernelGGL(k_real_to_complex, dim3(2048), dim3(256), 0, s, m ? map2_dev : map1_dev,
                            v + m * n3, n3);
-        for (int axis = 0; axis < 3 && r == BSX_OK; ++axis)
-            r = fft_axis_ex(v + m * n3, nullptr, n, n, n, 1, axis, -1, 1.0f, s);
+        for (int axis = 0; axis < (nz > 1 ? 3 : 2) && r == BSX_OK; ++axis)
+            r = fft_axis_ex(v + m * n3, nullptr, n, n, nz, 1, axis, -1, 1.0f, s);
     }
     if (r == BSX_OK) {
         (void)hipMemsetAsync(d, 0, sizeof(double) * 5 * maxrad, s);
         hipLaunchKernelGGL(k_fsc_dpr, dim3(1024), dim3(256), sizeof(double) * 5 * maxrad, s, v,
-                           v + n3, n, 1.0 / sampling_ratio, maxrad, d);
+                           v + n3, n, nz, 1.0 / sampling_ratio, maxrad, d);
         if (hipGetLastError() != hipSuccess) r = BSX_ERR_DEVICE;
     }
     std::vector<double> h(5 * maxrad);
@@ -389,6 +391,19 @@ int bsx_fsc_dpr(const float* map1_dev, const float* map2_dev, int n, double samp
     }
     *nrows = (int)maxrad;
     return BSX_OK;
+}
+
+int bsx_fsc_dpr(const float* map1_dev, const float* map2_dev, int n, double sampling,
+                double hi_res, double sampling_ratio, double* curves, int* nrows, void* stream) {
+    return fsc_dpr_impl(map1_dev, map2_dev, n, n, sampling, hi_res, sampling_ratio, curves, nrows,
+                        stream);
+}
+
+int bsx_fsc_dpr_2d(const float* img1_dev, const float* img2_dev, int n, double sampling,
+                   double hi_res, double sampling_ratio, double* curves, int* nrows,
+                   void* stream) {
+    return fsc_dpr_impl(img1_dev, img2_dev, n, 1, sampling, hi_res, sampling_ratio, curves, nrows,
+                        stream);
 }
 
 int bsx_score_frc(const float* part_dev, int64_t nparts, const float* ref_dev, int64_t nrefs,

@@ -241,6 +241,13 @@ int bsx_fsc(const float* a_dev, const float* b_dev, int n, double sampling, doub
 int bsx_fsc_dpr(const float* map1_dev, const float* map2_dev, int n, double sampling,
                 double hi_res, double sampling_ratio, double* curves, int* nrows, void* stream);
 
+/* The same for two real n x n images (Bimage::fsc_dpr of 2D images, as
+ * bresolve compares class averages or 2D reconstructions): shells in the
+ * plane, the resolution checked over two axes. */
+int bsx_fsc_dpr_2d(const float* img1_dev, const float* img2_dev, int n, double sampling,
+                   double hi_res, double sampling_ratio, double* curves, int* nrows,
+                   void* stream);
+
 /* ---- projection-matching score (img_recip_space_fom, fom_type 0,
  *      src/mg/mg_refine.cpp:660-781): ring-weighted FRC of particle transforms
  *      against CTF-free reference sections, FP32 on the device.

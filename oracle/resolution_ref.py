@@ -26,23 +26,28 @@ import math
 import numpy as np
 
 
-def check_resolution(res: float, n: int, sampling: float) -> float:
+def check_resolution(res: float, n: int, sampling: float, ndim: int = 3) -> float:
     if res > n * sampling:
         return n * sampling
-    return max(res, 2 / math.sqrt(3 / sampling ** 2))
+    return max(res, 2 / math.sqrt(ndim / sampling ** 2))
 
 
 def fsc_dpr(a: np.ndarray, b: np.ndarray, sampling: float, hi_res: float = 0.0,
             ratio: float = 1.0):
-    n = a.shape[0]
-    hi_res = check_resolution(hi_res, n, sampling)
+    """Cubic maps (z, y, x) or square images (y, x)."""
+    if a.ndim == 2:
+        a, b = a[None], b[None]
+    n, nz = a.shape[-1], a.shape[0]
+    hi_res = check_resolution(hi_res, n, sampling, 3 if nz > 1 else 2)
     rad_scale = n * sampling / ratio
     maxrad = int(2 + rad_scale / hi_res)
     F1 = np.fft.fftn(a.astype(np.float64))
     F2 = np.fft.fftn(b.astype(np.float64))
     f = np.arange(n)
     f = np.where(f > (n - 1) // 2, f - n, f).astype(np.float64)
-    l, k, h = np.meshgrid(f, f, f[: n // 2], indexing="ij")      # (z, y, x < n/2)
+    fz = np.arange(nz)
+    fz = np.where(fz > (nz - 1) // 2, fz - nz, fz).astype(np.float64)
+    l, k, h = np.meshgrid(fz, f, f[: n // 2], indexing="ij")     # (z, y, x < n/2)
     u, v = F1[:, :, : n // 2], F2[:, :, : n // 2]
     radius = np.sqrt(h * h + k * k + l * l) / ratio
     ir = radius.astype(np.int64)

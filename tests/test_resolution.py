@@ -186,3 +186,64 @@ def test_gpu_recon_stats_matches_oracle():
             np.testing.assert_allclose(rows[:, c], ref[:, c], rtol=1e-6,
                                        atol=1e-9 * max(1e-30, np.abs(ref[:, c]).max()))
         np.testing.assert_allclose(est, rest, rtol=1e-6)
+
+
+# ------------------------------------------------------------- 2D images
+def test_2d_identities_and_shift():
+    """Square images: FSC(a, a) = 1, FSC(a, -a) = -1 with DPR 180; a
+    one-pixel roll along x against a brute-force evaluation of the ring sums
+    (resolution checked over two axes)."""
+    n, samp = 16, 1.5
+    rng = np.random.default_rng(21)
+    a = rng.normal(size=(n, n))
+    hi = resolution_ref.check_resolution(0, n, samp, 2)
+    assert hi == pytest.approx(2 * samp / math.sqrt(2))
+    maxrad = int(2 + n * samp / hi)
+    # the last shell lies beyond the half-plane's corner (|(7, 8)| < 11): no
+    # data, FSC 0 there as in the reference
+    s, fsc, dpr = resolution_ref.fsc_dpr(a, a, samp)
+    assert len(s) == maxrad and fsc[-1] == 0
+    assert np.allclose(fsc[:-1], 1) and np.allclose(dpr, 0)
+    s, fsc, dpr = resolution_ref.fsc_dpr(a, -a, samp)
+    assert np.allclose(fsc[1:-1], -1) and np.allclose(dpr[1:-1], 180)
+    b = np.roll(a, 1, axis=1)
+    s, fsc, dpr = resolution_ref.fsc_dpr(a, b, samp)
+    F = np.fft.fft2(a)
+    num = np.zeros(maxrad); den = np.zeros(maxrad)
+    for y in range(n):
+        for x in range(n // 2):
+            h = x - n if x > (n - 1) // 2 else x
+            k = y - n if y > (n - 1) // 2 else y
+            r = math.sqrt(h * h + k * k)
+            i = int(r)
+            if i + 1 >= maxrad:
+                continue
+            f = r - i
+            I = abs(F[y, x]) ** 2
+            for j, wt in ((i, 1 - f), (i + 1, f)):
+                num[j] += wt * I * math.cos(2 * math.pi * h / n)
+                den[j] += wt * I
+    for j in range(1, maxrad):
+        if den[j] > 0:
+            assert fsc[j] == pytest.approx(num[j] / den[j], abs=1e-9)
+
+
+@pytest.mark.gpu
+def test_gpu_fsc_dpr_2d_matches_oracle(tmp_path, capsys):
+    import torch
+    from bsoft_amd import bresolve, formats
+    from bsoft_amd.bresolve import fsc_dpr
+    n = 64
+    rng = np.random.default_rng(23)
+    a = rng.normal(size=(n, n)).astype(np.float32)
+    b = (a + 0.8 * rng.normal(size=(n, n))).astype(np.float32)
+    for ratio, hi in ((1.0, 0.0), (2.0, 5.0)):
+        s, fsc, dpr = fsc_dpr(torch.as_tensor(a).cuda(), torch.as_tensor(b).cuda(), 1.5, hi, ratio)
+        rs, rf, rd = resolution_ref.fsc_dpr(a, b, 1.5, hi, ratio)
+        assert len(s) == len(rs)
+        np.testing.assert_allclose(fsc, rf, atol=2e-5)
+        np.testing.assert_allclose(dpr, rd, atol=2e-3)
+    formats.write_mrc(str(tmp_path / "a.mrc"), a[None], (1.5, 1.5, 1.0), (32, 32, 0))
+    formats.write_mrc(str(tmp_path / "b.mrc"), b[None], (1.5, 1.5, 1.0), (32, 32, 0))
+    assert bresolve.main([str(tmp_path / "a.mrc"), str(tmp_path / "b.mrc")]) == 0
+    assert "FSC(0.3):" in capsys.readouterr().out
