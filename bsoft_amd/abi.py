@@ -127,6 +127,7 @@ SIGNATURES = {
     "bsx_recon_stats": ([_vp, C.c_int, C.c_int, C.c_double, _dp, C.c_int32,
                          C.POINTER(C.c_int32), _dp, _vp], C.c_int),
     "bsx_symmetrize": ([_vp, _vp, C.c_int32, _dp, C.c_char_p, _vp], C.c_int),
+    "bsx_symmetrize_2d": ([_vp, _vp, C.c_int32, _dp, C.c_char_p, _vp], C.c_int),
     "bsx_symmetry_views": ([C.c_char_p, _dp, C.c_int64, _dp, C.c_int64, C.POINTER(C.c_int64)],
                            C.c_int),
     "bsx_recon_kernel_times": ([_vp, _dp, C.c_int], C.c_int),

@@ -120,7 +120,9 @@ def main(argv=None) -> int:
             print(f"FSC(0.3):                       {r_fsc:g} A\n")
         r = rec.finish(imap, nmaps, want_fom=a.fom and imap == 0)
         org = (size // 2, size // 2, 0 if a.TwoD else size // 2)
-        vol = symmetrize(r.map, a.symmetry, org) if a.mode == 1 else r.map
+        vol = r.map
+        if a.mode == 1:
+            vol = symmetrize(r.map, a.symmetry, org[:2] if a.TwoD else org)
         write_mrc(path, vol.cpu().numpy(), (sampling,) * 3, org)
         if r.fom is not None:
             write_mrc(f"{stem}_fom{ext or '.mrc'}", r.fom.cpu().numpy(), (sampling,) * 3, org)

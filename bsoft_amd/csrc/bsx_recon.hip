@@ -1589,11 +1589,11 @@ static void axis_angle_matrix(const double ax[3], double angle, double M[9]) {
 // the map, weights toggled as the reference's loops toggle them (an edge
 // neighbour missing in one axis leaves that axis's weight alternating),
 // normalised by their sum; outside the map: fill.
-__device__ inline double interp_fill(const float* __restrict__ a, int n, double xx, double yy,
-                                     double zz, double fill) {
-    if (xx < 0 || xx >= n || yy < 0 || yy >= n || zz < 0 || zz >= n) return fill;
+__device__ inline double interp_fill(const float* __restrict__ a, int n, int zn, double xx,
+                                     double yy, double zz, double fill) {
+    if (xx < 0 || xx >= n || yy < 0 || yy >= n || zz < 0 || zz >= zn) return fill;
     const long ix = (long)xx, iy = (long)yy, iz = (long)zz;
-    const int nx = (n < ix + 2) ? 1 : 2, ny = (n < iy + 2) ? 1 : 2, nz = (n < iz + 2) ? 1 : 2;
+    const int nx = (n < ix + 2) ? 1 : 2, ny = (n < iy + 2) ? 1 : 2, nz = (zn < iz + 2) ? 1 : 2;
     double fx = xx - ix, fy = yy - iy, fz = zz - iz;
     double value = 0, ws = 0;
     for (int zk = 0; zk < nz; ++zk) {
@@ -1617,17 +1617,17 @@ __device__ inline double interp_fill(const float* __restrict__ a, int n, double 
 // src/img/Bimage_background.cpp:27-80): mean of the voxels outside the
 // ellipsoid ((x - o) / (N/2))^2 + ... = 1. Per-block FP64 partial sums, then
 // one block adds them in a fixed order.
-__global__ __launch_bounds__(256) void k_bkg_partial(const float* __restrict__ a, int n, double o0,
+__global__ __launch_bounds__(256) void k_bkg_partial(const float* __restrict__ a, int n, int zn, double o0,
                                                      double o1, double o2,
                                                      double* __restrict__ part) {
     __shared__ double ss[256], sc[256];
-    const int64_t total = (int64_t)n * n * n;
+    const int64_t total = (int64_t)n * n * zn;
     const double h = (double)(n / 2);
     double s = 0, c = 0;
     for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < total;
          v += (int64_t)gridDim.x * 256) {
         const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
-        double x2 = (x - o0) / h, y2 = (y - o1) / h, z2 = (z - o2) / h;
+        double x2 = (x - o0) / h, y2 = (y - o1) / h, z2 = zn > 1 ? (z - o2) / (double)(zn / 2) : 0;
         x2 *= x2;
         y2 *= y2;
         z2 *= z2;
@@ -1667,11 +1667,11 @@ __global__ void k_bkg_final(const double* __restrict__ part, int nblocks, double
 // each term rounded to FP32 and added in FP32 as Bimage::sum does; the last
 // operator also multiplies by `scale` (1 / order).
 __global__ __launch_bounds__(256) void k_sym_op(const float* __restrict__ in,
-                                                float* __restrict__ out, int n, double o0,
+                                                float* __restrict__ out, int n, int zn, double o0,
                                                 double o1, double o2,
                                                 const double* __restrict__ mats, int nm,
                                                 const double* __restrict__ bkg, double scale) {
-    const int64_t total = (int64_t)n * n * n;
+    const int64_t total = (int64_t)n * n * zn;
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (v >= total) return;
     const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
@@ -1683,7 +1683,7 @@ __global__ __launch_bounds__(256) void k_sym_op(const float* __restrict__ in,
         const double a0 = M[0] * u0 + M[3] * u1 + M[6] * u2 + o0;
         const double a1 = M[1] * u0 + M[4] * u1 + M[7] * u2 + o1;
         const double a2 = M[2] * u0 + M[5] * u1 + M[8] * u2 + o2;
-        const float pj = (float)interp_fill(in, n, a0, a1, a2, fill);
+        const float pj = (float)interp_fill(in, n, zn, a0, a1, a2, fill);
         acc = (float)((double)acc + (double)pj);
     }
     if (scale != 1) acc = (float)((double)acc * scale);
@@ -2337,8 +2337,9 @@ int bsx_symmetry_views(const char* sym, const double* views, int64_t n, double* 
     return BSX_OK;
 }
 
-int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double* origin,
-                   const char* sym, void* stream) {
+// n x n x zn maps: zn = n (3D) or 1 (2D images, cyclic groups only).
+static int symmetrize_impl(const float* in_dev, float* out_dev, int32_t n, int32_t zn,
+                           const double* origin, const char* sym, void* stream) {
     BSX_REQUIRE(in_dev && out_dev && origin && n >= 1 && n <= 2048, BSX_ERR_ARG,
                 "bsx_symmetrize: bad arguments");
     BSX_REQUIRE(in_dev != out_dev, BSX_ERR_ARG, "bsx_symmetrize: input and output must differ");
@@ -2347,13 +2348,17 @@ int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double*
     std::vector<SymOp> all, ops;
     const int rc = parse_symmetry(sym, all);
     if (rc) return rc;
+    if (zn == 1)   // Bimage::symmetrize on a 2D image (Bimage_symmetry.cpp:40-46)
+        for (const SymOp& o : all)
+            BSX_REQUIRE(o.ax[0] == 0 && o.ax[1] == 0, BSX_ERR_ARG,
+                        "bsx_symmetrize_2d: only cyclic point groups apply to 2D images");
     int64_t order = 1;
     for (const SymOp& o : all) {
         order *= o.order;
         if (o.order > 1) ops.push_back(o);
     }
     hipStream_t s = (hipStream_t)stream;
-    const int64_t total = (int64_t)n * n * n;
+    const int64_t total = (int64_t)n * n * zn;
     if (ops.empty()) {   // C1 and Cs: Bimage::symmetrize leaves the map unchanged
         BSX_HIP(hipMemcpyAsync(out_dev, in_dev, sizeof(float) * total, hipMemcpyDeviceToDevice, s));
         return BSX_OK;
@@ -2389,7 +2394,7 @@ int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double*
     e = hipMemcpyAsync(dm, mats.data(), sizeof(double) * mats.size(), hipMemcpyHostToDevice, s);
     double* bkg = part + 2 * nblk;
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_bkg_partial, dim3(nblk), dim3(256), 0, s, in_dev, n, origin[0],
+        hipLaunchKernelGGL(k_bkg_partial, dim3(nblk), dim3(256), 0, s, in_dev, n, zn, origin[0],
                            origin[1], origin[2], part);
         hipLaunchKernelGGL(k_bkg_final, dim3(1), dim3(64), 0, s, part, nblk, bkg);
         e = hipGetLastError();
@@ -2400,7 +2405,7 @@ int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double*
         float* dst = ((K - 1 - k) % 2 == 0) ? out_dev : tmp;
         const double scale = k == K - 1 ? 1.0 / (double)order : 1.0;
         hipLaunchKernelGGL(k_sym_op, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src,
-                           dst, n, origin[0], origin[1], origin[2], dm + 9 * first[k],
+                           dst, n, zn, origin[0], origin[1], origin[2], dm + 9 * first[k],
                            ops[k].order - 1, (const double*)bkg, scale);
         e = hipGetLastError();
         src = dst;
@@ -2409,6 +2414,18 @@ int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double*
     release();
     BSX_HIP(e);
     return BSX_OK;
+}
+
+int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double* origin,
+                   const char* sym, void* stream) {
+    return symmetrize_impl(in_dev, out_dev, n, n, origin, sym, stream);
+}
+
+int bsx_symmetrize_2d(const float* in_dev, float* out_dev, int32_t n, const double* origin,
+                      const char* sym, void* stream) {
+    BSX_REQUIRE(origin, BSX_ERR_ARG, "bsx_symmetrize_2d: null origin");
+    const double o3[3] = {origin[0], origin[1], 0.0};
+    return symmetrize_impl(in_dev, out_dev, n, 1, o3, sym, stream);
 }
 
 int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n) {

@@ -230,12 +230,18 @@ def symmetrize(vol: torch.Tensor, sym: str, origin=None, stream=None) -> torch.T
     N^3 map on the device, about `origin` (voxels, x y z; default N/2):
     breconstruct's symmetry mode 1, applied after the inverse transform."""
     n = int(vol.shape[-1])
-    if tuple(vol.shape) != (n, n, n) or vol.dtype != torch.float32:
-        raise ValueError("a cubic float32 map is required")
+    if vol.dtype != torch.float32:
+        raise ValueError("a float32 map is required")
+    if tuple(vol.shape) == (n, n, n) and n > 1:
+        fn, dflt = "bsx_symmetrize", (n // 2,) * 3
+    elif tuple(vol.shape) in ((n, n), (1, n, n)):        # a 2D image: cyclic groups
+        fn, dflt = "bsx_symmetrize_2d", (n // 2,) * 2
+    else:
+        raise ValueError("a cubic map or a square image is required")
     vol = vol.contiguous()
     out = torch.empty_like(vol)
-    o = np.asarray(origin if origin is not None else (n // 2,) * 3, dtype=np.float64)
-    call("bsx_symmetrize", _ptr(vol), _ptr(out), n, o.ctypes.data_as(C.POINTER(C.c_double)),
+    o = np.asarray(origin if origin is not None else dflt, dtype=np.float64)
+    call(fn, _ptr(vol), _ptr(out), n, o.ctypes.data_as(C.POINTER(C.c_double)),
          sym.encode(), _stream(stream))
     return out
 

@@ -447,6 +447,12 @@ int bsx_views_c1(double theta_step, double phi_step, int32_t flag, double angle_
 int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double* origin,
                    const char* sym, void* stream);
 
+/* The same for an n x n image (breconstruct -TwoD -mode 1): cyclic point
+ * groups only, about origin (x, y), as Bimage::symmetrize restricts 2D
+ * images (Bimage_symmetry.cpp:40-46). Synchronises the stream. */
+int bsx_symmetrize_2d(const float* in_dev, float* out_dev, int32_t n, const double* origin,
+                      const char* sym, void* stream);
+
 /* symmetry_get_all_views (src/util/symmetry.cpp:277-319) for each of n views
  * (x, y, z, angle): out holds, view after view, the order(sym) related views
  * (the view itself first); cap = room in out (views); *count = n * order. */

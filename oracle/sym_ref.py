@@ -170,11 +170,12 @@ def axis_angle_matrix(axis, angle):
 
 def background(vol: np.ndarray, origin) -> float:
     """Mean of the voxels outside the ellipsoid with semi-axes N/2 (integer)
-    about origin (x, y, z)."""
-    n = vol.shape[0]
+    about origin (x, y, z); a single plane (z extent 1) has no z term."""
+    zn, n = vol.shape[0], vol.shape[-1]
     h = float(n // 2)
-    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
-    r = ((x - origin[0]) / h) ** 2 + ((y - origin[1]) / h) ** 2 + ((z - origin[2]) / h) ** 2 - 1
+    z, y, x = np.meshgrid(np.arange(zn), np.arange(n), np.arange(n), indexing="ij")
+    zt = ((z - origin[2]) / float(zn // 2)) ** 2 if zn > 1 else 0.0
+    r = ((x - origin[0]) / h) ** 2 + ((y - origin[1]) / h) ** 2 + zt - 1
     out = r > 0
     return float(vol[out].astype(np.float64).mean()) if out.any() else 0.0
 
@@ -183,12 +184,13 @@ def interpolate(vol: np.ndarray, xx, yy, zz, fill: float) -> np.ndarray:
     """Bimage::interpolate at arrays of positions: the (up to) 2 x 2 x 2
     neighbours inside the map, each axis weight toggled once per pass of its
     loop as the reference's loops do, normalised by the weight sum."""
-    n = vol.shape[0]
+    zn, n = vol.shape[0], vol.shape[-1]
     flat = vol.astype(np.float64).ravel()
-    inside = (xx >= 0) & (xx < n) & (yy >= 0) & (yy < n) & (zz >= 0) & (zz < n)
+    inside = (xx >= 0) & (xx < n) & (yy >= 0) & (yy < n) & (zz >= 0) & (zz < zn)
     xs, ys, zs = (np.where(inside, c, 0.0) for c in (xx, yy, zz))
     ix, iy, iz = xs.astype(np.int64), ys.astype(np.int64), zs.astype(np.int64)
-    nx, ny, nz = (np.where(n < i + 2, 1, 2) for i in (ix, iy, iz))
+    nx, ny = (np.where(n < i + 2, 1, 2) for i in (ix, iy))
+    nz = np.where(zn < iz + 2, 1, 2)
     fx, fy, fz = xs - ix, ys - iy, zs - iz
     value = np.zeros_like(xs)
     ws = np.zeros_like(xs)
@@ -213,10 +215,13 @@ def interpolate(vol: np.ndarray, xx, yy, zz, fill: float) -> np.ndarray:
 
 
 def symmetrize(vol: np.ndarray, label: str, origin) -> np.ndarray:
-    """Bimage::symmetrize(sym, View(), 1) of a real N^3 map (z, y, x): per
+    """Bimage::symmetrize(sym, View(), 1) of a real N^3 map (z, y, x) or
+    N x N image (y, x; cyclic groups, origin (x, y)): per
     operator the map plus its order - 1 rotations about origin (each rotated
     map samples R^T (x - o) + o, rounded to FP32, added in FP32), with the
     background of the input as the fill; finally x 1/order in FP32."""
+    if np.ndim(vol) == 2:                         # a 2D image
+        return symmetrize(np.asarray(vol)[None], label, origin)[0]
     ops = operators(label)
     order = 1
     for _, m in ops:
@@ -224,10 +229,10 @@ def symmetrize(vol: np.ndarray, label: str, origin) -> np.ndarray:
     cur = np.asarray(vol, dtype=np.float32).copy()
     if all(m == 1 for _, m in ops):
         return cur
-    n = cur.shape[0]
-    o = [float(c) for c in origin]
+    zn, n = cur.shape[0], cur.shape[-1]           # zn = 1: a 2D image
+    o = [float(c) for c in origin] + [0.0] * (3 - len(origin))
     fill = background(cur, o)
-    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    z, y, x = np.meshgrid(np.arange(zn), np.arange(n), np.arange(n), indexing="ij")
     u = [x - o[0], y - o[1], z - o[2]]
     for axis, m in ops:
         if m == 1:

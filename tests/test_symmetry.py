@@ -5,6 +5,7 @@ the k-th related view of a base view R_0, the set {R_k R_0^T} must be a
 rotation group of the point group's order that contains the identity and the
 generator rotations, and is closed under products. The oracle restatement
 (oracle/sym_ref.py) must agree with the library to 1e-12."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -196,3 +197,45 @@ def test_gpu_symmetrize_matches_oracle(label):
     out = symmetrize(torch.as_tensor(f).cuda(), label, o).cpu().numpy()
     ref = sym_ref.symmetrize(f, label, o)
     np.testing.assert_allclose(out, ref, rtol=0, atol=1e-6 * np.abs(f).max())
+
+
+def test_symmetrize_oracle_2d_images():
+    """A square image: C4 about the centre is the exact average of the four
+    quarter turns (np.rot90 about pixel n/2 of an image that is zero near its
+    border); C3 of a linear image leaves its mean b inside a disk that every
+    rotation keeps in the image."""
+    n, o = 16, 8
+    rng = np.random.default_rng(12)
+    f = rng.normal(size=(n, n)).astype(np.float32)
+    y, x = np.meshgrid(np.arange(n) - o, np.arange(n) - o, indexing="ij")
+    f[x * x + y * y > 36] = 0
+    out = sym_ref.symmetrize(f, "C4", (o, o))
+    g = np.zeros((n + 1, n + 1))
+    g[:n, :n] = f                     # pad so that pixel n/2 is the rotation centre
+    ref = sum(np.rot90(g[: 2 * o + 1, : 2 * o + 1], q) for q in range(4))[:n, :n] / 4
+    np.testing.assert_allclose(out, ref, atol=1e-6 * np.abs(f).max())
+    lin = (2.0 + 0.3 * x - 0.2 * y).astype(np.float32)
+    out = sym_ref.symmetrize(lin, "C3", (o, o))
+    inner = x * x + y * y <= 25
+    np.testing.assert_allclose(out[inner], 2.0, atol=2e-5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("label", ["C1", "C3", "C6"])
+def test_gpu_symmetrize_2d_matches_oracle(label):
+    import torch
+    from bsoft_amd.recon import symmetrize
+    n = 32
+    f = np.random.default_rng(8).normal(size=(n, n)).astype(np.float32) + 0.5
+    o = (n // 2, n // 2 - 0.5)
+    out = symmetrize(torch.as_tensor(f[None]).cuda(), label, o).cpu().numpy()[0]
+    np.testing.assert_allclose(out, sym_ref.symmetrize(f, label, o), rtol=0,
+                               atol=1e-6 * np.abs(f).max())
+
+
+def test_symmetrize_2d_rejects_non_cyclic_groups():
+    L = abi.lib()
+    org = (C.c_double * 2)(8, 8)
+    buf = C.c_void_p(0x1000)
+    assert L.bsx_symmetrize_2d(buf, C.c_void_p(0x2000), 16, org, b"D2", None) == abi.BSX_ERR_ARG
+    assert b"cyclic" in L.bsx_last_error()
