@@ -91,7 +91,7 @@ def main(argv=None) -> int:
            "wiener": abi.BSX_CTF_WIENER}[a.CTF]
     interp = {"nearest": 0, "weighted": 1, "trilinear": 2}[a.interpolate]
     flags = ((abi.BSX_FLAG_BOOTSTRAP if a.bootstrap else 0)
-             | (abi.BSX_FLAG_FOM if a.fom or (a.verbose and not a.TwoD) else 0)
+             | (abi.BSX_FLAG_FOM if a.fom or a.verbose else 0)
              | (abi.BSX_FLAG_2D if a.TwoD else 0))
     ft = part_ft_size(max(box, size), 1.0, a.pad)
     rec = Reconstruction(size, box, ft, sampling, hi_res=a.resolution, interp=interp,
@@ -110,7 +110,7 @@ def main(argv=None) -> int:
     if a.halfmaps:
         outs += [(1, 3, f"{stem}_1{ext or '.mrc'}"), (2, 3, f"{stem}_2{ext or '.mrc'}")]
     for imap, nmaps, path in outs:
-        if a.verbose and imap == 0 and not a.TwoD:   # img_reconstruction_sum_weigh :346-347
+        if a.verbose and imap == 0:        # img_reconstruction_sum_weigh :346-347
             rows, (r_snr, r_fsc) = rec.stats(imap, nmaps)
             print("Radius\ts(1/A)\tRes(A)\tWeight\tPower\tSignal\tNoise\tSNR\tFSC")
             for r in rows:

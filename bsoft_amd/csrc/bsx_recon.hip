@@ -1098,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_weigh(const float* __restrict__ acc, lo
 //    (sums 2: counts, 3: power, 4: weight, 5: weight2), nr2 <= nr1 shells.
 // FP64 sums in LDS per block; block partials to part[block][6][nr1].
 __global__ __launch_bounds__(256) void k_recon_stats(const float* __restrict__ acc, long long n3,
-                                                     int N, int NT, long long nrec, int stride,
+                                                     int N, int nz, int NT, long long nrec, int stride,
                                                      int hmask, int nhalves, double rad_scale,
                                                      double freq_scale, int nr1, int nr2,
                                                      double* __restrict__ part) {
@@ -1138,7 +1138,7 @@ __global__ __launch_bounds__(256) void k_recon_stats(const float* __restrict__ a
         // fspace_radial_power: wrap above (N - 1) / 2
         const long hx = xx > (N - 1) / 2 ? xx - N : xx;
         const long hy = yy > (N - 1) / 2 ? yy - N : yy;
-        const long hz = zz > (N - 1) / 2 ? zz - N : zz;
+        const long hz = zz > (nz - 1) / 2 ? zz - nz : zz;
         double rx = hx * freq_scale, ry = hy * freq_scale, rz = hz * freq_scale;
         rx *= rx;
         ry *= ry;
@@ -1157,7 +1157,7 @@ __global__ __launch_bounds__(256) void k_recon_stats(const float* __restrict__ a
         // Bimage::radial, origin 0 with wrap: d = x for x <= N/2, else x - N
         const long dx = xx > N / 2 ? xx - N : xx;
         const long dy = yy > N / 2 ? yy - N : yy;
-        const long dz = zz > N / 2 ? zz - N : zz;
+        const long dz = zz > nz / 2 ? zz - nz : zz;
         const double r2 = sqrt((double)(dx * dx) + (double)(dy * dy) + (double)(dz * dz));
         const long jr = (long)r2;
         if (jr < nr2) {
@@ -2481,7 +2481,6 @@ int bsx_recon_stats(bsx_recon* h, int imap, int nmaps, double resolution, double
                     int32_t cap, int32_t* nrows, double* estimates, void* stream) {
     BSX_REQUIRE(h && nrows, BSX_ERR_ARG, "bsx_recon_stats: null argument");
     BSX_REQUIRE(h->K.fom, BSX_ERR_STATE, "bsx_recon_stats needs BSX_FLAG_FOM");
-    BSX_REQUIRE(h->K.nz == h->K.N, BSX_ERR_STATE, "bsx_recon_stats: 3D maps only");
     BSX_REQUIRE(nmaps >= 1 && nmaps <= 3 && imap >= 0 && imap < nmaps, BSX_ERR_ARG,
                 "imap %d / nmaps %d invalid", imap, nmaps);
     BSX_REQUIRE(std::isfinite(resolution), BSX_ERR_ARG, "bsx_recon_stats: resolution not finite");
@@ -2492,14 +2491,15 @@ int bsx_recon_stats(bsx_recon* h, int imap, int nmaps, double resolution, double
         BSX_REQUIRE(hmask == 3, BSX_ERR_STATE, "half maps need nhalves = 2");
         hmask = 1;
     }
-    const int N = h->K.N;
+    const int N = h->K.N, nz = h->K.nz;   // nz = 1: a 2D map
     const double rs = N * h->p.sampling;   // real size = rad_scale (sampling ratio 1)
     // fspace_maximum_radius (Bimage_fspace.cpp:26-38) of the resolution as
     // given (the table's rows) and as check_resolution clamps it (the radial
     // power spectrum); Bimage::radial keeps at most N/2 shells
     auto maxrad = [&](double res) -> long { return res > 0.1 ? (long)(rs / res) : (long)(N / 2.0); };
     const long nr_stats = maxrad(resolution);
-    const long nr1 = std::max<long>(1, maxrad(check_resolution(resolution, N, h->p.sampling)));
+    const long nr1 = std::max<long>(
+        1, maxrad(check_resolution(resolution, N, h->p.sampling, nz > 1 ? 3 : 2)));
     long nr2 = nr_stats < 1 ? N / 2 : std::min<long>(nr_stats, N / 2);
     nr2 = std::min(nr2, nr1);
     const size_t shm = sizeof(double) * 6 * nr1;
@@ -2514,9 +2514,9 @@ int bsx_recon_stats(bsx_recon* h, int imap, int nmaps, double resolution, double
     double* part = nullptr;
     BSX_HIP(hipMalloc((void**)&part, sizeof(double) * ((size_t)nblk * m + m)));
     double* dsum = part + (size_t)nblk * m;
-    const long long n3 = (long long)N * N * N;
+    const long long n3 = (long long)N * N * nz;
     const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
-    hipLaunchKernelGGL(k_recon_stats, dim3(nblk), dim3(256), shm, s, h->acc, n3, N, h->K.NT, nrec,
+    hipLaunchKernelGGL(k_recon_stats, dim3(nblk), dim3(256), shm, s, h->acc, n3, N, nz, h->K.NT, nrec,
                        h->K.stride, hmask, h->p.nhalves, rs, 1.0 / rs, (int)nr1, (int)nr2, part);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {

@@ -172,7 +172,8 @@ int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n);
  * from img_reconstruction_sum_weigh (src/mg/mg_reconstruct.cpp:346-347):
  * radial averages of |F|^2 (fspace_radial_power), of the weighed power, the
  * weight and w - sum w^2 / w (Bimage::radial), then per shell i >= 1 the
- * signal, noise, SNR and FSC = SNR / (SNR + 1). Needs BSX_FLAG_FOM. imap /
+ * signal, noise, SNR and FSC = SNR / (SNR + 1); 2D maps (BSX_FLAG_2D) use
+ * rings in the plane. Needs BSX_FLAG_FOM. imap /
  * nmaps select the half sets as bsx_recon_finish does. resolution: the
  * limit as passed to img_reconstruction_sum_weigh (<= 0.1: N/2 shells).
  * rows (cap x 9, optional): radius, s (1/A), resolution (A), weight, power,

@@ -94,12 +94,13 @@ def _shell_split(radius: np.ndarray, values, nr: int):
 
 
 def reconstruction_stats(acc: np.ndarray, sampling: float, resolution: float):
-    """acc: (5, N, N, N) accumulator planes (z, y, x), summed over the half
+    """acc: (5, N, N, N) accumulator planes (z, y, x; (5, 1, N, N) for a 2D
+    map), summed over the half
     sets: sum w re, sum w im, sum w, sum w |F|^2, sum w^2. Returns the table
     rows (radius, s, res, weight, power, signal, noise, SNR, FSC) for shells
     1 .. min(maxrad, N/2) - 1 and the (SNR(0.5), FSC(0.3)) resolutions."""
     acc = np.asarray(acc, dtype=np.float64)
-    n = acc.shape[1]
+    n, zn = acc.shape[-1], acc.shape[1]           # zn = 1: a 2D map
     re, im, W, P, W2 = acc
     full = W > 1e-30
     Ws = np.where(full, W, 1.0)
@@ -119,15 +120,17 @@ def reconstruction_stats(acc: np.ndarray, sampling: float, resolution: float):
         return int(rs / res) if res > 0.1 else int(n / 2.0)
 
     nr_stats = maxrad(resolution)
-    nr1 = max(1, maxrad(check_resolution(resolution, n, sampling)))
+    nr1 = max(1, maxrad(check_resolution(resolution, n, sampling, 3 if zn > 1 else 2)))
     nr2 = min(nr_stats if nr_stats >= 1 else n // 2, n // 2, nr1)
-    idx = np.arange(n)
+    idx, iz = np.arange(n), np.arange(zn)
     hf = np.where(idx > (n - 1) // 2, idx - n, idx) / rs
-    z, y, x = np.meshgrid(hf, hf, hf, indexing="ij")
+    hz = np.where(iz > (zn - 1) // 2, iz - zn, iz) / rs
+    z, y, x = np.meshgrid(hz, hf, hf, indexing="ij")
     r1 = rs * np.sqrt(x * x + y * y + z * z)
     c1, s1 = _shell_split(r1, [pw], nr1)
     d = np.where(idx > n // 2, idx - n, idx).astype(np.float64)
-    z, y, x = np.meshgrid(d, d, d, indexing="ij")
+    dz = np.where(iz > zn // 2, iz - zn, iz).astype(np.float64)
+    z, y, x = np.meshgrid(dz, d, d, indexing="ij")
     r2 = np.sqrt(x * x + y * y + z * z)
     c2, sp, sw, sw2 = _shell_split(r2, [P, W, w2], nr2)
 

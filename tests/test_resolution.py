@@ -247,3 +247,43 @@ def test_gpu_fsc_dpr_2d_matches_oracle(tmp_path, capsys):
     formats.write_mrc(str(tmp_path / "b.mrc"), b[None], (1.5, 1.5, 1.0), (32, 32, 0))
     assert bresolve.main([str(tmp_path / "a.mrc"), str(tmp_path / "b.mrc")]) == 0
     assert "FSC(0.3):" in capsys.readouterr().out
+
+
+def test_stats_2d_closed_form():
+    """A 2D map (one plane): the same closed form, SNR = K (K - 2), weight
+    K, rows up to the in-plane Nyquist radius."""
+    n, K, samp = 16, 4, 1.5
+    rng = np.random.default_rng(31)
+    S = rng.normal(size=(1, n, n)) + 1j * rng.normal(size=(1, n, n))
+    acc = np.stack([K * S.real, K * S.imag, np.full(S.shape, float(K)), K * np.abs(S) ** 2,
+                    np.full(S.shape, float(K))])
+    rows, _ = resolution_ref.reconstruction_stats(acc, samp, samp)
+    assert len(rows) == n // 2 - 1
+    np.testing.assert_allclose(rows[:, 3], K)
+    np.testing.assert_allclose(rows[:, 7], K * (K - 2), rtol=1e-5)
+
+
+@pytest.mark.gpu
+def test_gpu_recon_stats_2d_matches_oracle():
+    import torch
+    from bsoft_amd import abi, synth
+    from bsoft_amd.recon import Reconstruction
+    n = 32
+    rng = np.random.default_rng(19)
+    bm = synth.make_blob_map(n, 10, rng)
+    ps = synth.make_particles(bm, 200, n, rng, snr=0.5, seed=19)
+    parts = ps.parts.copy()
+    ang = parts["view"][:, 3]
+    parts["view"] = np.stack([0 * ang, 0 * ang, np.ones_like(ang), ang], axis=1)
+    rec = Reconstruction(n, n, 64, 1.5, ctf_action=abi.BSX_CTF_FLIP,
+                         flags=abi.BSX_FLAG_FOM | abi.BSX_FLAG_2D)
+    rec.insert(ps.images.cuda().contiguous(), parts)
+    acc = rec.accumulators(0).cpu().numpy().astype(np.float64)[:, :1]
+    print("recon stats 2D n=32", flush=True)
+    rows, est = rec.stats(0, 1)
+    ref, rest = resolution_ref.reconstruction_stats(acc, 1.5, 1.5)
+    assert rows.shape == ref.shape and len(rows) > 3
+    for c in range(9):
+        np.testing.assert_allclose(rows[:, c], ref[:, c], rtol=1e-6,
+                                   atol=1e-9 * max(1e-30, np.abs(ref[:, c]).max()))
+    np.testing.assert_allclose(est, rest, rtol=1e-6)
