@@ -1406,25 +1406,32 @@ int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* p
     BSX_REQUIRE(n >= 0 && n < (1ll << 31) - GM, BSX_ERR_ARG, "bsx_search_stage: n %lld out of range",
                 (long long)n);
     const int N = h->p.box, Nd = h->Nd, Kp = h->Kp;
-    std::vector<SPart> sp((size_t)n);
     for (int64_t i = 0; i < n; ++i) {
         const bsx_particle& q = parts[i];
         BSX_REQUIRE(std::isfinite(q.ori[0]) && std::isfinite(q.ori[1]) && q.pixel_size > 0 &&
                         std::isfinite(q.def_avg) && std::isfinite(q.def_dev) &&
                         std::isfinite(q.ast_angle) && std::isfinite(q.amp_shift),
                     BSX_ERR_ARG, "bsx_search_stage: particle %lld has invalid parameters", (long long)i);
-        SPart& r = sp[i];
-        r.phx = q.ori[0] / N * 2 * M_PI;
-        r.phy = q.ori[1] / N * 2 * M_PI;
-        r.fscale = 1.0 / (N * q.pixel_size);
-        r.c = ctf_coef(q);
-        r.ox = q.ori[0];
-        r.oy = q.ori[1];
-        r.def_avg = q.def_avg;
-        r.c20u = -M_PI * lambda_of(q.volt);
-        r.flag = (h->p.flags & BSX_SEARCH_CTF) && q.volt > 0;
-        r.apply = r.flag && q.def_avg >= 1;
     }
+    // records (CTF coefficients, wavelength) built on host threads
+    std::vector<SPart> sp((size_t)n);
+    const bool ctf_flag = (h->p.flags & BSX_SEARCH_CTF) != 0;
+    host_parallel(n, 4096, [&](int64_t b, int64_t e, unsigned) {
+        for (int64_t i = b; i < e; ++i) {
+            const bsx_particle& q = parts[i];
+            SPart& r = sp[i];
+            r.phx = q.ori[0] / N * 2 * M_PI;
+            r.phy = q.ori[1] / N * 2 * M_PI;
+            r.fscale = 1.0 / (N * q.pixel_size);
+            r.c = ctf_coef(q);
+            r.ox = q.ori[0];
+            r.oy = q.ori[1];
+            r.def_avg = q.def_avg;
+            r.c20u = -M_PI * lambda_of(q.volt);
+            r.flag = ctf_flag && q.volt > 0;
+            r.apply = r.flag && q.def_avg >= 1;
+        }
+    });
     hipStream_t s = (hipStream_t)stream;
     const int64_t ppad = round_up(std::max<int64_t>(n, 1), GM);
     int rc = grow(h, &h->d_fp, &h->fp_cap, ppad * Nd, "particle values");
