@@ -93,8 +93,9 @@ def main() -> int:
                     help="also run a 1 GiB device copy (reference kernel for PMC traffic scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the path "
+              "shards over particles (private volumes, one all-reduce of the sums), which "
+              "is designed but not built", file=sys.stderr)
         return 2
 
     import torch

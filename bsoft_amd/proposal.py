@@ -6,7 +6,7 @@ whose top proposal is not confident fall back to the exhaustive search.
 
 No counterpart in the reference: its exhaustive search
 (part_determine_orientation, src/mg/mg_orient.cpp:545-798) stays the
-fallback and the judge of angular error.
+fallback and the yardstick of angular error.
 """
 from __future__ import annotations
 
