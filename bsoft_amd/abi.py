@@ -23,6 +23,7 @@ BSX_CTF_FLIP = 1
 BSX_CTF_APPLY = 2
 BSX_CTF_WIENER = 4
 
+BSX_FLAG_RESCALE = 1
 BSX_FLAG_2D = 2
 BSX_FLAG_BOOTSTRAP = 4
 BSX_FLAG_EWALD = 8
@@ -116,11 +117,15 @@ SIGNATURES = {
     "bsx_recon_insert": ([_vp, _vp, C.POINTER(Particle), C.c_int64, _vp], C.c_int),
     "bsx_recon_finish": ([_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.POINTER(Stats), _vp], C.c_int),
     "bsx_recon_accumulators": ([_vp, C.c_int, _vp, _vp], C.c_int),
+    "bsx_recon_set_accumulators": ([_vp, C.c_int, _vp, C.c_int64, _vp], C.c_int),
     "bsx_recon_device_bytes": ([_vp, C.POINTER(C.c_int64)], C.c_int),
     "bsx_recon_last_timing": ([_vp, _dp, C.POINTER(C.c_int64), _dp], C.c_int),
     "bsx_recon_autotune": ([_vp, _vp, C.POINTER(Particle), C.c_int64, C.POINTER(C.c_int32), _dp,
                             _vp], C.c_int),
     "bsx_recon_graph_launches": ([_vp, C.POINTER(C.c_int64)], C.c_int),
+    "bsx_recon_fallbacks": ([_vp, C.POINTER(C.c_int32)], C.c_int),
+    "bsx_rescale_to_avg_std": ([_vp, C.c_int64, C.c_double, C.c_double, _vp], C.c_int),
+    "bsx_filter_sinc": ([_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _vp], C.c_int),
     "bsx_recon_set_symmetry": ([_vp, C.c_char_p], C.c_int),
     "bsx_recon_tuning": ([_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)], C.c_int),
     "bsx_recon_set_tuning": ([_vp, C.c_int32, C.c_int32], C.c_int),

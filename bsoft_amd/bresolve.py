@@ -41,9 +41,12 @@ def fsc_dpr(map1, map2, sampling: float, hi_res: float = 0.0, sampling_ratio: fl
         fn = "bsx_fsc_dpr_2d"
     else:
         raise ValueError("cubic maps or square images are required")
+    # contiguous copies bound to names so that both stay allocated (and
+    # distinct) until the call has returned
+    a, b = map1.contiguous(), map2.contiguous()
     cap = C.c_int(4 * n + 8)
     out = np.zeros((cap.value, 3))
-    call(fn, _ptr(map1.contiguous()), _ptr(map2.contiguous()), n, float(sampling),
+    call(fn, _ptr(a), _ptr(b), n, float(sampling),
          float(hi_res), float(sampling_ratio), out.ctypes.data_as(C.POINTER(C.c_double)),
          C.byref(cap), _stream(None))
     out = out[: cap.value]

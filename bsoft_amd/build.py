@@ -8,6 +8,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -32,10 +33,24 @@ def _newer(src: list[str], dst: str) -> bool:
     return any(os.path.getmtime(s) > t for s in src if os.path.exists(s))
 
 
+def _compile(src: str, op: str, verbose: bool) -> None:
+    cmd = [HIPCC, *FLAGS, "-c", src, "-o", op]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {os.path.basename(src)}:\n{err}")
+    if verbose and err.strip():
+        print(err)
+
+
 def build(force: bool = False, verbose: bool = False) -> str:
+    """Compiles the HIP sources (in parallel, one hipcc per source) and links
+    libbsx.so; without `force` only sources newer than their objects."""
     os.makedirs(OBJ, exist_ok=True)
     hdrs = [os.path.join(CSRC, h) for h in HEADERS] + [os.path.join(HERE, "..", "include", "bsx.h")]
-    objs = []
+    objs, todo = [], []
     for src in SOURCES:
         sp = os.path.join(CSRC, src)
         if not os.path.exists(sp):
@@ -43,15 +58,11 @@ def build(force: bool = False, verbose: bool = False) -> str:
         op = os.path.join(OBJ, src.replace(".hip", ".o"))
         objs.append(op)
         if force or _newer([sp] + hdrs, op):
-            cmd = [HIPCC, *FLAGS, "-c", sp, "-o", op]
-            if verbose:
-                print(" ".join(cmd), flush=True)
-            r = subprocess.run(cmd, capture_output=True, text=True)
-            err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
-            if r.returncode != 0:
-                raise RuntimeError(f"hipcc failed for {src}:\n{err}")
-            if verbose and err.strip():
-                print(err)
+            todo.append((sp, op))
+    jobs = max(1, min(len(todo), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_compile, sp, op, verbose) for sp, op in todo]:
+            f.result()
     if force or _newer(objs, LIB):
         tmp = LIB + ".tmp"
         cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

@@ -113,37 +113,33 @@ __host__ __device__ inline double ctf_chi(const CtfCoef& c, double fx, double fy
     return dphi;
 }
 
-// FSI_Kernel table (include/FSI_Kernel.h:28-68), computed in double and
-// stored FP32: (ndiv + 1) rows of `wd` weights, row = round(frac * 512).
+// FSI interpolation kernel table. Its values are those of FSI_Kernel
+// (include/FSI_Kernel.h:28-68), evaluated here from the closed form
+//   K(x) = sin(pi x) / (wd sin(pi x / wd)) * cos(pi x / wd)^power
+// at the tap distance x = frac + (wd - 1)/2 - j of tap j, for the (ndiv + 1)
+// fractions frac = row / 512 (row = round(frac * 512) at lookup). At integer
+// distances (rows 0 and 512) K takes its limit: 1 at x = 0, else 0.
+// power < 0: the nearest-neighbour table, one tap of weight 1.
+// Computed in double, stored FP32, row-major by fraction.
 constexpr int FSI_NDIV = 512;
 
+inline double fsi_weight(double x, int wd, int power) {
+    const double r = std::nearbyint(x);
+    if (std::fabs(x - r) < 1e-12) return r == 0 ? 1.0 : 0.0;
+    const double a = M_PI * x / wd;
+    return std::sin(M_PI * x) / (wd * std::sin(a)) * std::pow(std::cos(a), power);
+}
+
 inline std::vector<float> fsi_table(int wd, int power) {
-    const int ndiv = FSI_NDIV;
-    if (power < 0) wd = 1;
-    std::vector<double> data((size_t)(ndiv + 1) * wd, 0.0);
-    if (power < 0) {
-        for (auto& d : data) d = 1;
-    } else {
-        int wd_2 = (wd - 1) / 2;
-        double iwd = 1.0 / wd, piwd = M_PI * iwd, incr = 1.0 / ndiv, arg = incr;
-        data[wd_2] = 1;
-        long ik = wd;
-        for (long i = 1; i < ndiv; i++) {
-            double arg1 = M_PI * (arg + wd_2);
-            double s1 = iwd * sin(arg1);
-            double arg2 = arg1 * iwd;
-            for (int j = 0; j < wd; j++, ik++) {
-                double s2 = sin(arg2), c = cos(arg2);
-                data[ik] = (s1 / s2) * pow(c, power);
-                arg2 -= piwd;
-                s1 = -s1;
-            }
-            arg += incr;
-        }
-        ik += wd_2 + 1;
-        data[ik] = 1;
+    if (power < 0) return std::vector<float>(FSI_NDIV + 1, 1.0f);
+    const int centre = (wd - 1) / 2;
+    std::vector<float> t((size_t)(FSI_NDIV + 1) * wd);
+    for (int row = 0; row <= FSI_NDIV; ++row) {
+        const double frac = (double)row / FSI_NDIV;
+        for (int j = 0; j < wd; ++j)
+            t[(size_t)row * wd + j] = (float)fsi_weight(frac + centre - j, wd, power);
     }
-    return std::vector<float>(data.begin(), data.end());
+    return t;
 }
 
 // Periodic index in [0, N): one compare in the usual in-range case, a

@@ -67,14 +67,56 @@ struct ReconConst {
 };
 
 // ------------------------------------------------------------------- K1
+// Block-wide sums of 256 threads (FP64 and int).
+__device__ __forceinline__ void block_sum256(double& a, double& b, int& c, int& d) {
+    __shared__ double sa[256], sb[256];
+    __shared__ int sc[256], sd[256];
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    sc[threadIdx.x] = c;
+    sd[threadIdx.x] = d;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sa[threadIdx.x] += sa[threadIdx.x + o];
+            sb[threadIdx.x] += sb[threadIdx.x + o];
+            sc[threadIdx.x] += sc[threadIdx.x + o];
+            sd[threadIdx.x] |= sd[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    a = sa[0];
+    b = sb[0];
+    c = sc[0];
+    d = sd[0];
+    __syncthreads();
+}
+
+// Rescaled pixel value: the FP64 product and sum of Bimage::rescale
+// (Bimage_rescale.cpp:27-45) stored back as FP32.
+__device__ __forceinline__ float rescaled(float v, double2 r) {
+    return (float)((double)v * r.x + r.y);
+}
+
+// One workgroup per particle record: the background (mean of the pixels
+// outside the inscribed ellipse about the image-header origin,
+// Bimage::calculate_background, Bimage_background.cpp:27-89). With `resc`
+// (breconstruct -rescale, flags bit 1) the image is first rescaled to
+// average 0 and standard deviation 1 (rescale_to_avg_std(0, 1),
+// mg_reconstruct.cpp:160; statistics as Bimage::statistics,
+// Bimage_stats.cpp:104-148: FP64 sums, population deviation): the
+// (scale, shift) pair is stored in resc[p] and the background is that of
+// the rescaled FP32 pixels.
 __global__ __launch_bounds__(256) void k_background(const float* __restrict__ imgs,
                                                     const PartDev* __restrict__ recs, int box,
                                                     float* __restrict__ bkg,
-                                                    int* __restrict__ nonfinite) {
+                                                    int* __restrict__ nonfinite,
+                                                    double2* __restrict__ resc) {
     const int p = blockIdx.x;
     const float* img = imgs + (size_t)recs[p].src * box * box;
     const double ox = recs[p].hox, oy = recs[p].hoy;
     const long hx = box / 2, hy = box / 2;
+    const int npix = box * box;
     // squared normalised distances per column and per row (the same FP64
     // operations as a per-pixel evaluation, computed once)
     __shared__ double cx2[1024], ry2[1024];
@@ -84,36 +126,43 @@ __global__ __launch_bounds__(256) void k_background(const float* __restrict__ im
         double y2 = (i - oy) / hy;
         ry2[i] = y2 * y2;
     }
+    double2 r = make_double2(1.0, 0.0);
+    int bad = 0;
+    if (resc) {
+        double s1 = 0, s2 = 0;
+        int unused = 0;
+        for (int i = threadIdx.x; i < npix; i += blockDim.x) {
+            const double v = img[i];
+            if (!isfinite(v)) bad = 1;
+            s1 += v;
+            s2 += v * v;
+        }
+        block_sum256(s1, s2, unused, bad);
+        const double avg = s1 / npix;
+        double var = s2 / npix - avg * avg;
+        const double sd = var > 0 ? sqrt(var) : 0.0;
+        r.x = sd > 1e-30 ? 1.0 / sd : 1.0;
+        r.y = 0.0 - avg * r.x;
+        if (threadIdx.x == 0) resc[p] = r;
+    }
     __syncthreads();
-    double sum = 0;
-    int cnt = 0, bad = 0;
+    double sum = 0, dummy = 0;
+    int cnt = 0;
     int yy = (int)threadIdx.x / box, xx = (int)threadIdx.x - yy * box;
     const int step_y = (int)blockDim.x / box, step_x = (int)blockDim.x - step_y * box;
-    for (int i = threadIdx.x; i < box * box; i += blockDim.x) {
-        const float v = img[i];
+    for (int i = threadIdx.x; i < npix; i += blockDim.x) {
+        float v = img[i];
         if (!isfinite(v)) bad = 1;
+        if (resc) v = rescaled(v, r);
         if (cx2[xx] + ry2[yy] - 1 > 0) { sum += v; ++cnt; }
         xx += step_x;
         yy += step_y;
         if (xx >= box) { xx -= box; ++yy; }
     }
-    __shared__ double ssum[256];
-    __shared__ int scnt[256], sbad[256];
-    ssum[threadIdx.x] = sum;
-    scnt[threadIdx.x] = cnt;
-    sbad[threadIdx.x] = bad;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) {
-            ssum[threadIdx.x] += ssum[threadIdx.x + o];
-            scnt[threadIdx.x] += scnt[threadIdx.x + o];
-            sbad[threadIdx.x] |= sbad[threadIdx.x + o];
-        }
-        __syncthreads();
-    }
+    block_sum256(sum, dummy, cnt, bad);
     if (threadIdx.x == 0) {
-        bkg[p] = scnt[0] ? (float)(ssum[0] / scnt[0]) : 0.0f;
-        if (sbad[0] && nonfinite) atomicOr(nonfinite, 1);
+        bkg[p] = cnt ? (float)(sum / cnt) : 0.0f;
+        if (bad && nonfinite) atomicOr(nonfinite, 1);
     }
 }
 
@@ -125,7 +174,8 @@ __global__ __launch_bounds__(256) void k_rows(const float* __restrict__ imgs,
                                               const PartDev* __restrict__ recs,
                                               const float* __restrict__ bkg, int box,
                                               int tiles_per_part, float2* __restrict__ inter,
-                                              const float2* __restrict__ twg) {
+                                              const float2* __restrict__ twg,
+                                              const double2* __restrict__ resc) {
     constexpr int T = FftThreads<FT>::value;
     constexpr int G = 256 / T;
     constexpr int NT = G * T;
@@ -138,17 +188,32 @@ __global__ __launch_bounds__(256) void k_rows(const float* __restrict__ imgs,
     const float* img = imgs + (size_t)recs[p].src * box * box;
     const float b = bkg[p];
     for (int i = tid; i < FT; i += NT) tw[i] = twg[i];
-    // load: line j = rows (y0+2j, y0+2j+1), x < box from the image minus the
-    // background, zero elsewhere (the padded region equals the background).
-    for (int idx = tid; idx < LINES * FT; idx += NT) {
-        const int j = idx / FT, x = idx - j * FT;
-        const int ya = y0 + 2 * j, yb = ya + 1;
-        float a = 0.f, c = 0.f;
-        if (x < box) {
-            if (ya < box) a = img[(size_t)ya * box + x] - b;
-            if (yb < box) c = img[(size_t)yb * box + x] - b;
+    // load: line j = rows (y0+2j, y0+2j+1), x < box from the image (rescaled
+    // when resc is given) minus the background, zero elsewhere (the padded
+    // region equals the background).
+    if (resc) {
+        const double2 rs = resc[p];
+        for (int idx = tid; idx < LINES * FT; idx += NT) {
+            const int j = idx / FT, x = idx - j * FT;
+            const int ya = y0 + 2 * j, yb = ya + 1;
+            float a = 0.f, c = 0.f;
+            if (x < box) {
+                if (ya < box) a = rescaled(img[(size_t)ya * box + x], rs) - b;
+                if (yb < box) c = rescaled(img[(size_t)yb * box + x], rs) - b;
+            }
+            lds[j * FT + x] = make_float2(a, c);
         }
-        lds[j * FT + x] = make_float2(a, c);
+    } else {
+        for (int idx = tid; idx < LINES * FT; idx += NT) {
+            const int j = idx / FT, x = idx - j * FT;
+            const int ya = y0 + 2 * j, yb = ya + 1;
+            float a = 0.f, c = 0.f;
+            if (x < box) {
+                if (ya < box) a = img[(size_t)ya * box + x] - b;
+                if (yb < box) c = img[(size_t)yb * box + x] - b;
+            }
+            lds[j * FT + x] = make_float2(a, c);
+        }
     }
     __syncthreads();
     const int g = tid / T, t = tid - g * T;
@@ -1280,6 +1345,16 @@ __global__ void k_planes(const float* __restrict__ acc, long long n3, int N, int
     }
 }
 
+// The inverse of k_planes: planes (z, y, x order) into the tiled records.
+__global__ void k_set_planes(float* __restrict__ acc, long long n3, int N, int NT, int stride,
+                             const float* __restrict__ in) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+         i += (long long)gridDim.x * blockDim.x) {
+        const long long ri = rec_index(i % N, (i / N) % N, i / ((long long)N * N), NT);
+        for (int q = 0; q < stride; ++q) acc[ri * stride + q] = in[q * n3 + i];
+    }
+}
+
 // ================================================================== host side
 struct bsx_recon {
     bsx_recon_params p;
@@ -1292,6 +1367,7 @@ struct bsx_recon {
     size_t acc_elems = 0;
     float2* inter = nullptr;
     float* bkg = nullptr;
+    double2* resc = nullptr;       // per record (scale, shift) of -rescale (flags bit 1)
     PartDev* recs = nullptr;
     int* nonfinite = nullptr;
     int64_t recs_cap = 0;
@@ -1310,6 +1386,7 @@ struct bsx_recon {
     int batch_cap = 0;               // batch the buffers were sized for
     int piece = BSX_PIECE_DEFAULT;   // tile-pass piece length (entries)
     int64_t inserted[2] = {0, 0};
+    int fallback_flags = 0;        // atomic-fallback flags seen (1 table, 2 pool, 4 wait bound)
     double ktime[7] = {0, 0, 0, 0, 0, 0, 0};   // background rows fallback scan fill tile direct
     double last_entries = 0;
     int64_t last_launches = 0;
@@ -1703,7 +1780,8 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
     BSX_REQUIRE(tw, BSX_ERR_DEVICE, "twiddles(%d) failed", FT);
     const int tiles = (box + 2 * LINES - 1) / (2 * LINES);
     hipLaunchKernelGGL((k_rows<FT, LINES>), dim3(nb * tiles), dim3(NT), 0, s, imgs,
-                       h->recs + p0, h->bkg + p0, box, tiles, h->inter, tw);
+                       h->recs + p0, h->bkg + p0, box, tiles, h->inter, tw,
+                       h->resc ? h->resc + p0 : nullptr);
     BSX_HIP(hipGetLastError());
     int r = tmark(h, s, KT_ROWS);
     if (r) return r;
@@ -1834,9 +1912,11 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     h->binned = p->method == 0 && !K.fom;
     K.interp = p->interp;
     K.nhalves = p->nhalves;
-    K.ctf_mode = p->ctf_action == 0 ? 0 : p->ctf_action == 1 ? 1
-                 : (p->ctf_action > 2 && p->wiener > 0) ? 3 : 2;
-    K.wiener = p->ctf_action > 2 ? p->wiener : 0;
+    // img_ctf_calculate (mg_ctf.cpp:100-107) as particle_reconstruct calls it
+    // (flip = action 1): any other action is a Wiener filter when the factor is
+    // positive, else the CTF is applied
+    K.ctf_mode = p->ctf_action == 0 ? 0 : p->ctf_action == 1 ? 1 : p->wiener > 0 ? 3 : 2;
+    K.wiener = p->ctf_action >= 2 ? p->wiener : 0;
     double hi = h->hi_res < 0.5 ? 0.5 : h->hi_res;   // fspace_pack_2D limit (:116)
     K.mins2 = p->lo_res > 0 ? 1.0 / (p->lo_res * p->lo_res) : 0;
     K.maxs2 = 1.0 / (hi * hi);
@@ -1926,6 +2006,7 @@ int bsx_recon_destroy(bsx_recon* h) {
     if (h->xacc) (void)hipFree(h->xacc);
     if (h->inter) (void)hipFree(h->inter);
     if (h->bkg) (void)hipFree(h->bkg);
+    if (h->resc) (void)hipFree(h->resc);
     if (h->recs) (void)hipFree(h->recs);
     if (h->nonfinite) (void)hipFree(h->nonfinite);
     if (h->bins) (void)hipFree(h->bins);
@@ -2029,19 +2110,24 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
         parts = expanded.data();
         n = (int64_t)expanded.size();
     }
+    const bool rescale = h->p.flags & BSX_FLAG_RESCALE;
     if (n > h->recs_cap) {
         if (h->recs) (void)hipFree(h->recs);
         if (h->bkg) (void)hipFree(h->bkg);
+        if (h->resc) (void)hipFree(h->resc);
         h->recs = nullptr;
         h->bkg = nullptr;
-        int64_t extra = n * (int64_t)(sizeof(PartDev) + sizeof(float));
+        h->resc = nullptr;
+        int64_t extra = n * (int64_t)(sizeof(PartDev) + sizeof(float) +
+                                      (rescale ? sizeof(double2) : 0));
         if (h->p.mem_cap > 0 && (int64_t)h->bytes + extra > h->p.mem_cap) {
             h->recs_cap = 0;
             set_error("particle records (%lld bytes) exceed the device-memory cap", (long long)extra);
             return BSX_ERR_NOMEM;
         }
         if (hipMalloc(&h->recs, sizeof(PartDev) * n) != hipSuccess ||
-            hipMalloc(&h->bkg, sizeof(float) * n) != hipSuccess) {
+            hipMalloc(&h->bkg, sizeof(float) * n) != hipSuccess ||
+            (rescale && hipMalloc(&h->resc, sizeof(double2) * n) != hipSuccess)) {
             h->recs_cap = 0;
             set_error("allocation of %lld particle records failed", (long long)n);
             return BSX_ERR_NOMEM;
@@ -2114,7 +2200,8 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
     int r = tmark(h, s, -1);
     if (r) return r;
     hipLaunchKernelGGL(k_background, dim3((unsigned)n), dim3(256), 0, s, imgs_dev, h->recs,
-                       h->p.box, h->bkg, check_nan ? h->nonfinite : nullptr);
+                       h->p.box, h->bkg, check_nan ? h->nonfinite : nullptr,
+                       rescale ? h->resc : nullptr);
     BSX_HIP(hipGetLastError());
     if ((r = tmark(h, s, KT_BG))) return r;
     if (check_nan) {
@@ -2193,6 +2280,7 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
             BSX_HIP(hipMemcpy(&flags, h->pool + 1, sizeof(int), hipMemcpyDeviceToHost));
             BSX_HIP(hipMemset(h->pool + 1, 0, sizeof(int)));
             h->ktime[KT_FALLBACK] = flags;
+            h->fallback_flags |= flags;
         }
         h->last_entries = (double)ne;
     }
@@ -2428,6 +2516,134 @@ int bsx_symmetrize_2d(const float* in_dev, float* out_dev, int32_t n, const doub
     return symmetrize_impl(in_dev, out_dev, n, 1, o3, sym, stream);
 }
 
+// ------------------------------------------------ output map post-processing
+// FP64 sum and sum of squares of a float array, per workgroup.
+__global__ __launch_bounds__(256) void k_moments(const float* __restrict__ d, long long n,
+                                                 double2* __restrict__ part) {
+    double s1 = 0, s2 = 0;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const double v = d[i];
+        s1 += v;
+        s2 += v * v;
+    }
+    __shared__ double a[256], b[256];
+    a[threadIdx.x] = s1;
+    b[threadIdx.x] = s2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            a[threadIdx.x] += a[threadIdx.x + o];
+            b[threadIdx.x] += b[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = make_double2(a[0], b[0]);
+}
+
+// Scale and shift of Bimage::rescale_to_avg_std (Bimage_rescale.cpp:163-183)
+// from the partial moments: average and population standard deviation as
+// Bimage::statistics (Bimage_stats.cpp:140-148).
+__global__ void k_rescale_coef(const double2* __restrict__ part, int nblk, long long n,
+                               double nuavg, double nustd, double2* __restrict__ coef) {
+    __shared__ double a[256], b[256];
+    double s1 = 0, s2 = 0;
+    for (int i = threadIdx.x; i < nblk; i += 256) {
+        s1 += part[i].x;
+        s2 += part[i].y;
+    }
+    a[threadIdx.x] = s1;
+    b[threadIdx.x] = s2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            a[threadIdx.x] += a[threadIdx.x + o];
+            b[threadIdx.x] += b[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double avg = a[0] / n;
+        const double var = b[0] / n - avg * avg;
+        const double sd = var > 0 ? sqrt(var) : 0.0;
+        const double sc = sd > 1e-30 ? nustd / sd : 1.0;
+        coef[0] = make_double2(sc, nuavg - avg * sc);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rescale_apply(float* __restrict__ d, long long n,
+                                                       const double2* __restrict__ coef) {
+    const double2 c = coef[0];
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        d[i] = (float)((double)d[i] * c.x + c.y);
+}
+
+// Bimage::filter_sinc (Bimage_filter.cpp:484-509): every voxel times
+// (r / sin r)^2, r = pi |d|, d = (voxel - origin) / size per axis; the
+// origin voxel (r = 0) is left as is.
+__global__ __launch_bounds__(256) void k_filter_sinc(float* __restrict__ d, int nx, int ny, int nz,
+                                                     double ox, double oy, double oz) {
+    const long long n = (long long)nx * ny * nz;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int x = (int)(i % nx);
+        const long long t = i / nx;
+        const int y = (int)(t % ny), z = (int)(t / ny);
+        const double dx = (x - ox) / nx, dy = (y - oy) / ny, dz = (z - oz) / nz;
+        const double r = M_PI * sqrt(dx * dx + dy * dy + dz * dz);
+        if (r != 0) {
+            const double w = r / sin(r);
+            d[i] = (float)(w * w * (double)d[i]);
+        }
+    }
+}
+
+int bsx_rescale_to_avg_std(float* data_dev, int64_t n, double avg, double std, void* stream) {
+    BSX_REQUIRE(data_dev && n > 0, BSX_ERR_ARG, "bsx_rescale_to_avg_std: bad arguments");
+    BSX_REQUIRE(std::isfinite(avg) && std::isfinite(std), BSX_ERR_ARG,
+                "bsx_rescale_to_avg_std: average and deviation must be finite");
+    BSX_REQUIRE(std >= 0, BSX_ERR_ARG,
+                "bsx_rescale_to_avg_std: cannot use a negative standard deviation (%g)", std);
+    hipStream_t s = (hipStream_t)stream;
+    const int nblk = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    double2* part = nullptr;
+    BSX_HIP(hipMalloc((void**)&part, sizeof(double2) * (nblk + 1)));
+    hipLaunchKernelGGL(k_moments, dim3(nblk), dim3(256), 0, s, data_dev, (long long)n, part);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_rescale_coef, dim3(1), dim3(256), 0, s, part, nblk, (long long)n, avg,
+                           std, part + nblk);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_rescale_apply, dim3(nblk), dim3(256), 0, s, data_dev, (long long)n,
+                           part + nblk);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(part);
+    BSX_HIP(e);
+    return BSX_OK;
+}
+
+int bsx_filter_sinc(float* data_dev, int32_t nx, int32_t ny, int32_t nz, const double* origin,
+                    void* stream) {
+    BSX_REQUIRE(data_dev && origin && nx > 0 && ny > 0 && nz > 0, BSX_ERR_ARG,
+                "bsx_filter_sinc: bad arguments");
+    BSX_REQUIRE(std::isfinite(origin[0]) && std::isfinite(origin[1]) && std::isfinite(origin[2]),
+                BSX_ERR_ARG, "bsx_filter_sinc: origin not finite");
+    const long long n = (long long)nx * ny * nz;
+    const int nblk = (int)std::min<long long>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_filter_sinc, dim3(nblk), dim3(256), 0, (hipStream_t)stream, data_dev, nx,
+                       ny, nz, origin[0], origin[1], origin[2]);
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_recon_fallbacks(const bsx_recon* h, int32_t* flags) {
+    BSX_REQUIRE(h && flags, BSX_ERR_ARG, "null argument");
+    *flags = h->fallback_flags;
+    return BSX_OK;
+}
+
 int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n) {
     BSX_REQUIRE(h && n, BSX_ERR_ARG, "bsx_recon_graph_launches: null argument");
     *n = h->graph_launches;
@@ -2474,6 +2690,25 @@ int bsx_recon_accumulators(bsx_recon* h, int half, float* planes_dev, void* stre
                        h->acc + (size_t)half * nrec * h->K.stride, n3, h->K.N, h->K.NT,
                        h->K.stride, h->K.fom, planes_dev);
     BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_recon_set_accumulators(bsx_recon* h, int half, const float* planes_dev, int64_t count,
+                               void* stream) {
+    BSX_REQUIRE(h && planes_dev, BSX_ERR_ARG, "null argument");
+    BSX_REQUIRE(half >= 0 && half < h->p.nhalves, BSX_ERR_ARG, "half %d out of range", half);
+    BSX_REQUIRE(count >= 0, BSX_ERR_ARG, "negative particle count");
+    hipStream_t s = (hipStream_t)stream;
+    BSX_HIP(hipSetDevice(h->dev));
+    const int rf = fold(h, s);   // the pair accumulator is empty afterwards
+    if (rf) return rf;
+    const long long n3 = (long long)h->K.N * h->K.N * h->K.N;
+    const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
+    hipLaunchKernelGGL(k_set_planes, dim3(1024), dim3(256), 0, s,
+                       h->acc + (size_t)half * nrec * h->K.stride, n3, h->K.N, h->K.NT,
+                       h->K.stride, planes_dev);
+    BSX_HIP(hipGetLastError());
+    h->inserted[half] = count;
     return BSX_OK;
 }
 
@@ -2617,6 +2852,13 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
     double hsum[4] = {0, 0, 0, 0};
     BSX_HIP(hipMemcpyAsync(&hcov, dcov, sizeof(hcov), hipMemcpyDeviceToHost, s));
     BSX_HIP(hipMemcpyAsync(hsum, dsum, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    // atomic-fallback flags of the inserts since the last read (graph
+    // replays included); cleared for the next inserts
+    int hflags = 0;
+    if (h->binned) {
+        BSX_HIP(hipMemcpyAsync(&hflags, h->pool + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        BSX_HIP(hipMemsetAsync(h->pool + 1, 0, sizeof(int), s));
+    }
     int r = BSX_OK;
     if (map_dev) {
         // inverse transform: work on a copy when the caller wants the transform back
@@ -2639,6 +2881,7 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
         if (wtmp) BSX_HIP(hipFree(wtmp));
     }
     BSX_HIP(hipStreamSynchronize(s));
+    h->fallback_flags |= hflags;
     if (tmp) BSX_HIP(hipFree(tmp));
     BSX_HIP(hipFree(dcov));
     if (r != BSX_OK) return r;

@@ -264,13 +264,65 @@ def _real(d: dict, *tags, default=None):
     return default
 
 
+# Deprecated micrograph CTF tags and the tags that replace them, as the
+# STAR reader renames them before reading (rwmgSTAR.cpp:113-126, tag names
+# include/mg_tags.h:129-142, 169-189).
+_DEPRECATED_CTF_TAGS = {
+    "micrograph.voltage": "ctf.voltage",
+    "micrograph.ctf.Cs": "ctf.Cs",
+    "micrograph.ctf.Cc": "ctf.Cc",
+    "micrograph.ctf.alpha": "ctf.alpha",
+    "micrograph.ctf.energy_spread": "ctf.energy_spread",
+    "micrograph.ctf.amp_contrast": "ctf.amp_contrast",
+    "micrograph.ctf.first_zero": "ctf.first_zero",
+    "micrograph.ctf.defocus_average": "ctf.defocus_average",
+    "micrograph.ctf.defocus_deviation": "ctf.defocus_deviation",
+    "micrograph.ctf.defocus_min": "ctf.defocus_min",
+    "micrograph.ctf.defocus_max": "ctf.defocus_max",
+    "micrograph.ctf.astigmatism_angle": "ctf.astigmatism_angle",
+    "micrograph.ctf.baseline": "ctf.baseline",
+    "micrograph.ctf.envelope": "ctf.envelope",
+}
+
+
+def _micrograph_ctf(items: dict):
+    """CTF of a micrograph block as ctf_from_starblock reads it
+    (rwmgSTAR.cpp:458-518) after the deprecated tags are renamed: (volt,
+    Cs, amplitude phase shift, defocus average, defocus deviation,
+    astigmatism angle in radians). Without ctf.voltage the micrograph has no
+    CTF and the reconstruction's default CTFparam applies (include/ctf.h:
+    49-72: 300 kV, every aberration weight 0)."""
+    it = dict(items)
+    for old, new in _DEPRECATED_CTF_TAGS.items():
+        if old in it and new not in it:
+            it[new] = it.pop(old)
+    if "ctf.voltage" not in it:
+        return 300000.0, 0.0, 0.0, 0.0, 0.0, 0.0
+    volt = _real(it, "ctf.voltage", default=0.0)
+    cs = _real(it, "ctf.Cs", default=0.0)
+    if "ctf.amplitude_phase_shift" in it:
+        amp_shift = float(it["ctf.amplitude_phase_shift"])
+    else:
+        amp_shift = math.asin(_real(it, "ctf.amp_contrast", default=0.0))
+    if "ctf.defocus_average" in it:
+        def_avg = _real(it, "ctf.defocus_average")
+        def_dev = _real(it, "ctf.defocus_deviation", default=0.0)
+    else:
+        lo = _real(it, "ctf.defocus_min", default=0.0)
+        hi = _real(it, "ctf.defocus_max", default=0.0)
+        def_avg, def_dev = (hi + lo) / 2, abs(hi - lo) / 2
+    ast = math.radians(_real(it, "ctf.astigmatism_angle", default=0.0))
+    return volt, cs, amp_shift, def_avg, def_dev, ast
+
+
 def project_particles(path: str):
     """The selected particles of a Bsoft STAR project: a PARTICLE_DTYPE record
     per particle (view, origin, CTF of its micrograph with the particle's own
     defocus average, pixel size, selection) and where its image lives.
     Micrograph CTF: ctf.voltage (V), ctf.Cs (A), ctf.amplitude_phase_shift or
-    asin(ctf.amp_contrast), ctf.defocus_average / ctf.defocus_deviation (A),
-    ctf.astigmatism_angle (deg). Particle loop: particle.id, particle.file_name,
+    asin(ctf.amp_contrast), ctf.defocus_average / ctf.defocus_deviation (A)
+    or ctf.defocus_min / _max, ctf.astigmatism_angle (deg); the deprecated
+    micrograph.voltage and micrograph.ctf.* tags are read as these. Particle loop: particle.id, particle.file_name,
     particle.defocus, particle.origin_x/y, particle.view_x/y/z,
     particle.view_angle (deg), particle.select, particle.pixel_size_x."""
     base = os.path.dirname(os.path.abspath(path))
@@ -282,21 +334,7 @@ def project_particles(path: str):
             continue
         tags, rows = lp
         col = {t: k for k, t in enumerate(tags)}
-        volt = _real(it, "ctf.voltage", "micrograph.voltage", default=0.0)
-        cs = _real(it, "ctf.Cs", "micrograph.ctf_Cs", default=0.0)
-        if "ctf.amplitude_phase_shift" in it:
-            amp_shift = float(it["ctf.amplitude_phase_shift"])
-        else:
-            amp_shift = math.asin(_real(it, "ctf.amp_contrast", "micrograph.ctf_amp_contrast",
-                                        default=0.0))
-        if "ctf.defocus_average" in it or "micrograph.ctf_defocus_average" in it:
-            def_avg = _real(it, "ctf.defocus_average", "micrograph.ctf_defocus_average")
-            def_dev = _real(it, "ctf.defocus_deviation", default=0.0)
-        else:
-            lo = _real(it, "ctf.defocus_min", default=0.0)
-            hi = _real(it, "ctf.defocus_max", default=0.0)
-            def_avg, def_dev = (hi + lo) / 2, abs(hi - lo) / 2
-        ast = math.radians(_real(it, "ctf.astigmatism_angle", default=0.0))
+        volt, cs, amp_shift, def_avg, def_dev, ast = _micrograph_ctf(it)
         pix = _real(it, "micrograph.pixel_size_x", "micrograph.pixel_size", default=1.0)
         mg_file = it.get("particle.file_name", it.get("micrograph_particle.file_name", ""))
         for r in rows:

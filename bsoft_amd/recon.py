@@ -115,7 +115,8 @@ class Reconstruction:
         the handle must be empty and is left empty."""
         parts = np.ascontiguousarray(parts, dtype=PARTICLE_DTYPE)
         b, ms = C.c_int32(), C.c_double()
-        call("bsx_recon_autotune", self.h, _ptr(imgs.contiguous()), particles_ctypes(parts),
+        imgs = imgs.contiguous()
+        call("bsx_recon_autotune", self.h, _ptr(imgs), particles_ctypes(parts),
              len(parts), C.byref(b), C.byref(ms), _stream(stream))
         self._batch = b.value
         return b.value
@@ -170,6 +171,22 @@ class Reconstruction:
         torch.cuda.synchronize()
         return out
 
+    def set_accumulators(self, planes: torch.Tensor, half: int = 0, count: int = 0, stream=None):
+        """Replace one half-set's sums with hand-built planes (5, N, N, N)."""
+        n = self.size
+        if planes.dtype != torch.float32 or tuple(planes.shape) != (5, n, n, n):
+            raise ValueError(f"planes must be float32 (5, {n}, {n}, {n})")
+        planes = planes.contiguous()
+        call("bsx_recon_set_accumulators", self.h, half, _ptr(planes), int(count), _stream(stream))
+        torch.cuda.synchronize()
+
+    def fallbacks(self) -> int:
+        """Atomic-fallback flags of the binned insertion seen so far
+        (bsx_recon_fallbacks: 1 chunk table, 2 chunk pool, 4 wait bound)."""
+        f = C.c_int32()
+        call("bsx_recon_fallbacks", self.h, C.byref(f))
+        return f.value
+
     def finish(self, imap: int = 0, nmaps: int = 1, want_fom: bool = False,
                want_ft: bool = False, want_map: bool = True, stream=None) -> FinishResult:
         n = self.size
@@ -210,6 +227,27 @@ def particle_reconstruct(imgs: torch.Tensor, parts: np.ndarray, size: int, sampl
                          batch=batch)
     rec.insert(imgs, parts)
     return rec
+
+
+def rescale_to_avg_std(vol: torch.Tensor, avg: float, std: float, stream=None) -> torch.Tensor:
+    """Bimage::rescale_to_avg_std (Bimage_rescale.cpp:163) of a float32
+    device array, in place (breconstruct -rescale on each output map)."""
+    if vol.dtype != torch.float32 or not vol.is_contiguous():
+        raise ValueError("a contiguous float32 array is required")
+    call("bsx_rescale_to_avg_std", _ptr(vol), vol.numel(), float(avg), float(std), _stream(stream))
+    return vol
+
+
+def filter_sinc(vol: torch.Tensor, origin=None, stream=None) -> torch.Tensor:
+    """Bimage::filter_sinc (Bimage_filter.cpp:484-509) of a float32 (nz, ny,
+    nx) device map, in place: the correction of trilinear insertion."""
+    if vol.dtype != torch.float32 or not vol.is_contiguous() or vol.dim() != 3:
+        raise ValueError("a contiguous float32 (nz, ny, nx) map is required")
+    nz, ny, nx = vol.shape
+    o = np.asarray(origin if origin is not None else (nx // 2, ny // 2, nz // 2), dtype=np.float64)
+    call("bsx_filter_sinc", _ptr(vol), nx, ny, nz, o.ctypes.data_as(C.POINTER(C.c_double)),
+         _stream(stream))
+    return vol
 
 
 def symmetry_views(sym: str, views: np.ndarray) -> np.ndarray:
@@ -285,7 +323,8 @@ def map_transform(vol: torch.Tensor, origin=None, stream=None) -> torch.Tensor:
         origin = (n // 2, n // 2, n // 2)
     o = (C.c_double * 3)(*origin)
     out = torch.empty((n, n, n), device=vol.device, dtype=torch.complex64)
-    call("bsx_map_transform", _ptr(vol.contiguous()), n, o, _ptr(out), _stream(stream))
+    vol = vol.contiguous()          # held until the call has returned
+    call("bsx_map_transform", _ptr(vol), n, o, _ptr(out), _stream(stream))
     return out
 
 

@@ -28,7 +28,12 @@ extern "C" {
 #define BSX_ERR_STATE       -4   /* call out of order / unsupported mode */
 #define BSX_ERR_NONFINITE   -5   /* non-finite particle data detected */
 
-/* ---- CTF actions: include/options.h:418-442 (flip=1, apply=2, >2 Wiener) ---- */
+/* ---- CTF actions: include/options.h:418-442. particle_reconstruct passes
+ *      flip = (action == 1) and the Wiener factor to img_ctf_calculate
+ *      (mg_ctf.cpp:46-110): action 1 flips phases; any other nonzero action
+ *      applies sin(chi) when wiener == 0 and the Wiener filter when wiener > 0
+ *      (breconstruct sets wiener 0.2 for actions > 2 given no factor,
+ *      breconstruct.cpp:187; the library takes the factor as given). ---- */
 #define BSX_CTF_NONE   0
 #define BSX_CTF_FLIP   1
 #define BSX_CTF_APPLY  2
@@ -36,6 +41,10 @@ extern "C" {
 
 /* ---- reconstruction flags: src/mg/mg_reconstruct.cpp:83 (1 rescale,
  *      2 2D, 4 bootstrap, 8 Ewald) plus library-specific bits ---- */
+#define BSX_FLAG_RESCALE     1   /* rescale each particle image to average 0 and
+                                    standard deviation 1 before its background
+                                    (rescale_to_avg_std(0, 1),
+                                    mg_reconstruct.cpp:160)                   */
 #define BSX_FLAG_2D          2   /* 2D map (breconstruct -TwoD): one z plane,
                                     map / FOM / transform outputs N x N     */
 #define BSX_FLAG_BOOTSTRAP   4
@@ -64,7 +73,7 @@ typedef struct bsx_recon_params {
     double  lo_res;        /* low resolution limit (A), 0 = none                  */
     int32_t ctf_action;    /* BSX_CTF_*                                           */
     int32_t flags;         /* BSX_FLAG_*                                          */
-    double  wiener;        /* Wiener factor (used when ctf_action > 2)            */
+    double  wiener;        /* Wiener factor (> 0: Wiener for any action but flip) */
     int32_t nhalves;       /* 1: one accumulator; 2: half-set accumulators        */
     int32_t batch;         /* particles per internal batch (0 = autotuned)        */
     int64_t mem_cap;       /* device-memory cap for this handle (bytes, 0 = none) */
@@ -148,6 +157,13 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
  * (planes 3-4 are zero without BSX_FLAG_FOM). For verification. */
 int bsx_recon_accumulators(bsx_recon* h, int half, float* planes_dev, void* stream);
 
+/* The inverse: replaces the accumulators of one half-set with planes_dev
+ * (the same [5][N^3] layout; planes 3-4 are ignored without BSX_FLAG_FOM)
+ * and sets its particle count. For verification of the finish on
+ * hand-built sums (the reference's partial maps are plain images). */
+int bsx_recon_set_accumulators(bsx_recon* h, int half, const float* planes_dev, int64_t count,
+                               void* stream);
+
 /* Device bytes held by a handle. */
 int bsx_recon_device_bytes(const bsx_recon* h, int64_t* bytes);
 
@@ -166,6 +182,13 @@ int bsx_recon_tuning(const bsx_recon* h, int32_t* batch, int32_t* piece);
 int bsx_recon_set_tuning(bsx_recon* h, int32_t batch, int32_t piece);
 /* Number of inserts served by graph replay so far (BSX_FLAG_GRAPH). */
 int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n);
+/* Capacity fallbacks of the tile-binned insertion (no counterpart in the
+ * reference): the OR of the flags raised by the inserts read so far — by
+ * each bsx_recon_finish and by each timed insert (BSX_FLAG_TIMING), graph
+ * replays included: 1 a bin's chunk table was full, 2 the chunk pool was
+ * exhausted, 4 a chunk wait reached its bound. The entries concerned were
+ * added with float atomics: the sums are complete, the insert was slower. */
+int bsx_recon_fallbacks(const bsx_recon* h, int32_t* flags);
 
 /* Fourier shell statistics of a weighed reconstruction, the table that
  * fspace_reconstruction_stats (src/img/Bimage_reconstruct.cpp:526-599) prints
@@ -453,6 +476,24 @@ int bsx_symmetrize(const float* in_dev, float* out_dev, int32_t n, const double*
  * images (Bimage_symmetry.cpp:40-46). Synchronises the stream. */
 int bsx_symmetrize_2d(const float* in_dev, float* out_dev, int32_t n, const double* origin,
                       const char* sym, void* stream);
+
+/* Bimage::rescale_to_avg_std (src/img/Bimage_rescale.cpp:163-183), as
+ * breconstruct -rescale avg,std applies it to each output map
+ * (src/breconstruct.cpp:336): v <- v * (std / s) + avg - a * std / s with
+ * a, s the average and population standard deviation of the n values
+ * (FP64 sums, Bimage::statistics) and the factor 1 when s <= 1e-30.
+ * data_dev: n FP32 values (device), rescaled in place. std < 0 is an
+ * error, as in the reference. Synchronises the stream. */
+int bsx_rescale_to_avg_std(float* data_dev, int64_t n, double avg, double std, void* stream);
+
+/* Bimage::filter_sinc (src/img/Bimage_filter.cpp:484-509), the correction
+ * breconstruct applies to maps from trilinear insertion
+ * (src/breconstruct.cpp:359): each voxel times (r / sin r)^2 with
+ * r = pi |(x - ox) / nx, (y - oy) / ny, (z - oz) / nz|, the origin voxel
+ * unchanged. data_dev: nz x ny x nx FP32 (device), in place; origin: 3
+ * doubles (voxels). Asynchronous. */
+int bsx_filter_sinc(float* data_dev, int32_t nx, int32_t ny, int32_t nz, const double* origin,
+                    void* stream);
 
 /* symmetry_get_all_views (src/util/symmetry.cpp:277-319) for each of n views
  * (x, y, z, angle): out holds, view after view, the order(sym) related views

@@ -285,6 +285,73 @@ int or_fft(orr* data, long nx, long ny, long nz, long batch, int dir, int norm)
 /* (mg_ctf.cpp:661-676), in the order of particle_reconstruct                 */
 /* (mg_reconstruct.cpp:159-214).                                              */
 /* ------------------------------------------------------------------------ */
+/* Bimage::statistics (Bimage_stats.cpp:104-148): average and population
+ * standard deviation from FP64 sums of v and v^2. */
+static void moments(const double* d, const float* f, long n, double* avg, double* sd)
+{
+    double s1 = 0, s2 = 0;
+    for (long i = 0; i < n; i++) {
+        double v = d ? d[i] : (double)f[i];
+        s1 += v;
+        s2 += v * v;
+    }
+    *avg = s1 / n;
+    double var = s2 / n - *avg * *avg;
+    *sd = var > 0 ? sqrt(var) : 0;
+}
+
+/* Bimage::rescale_to_avg_std (Bimage_rescale.cpp:163-183) with
+ * Bimage::rescale (:27-45): v <- v * scale + shift, scale = nustd / sd
+ * (1 when sd <= 1e-30), shift = nuavg - avg * scale. */
+static void rescale_coef(double avg, double sd, double nuavg, double nustd, double* sc, double* sh)
+{
+    *sc = sd > 1e-30 ? nustd / sd : 1;
+    *sh = nuavg - avg * *sc;
+}
+
+void or_rescale_f32(float* d, long n, double nuavg, double nustd)
+{
+    double avg, sd, sc, sh;
+    moments(NULL, d, n, &avg, &sd);
+    rescale_coef(avg, sd, nuavg, nustd, &sc, &sh);
+    for (long i = 0; i < n; i++) d[i] = (float)((double)d[i] * sc + sh);
+}
+
+int or_rescale_to_avg_std(orr* d, long n, double nuavg, double nustd)
+{
+    if (nustd < 0 || n < 1) return -1;
+    double avg, sd, sc, sh;
+#ifdef OR_FP32
+    moments(NULL, d, n, &avg, &sd);
+#else
+    moments(d, NULL, n, &avg, &sd);
+#endif
+    rescale_coef(avg, sd, nuavg, nustd, &sc, &sh);
+    for (long i = 0; i < n; i++) d[i] = (orr)((double)d[i] * sc + sh);
+    return 0;
+}
+
+/* Bimage::filter_sinc (Bimage_filter.cpp:484-509): v <- v (r / sin r)^2,
+ * r = pi |(x - o) / size| per axis, the origin voxel unchanged. */
+int or_filter_sinc(orr* d, long nx, long ny, long nz, const double o[3])
+{
+    for (long z = 0, i = 0; z < nz; z++) {
+        double dz = (z - o[2]) / nz;
+        for (long y = 0; y < ny; y++) {
+            double dy = (y - o[1]) / ny;
+            for (long x = 0; x < nx; x++, i++) {
+                double dx = (x - o[0]) / nx;
+                double r = M_PI * sqrt(dx * dx + dy * dy + dz * dz);
+                if (r != 0) {
+                    double w = r / sin(r);
+                    d[i] = (orr)(w * w * (double)d[i]);
+                }
+            }
+        }
+    }
+    return 0;
+}
+
 double or_background(const float* img, long box, const double ori[2])
 {
     long hx = box / 2, hy = box / 2, nm = 0;
@@ -311,6 +378,16 @@ int or_stage_particle(const bsx_recon_params* rp, const bsx_particle* pp, const 
     if (box < 2 || ft < box) return -1;
     /* a 2D map (flags & 2) has one z plane: its check_resolution counts two axes */
     double hi = or_check_resolution(rp->hi_res, rp->size, rp->sampling, (rp->flags & 2) ? 2 : 3);
+    /* -rescale (flags bit 1): rescale_to_avg_std(0, 1) of the image before
+     * its background (mg_reconstruct.cpp:160-161), the values stored FP32 */
+    float* scaled = NULL;
+    if (rp->flags & BSX_FLAG_RESCALE) {
+        scaled = (float*)malloc(sizeof(float) * box * box);
+        if (!scaled) return -1;
+        for (long i = 0; i < box * box; i++) scaled[i] = img[i];
+        or_rescale_f32(scaled, box * box, 0.0, 1.0);
+        img = scaled;
+    }
     double bkg = or_background(img, box, pp->hdr_ori);
     double w = 1;
     if (rp->flags & BSX_FLAG_BOOTSTRAP) w = pp->sel;
@@ -323,11 +400,14 @@ int or_stage_particle(const bsx_recon_params* rp, const bsx_particle* pp, const 
             F[2 * (y * ft + x)] = v;
             F[2 * (y * ft + x) + 1] = 0;
         }
+    free(scaled);
     if (or_fft(F, ft, ft, 1, 1, -1, 1)) return -1;
     /* phase shift to origin: shift = origin/size * 2pi */
     double sx = pp->ori[0] / ft * 2 * M_PI, sy = pp->ori[1] / ft * 2 * M_PI;
+    /* img_ctf_calculate(cp, flip = (action == 1), wiener, ...): a positive
+     * factor makes any other action a Wiener filter (mg_ctf.cpp:100-107) */
     int flip = rp->ctf_action == BSX_CTF_FLIP;
-    double wiener = rp->ctf_action > 2 ? rp->wiener : 0;
+    double wiener = rp->ctf_action >= 2 ? rp->wiener : 0;
     double pix = pp->pixel_size, shi = 1 / hi, shi2 = shi * shi;
     double fscale = 1.0 / (ft * pix);
     for (long y = 0; y < ft; y++) {
@@ -560,6 +640,24 @@ int or_recon_partial_acc(or_recon* r, int t, orr* planes)
     return 0;
 }
 
+/* Replaces the accumulators of partial map t (planes as or_recon_partial_acc
+ * returns them; the particle count is set to `count`): hand-built sums for
+ * the tests of the finish. */
+int or_recon_set_partial_acc(or_recon* r, int t, const orr* planes, long count)
+{
+    if (t < 0 || t >= r->T) return -1;
+    long N3 = r->N3;
+    for (long j = 0; j < N3; j++) {
+        r->acc[t].F[2 * j] = planes[j];
+        r->acc[t].F[2 * j + 1] = planes[N3 + j];
+        r->acc[t].W[j] = planes[2 * N3 + j];
+        r->acc[t].P[j] = planes[3 * N3 + j];
+        r->acc[t].W2[j] = planes[4 * N3 + j];
+    }
+    r->acc[t].count = count;
+    return 0;
+}
+
 /* Bimage::friedel_check, Bimage_fspace.cpp:2024-2074 (overall residual R) */
 double or_friedel(const orr* d, long x, long y, long z)
 {
@@ -730,32 +828,29 @@ int or_fsc(const orr* a, const orr* b, long n, double sampling, double hi_res,
 /* extraction: FSI_Kernel (include/FSI_Kernel.h:28-68), central_section       */
 /* (Bimage_project.cpp:288-338), fspace_interpolate (Bimage_fspace.cpp:85-147)*/
 /* ------------------------------------------------------------------------ */
+/* The kernel table of FSI_Kernel (include/FSI_Kernel.h:28-68) from its
+ * closed form: row r (frac = r/512), tap j at distance x = frac + (wd-1)/2 - j,
+ * K(x) = sin(pi x)/(wd sin(pi x/wd)) cos^p(pi x/wd), limit 1 at x = 0 and 0
+ * at other integers; power < 0 gives one tap of weight 1. */
+static double fsi_k(double x, int wd, int power)
+{
+    double r = nearbyint(x);
+    if (fabs(x - r) < 1e-12) return r == 0 ? 1.0 : 0.0;
+    double a = M_PI * x / wd;
+    return sin(M_PI * x) / (wd * sin(a)) * pow(cos(a), power);
+}
+
 int or_fsi_kernel(int wd, int power, double* data)
 {
-    int ndiv = 512;
-    if (power < 0) wd = 1;
-    long size = (long)(ndiv + 1) * wd;
-    for (long i = 0; i < size; i++) data[i] = 0;
-    if (power < 0) { for (long i = 0; i < size; i++) data[i] = 1; return (int)size; }
-    int wd_2 = (wd - 1) / 2;
-    double iwd = 1.0 / wd, piwd = M_PI * iwd, incr = 1.0 / ndiv, arg = incr;
-    data[wd_2] = 1;
-    long ik = wd;
-    for (long i = 1; i < ndiv; i++) {
-        double arg1 = M_PI * (arg + wd_2);
-        double s1 = iwd * sin(arg1);
-        double arg2 = arg1 * iwd;
-        for (int j = 0; j < wd; j++, ik++) {
-            double s2 = sin(arg2), c = cos(arg2);
-            data[ik] = (s1 / s2) * pow(c, power);
-            arg2 -= piwd;
-            s1 = -s1;
-        }
-        arg += incr;
+    const int ndiv = 512;
+    if (power < 0) {
+        for (int r = 0; r <= ndiv; r++) data[r] = 1;
+        return ndiv + 1;
     }
-    ik += wd_2 + 1;
-    data[ik] = 1;
-    return (int)size;
+    int c = (wd - 1) / 2;
+    for (int r = 0; r <= ndiv; r++)
+        for (int j = 0; j < wd; j++) data[(long)r * wd + j] = fsi_k((double)r / ndiv + c - j, wd, power);
+    return (ndiv + 1) * wd;
 }
 
 static void fsi_interp(const orr* ft, long n, const double m[3], const double* K, int wd,

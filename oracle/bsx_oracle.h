@@ -33,6 +33,9 @@ double or_check_resolution(double res, int n, double sampling, int ndim);
 double or_ctf_value(const bsx_particle* p, double sx, double sy, int flip, double wiener);
 int    or_fft(orr* data, long nx, long ny, long nz, long batch, int dir, int norm);
 double or_background(const float* img, long box, const double hdr_ori[2]);
+void   or_rescale_f32(float* img, long n, double nuavg, double nustd);
+int    or_rescale_to_avg_std(orr* data, long n, double nuavg, double nustd);
+int    or_filter_sinc(orr* data, long nx, long ny, long nz, const double origin[3]);
 int    or_stage_particle(const bsx_recon_params* rp, const bsx_particle* p, const float* img,
                          orr* out_ft, double* part_weight);
 
@@ -42,6 +45,7 @@ void   or_recon_free(or_recon* r);
 int    or_recon_insert(or_recon* r, const bsx_particle* parts, const float* imgs, long n,
                        int nthreads);
 int    or_recon_partial_acc(or_recon* r, int ipartial, orr* planes);
+int    or_recon_set_partial_acc(or_recon* r, int ipartial, const orr* planes, long count);
 int    or_recon_finish(or_recon* r, int imap, int nmaps, orr* map_out, orr* fom_out,
                        orr* ft_out, bsx_stats* st);
 

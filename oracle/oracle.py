@@ -57,6 +57,10 @@ def _load(prec: int) -> C.CDLL:
                                C.c_double),
         "or_asu_views_c1": ([C.c_double, C.c_double, C.c_int, _dp, C.c_long], C.c_long),
         "or_friedel": ([_vp, C.c_long, C.c_long, C.c_long], C.c_double),
+        "or_recon_set_partial_acc": ([_vp, C.c_int, _vp, C.c_long], C.c_int),
+        "or_rescale_to_avg_std": ([_vp, C.c_long, C.c_double, C.c_double], C.c_int),
+        "or_rescale_f32": ([_vp, C.c_long, C.c_double, C.c_double], None),
+        "or_filter_sinc": ([_vp, C.c_long, C.c_long, C.c_long, _dp], C.c_int),
     }
     for name, (a, r) in sig.items():
         f = getattr(L, name)
@@ -187,6 +191,26 @@ class Oracle:
         n = self.L.or_asu_views_c1(float(theta_step), float(phi_step), int(flag), self._d(v), cap)
         return v[: 4 * min(n, cap)].reshape(-1, 4)
 
+    def rescale_to_avg_std(self, d: np.ndarray, avg: float, std: float) -> np.ndarray:
+        """Bimage::rescale_to_avg_std of a copy of d (oracle precision)."""
+        d = np.ascontiguousarray(d, dtype=self.dt).copy()
+        assert self.L.or_rescale_to_avg_std(self._p(d), d.size, float(avg), float(std)) == 0
+        return d
+
+    def rescale_f32(self, img: np.ndarray, avg: float = 0.0, std: float = 1.0) -> np.ndarray:
+        """The per-particle rescale of breconstruct -rescale (FP32 storage)."""
+        img = np.ascontiguousarray(img, dtype=np.float32).copy()
+        self.L.or_rescale_f32(self._p(img), img.size, float(avg), float(std))
+        return img
+
+    def filter_sinc(self, d: np.ndarray, origin) -> np.ndarray:
+        """Bimage::filter_sinc of a copy of a (nz, ny, nx) map."""
+        d = np.ascontiguousarray(d, dtype=self.dt).copy()
+        nz, ny, nx = d.shape
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        assert self.L.or_filter_sinc(self._p(d), nx, ny, nz, self._d(o)) == 0
+        return d
+
     def friedel(self, ft: np.ndarray) -> float:
         ft = np.ascontiguousarray(ft, dtype=self.cdt)
         nz, ny, nx = ft.shape
@@ -218,6 +242,13 @@ class OracleRecon:
     def _shape(self):
         n = self.rp.size
         return (1 if self.rp.flags & 2 else n, n, n)     # flags & 2: a 2D map
+
+    def set_partial_acc(self, t: int, planes: np.ndarray, count: int = 0) -> None:
+        """Hand-built accumulators (5, *shape): sum w re, sum w im, sum w,
+        sum w|F|^2, sum w^2."""
+        planes = np.ascontiguousarray(planes, dtype=self.o.dt)
+        assert planes.shape == (5, *self._shape())
+        assert self.o.L.or_recon_set_partial_acc(self.h, int(t), self.o._p(planes), int(count)) == 0
 
     def partial_acc(self, t: int) -> np.ndarray:
         shp = self._shape()

@@ -57,7 +57,12 @@ def test_mrc_big_endian_and_modes(tmp_path):
     assert np.array_equal(m.data[0, 0], np.arange(12, dtype=np.float32).reshape(3, 4))
 
 
-def _project(tmp_path, n=5):
+CTF_TAGS = {"ctf.voltage": "300000", "ctf.Cs": "2.7e7", "ctf.amp_contrast": "0.07",
+            "ctf.defocus_average": "20000", "ctf.defocus_deviation": "500",
+            "ctf.astigmatism_angle": "30"}
+
+
+def _project(tmp_path, n=5, ctf_tags=None):
     rng = np.random.default_rng(2)
     imgs = rng.normal(size=(n, 16, 16)).astype(np.float32)
     formats.write_mrc(str(tmp_path / "parts.mrcs"), imgs, (1.5, 1.5, 1.0), (8, 8, 0), images=n)
@@ -66,9 +71,7 @@ def _project(tmp_path, n=5):
         rows.append([i + 1, 1.1e4 + 100 * i, 8.0, 7.5, 0.0, 0.6, 0.8, 30.0 * i, 1 if i != 2 else 0])
     b = formats.StarBlock(name="mg1", items={
         "micrograph.id": "mg1", "micrograph.pixel_size_x": "1.5",
-        "particle.file_name": "parts.mrcs", "ctf.voltage": "300000", "ctf.Cs": "2.7e7",
-        "ctf.amp_contrast": "0.07", "ctf.defocus_average": "20000", "ctf.defocus_deviation": "500",
-        "ctf.astigmatism_angle": "30"},
+        "particle.file_name": "parts.mrcs", **(CTF_TAGS if ctf_tags is None else ctf_tags)},
         loops=[(["particle.id", "particle.defocus", "particle.origin_x", "particle.origin_y",
                  "particle.view_x", "particle.view_y", "particle.view_z", "particle.view_angle",
                  "particle.select"], rows)])
@@ -95,6 +98,38 @@ def test_star_project_particles(tmp_path):
     got, hdr = formats.load_particle_images(srcs)
     assert np.array_equal(got, imgs)
     assert np.allclose(hdr, 8.0)
+
+
+def test_star_deprecated_ctf_tags(tmp_path):
+    """Old-style projects keep the micrograph CTF under micrograph.voltage and
+    micrograph.ctf.* (include/mg_tags.h:129-142); they read as the ctf.* tags
+    (rwmgSTAR.cpp:113-126)."""
+    old = {"micrograph.voltage": "200000", "micrograph.ctf.Cs": "2.0e7",
+           "micrograph.ctf.amp_contrast": "0.1", "micrograph.ctf.defocus_average": "15000",
+           "micrograph.ctf.defocus_deviation": "300", "micrograph.ctf.astigmatism_angle": "45"}
+    p, _ = _project(tmp_path, ctf_tags=old)
+    parts, _ = formats.project_particles(p)
+    assert parts["volt"][0] == 2e5 and parts["cs"][0] == 2.0e7
+    assert parts["amp_shift"][0] == pytest.approx(math.asin(0.1))
+    assert parts["def_dev"][0] == pytest.approx(300.0)
+    assert parts["ast_angle"][0] == pytest.approx(math.radians(45))
+    assert parts["def_avg"][0] == pytest.approx(1.1e4)         # the particle's own defocus
+    # defocus minimum / maximum instead of average / deviation
+    p, _ = _project(tmp_path, ctf_tags={"micrograph.voltage": "300000",
+                                        "micrograph.ctf.defocus_min": "10000",
+                                        "micrograph.ctf.defocus_max": "12000"})
+    parts, _ = formats.project_particles(p)
+    assert parts["def_dev"][0] == pytest.approx(1000.0)
+    assert parts["cs"][0] == 0.0 and parts["amp_shift"][0] == 0.0
+
+
+def test_star_without_ctf(tmp_path):
+    """No voltage tag: the micrograph has no CTF (ctf_from_starblock returns
+    NULL) and the default CTFparam applies: 300 kV, no aberrations."""
+    p, _ = _project(tmp_path, ctf_tags={"ctf.Cs": "2.7e7"})
+    parts, _ = formats.project_particles(p)
+    assert parts["volt"][0] == 3e5 and parts["cs"][0] == 0.0 and parts["def_dev"][0] == 0.0
+    assert parts["amp_shift"][0] == 0.0
 
 
 def test_star_syntax(tmp_path):

@@ -93,7 +93,8 @@ def test_interpolation_modes(interp):
     check_map(g, ref)
 
 
-@pytest.mark.parametrize("ctf,wiener", [(0, 0.0), (2, 0.0), (4, 0.2), (4, 0.05)])
+@pytest.mark.parametrize("ctf,wiener", [(0, 0.0), (2, 0.0), (4, 0.2), (4, 0.05), (2, 0.3),
+                                        (7, 0.2)])
 def test_ctf_actions(ctf, wiener):
     bm, ps = dataset(32, 10, 300, 32, seed=2, astig=(0.0, 2000.0))
     rec, g, ref, _ = run_pair(ps, size=32, box=32, ft=64, ctf=ctf, wiener=wiener)
@@ -276,29 +277,87 @@ def _star_project(tmp_path):
     return star, ps
 
 
+def _oracle_maps(ps, nmaps, **kw):
+    """The FP64 oracle's maps (imap 0..nmaps-1) of the particles in project
+    order with T = 2 round-robin partial maps, as breconstruct runs it."""
+    d = dict(size=32, box=32, ft_size=64, interp=0, sampling=1.5, hi_res=1.5, lo_res=0.0,
+             ctf_action=1, flags=0, wiener=0.0, nhalves=1, batch=0, mem_cap=0)
+    d.update(kw)
+    o = OracleRecon(ReconParams(**d), 2, 64)
+    o.insert(synth.particles_list(ps.parts), ps.images.numpy())
+    return [o.finish(i, nmaps, want_fom=bool(d["flags"] & abi.BSX_FLAG_FOM)) for i in range(nmaps)]
+
+
 def test_breconstruct_cli_on_star_project(tmp_path, capsys):
-    """breconstruct over a STAR project and an MRC stack gives the map of the
-    direct API call on the same particles, written as MRC with the sampling
-    and origin of the reference's output."""
+    """breconstruct over a STAR project and an MRC stack, full map and half
+    maps (-fullmap -halfmaps: <name>, <name>_01, <name>_02) against the FP64
+    oracle on the same particles, written as MRC with the sampling and
+    origin of the reference's output; the FOM maps too (-fom)."""
     from bsoft_amd import breconstruct, formats
     star, ps = _star_project(tmp_path)
-    n = len(ps.parts)
     out = str(tmp_path / "map.mrc")
-    assert breconstruct.main(["-reconstruction", out, "-CTF", "flip", "-halfmaps", "-verbose",
-                              star]) == 0
+    fom = str(tmp_path / "fom.mrc")
+    assert breconstruct.main(["-reconstruction", out, "-CTF", "flip", "-fullmap", "-halfmaps",
+                              "-fom", fom, "-verbose", star]) == 0
     printed = capsys.readouterr().out
     assert "SNR(0.5):" in printed and "FSC(0.3):" in printed and "Radius\ts(1/A)" in printed
-    m = formats.read_mrc(out)
-    assert m.data.shape == (1, 32, 32, 32) and m.sampling == pytest.approx((1.5,) * 3)
-    assert m.origin == pytest.approx((16, 16, 16))
-    rec = Reconstruction(32, 32, 64, 1.5, ctf_action=1, nhalves=2)
-    parts = ps.parts.copy()
-    parts["half"] = np.arange(n) % 2
-    rec.insert(ps.images.cuda().contiguous(), parts)
-    ref = rec.finish(0, 3).map.cpu().numpy()
-    assert np.abs(m.data[0] - ref).max() <= 1e-5 * np.abs(ref).max()
-    h1 = formats.read_mrc(str(tmp_path / "map_1.mrc")).data[0]
-    assert np.abs(h1 - rec.finish(1, 3).map.cpu().numpy()).max() <= 1e-5 * np.abs(ref).max()
+    ref = _oracle_maps(ps, 3, flags=abi.BSX_FLAG_FOM)
+    for name, (m, f, _, st) in zip(("map.mrc", "map_01.mrc", "map_02.mrc"), ref):
+        g = formats.read_mrc(str(tmp_path / name))
+        assert g.data.shape == (1, 32, 32, 32) and g.sampling == pytest.approx((1.5,) * 3)
+        assert g.origin == pytest.approx((16, 16, 16))
+        err = np.abs(g.data[0] - m).max() / np.abs(m).max()
+        say(f"  {name}: max rel err {err:.2e} vs the oracle")
+        assert err <= 1e-4
+    gf = formats.read_mrc(str(tmp_path / "fom_01.mrc")).data[0].astype(np.float64)
+    of = ref[1][1]
+    sel = np.abs(of) > 1e-3
+    assert np.median(np.abs(gf[sel] - of[sel]) / np.abs(of[sel])) < 1e-3
+    # -halfmaps alone: the two half maps only
+    out2 = str(tmp_path / "h.mrc")
+    assert breconstruct.main(["-reconstruction", out2, "-CTF", "1", "-halfmaps", star]) == 0
+    assert not (tmp_path / "h.mrc").exists()
+    ref2 = _oracle_maps(ps, 2)
+    for name, (m, _, _, _) in zip(("h_01.mrc", "h_02.mrc"), ref2):
+        g = formats.read_mrc(str(tmp_path / name)).data[0]
+        assert np.abs(g - m).max() <= 1e-4 * np.abs(m).max()
+
+
+@pytest.mark.parametrize("ctf,action,wiener", [("baseflip", 7, 0.2), ("correct", 3, 0.2),
+                                               ("apply", 2, 0.0), ("apply -wiener", 2, 0.3)])
+def test_breconstruct_ctf_actions(tmp_path, ctf, action, wiener):
+    """-CTF spellings of include/options.h:418-442: baseflip and correct are
+    Wiener filters with the default factor 0.2 (breconstruct.cpp:187),
+    apply applies the CTF, and apply with -wiener is a Wiener filter."""
+    from bsoft_amd import breconstruct, formats
+    star, ps = _star_project(tmp_path)
+    out = str(tmp_path / "c.mrc")
+    args = ["-reconstruction", out, "-CTF", ctf.split()[0], star]
+    if "-wiener" in ctf:
+        args[-1:-1] = ["-wiener", str(wiener)]
+    assert breconstruct.main(args) == 0
+    (m, _, _, _), = _oracle_maps(ps, 1, ctf_action=action, wiener=wiener)
+    g = formats.read_mrc(out).data[0]
+    assert np.abs(g - m).max() <= 1e-4 * np.abs(m).max()
+
+
+def test_breconstruct_trilinear_rescale(tmp_path):
+    """-interpolate trilinear -rescale avg,std: particles normalised before
+    insertion, the map rescaled to (avg, std) and then given the sinc^2
+    correction of trilinear insertion (Bimage::filter_sinc), as
+    breconstruct.cpp:336 and :359 do, against the oracle's chain."""
+    from bsoft_amd import breconstruct, formats
+    star, ps = _star_project(tmp_path)
+    out = str(tmp_path / "t.mrc")
+    assert breconstruct.main(["-reconstruction", out, "-CTF", "flip", "-interpolate", "trilinear",
+                              "-rescale", "0.5,2", star]) == 0
+    (m, _, _, _), = _oracle_maps(ps, 1, interp=2, flags=abi.BSX_FLAG_RESCALE)
+    o = Oracle(64)
+    m = o.filter_sinc(o.rescale_to_avg_std(m, 0.5, 2.0), (16, 16, 16))
+    g = formats.read_mrc(out).data[0].astype(np.float64)
+    err = np.abs(g - m).max() / np.abs(m).max()
+    say(f"  trilinear + rescale + sinc: max rel err {err:.2e}")
+    assert err <= 1e-4
 
 
 @pytest.mark.parametrize("mode", [1, 2])

@@ -93,6 +93,27 @@ def test_gpu_fsc_dpr_matches_oracle():
 
 
 @pytest.mark.gpu
+def test_gpu_fsc_dpr_non_contiguous_inputs():
+    """Two strided views (transposed maps): each is copied to its own
+    contiguous buffer for the call, so the curves are those of the two
+    transposed maps, not of one map with itself."""
+    import torch
+    from bsoft_amd.bresolve import fsc_dpr
+    n = 24
+    a = _map(n, 7).astype(np.float32)
+    b = (a + 0.8 * _map(n, 8)).astype(np.float32)
+    ta = torch.as_tensor(a).cuda().permute(2, 1, 0)
+    tb = torch.as_tensor(b).cuda().permute(2, 1, 0)
+    assert not ta.is_contiguous() and not tb.is_contiguous()
+    s, fsc, dpr = fsc_dpr(ta, tb, 1.5)
+    rs, rf, rd = resolution_ref.fsc_dpr(np.ascontiguousarray(a.transpose(2, 1, 0)),
+                                        np.ascontiguousarray(b.transpose(2, 1, 0)), 1.5)
+    np.testing.assert_allclose(fsc, rf, atol=2e-5)
+    np.testing.assert_allclose(dpr, rd, atol=2e-3)
+    assert fsc[1:].min() < 0.99
+
+
+@pytest.mark.gpu
 def test_bresolve_cli(tmp_path, capsys):
     from bsoft_amd import bresolve, formats
     n = 32
