@@ -87,8 +87,6 @@ def main() -> int:
     ap.add_argument("--recon-only", action="store_true")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed-size run: boxes 128/192/256 in a 50/30/20 ratio")
-    ap.add_argument("--exhaustive-sample", type=int, default=1024,
-                    help="particles in the timed FP32 exhaustive-search sample")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (reference kernel for PMC traffic scaling)")
     args = ap.parse_args()
@@ -101,7 +99,7 @@ def main() -> int:
     import torch
     from bsoft_amd import abi, search, synth
     from bsoft_amd.proposal import OrientationClasses, ProposalNet, learned_search, training_set
-    from bsoft_amd.recon import Reconstruction, map_transform, part_ft_size
+    from bsoft_amd.recon import Reconstruction, check_resolution, fsc, map_transform, part_ft_size
 
     if args.mixed:
         return run_mixed(args)
@@ -149,7 +147,8 @@ def main() -> int:
     T_rec = (time.perf_counter() - t0) / args.steps
     log(f"[bench] reconstruction at known views: {T_rec * 1e3:.1f} ms/step; kernels " +
         ", ".join(f"{k} {v:.1f}" for k, v in kt_sum.items()))
-    recon_roof = insertion_roofline(kt_sum, launches, P * args.steps, box, ft, N)
+    n_s = band_samples(ft, pix, check_resolution(rec.params.hi_res, N, pix))
+    recon_roof = insertion_roofline(kt_sum, launches, P * args.steps, box, ft, N, n_s)
 
     if args.recon_only:
         result.update({"value": round(P / T_rec, 1), "ms_per_step": round(1e3 * T_rec, 3),
@@ -193,7 +192,9 @@ def main() -> int:
     # their reconstruction handle replays the insertion from a hipGraph and
     # uses the autotuned batch size
     recp = Reconstruction(N, box, ft, pix, ctf_action=abi.BSX_CTF_FLIP, flags=abi.BSX_FLAG_GRAPH)
-    tb = recp.autotune(ps.images[:8192].contiguous(), ps.parts[:8192])
+    t0 = time.perf_counter()
+    recp.autotune(ps.images[:8192].contiguous(), ps.parts[:8192])
+    setup["autotune_s"] = round(time.perf_counter() - t0, 2)
     setup["autotuned_batch"], setup["autotuned_piece"] = recp.tuning()
 
     def pipeline(kind, timer):
@@ -223,6 +224,10 @@ def main() -> int:
         timer("finish")
         return found, stats, fin
 
+    # the map from the true orientations (accumulators of the last known-view
+    # step, still held by `rec`): what each pipeline's map is compared with
+    true_ft = rec.finish(want_map=False, want_ft=True).ft
+    hi = check_resolution(rec.params.hi_res, N, pix)
     out = {}
     for kind in ("plain", "learned"):
         for _ in range(args.warmup):
@@ -238,10 +243,15 @@ def main() -> int:
         torch.cuda.synchronize()
         T = (time.perf_counter() - t0) / args.steps
         err = search.rotation_angle_deg(found, truth)
+        # finish again (outside the timed steps) for the weighed transform of
+        # this pipeline's map; the accumulators are those of its last step
+        f_true = fsc(recp.finish(want_map=False, want_ft=True).ft, true_ft, pix, hi)
         out[kind] = {"T": T, "found": found, "stages": {k: round(v / args.steps, 2) for k, v in stages.items()},
-                     "median_err_vs_truth_deg": round(float(np.median(err)), 3), "stats": stats}
+                     "median_err_vs_truth_deg": round(float(np.median(err)), 3), "stats": stats,
+                     "fsc_vs_true_views": fsc_summary(f_true, N, pix)}
         log(f"[bench] {kind}: {T * 1e3:.1f} ms/step ({P / T:.0f} particles/s); median error vs "
-            f"truth {np.median(err):.2f} deg; stages ms {out[kind]['stages']}")
+            f"truth {np.median(err):.2f} deg; stages ms {out[kind]['stages']}; FSC vs the "
+            f"true-view map {out[kind]['fsc_vs_true_views']}")
     d = search.rotation_angle_deg(out["learned"]["found"], out["plain"]["found"])
     T_plain, T_learn = out["plain"]["T"], out["learned"]["T"]
     st = out["learned"]["stats"]
@@ -258,53 +268,68 @@ def main() -> int:
                    "particles": P, "box": box, "map": N, "ft": ft, "views": len(lib_mats)},
         "plain_gpu": {"value": round(P / T_plain, 1), "ms_per_step": round(1e3 * T_plain, 3),
                       "stages_ms": out["plain"]["stages"],
-                      "median_err_vs_truth_deg": out["plain"]["median_err_vs_truth_deg"]},
+                      "median_err_vs_truth_deg": out["plain"]["median_err_vs_truth_deg"],
+                      "fsc_vs_true_views": out["plain"]["fsc_vs_true_views"]},
         "learned": {"stages_ms": out["learned"]["stages"],
                     "median_err_vs_truth_deg": out["learned"]["median_err_vs_truth_deg"],
                     "median_err_vs_exhaustive_deg": round(float(np.median(d)), 3),
+                    "fsc_vs_true_views": out["learned"]["fsc_vs_true_views"],
                     "fallback_particles": st.fallback, "candidate_slots": st.candidates},
+        # the target of >= 20x is not met: insertion and refinement are shared
+        # by both pipelines and bound the ratio
         "speedup_learned_vs_plain": {"search_plus_recon": round(T_plain / T_learn, 3),
-                                     "search_only": round(search_ms["plain"] / search_ms["learned"], 3)},
+                                     "search_only": round(search_ms["plain"] / search_ms["learned"], 3),
+                                     "target": 20.0},
         "recon_known_views": {"value": round(P / T_rec, 1), "ms_per_step": round(1e3 * T_rec, 3)},
         "setup_s": setup,
+        "amortised": amortised(setup, P, T_plain, T_learn),
         "graph_replays": recp.graph_launches(),
     })
     result["roofline"] = pipeline_roofline(out, s, P, args, recon_roof)
-    result["fp32_exhaustive_baseline"], result["speedup_learned_vs_fp32_exhaustive"] = \
-        fp32_exhaustive_baseline(s, ps, P / T_rec, result["value"], args)
     finish_result(args, result, ps, rec, ft, dev)
     return 0
 
 
-def fp32_exhaustive_baseline(s, ps, recon_rate, learned_rate, args):
-    """The plain GPU exhaustive search of the reference's method: the FP32
-    FRC (img_recip_space_fom, k_rescore) of each particle against every
-    library view, timed on a sample of particles that fills the GPU, then
-    combined with the reconstruction time per particle."""
-    import torch
-    n = min(args.exhaustive_sample, len(ps.parts))
-    s.stage(ps.images[:n].contiguous(), ps.parts[:n])
-    nv = s.nviews
-    blk = 4096                                             # candidates per rescore call
-    views = torch.arange(nv, dtype=torch.int32, device=s.device)
-    s.rescore(views[:256].expand(n, 256).contiguous())     # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for j in range(0, nv, blk):
-        s.rescore(views[j:j + blk].expand(n, min(blk, nv - j)).contiguous())
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
-    search_rate = n / t
-    combined = 1.0 / (1.0 / search_rate + 1.0 / recon_rate)
-    log(f"[bench] FP32 exhaustive FRC search: {n} particles x {nv} views in {t:.2f} s "
-        f"({search_rate:.0f} particles/s; with reconstruction {combined:.0f}/s)")
-    return ({"particles": n, "views": nv, "search_s": round(t, 3),
-             "search_particles_per_s": round(search_rate, 1),
-             "search_plus_recon_particles_per_s": round(combined, 1),
-             "method": "k_rescore FP32 FRC (reference img_recip_space_fom formula) of every particle "
-                       "against all library views; reconstruction at the measured known-view rate"},
-            round(learned_rate / combined, 1))
 
+def fsc_summary(f: np.ndarray, N: int, pix: float) -> dict:
+    """Shell FSC (Bimage::fsc, shell r at N pix |s|) summarised: the minimum
+    up to half Nyquist, the value in the Nyquist shell, and the first shells
+    where it falls below 0.5 and 0.143 (as resolutions in A)."""
+    f = np.asarray(f)
+    nyq = N // 2
+    def cross(t):
+        below = np.nonzero(f[1:] < t)[0]
+        return None if len(below) == 0 else round(N * pix / (below[0] + 1), 3)
+    return {"min_to_half_nyquist": round(float(f[1:nyq // 2 + 1].min()), 5),
+            "at_nyquist": round(float(f[min(nyq, len(f) - 1)]), 5),
+            "res_fsc_0.5_A": cross(0.5), "res_fsc_0.143_A": cross(0.143), "shells": len(f)}
+
+
+def amortised(setup: dict, P: int, T_plain: float, T_learn: float) -> dict:
+    """The learned pipeline's one-time set-up per map (library, training set,
+    network training, autotuning) spread over one or more passes."""
+    learn_setup = sum(setup.get(k, 0.0) for k in ("library_s", "training_set_s", "training_s",
+                                                  "autotune_s"))
+    plain_setup = sum(setup.get(k, 0.0) for k in ("library_s", "autotune_s"))
+    gain = T_plain - T_learn
+    return {"learned_setup_s": round(learn_setup, 2), "plain_setup_s": round(plain_setup, 2),
+            "learned_one_pass_particles_per_s": round(P / (T_learn + learn_setup), 1),
+            "plain_one_pass_particles_per_s": round(P / (T_plain + plain_setup), 1),
+            "passes_to_break_even": (round((learn_setup - plain_setup) / gain, 1)
+                                     if gain > 0 else None)}
+
+
+def band_samples(ft: int, pix: float, hi_res: float, lo_res: float = 0.0) -> int:
+    """Samples of one ft x ft particle transform inside the resolution band
+    (fspace_pack_2D, Bimage_reconstruct.cpp:129-155: lo^2 <= s^2 <= 1/hi^2,
+    s = (h, k) / (ft pix))."""
+    h = np.arange(ft)
+    h = np.where(h <= (ft - 1) // 2, h, h - ft).astype(np.float64)
+    s2 = (h[:, None] ** 2 + h[None, :] ** 2) / (ft * pix) ** 2
+    keep = s2 <= 1.0 / hi_res ** 2
+    if lo_res > 0:
+        keep &= s2 >= 1.0 / lo_res ** 2
+    return int(keep.sum())
 
 def run_mixed(args) -> int:
     """Boxes {128, 192, 256} in a 50/30/20 ratio (default 60k particles) of one
@@ -405,36 +430,47 @@ def run_mixed(args) -> int:
     return 0
 
 
-def insertion_roofline(kt_sum, launches, nparticles, box, ft, N):
-    """Roofline of the dominant insertion kernel: fill reads the half-spectrum
-    columns ((ft/2+1)*box*8 B per particle) and writes a 16-B bin entry per
-    merged sample run; tile-accumulate reads the entries and read-modify-writes
-    the touched 48-KB tiles."""
+def insertion_roofline(kt_sum, launches, nparticles, box, ft, N, n_s):
+    """Roofline of the dominant insertion kernel, the fill pass (`k_cols`, the
+    column transform, per-sample geometry and CTF, and the emission of every
+    sample into the bins). `achieved` counts the algorithmic bytes of
+    nearest-neighbour insertion, 32 B per in-band sample (the 8-B sample plus
+    a read and a write of the 12-B voxel record re, im, w), for the particles
+    one launch processes, over the fill's average launch time. The bytes this
+    implementation moves (half-spectrum columns in, 16-B bin entries out) are
+    reported beside it, as is the whole insertion stage (all its kernels)
+    against the algorithmic bytes of staging, 2D FFT and insertion."""
     entries = kt_sum.get("entries", 0.0)
     nbatch = max(1, launches)
+    ppl = nparticles / nbatch                       # particles per launch
+    fill_ms = max(kt_sum.get("fill", 0.0), 1e-9)
+    k3 = 32 * n_s                                   # algorithmic bytes per particle
+    achieved = k3 * nparticles / (fill_ms * 1e-3) / 1e9
     col_bytes = (ft // 2 + 1) * box * 8
-    kernel_bytes = {
-        "fill": col_bytes * nparticles + 16 * entries,
-        "tile": 16 * entries + 2 * 48 * 1024 * (N // 16) ** 3 * nbatch,
-        "direct": 32 * ft * ft * nparticles,
-    }
-    cand = {k: kt_sum.get(k, 0.0) for k in kernel_bytes}
-    dom = max(cand, key=cand.get)
-    dom_ms = max(cand[dom], 1e-9)
-    achieved = kernel_bytes[dom] / (dom_ms * 1e-3) / 1e9
-    ins_ms = sum(kt_sum.get(k, 0.0) for k in ("scan", "fill", "tile", "direct"))
-    # the fill of nearest-neighbour insertion runs the staged specialisation
-    kname = {"fill": f"k_cols<{ft}, false, 2, true>", "tile": "k_tile_accum",
-             "direct": f"k_cols<{ft}, false, 0>"}[dom]
+    impl = col_bytes * nparticles + 16 * entries
+    stage_keys = ("background", "rows", "fill", "scan", "tile", "direct", "fallback")
+    ins_ms = max(sum(kt_sum.get(k, 0.0) for k in stage_keys), 1e-9)
+    stage_bytes = (4 * box * box + 16 * ft * ft + k3) * nparticles
+    kname = f"k_cols<{ft}, false, 2, true>"
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(kname, nparticles / nbatch) if ft == 512 else None,
-            "kernel": kname,
-            "stage": dom, "avg_launch_ms": round(dom_ms / nbatch, 4), "launches": nbatch,
-            "algorithmic_bytes_per_launch": round(kernel_bytes[dom] / nbatch),
-            "bin_entries_per_particle": round(entries / max(1, nparticles), 1),
-            "kernel_ms": {k: round(v, 2) for k, v in kt_sum.items() if k != "entries"},
-            "insertion_ms": round(ins_ms, 2)}
+            "traffic": measured_traffic(kname, ppl) if ft == 512 else None,
+            "kernel": kname, "avg_launch_ms": round(fill_ms / nbatch, 4), "launches": nbatch,
+            "particles_per_launch": round(ppl, 1), "samples_per_particle": n_s,
+            "algorithmic_bytes_per_launch": round(k3 * ppl),
+            "algorithmic_basis": "32 B per in-band sample (8-B sample + 12-B voxel read + 12-B "
+                                 "voxel write), nearest-neighbour insertion",
+            "implementation_bytes": {"per_launch": round(impl / nbatch),
+                                     "GBs": round(impl / (fill_ms * 1e-3) / 1e9, 1),
+                                     "basis": "half-spectrum columns read + 16-B bin entries "
+                                              "written",
+                                     "bin_entries_per_particle": round(entries / max(1, nparticles), 1)},
+            "insertion_stage": {"ms": round(ins_ms, 2),
+                                "algorithmic_GBs": round(stage_bytes / (ins_ms * 1e-3) / 1e9, 1),
+                                "frac": round(stage_bytes / (ins_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                "basis": "4 box^2 staging + 16 ft^2 FFT + 32 n_s insertion bytes "
+                                         "per particle over all insertion kernels"},
+            "kernel_ms": {k: round(v, 2) for k, v in kt_sum.items() if k != "entries"}}
 
 
 def pipeline_roofline(out, s, P, args, recon_roof):
@@ -490,6 +526,20 @@ def finish_result(args, result, ps, rec, ft, dev):
     print(json.dumps(result), flush=True)
 
 
+def host_cpu() -> dict:
+    """nproc, the affinity mask's size and the CPU model of the host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
 def cpu_baseline(args, ps, params, ft):
     """The FP32 CPU restatement of the reconstruction (OpenMP over partial
     maps, like breconstruct -threads T) timed on a bounded sample of the same
@@ -500,7 +550,10 @@ def cpu_baseline(args, ps, params, ft):
     from bsoft_amd.recon import Reconstruction
     from oracle.oracle import Oracle, OracleRecon
 
+    # 16 threads: the host-core share of one GPU on the measurement box
+    # (nproc and the affinity mask there show more, shared by other GPUs)
     threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+    host = host_cpu()
     rp = params
     rp_cpu = type(rp)()
     for f, _ in rp._fields_:
@@ -541,6 +594,7 @@ def cpu_baseline(args, ps, params, ft):
     fsc = fsc[sup]
     err = float(np.abs(gr.map.cpu().numpy().astype(np.float64) - m).max() / np.abs(m).max())
     cpu = {"value": round(P / full, 3), "unit": "particles/s", "cores": threads, "kind": "port",
+           "host": host,
            "sample": f"reconstruction only (the CPU path has no search): FP32 CPU restatement, "
                      f"OpenMP over {threads} partial maps, {nsamp} of the {P} particles at their "
                      f"known views inserted in {tins:.1f} s (after a {ncal}-particle warm-up), "

@@ -20,85 +20,99 @@ from bsoft_amd.synth import PARTICLE_DTYPE
 _H, _E, _M, _C = 6.626070e-34, 1.602177e-19, 9.109384e-31, 299792458.0
 LAMBDA_1KV = _H * 1e10 / math.sqrt(2 * _M * _E * 1e3 * (1 + _E * 1e3 / (2 * _M * _C * _C)))
 
-# Geometry shared by the insertion cases: map N = 16 at 1 A/voxel, particles
-# of box 8 padded to ft 16 at 1 A/pixel, identity view. A sample at Fourier
-# index (h, k) has s = (h, k) / (ft mag) and lands at m = s N = (h, k) / mag.
-N, BOX, FT = 16, 8, 16
+# Geometry shared by the insertion cases: identity view, ft = N = 2 box,
+# ft pix = N pix_map = 16 A. A sample at Fourier index (h, k) has
+# s = (h, k) / (16 mag) and lands at m = s N pix_map = (h, k) / mag, so the
+# hand counts below hold for both geometries: the oracle's (N = 16 at 1 A)
+# and the library's (N = 32 at 0.5 A: its insertion takes ft >= 32); only
+# the periodic wrap of negative voxel indices (by N) differs, and the extra
+# samples of the larger transform (|h| or |k| > 8) land away from every
+# counted voxel.
+class Geometry:
+    def __init__(self, n: int, pix: float):
+        self.N, self.BOX, self.FT, self.PIX = n, n // 2, n, pix
+
+
+CPU = Geometry(16, 1.0)
+GPU = Geometry(32, 0.5)
+N, BOX, FT = CPU.N, CPU.BOX, CPU.FT
 PAD_WEIGHT = 1.0 / (0.6 * (FT / BOX) ** 2)     # 1 / (0.6 (ft/box)^2), mg_reconstruct.cpp:192-199
 
 
-def particle(**kw) -> np.ndarray:
+def particle(g: Geometry = CPU, **kw) -> np.ndarray:
     p = np.zeros(1, dtype=PARTICLE_DTYPE)
     p["view"] = kw.get("view", (0.0, 0.0, 1.0, 0.0))
-    p["pixel_size"] = 1.0
+    p["pixel_size"] = g.PIX
     p["mag"] = kw.get("mag", 1.0)
     p["volt"] = kw.get("volt", 300e3)
     p["sel"] = kw.get("sel", 1.0)
-    p["ori"] = (BOX // 2, BOX // 2)
-    p["hdr_ori"] = (BOX // 2, BOX // 2)
+    p["ori"] = (g.BOX // 2, g.BOX // 2)
+    p["hdr_ori"] = (g.BOX // 2, g.BOX // 2)
     return p
 
 
-def image(seed: int = 0) -> np.ndarray:
-    return np.random.default_rng(seed).normal(size=(1, BOX, BOX)).astype(np.float32)
+def image(seed: int = 0, g: Geometry = CPU) -> np.ndarray:
+    return np.random.default_rng(seed).normal(size=(1, g.BOX, g.BOX)).astype(np.float32)
 
 
 # Each case: reconstruction parameters, the particle, and {(x, y, z): weight
-# sum at that voxel} for voxels whose contributions were counted by hand.
-def insertion_cases():
+# sum at that voxel} for voxels whose contributions were counted by hand
+# (signed voxel coordinates are wrapped by N at the end).
+def insertion_cases(g: Geometry = CPU):
     pw = PAD_WEIGHT
     cases = {}
     # Weighted nearest neighbour (interp 1): mag 1/0.7 puts sample (3, 0) at
     # m = (2.1, 0, 0). Voxel (2, 0, 0) receives only it: round(0.7 h) = 2
     # needs h = 3, round(0.7 k) = 0 needs k = 0. Weight pw (1 - |m - v|) =
-    # 0.9 pw. Its mate (-3, 0) lands at (-2.1) -> voxel -2 = 14, also 0.9 pw.
+    # 0.9 pw. Its mate (-3, 0) lands at (-2.1) -> voxel -2, also 0.9 pw.
     # Voxel (2, 1, 0) receives (3, 1) at (2.1, 0.7) and (3, 2) at (2.1, 1.4):
     # pw (2 - sqrt(0.01 + 0.09) - sqrt(0.01 + 0.16)).
-    cases["weighted_nn"] = (dict(interp=1), particle(mag=1 / 0.7), {
-        (2, 0, 0): 0.9 * pw, (14, 0, 0): 0.9 * pw,
+    cases["weighted_nn"] = (dict(interp=1), particle(g, mag=1 / 0.7), {
+        (2, 0, 0): 0.9 * pw, (-2, 0, 0): 0.9 * pw,
         (2, 1, 0): pw * (2 - math.sqrt(0.1) - math.sqrt(0.17)),
     })
     # Plain nearest neighbour at the same coordinates: each sample adds pw.
     # Voxel (2, 1, 0) gets 2 pw; voxel (1, 0, 0) gets h in {1, 2} (0.7, 1.4)
     # with k = 0: 2 pw.
-    cases["nearest_mag"] = (dict(interp=0), particle(mag=1 / 0.7), {
+    cases["nearest_mag"] = (dict(interp=0), particle(g, mag=1 / 0.7), {
         (2, 0, 0): pw, (2, 1, 0): 2 * pw, (1, 0, 0): 2 * pw, (0, 0, 0): pw,
     })
     # Magnification 1: sample (h, k) lands on voxel (h, k, 0) exactly.
-    cases["nearest_mag1"] = (dict(interp=0), particle(), {
-        (3, 1, 0): pw, (7, 7, 0): pw, (9, 0, 0): pw,          # h = -7 wraps to 9
+    cases["nearest_mag1"] = (dict(interp=0), particle(g), {
+        (3, 1, 0): pw, (7, 7, 0): pw, (-7, 0, 0): pw,
     })
     # Bootstrap (flags 4): the weight is the selection number times pw.
-    cases["bootstrap"] = (dict(interp=0, flags=4), particle(sel=3.0), {
+    cases["bootstrap"] = (dict(interp=0, flags=4), particle(g, sel=3.0), {
         (3, 1, 0): 3 * pw, (1, 0, 0): 3 * pw,
     })
     # Low-resolution limit: lo_res = 16 / sqrt(10.5) keeps s^2 >= 10.5 / 256,
     # i.e. h^2 + k^2 >= 10.5: (3, 1) (10) is cut, (3, 2) (13) kept, DC cut.
-    cases["lo_res"] = (dict(interp=0, lo_res=16 / math.sqrt(10.5)), particle(), {
+    cases["lo_res"] = (dict(interp=0, lo_res=16 / math.sqrt(10.5)), particle(g), {
         (3, 1, 0): 0.0, (3, 2, 0): pw, (0, 0, 0): 0.0, (1, 0, 0): 0.0,
     })
     # Resolution taper: hi_res = 16 / sqrt(50.5) (maxs2 = 50.5 / 256). Sample
     # (7, 1) has s^2 = 50 / 256: d = (maxs2 - s^2) N = 0.5 / 256 * 16 =
     # 1/32 < 1, weight pw sqrt(1/32). (5, 5) (50 as well): the same. (6, 4)
     # (52) is beyond the limit: 0. (1, 1): d = 48.5 / 16 > 1, weight pw.
-    cases["taper"] = (dict(interp=0, hi_res=16 / math.sqrt(50.5)), particle(), {
+    cases["taper"] = (dict(interp=0, hi_res=16 / math.sqrt(50.5)), particle(g), {
         (7, 1, 0): pw * math.sqrt(1 / 32), (5, 5, 0): pw * math.sqrt(1 / 32),
         (6, 4, 0): 0.0, (1, 1, 0): pw,
     })
     # Ewald sphere (flags 8) at 1 kV: each sample is inserted twice, at
     # s_z = +-(lambda/2) s^2, i.e. m_z = +-(lambda/2) (h^2 + k^2) / 16.
-    # (7, 7): m_z = +-1.187 -> voxels z = 1 and z = -1 = 15, none at z = 0.
+    # (7, 7): m_z = +-1.187 -> voxels z = 1 and z = -1, none at z = 0.
     # (2, 0): m_z = +-0.048 -> both at z = 0: 2 pw.
     lam2 = LAMBDA_1KV / 2
     assert round(lam2 * 98 / 16) == 1 and round(lam2 * 4 / 16) == 0
-    cases["ewald"] = (dict(interp=0, flags=8), particle(volt=1e3), {
-        (7, 7, 1): pw, (7, 7, 15): pw, (7, 7, 0): 0.0, (2, 0, 0): 2 * pw,
+    cases["ewald"] = (dict(interp=0, flags=8), particle(g, volt=1e3), {
+        (7, 7, 1): pw, (7, 7, -1): pw, (7, 7, 0): 0.0, (2, 0, 0): 2 * pw,
     })
-    return cases
+    return {name: (params, part, {(x % g.N, y % g.N, z % g.N): w for (x, y, z), w in exp.items()})
+            for name, (params, part, exp) in cases.items()}
 
 
-def recon_params(**kw) -> dict:
-    d = dict(size=N, box=BOX, ft_size=FT, interp=0, sampling=1.0, hi_res=1.0, lo_res=0.0,
+def recon_params(g: Geometry = CPU, **kw) -> dict:
+    d = dict(size=g.N, box=g.BOX, ft_size=g.FT, interp=0, sampling=g.PIX, hi_res=1.0, lo_res=0.0,
              ctf_action=0, flags=0, wiener=0.0, nhalves=1, batch=0, mem_cap=0)
     d.update(kw)
     return d

@@ -1,0 +1,120 @@
+"""The benchmark configurations as GPU tests:
+
+* the learned pipeline's low-confidence fallback (particles whose top class
+  probability is below the threshold also get the exhaustive candidates,
+  bsx_search_exhaustive_rows) against the exhaustive search;
+* batched projection matching at 128^3: 20,000 particles of 128^2 (SNR 0.1)
+  against the 4-degree grid with 90 in-plane angles (230,940 views), BF16
+  GEMM top-8 then FP32 FRC rescoring, checked against the FP64 scoring on a
+  subset, against ground truth, and by properties over every particle.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from bsoft_amd import search, synth
+from bsoft_amd.proposal import OrientationClasses, ProposalNet, learned_search
+from bsoft_amd.recon import map_transform
+from oracle import search_ref as sr
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_fallback_rows_and_learned_fallback(gpu):
+    """bsx_search_exhaustive_rows equals the exhaustive GEMM's rows; with an
+    untrained network and a confidence threshold that every particle misses,
+    the learned pipeline sends every particle through the fallback, and with
+    a threshold at the median probability about half of them; each particle
+    that fell back ends at least as high in FP32 FRC as the exhaustive
+    search's best grid view (its candidates include the exhaustive top-8)."""
+    n, R = 64, 14
+    rng = np.random.default_rng(41)
+    bm = synth.make_blob_map(n, 30, rng)
+    ft = map_transform(bm.volume().cuda())
+    lm = search.view_matrices(search.views_c1(8.0, angle_step_deg=8.0))
+    classes = OrientationClasses(20.0)
+    s = search.Search(n, R, 1.5, topk=8)
+    s.set_reference(ft, lm)
+    s.set_classes(classes.members(lm))
+    ps = synth.make_particles(bm, 96, n, rng, snr=0.5, seed=41)
+    s.stage(ps.images.cuda().contiguous(), ps.parts)
+    say(f"fallback: {len(lm)} views, {classes.n} classes, {len(ps.parts)} particles")
+    cand, score = s.exhaustive()
+    rows = torch.as_tensor(np.array([5, 0, 77, 31, 31, 95], dtype=np.int32))
+    rc, rsc = s.exhaustive_rows(rows)
+    assert torch.equal(rc, cand[rows.long().cuda()])
+    assert torch.equal(rsc, score[rows.long().cuda()])
+    _, ebest, ebf = s.rescore(cand)
+    net = ProposalNet(32, classes.n, (16, 32, 64, 64), max_batch=128, seed=3)
+    _, prob = net.topk(s.cnn_input(32), 16)
+    p0 = prob[:, 0].cpu().numpy()
+    for thr, expect in ((1.01, len(p0)), (float(np.median(p0)), int(np.sum(p0 < np.median(p0))))):
+        res, st, _, _ = learned_search(s, net, k=16, min_prob=thr, steps_deg=())
+        say(f"  threshold {thr:.5f}: {st.fallback} particles fell back, "
+            f"{st.candidates} candidates per particle")
+        assert st.fallback == expect and st.candidates == 16   # proposed 8 + fallback 8
+        low = p0 < thr
+        gf = res.grid_frc.cpu().numpy()
+        assert np.all(gf[low] >= ebf.cpu().numpy()[low] - 1e-6)
+        # where the exhaustive best wins, it is the view the pipeline returns
+        same = low & (gf <= ebf.cpu().numpy() + 1e-7)
+        assert np.array_equal(res.grid_view.cpu().numpy()[same], ebest.cpu().numpy()[same])
+
+
+def test_projection_matching_128_full_grid(gpu):
+    """128^3 map, 20,000 particles 128^2 at SNR 0.1, 230,940 library views."""
+    n, R, P = 128, 16, 20000
+    rng = np.random.default_rng(51)
+    bm = synth.make_blob_map(n, 60, rng)
+    ft = map_transform(bm.volume().cuda())
+    lm = search.view_matrices(search.views_c1(4.0, angle_step_deg=4.0))
+    assert len(lm) == 2566 * 90
+    s = search.Search(n, R, 1.5, kernel_width=3, topk=8)
+    s.set_reference(ft, lm)
+    say(f"C3: {len(lm)} views, {s.npoints} disk points, GEMM K {s.kdim}; generating {P} particles")
+    ps = synth.make_particles(bm, P, n, rng, snr=0.1, seed=51, device=torch.device("cuda"),
+                              chunk=2048)
+    s.stage(ps.images, ps.parts)
+    cand, score = s.exhaustive()
+    frc, best, bf = s.rescore(cand)
+    torch.cuda.synchronize()
+    c, sc, f = cand.cpu().numpy(), score.cpu().numpy(), frc.cpu().numpy()
+    # properties over every particle
+    assert np.all((c >= 0) & (c < len(lm)))
+    assert np.all(np.diff(sc, axis=1) <= 1e-6)                  # top-8 in descending order
+    assert np.all([len(set(r)) == 8 for r in c[::97]])           # distinct views
+    assert np.all(np.abs(f) <= 1 + 1e-5)
+    bi = np.argmax(f, axis=1)
+    assert np.array_equal(best.cpu().numpy(), c[np.arange(P), bi])
+    # against the FP64 scoring of the same candidates on a subset
+    o = Oracle(64)
+    hk, rs = s.points()
+    ftc = np.ascontiguousarray(ft.cpu().numpy(), dtype=np.complex128)   # converted once
+    parts = synth.particles_list(ps.parts[:16])
+    imgs = ps.images[:16].cpu().numpy()
+    for i in range(16):
+        pv, pc = sr.particle_disk(imgs[i], parts[i], hk, o)
+        ref = []
+        for v in c[i]:
+            osec = o.central_section(ftc, 1.5, lm[v], 1.5, 3, 2, 0)
+            ref.append(sr.frc_half(pv, pc, osec[hk[:, 1] % n, hk[:, 0] % n], rs))
+        np.testing.assert_allclose(f[i], ref, atol=1e-4)
+        # the FP32 choice is the FP64 optimum up to the FP32 scoring error
+        assert ref[int(bi[i])] >= max(ref) - 2e-4
+    # angular error against the truth, grid and refined
+    truth = synth.view_to_matrix(ps.parts["view"])
+    gerr = search.rotation_angle_deg(lm[best.cpu().numpy()], truth)
+    mats, _ = s.refine(s.view_mats(best), (2.0, 1.0))
+    err = search.rotation_angle_deg(mats.cpu().numpy(), truth)
+    say(f"  grid view median error {np.median(gerr):.2f} deg, refined {np.median(err):.2f} deg; "
+        f"within 5 deg: {np.mean(err < 5):.3f}")
+    assert np.median(gerr) <= 3.0
+    assert np.median(err) <= 1.5

@@ -26,7 +26,7 @@ def say(*a):
 
 
 def gpu_weights(params: dict, part: np.ndarray, img: np.ndarray, method: int = 0) -> np.ndarray:
-    d = pc.recon_params(**params)
+    d = pc.recon_params(pc.GPU, **params)
     rec = Reconstruction(d["size"], d["box"], d["ft_size"], d["sampling"], hi_res=d["hi_res"],
                          lo_res=d["lo_res"], interp=d["interp"], flags=d["flags"], method=method)
     rec.insert(torch.as_tensor(img).cuda().contiguous(), part)
@@ -39,13 +39,14 @@ def gpu_weights(params: dict, part: np.ndarray, img: np.ndarray, method: int = 0
 @pytest.mark.parametrize("name", sorted(pc.insertion_cases()))
 def test_insertion_weights_by_hand(gpu, name, method):
     say(f"case {name} method {method}")
-    params, part, expect = pc.insertion_cases()[name]
-    acc = gpu_weights(params, part, pc.image(), method)
+    params, part, expect = pc.insertion_cases(pc.GPU)[name]
+    img = pc.image(g=pc.GPU)
+    acc = gpu_weights(params, part, img, method)
     for (x, y, z), w in expect.items():
         assert acc[2, z, y, x] == pytest.approx(w, rel=2e-6, abs=1e-7), (name, (x, y, z))
     # the whole weight plane equals the FP64 oracle's (float sums)
-    o = OracleRecon(ReconParams(**pc.recon_params(**params)), 1, 64)
-    o.insert(synth.particles_list(part), pc.image())
+    o = OracleRecon(ReconParams(**pc.recon_params(pc.GPU, **params)), 1, 64)
+    o.insert(synth.particles_list(part), img)
     ref = o.partial_acc(0)
     np.testing.assert_allclose(acc[2], ref[2], rtol=2e-6, atol=1e-6)
     np.testing.assert_allclose(acc[:2], ref[:2], atol=2e-6 * np.abs(ref[:2]).max())
