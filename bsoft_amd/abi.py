@@ -17,6 +17,7 @@ BSX_ERR_DEVICE = -2
 BSX_ERR_NOMEM = -3
 BSX_ERR_STATE = -4
 BSX_ERR_NONFINITE = -5
+BSX_NOALLOC_MAX_VIEWS = 4096     # include/bsx.h
 
 BSX_CTF_NONE = 0
 BSX_CTF_FLIP = 1
@@ -140,6 +141,8 @@ SIGNATURES = {
     "bsx_map_transform": ([_vp, C.c_int, _dp, _vp, _vp], C.c_int),
     "bsx_extract": ([_vp, C.c_int, C.c_double, _dp, C.c_int64, C.c_double, C.c_int, C.c_int,
                      C.c_int, _vp, _vp], C.c_int),
+    "bsx_extract_noalloc": ([_vp, C.c_int, C.c_double, _dp, C.c_int64, C.c_double, C.c_int,
+                             C.c_int, C.c_int, _vp, _vp], C.c_int),
     "bsx_fsc": ([_vp, _vp, C.c_int, C.c_double, C.c_double, _dp, C.POINTER(C.c_int), _vp], C.c_int),
     "bsx_fsc_dpr": ([_vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double, _dp,
                      C.POINTER(C.c_int), _vp], C.c_int),

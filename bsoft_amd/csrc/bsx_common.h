@@ -130,15 +130,24 @@ inline double fsi_weight(double x, int wd, int power) {
     return std::sin(M_PI * x) / (wd * std::sin(a)) * std::pow(std::cos(a), power);
 }
 
-inline std::vector<float> fsi_table(int wd, int power) {
-    if (power < 0) return std::vector<float>(FSI_NDIV + 1, 1.0f);
+// Writes the table into `t` ((FSI_NDIV + 1) x wd floats, or FSI_NDIV + 1
+// when power < 0).
+inline void fsi_table_into(float* t, int wd, int power) {
+    if (power < 0) {
+        for (int row = 0; row <= FSI_NDIV; ++row) t[row] = 1.0f;
+        return;
+    }
     const int centre = (wd - 1) / 2;
-    std::vector<float> t((size_t)(FSI_NDIV + 1) * wd);
     for (int row = 0; row <= FSI_NDIV; ++row) {
         const double frac = (double)row / FSI_NDIV;
         for (int j = 0; j < wd; ++j)
             t[(size_t)row * wd + j] = (float)fsi_weight(frac + centre - j, wd, power);
     }
+}
+
+inline std::vector<float> fsi_table(int wd, int power) {
+    std::vector<float> t((size_t)(FSI_NDIV + 1) * (power < 0 ? 1 : wd));
+    fsi_table_into(t.data(), wd, power);
     return t;
 }
 

@@ -8,6 +8,7 @@
 //   k_fsc            : FP64 shell sums of Bimage::fsc (Bimage_resolution.cpp:330-406)
 //   k_frc_pairs      : img_recip_space_fom, fom_type 0 (src/mg/mg_refine.cpp:660-781)
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "bsx_common.h"
@@ -254,29 +255,30 @@ int bsx_map_transform(const float* map_dev, int n, const double origin[3], float
     return BSX_OK;
 }
 
-int bsx_extract(const float* ft_dev, int n, double sampling, const double* mats, int64_t nviews,
-                double resolution, int kernel_width, int kernel_power, int center, float* out_dev,
-                void* stream) {
-    BSX_REQUIRE(ft_dev && mats && out_dev, BSX_ERR_ARG, "bsx_extract: null argument");
+// Argument checks shared by bsx_extract and bsx_extract_noalloc; normalises
+// width, sampling and resolution as the reference's defaults do.
+static int extract_args(const float* ft_dev, int n, double& sampling, const double* mats,
+                        int64_t nviews, double& resolution, int& kernel_width, int kernel_power,
+                        const float* out_dev, const char* who) {
+    BSX_REQUIRE(ft_dev && mats && out_dev, BSX_ERR_ARG, "%s: null argument", who);
     BSX_REQUIRE(n >= 2 && nviews >= 0 && nviews < 65536, BSX_ERR_ARG,
-                "bsx_extract: n %d / nviews %lld out of range", n, (long long)nviews);
-    if (nviews == 0) return BSX_OK;
+                "%s: n %d / nviews %lld out of range", who, n, (long long)nviews);
     if (kernel_power < 0) kernel_width = 1;
     BSX_REQUIRE(kernel_width >= 1 && kernel_width <= 10, BSX_ERR_ARG,
-                "bsx_extract: kernel width %d out of range 1..10", kernel_width);
+                "%s: kernel width %d out of range 1..10", who, kernel_width);
     if (sampling < 1e-10) sampling = 1;
     if (resolution < 1e-10) resolution = sampling;
     for (int64_t i = 0; i < 9 * nviews; ++i)
         BSX_REQUIRE(std::isfinite(mats[i]) && fabs(mats[i]) <= 1.0 + 1e-9, BSX_ERR_ARG,
-                    "bsx_extract: matrix %lld is not a rotation", (long long)(i / 9));
-    hipStream_t s = (hipStream_t)stream;
-    std::vector<float> tab = fsi_table(kernel_width, kernel_power);
-    float* dtab = nullptr;
-    double* dmats = nullptr;
-    BSX_HIP(hipMalloc(&dtab, sizeof(float) * tab.size()));
-    BSX_HIP(hipMalloc(&dmats, sizeof(double) * 9 * nviews));
-    BSX_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
-    BSX_HIP(hipMemcpyAsync(dmats, mats, sizeof(double) * 9 * nviews, hipMemcpyHostToDevice, s));
+                    "%s: matrix %lld is not a rotation", who, (long long)(i / 9));
+    return BSX_OK;
+}
+
+// Launches k_extract<width> over nviews sections; dtab / dmats are device
+// copies of the kernel table and the view matrices.
+static int extract_launch(const float* ft_dev, int n, double sampling, const double* dmats,
+                          int64_t nviews, double resolution, int kernel_width, int center,
+                          const float* dtab, float* out_dev, hipStream_t s) {
     const double inv = 1.0 / (n * sampling), rs = n * sampling;
     const double maxs2 = 1.0 / (resolution * resolution);
     dim3 grid((unsigned)(((long long)n * n + 255) / 256), (unsigned)nviews);
@@ -288,11 +290,91 @@ int bsx_extract(const float* ft_dev, int n, double sampling, const double* mats,
         BSX_EXT(6) BSX_EXT(7) BSX_EXT(8) BSX_EXT(9) BSX_EXT(10)
 #undef BSX_EXT
     }
-    hipError_t le = hipGetLastError();
-    BSX_HIP(hipStreamSynchronize(s));
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_extract(const float* ft_dev, int n, double sampling, const double* mats, int64_t nviews,
+                double resolution, int kernel_width, int kernel_power, int center, float* out_dev,
+                void* stream) {
+    int r = extract_args(ft_dev, n, sampling, mats, nviews, resolution, kernel_width,
+                         kernel_power, out_dev, "bsx_extract");
+    if (r != BSX_OK || nviews == 0) return r;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<float> tab = fsi_table(kernel_width, kernel_power);
+    float* dtab = nullptr;
+    double* dmats = nullptr;
+    BSX_HIP(hipMalloc(&dtab, sizeof(float) * tab.size()));
+    BSX_HIP(hipMalloc(&dmats, sizeof(double) * 9 * nviews));
+    BSX_HIP(hipMemcpyAsync(dtab, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
+    BSX_HIP(hipMemcpyAsync(dmats, mats, sizeof(double) * 9 * nviews, hipMemcpyHostToDevice, s));
+    r = extract_launch(ft_dev, n, sampling, dmats, nviews, resolution, kernel_width, center, dtab,
+                       out_dev, s);
+    hipError_t se = hipStreamSynchronize(s);
     (void)hipFree(dtab);
     (void)hipFree(dmats);
-    BSX_HIP(le);
+    if (r != BSX_OK) return r;
+    BSX_HIP(se);
+    return BSX_OK;
+}
+
+// ---------------------------------------------- allocation-free extraction
+// Scratch of bsx_extract_noalloc: the view matrices (FP64, at most
+// BSX_NOALLOC_MAX_VIEWS) followed by the FSI kernel table (at most width 10).
+// One copy per GPU, held until the process ends. In host code `scratch` is
+// only a symbol: its device address comes from hipGetSymbolAddress, once per
+// device, and only that address is handed to copies and kernels.
+constexpr size_t kScratchMats = sizeof(double) * 9 * BSX_NOALLOC_MAX_VIEWS;
+constexpr size_t kScratchTab = sizeof(float) * (FSI_NDIV + 1) * 10;
+static __device__ __align__(256) unsigned char scratch[kScratchMats + kScratchTab];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static void* scratch_ptr[kMaxDevices];
+
+// Device address of the current device's scratch (looked up on the first
+// call on that device).
+static int scratch_address(void** out) {
+    int dev = 0;
+    BSX_HIP(hipGetDevice(&dev));
+    BSX_REQUIRE(dev >= 0 && dev < kMaxDevices, BSX_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> g(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        BSX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return BSX_OK;
+}
+
+int bsx_extract_noalloc(const float* ft_dev, int n, double sampling, const double* mats,
+                        int64_t nviews, double resolution, int kernel_width, int kernel_power,
+                        int center, float* out_dev, void* stream) {
+    int r = extract_args(ft_dev, n, sampling, mats, nviews, resolution, kernel_width,
+                         kernel_power, out_dev, "bsx_extract_noalloc");
+    if (r != BSX_OK) return r;
+    BSX_REQUIRE(nviews <= BSX_NOALLOC_MAX_VIEWS, BSX_ERR_ARG,
+                "bsx_extract_noalloc: %lld views exceed the scratch capacity of %d",
+                (long long)nviews, BSX_NOALLOC_MAX_VIEWS);
+    if (nviews == 0) return BSX_OK;
+    void* base = nullptr;
+    r = scratch_address(&base);
+    if (r != BSX_OK) return r;
+    double* dmats = (double*)base;
+    float* dtab = (float*)((unsigned char*)base + kScratchMats);
+    // the kernel table on the host stack (no heap), then both into scratch
+    float tab[(FSI_NDIV + 1) * 10];
+    fsi_table_into(tab, kernel_width, kernel_power);
+    const size_t ntab = (size_t)(FSI_NDIV + 1) * (kernel_power < 0 ? 1 : kernel_width);
+    hipStream_t s = (hipStream_t)stream;
+    BSX_HIP(hipMemcpyAsync(dtab, tab, sizeof(float) * ntab, hipMemcpyHostToDevice, s));
+    BSX_HIP(hipMemcpyAsync(dmats, mats, sizeof(double) * 9 * nviews, hipMemcpyHostToDevice, s));
+    r = extract_launch(ft_dev, n, sampling, dmats, nviews, resolution, kernel_width, center, dtab,
+                       out_dev, s);
+    if (r != BSX_OK) return r;
+    // the host copies above read `tab` and `mats`: wait before returning
+    BSX_HIP(hipStreamSynchronize(s));
     return BSX_OK;
 }
 

@@ -249,6 +249,20 @@ int bsx_extract(const float* ft_dev, int n, double sampling, const double* mats,
                 int64_t nviews, double resolution, int kernel_width, int kernel_power,
                 int center, float* out_dev, void* stream);
 
+/* bsx_extract without any device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments and results as bsx_extract; the view matrices and the kernel
+ * table go to a static per-GPU scratch array (about 315 KB, held until the
+ * process ends), so nviews is limited to BSX_NOALLOC_MAX_VIEWS (larger counts
+ * return BSX_ERR_ARG). Every call on one GPU shares that scratch: calls that
+ * overlap on one GPU (other host threads, or other streams still running)
+ * race. Synchronises `stream` before returning. */
+#define BSX_NOALLOC_MAX_VIEWS 4096
+int bsx_extract_noalloc(const float* ft_dev, int n, double sampling, const double* mats,
+                        int64_t nviews, double resolution, int kernel_width, int kernel_power,
+                        int center, float* out_dev, void* stream);
+
 /* ---- FSC (Bimage::fsc, src/img/Bimage_resolution.cpp:330-406 with
  *      fspace_radial_power, src/img/Bimage_fspace.cpp:848-958). FP64 shell sums
  *      on the device. a_dev/b_dev: N^3 complex64. fsc: host array of length
