@@ -167,6 +167,9 @@ SIGNATURES = {
     "bsx_search_library_data": ([_vp, _vp, _vp, _vp], C.c_int),
     "bsx_search_view_mats": ([_vp, _vp, C.c_int64, _vp, _vp], C.c_int),
     "bsx_search_refine": ([_vp, _vp, _vp, _vp, _dp, C.c_int32, _vp], C.c_int),
+    "bsx_search_refine_views": ([_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int32,
+                                 C.POINTER(C.c_int32), _vp], C.c_int),
+    "bsx_views_for_refinement": ([_dp, C.c_double, _dp], C.c_int),
     "bsx_search_fit_origin": ([_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp, _vp,
                                _vp], C.c_int),
     "bsx_search_cnn_input": ([_vp, C.c_int32, _vp, _vp], C.c_int),

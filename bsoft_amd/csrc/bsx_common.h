@@ -47,7 +47,7 @@ inline void host_parallel(int64_t n, int64_t grain, F f) {
 
 // View::quaternion + Matrix3::from_quaternion (include/View.h:131-157,
 // include/Matrix3.h:30-42): row-major d[3*i+j].
-inline void view_matrix(const double view[4], double d[9]) {
+__host__ __device__ inline void view_matrix(const double view[4], double d[9]) {
     double v0 = view[0], v1 = view[1], v2 = view[2], a = view[3];
     double size = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
     if (size < 1e-30) { v0 = v1 = 0; v2 = 1; }
@@ -76,6 +76,145 @@ inline void view_matrix(const double view[4], double d[9]) {
         if (d[i] > 1) d[i] = 1;
         if (d[i] < -1) d[i] = -1;
     }
+}
+
+// View (x, y, z, angle) after View::normalize (include/View.h:110-116): a
+// unit view vector ((0, 0, 1) when it vanishes), components below 1e-30
+// set to 0, the angle brought to (-pi, pi] by subtracting an even multiple
+// of pi (angle_set_negPI_to_PI, src/util/utilities.cpp:606-615) and zeroed
+// below 1e-30.
+__host__ __device__ inline void view_normalize4(double v[4]) {
+    const double s = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (s < 1e-30) {
+        v[0] = v[1] = 0;
+        v[2] = 1;
+    } else {
+        v[0] /= s;
+        v[1] /= s;
+        v[2] /= s;
+    }
+    for (int i = 0; i < 3; ++i)
+        if (fabs(v[i]) < 1e-30) v[i] = 0;
+    double a = v[3];
+    if (!(fabs(a) <= 1.7976931348623157e308)) {   // not finite
+        a = 0;
+    } else {
+        const long half_turns = (long)(a / M_PI);            // truncated
+        a -= (double)(half_turns + half_turns % 2) * M_PI;   // an even multiple of pi
+    }
+    v[3] = fabs(a) < 1e-30 ? 0 : a;
+}
+
+// Quaternions (s, x, y, z) as the reference's Quaternion class multiplies
+// and rotates them (include/Quaternion.h:34-41, 117-124, 161-177).
+struct Quat {
+    double s, x, y, z;
+};
+
+// Rotation by `angle` about `axis` (normalised here); parts below 1e-30 set
+// to 0.
+__host__ __device__ inline Quat quat_axis_angle(const double axis[3], double angle) {
+    const double n = sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    const double half = angle / 2.0, sh = sin(half);
+    Quat q{cos(half), axis[0] / n * sh, axis[1] / n * sh, axis[2] / n * sh};
+    if (fabs(q.s) < 1e-30) q.s = 0;
+    if (fabs(q.x) < 1e-30) q.x = 0;
+    if (fabs(q.y) < 1e-30) q.y = 0;
+    if (fabs(q.z) < 1e-30) q.z = 0;
+    return q;
+}
+
+__host__ __device__ inline Quat quat_mul(const Quat& a, const Quat& b) {
+    Quat r;
+    r.s = a.s * b.s - (a.x * b.x + a.y * b.y + a.z * b.z);
+    r.x = a.s * b.x + a.x * b.s + a.y * b.z - a.z * b.y;
+    r.y = a.s * b.y + a.y * b.s + a.z * b.x - a.x * b.z;
+    r.z = a.s * b.z + a.z * b.s + a.x * b.y - a.y * b.x;
+    return r;
+}
+
+// q p q^-1, with q^-1 = conj(q) / |q|^2.
+__host__ __device__ inline Quat quat_rotate(const Quat& q, const Quat& p) {
+    const double n2 = q.s * q.s + q.x * q.x + q.y * q.y + q.z * q.z, d = 1 / n2;
+    const Quat qi{q.s * d, -q.x * d, -q.y * d, -q.z * d};
+    return quat_mul(quat_mul(q, p), qi);
+}
+
+// views_for_refinement (src/util/View.cpp:371-436): the 27 views around
+// `view` for an angular step > 0. With v the normalised view vector,
+// axis1 = v x (0, 0, 1) and axis2 = v x axis1 (each set to (1, ., .) /
+// (., 1, .) when shorter than 0.001, then normalised); for a1, a2 in
+// {-step, 0, step} (the reference's floating loop) the vector is turned by
+// R(axis2, a2) R(axis1, a1), and for a3 in the same set the view carries the
+// in-plane angle angle + a3. out: 27 x (x, y, z, angle) in that loop order,
+// each normalised as a View; the centre (0, 0, 0) is entry 13.
+__host__ __device__ inline void refine_view_grid(const double view[4], double step,
+                                                 double* out) {
+    double v[4] = {view[0], view[1], view[2], view[3]};
+    view_normalize4(v);
+    double ax1[3] = {v[1], -v[0], 0.0};                 // v x (0, 0, 1)
+    if (sqrt(ax1[0] * ax1[0] + ax1[1] * ax1[1] + ax1[2] * ax1[2]) < 0.001) ax1[0] = 1;
+    double n1 = sqrt(ax1[0] * ax1[0] + ax1[1] * ax1[1] + ax1[2] * ax1[2]);
+    for (double& c : ax1) c /= n1;
+    double ax2[3] = {v[1] * ax1[2] - v[2] * ax1[1], v[2] * ax1[0] - v[0] * ax1[2],
+                     v[0] * ax1[1] - v[1] * ax1[0]};   // v x axis1
+    if (sqrt(ax2[0] * ax2[0] + ax2[1] * ax2[1] + ax2[2] * ax2[2]) < 0.001) ax2[1] = 1;
+    double n2 = sqrt(ax2[0] * ax2[0] + ax2[1] * ax2[1] + ax2[2] * ax2[2]);
+    for (double& c : ax2) c /= n2;
+    const Quat qv{0.0, v[0], v[1], v[2]};
+    int i = 0;
+    for (double a1 = -step; a1 <= step; a1 += step) {
+        const Quat q1 = quat_axis_angle(ax1, a1);
+        for (double a2 = -step; a2 <= step; a2 += step) {
+            const Quat r = quat_rotate(quat_mul(quat_axis_angle(ax2, a2), q1), qv);
+            for (double a3 = -step; a3 <= step; a3 += step) {
+                if (i < 27) {
+                    double* o = out + 4 * i;
+                    o[0] = r.x;
+                    o[1] = r.y;
+                    o[2] = r.z;
+                    o[3] = v[3] + a3;
+                    view_normalize4(o);
+                }
+                ++i;
+            }
+        }
+    }
+}
+
+// Matrix3::quaternion (include/Matrix3.h:129-166) and View(Quaternion&)
+// (include/View.h:52-63): the view (x, y, z, angle) of a rotation matrix
+// (row-major), angle in [-pi, pi].
+__host__ __device__ inline void matrix_view(const double* d, double* view) {
+    double q[4];
+    const double tr = 1 + d[0] + d[4] + d[8];
+    if (tr > 1) {
+        const double t = 0.5 * sqrt(tr), u = 0.25 / t;
+        q[0] = t; q[1] = (d[7] - d[5]) * u; q[2] = (d[2] - d[6]) * u; q[3] = (d[3] - d[1]) * u;
+    } else if (d[0] > d[4] && d[0] > d[8]) {
+        const double t = 0.5 * sqrt(1 + d[0] - d[4] - d[8]), u = 0.25 / t;
+        q[1] = t; q[0] = (d[7] - d[5]) * u; q[2] = (d[1] + d[3]) * u; q[3] = (d[2] + d[6]) * u;
+    } else if (d[4] > d[8]) {
+        const double t = 0.5 * sqrt(1 - d[0] + d[4] - d[8]), u = 0.25 / t;
+        q[2] = t; q[0] = (d[2] - d[6]) * u; q[1] = (d[1] + d[3]) * u; q[3] = (d[5] + d[7]) * u;
+    } else {
+        const double t = 0.5 * sqrt(1 - d[0] - d[4] + d[8]), u = 0.25 / t;
+        q[3] = t; q[0] = (d[3] - d[1]) * u; q[1] = (d[2] + d[6]) * u; q[2] = (d[5] + d[7]) * u;
+    }
+    const double qn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (double& x : q) x /= qn;
+    double ang = atan2(q[3], q[0]);
+    const double ca = cos(ang), sa = sin(ang);
+    const double q2 = q[0] * q[0] + q[3] * q[3];
+    const double f2 = 2 * sqrt(q2);
+    if (q2 < 1e-30) ang = atan2(q[1], q[2]);
+    double v0 = f2 * (q[2] * ca + q[1] * sa), v1 = f2 * (q[2] * sa - q[1] * ca), v2 = 2 * q2 - 1;
+    const double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+    if (r > 0) { v0 /= r; v1 /= r; v2 /= r; }
+    view[0] = v0;
+    view[1] = v1;
+    view[2] = v2;
+    view[3] = remainder(2 * ang, 2 * M_PI);
 }
 
 // Even aberration weights of CTFparam (include/ctf.h:79-83, 321-335;

@@ -1058,6 +1058,78 @@ __global__ void k_local_pick(const double* __restrict__ mats, const float* __res
     if (best) best[p] = bf;
 }
 
+// img_refine_grid's view loop (src/mg/mg_refine.cpp:1116-1140) on the GPU,
+// one thread per particle. State per particle: the best view (x, y, z,
+// angle), the angular step, the best FOM of the last pass (-1 before the
+// first) and whether it still iterates (step >= accuracy).
+__global__ void k_rv_init(const double* __restrict__ mats, int64_t n, double step,
+                          double accuracy, double* __restrict__ view, double* __restrict__ st,
+                          float* __restrict__ fom, int* __restrict__ active) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    matrix_view(mats + 9 * p, view + 4 * p);
+    st[p] = step;
+    fom[p] = -1.f;
+    active[p] = step >= accuracy;
+}
+
+// The 27 rotation matrices of views_for_refinement around each iterating
+// particle's best view.
+__global__ void k_rv_mats(const double* __restrict__ view, const double* __restrict__ st,
+                          const int* __restrict__ active, int64_t n, double* __restrict__ mats) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !active[p]) return;
+    double grid[27 * 4];
+    refine_view_grid(view + 4 * p, st[p], grid);
+    for (int c = 0; c < 27; ++c) view_matrix(grid + 4 * c, mats + (p * 27 + c) * 9);
+}
+
+// One pass's decision: the first view of the highest FOM (strict >, from
+// -1) becomes the best view; the step halves when that is the centre
+// (entry 13) or the FOM moved by less than 1e-6; the particle stops once
+// the step is below the accuracy. `left` counts particles still iterating.
+__global__ void k_rv_pick(const float* __restrict__ frc, int64_t n, double accuracy,
+                          double* __restrict__ view, double* __restrict__ st,
+                          float* __restrict__ fom, int* __restrict__ active,
+                          int* __restrict__ left) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || !active[p]) return;
+    const double prefom = fom[p];
+    double best = -1;
+    int ib = 0;
+    for (int c = 0; c < 27; ++c) {
+        const double f = frc[p * 27 + c];
+        if (best < f) { best = f; ib = c; }
+    }
+    double grid[27 * 4];
+    refine_view_grid(view + 4 * p, st[p], grid);
+    for (int i = 0; i < 4; ++i) view[4 * p + i] = grid[4 * ib + i];
+    fom[p] = (float)best;
+    if (ib == 13 || fabs(prefom - best) < 1e-6) st[p] /= 2;
+    active[p] = st[p] >= accuracy;
+    if (active[p]) atomicAdd(left, 1);
+}
+
+// Matrices of the current view for particles that do not iterate (all 27
+// entries), so that every matrix scored is a rotation.
+__global__ void k_rv_fill(const double* __restrict__ view, const int* __restrict__ active,
+                          int64_t n, double* __restrict__ mats) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || active[p]) return;
+    double m[9];
+    view_matrix(view + 4 * p, m);
+    for (int c = 0; c < 27; ++c)
+        for (int i = 0; i < 9; ++i) mats[(p * 27 + c) * 9 + i] = m[i];
+}
+
+__global__ void k_rv_finish(const double* __restrict__ view, const float* __restrict__ fom,
+                            int64_t n, double* __restrict__ mats, float* __restrict__ frc) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    view_matrix(view + 4 * p, mats + 9 * p);
+    if (frc) frc[p] = fom[p];
+}
+
 // Library view matrices of the chosen views (negative index: identity).
 __global__ void k_view_mats(const double* __restrict__ lm, int64_t nviews,
                             const int* __restrict__ idx, int64_t n, double* __restrict__ out) {
@@ -1164,6 +1236,10 @@ struct bsx_search {
     int64_t lmats_cap = 0;
     float* d_lfrc = nullptr;
     int64_t lfrc_cap = 0;
+    double* d_rv = nullptr;          // views_for_refinement loop: views (4n), steps (n)
+    int64_t rv_cap = 0;
+    int* d_rvi = nullptr;            // flags (n), FOMs (n floats), counter
+    int64_t rvi_cap = 0;
     uint16_t* d_sub = nullptr;       // gathered rows for a partial exhaustive search
     int64_t sub_cap = 0;
     // proposal classes: member views per class
@@ -1333,7 +1409,8 @@ int bsx_search_destroy(bsx_search* h) {
                     h->d_ctf,  h->d_pbf,     h->d_spec,   h->d_sp,
                     h->d_ktab, h->d_pscore,  h->d_pidx,   h->d_libmats, h->d_lmats,
                     h->d_lfrc, h->d_sub,     h->d_members, h->d_mcount, h->d_counts,
-                    h->d_cursors, h->d_pairs, h->d_items, h->d_zrow, h->d_cube};
+                    h->d_cursors, h->d_pairs, h->d_items, h->d_zrow, h->d_cube,
+                    h->d_rv,   h->d_rvi};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     delete h;
@@ -1809,6 +1886,71 @@ int bsx_search_refine(bsx_search* h, const float* ft_dev, double* mats_dev, floa
     return BSX_OK;
 }
 
+int bsx_search_refine_views(bsx_search* h, const float* ft_dev, double* mats_dev,
+                            float* frc_dev, double alpha_step, double accuracy, int32_t max_iter,
+                            int32_t* iterations, void* stream) {
+    BSX_REQUIRE(h && ft_dev && mats_dev, BSX_ERR_ARG, "bsx_search_refine_views: null argument");
+    BSX_REQUIRE(std::isfinite(alpha_step) && std::isfinite(accuracy) && alpha_step > 0 &&
+                    alpha_step < 1 && accuracy > 0 && alpha_step >= accuracy,
+                BSX_ERR_ARG,
+                "bsx_search_refine_views: step %g / accuracy %g rad invalid (0 < accuracy <= step < 1)",
+                alpha_step, accuracy);
+    BSX_REQUIRE(max_iter >= 1 && max_iter <= 10000, BSX_ERR_ARG,
+                "bsx_search_refine_views: max_iter %d out of range 1..10000", max_iter);
+    if (iterations) *iterations = 0;
+    const int64_t n = h->nparts;
+    if (n == 0) return BSX_OK;
+    BSX_REQUIRE(n * 27 < (1ll << 31), BSX_ERR_ARG, "bsx_search_refine_views: too many particles");
+    int rc = grow(h, &h->d_lmats, &h->lmats_cap, n * 27 * 9, "refinement matrices");
+    if (!rc) rc = grow(h, &h->d_lfrc, &h->lfrc_cap, n * 27, "refinement scores");
+    if (!rc) rc = grow(h, &h->d_rv, &h->rv_cap, n * 5, "refinement views and steps");
+    if (!rc) rc = grow(h, &h->d_rvi, &h->rvi_cap, 2 * n + 1, "refinement state");
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    double* view = h->d_rv;
+    double* st = h->d_rv + 4 * n;
+    float* fom = (float*)(h->d_rvi + n);     // n floats beside the n flags
+    int* active = h->d_rvi;
+    int* left = h->d_rvi + 2 * n;
+    const unsigned gp = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_rv_init, dim3(gp), dim3(256), 0, s, mats_dev, n, alpha_step, accuracy,
+                       view, st, fom, active);
+    BSX_HIP(hipGetLastError());
+    int it = 0, remaining = 1;
+    while (it < max_iter && remaining > 0) {
+        hipLaunchKernelGGL(k_rv_mats, dim3(gp), dim3(256), 0, s, view, st, active, n, h->d_lmats);
+        BSX_HIP(hipGetLastError());
+        if (it == 0) {
+            // particles that never iterate still need valid matrices to score
+            hipLaunchKernelGGL(k_rv_fill, dim3(gp), dim3(256), 0, s, view, active, n, h->d_lmats);
+            BSX_HIP(hipGetLastError());
+        }
+        rc = bsx_search_score_mats(h, ft_dev, h->d_lmats, 27, h->d_lfrc, stream);
+        if (rc) return rc;
+        BSX_HIP(hipMemsetAsync(left, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_rv_pick, dim3(gp), dim3(256), 0, s, h->d_lfrc, n, accuracy, view, st,
+                           fom, active, left);
+        BSX_HIP(hipGetLastError());
+        BSX_HIP(hipMemcpyAsync(&remaining, left, sizeof(int), hipMemcpyDeviceToHost, s));
+        BSX_HIP(hipStreamSynchronize(s));
+        ++it;
+    }
+    hipLaunchKernelGGL(k_rv_finish, dim3(gp), dim3(256), 0, s, view, fom, n, mats_dev, frc_dev);
+    BSX_HIP(hipGetLastError());
+    if (iterations) *iterations = it;
+    return BSX_OK;
+}
+
+int bsx_views_for_refinement(const double view[4], double alpha_step, double* out) {
+    BSX_REQUIRE(view && out, BSX_ERR_ARG, "bsx_views_for_refinement: null argument");
+    BSX_REQUIRE(std::isfinite(alpha_step) && alpha_step > 0 && std::isfinite(view[0]) &&
+                    std::isfinite(view[1]) && std::isfinite(view[2]) && std::isfinite(view[3]),
+                BSX_ERR_ARG, "bsx_views_for_refinement: step %g or view not finite / positive",
+                alpha_step);
+    refine_view_grid(view, alpha_step, out);
+    return BSX_OK;
+}
+
 int bsx_search_cnn_input(bsx_search* h, int32_t S, float* out_dev, void* stream) {
     BSX_REQUIRE(h && (out_dev || h->nparts == 0), BSX_ERR_ARG, "bsx_search_cnn_input: null argument");
     BSX_REQUIRE(S >= 8 && S <= 256 && S % 2 == 0 && fft_length_supported(S), BSX_ERR_ARG,
@@ -1872,38 +2014,7 @@ int bsx_matrix_views(const double* mats, int64_t n, double* views) {
         BSX_REQUIRE(std::isfinite(mats[i]), BSX_ERR_ARG, "bsx_matrix_views: matrix %lld is not finite",
                     (long long)(i / 9));
     host_parallel(n, 8192, [&](int64_t b, int64_t e, unsigned) {
-      for (int64_t i = b; i < e; ++i) {
-        const double* d = mats + 9 * i;
-        double q[4];
-        const double tr = 1 + d[0] + d[4] + d[8];
-        if (tr > 1) {
-            const double t = 0.5 * sqrt(tr), u = 0.25 / t;
-            q[0] = t; q[1] = (d[7] - d[5]) * u; q[2] = (d[2] - d[6]) * u; q[3] = (d[3] - d[1]) * u;
-        } else if (d[0] > d[4] && d[0] > d[8]) {
-            const double t = 0.5 * sqrt(1 + d[0] - d[4] - d[8]), u = 0.25 / t;
-            q[1] = t; q[0] = (d[7] - d[5]) * u; q[2] = (d[1] + d[3]) * u; q[3] = (d[2] + d[6]) * u;
-        } else if (d[4] > d[8]) {
-            const double t = 0.5 * sqrt(1 - d[0] + d[4] - d[8]), u = 0.25 / t;
-            q[2] = t; q[0] = (d[2] - d[6]) * u; q[1] = (d[1] + d[3]) * u; q[3] = (d[5] + d[7]) * u;
-        } else {
-            const double t = 0.5 * sqrt(1 - d[0] - d[4] + d[8]), u = 0.25 / t;
-            q[3] = t; q[0] = (d[3] - d[1]) * u; q[1] = (d[2] + d[6]) * u; q[2] = (d[5] + d[7]) * u;
-        }
-        const double qn = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-        for (double& x : q) x /= qn;
-        double ang = atan2(q[3], q[0]);
-        const double ca = cos(ang), sa = sin(ang);
-        const double q2 = q[0] * q[0] + q[3] * q[3];
-        const double f2 = 2 * sqrt(q2);
-        if (q2 < 1e-30) ang = atan2(q[1], q[2]);
-        double v0 = f2 * (q[2] * ca + q[1] * sa), v1 = f2 * (q[2] * sa - q[1] * ca), v2 = 2 * q2 - 1;
-        const double r = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
-        if (r > 0) { v0 /= r; v1 /= r; v2 /= r; }
-        views[4 * i] = v0;
-        views[4 * i + 1] = v1;
-        views[4 * i + 2] = v2;
-        views[4 * i + 3] = remainder(2 * ang, 2 * M_PI);
-      }
+        for (int64_t i = b; i < e; ++i) matrix_view(mats + 9 * i, views + 4 * i);
     });
     return BSX_OK;
 }

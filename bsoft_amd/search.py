@@ -210,6 +210,20 @@ class Search:
              st.ctypes.data_as(C.POINTER(C.c_double)), len(st), _stream())
         return mats, frc
 
+    def refine_views(self, mats: torch.Tensor, step_deg: float = 2.0, accuracy_deg: float = 0.25,
+                     max_iter: int = 64, ft: torch.Tensor | None = None):
+        """img_refine_grid's view loop (mg_refine.cpp:1116-1140) over the
+        views_for_refinement grid (View.cpp:371-436), the origin kept:
+        returns (mats (n, 3, 3), frc (n,), passes)."""
+        ft = self._ft if ft is None else ft
+        mats = mats.to(device=self.device, dtype=torch.float64).contiguous().clone()
+        frc = torch.empty(self.nparts, dtype=torch.float32, device=self.device)
+        it = C.c_int32()
+        call("bsx_search_refine_views", self.h, _dev_ptr(ft), _dev_ptr(mats), _dev_ptr(frc),
+             math.radians(step_deg), math.radians(accuracy_deg), int(max_iter), C.byref(it),
+             _stream())
+        return mats, frc, it.value
+
     def fit_origin(self, mats: torch.Tensor, step: float = 1.0, accuracy: float = 0.25,
                    def_std: float = 0.0, ft: torch.Tensor | None = None):
         """img_fit_in_recip_space_grid at rotations mats (n, 3, 3): the best

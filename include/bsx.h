@@ -398,6 +398,27 @@ int bsx_search_view_mats(const bsx_search* h, const int32_t* views_dev, int64_t 
 int bsx_search_refine(bsx_search* h, const float* ft_dev, double* mats_dev, float* frc_dev,
                       const double* steps, int32_t nsteps, void* stream);
 
+/* img_refine_grid's view loop (src/mg/mg_refine.cpp:1116-1140) over the
+ * views_for_refinement grid (src/util/View.cpp:371-436), for every staged
+ * particle at once: starting from mats_dev (n x 9, FP64, device) with angular
+ * step alpha_step (rad), each pass scores the 27 views around the best view
+ * (the view vector tilted by -step, 0, +step about two axes perpendicular to
+ * it, times the in-plane angle +-step) by the FP32 FRC of the staged
+ * particle (as bsx_search_score_mats, the origin kept); the first highest
+ * score becomes the best view, and the step halves when that is the centre
+ * or the score changed by less than 1e-6; a particle stops once its step is
+ * below `accuracy`. At most max_iter passes (the reference has no bound).
+ * Out: mats_dev = the best views' matrices, frc_dev (n, may be NULL) their
+ * scores (-1 where alpha_step < accuracy), *iterations = passes run.
+ * Synchronises the stream once per pass. */
+int bsx_search_refine_views(bsx_search* h, const float* ft_dev, double* mats_dev,
+                            float* frc_dev, double alpha_step, double accuracy, int32_t max_iter,
+                            int32_t* iterations, void* stream);
+/* views_for_refinement (src/util/View.cpp:371-436) on the host: the 27 views
+ * (x, y, z, angle; angle in rad) around `view` at angular step alpha_step > 0,
+ * in the reference's order (centre = entry 13). out: 27 x 4 doubles. */
+int bsx_views_for_refinement(const double view[4], double alpha_step, double* out);
+
 /* Origin and defocus fit of every staged particle at its orientation, as
  * img_fit_in_recip_space_grid (src/mg/mg_refine.cpp:862-946) does: the
  * section at mats_dev (n x 9 FP64, device) is taken once; when def_std > 0

@@ -166,3 +166,96 @@ def fit_grid(v0: np.ndarray, part: Particle, svals: np.ndarray, hk: np.ndarray,
         if ibest == i // 2 or abs(prevfom - bestfom) < 1e-6:
             step /= 2
     return best, bestdef, bestfom, number
+
+
+# ------------------------------------------------------------ local view grid
+def _view_normalize(v):
+    """View::normalize (include/View.h:110-116, check :34-38) and
+    angle_set_negPI_to_PI (src/util/utilities.cpp:606-615)."""
+    x, y, z, a = (float(c) for c in v)
+    s = math.sqrt(x * x + y * y + z * z)
+    if s < 1e-30:
+        x, y, z = 0.0, 0.0, 1.0
+    else:
+        x, y, z = x / s, y / s, z / s
+    x, y, z = (0.0 if abs(c) < 1e-30 else c for c in (x, y, z))
+    if not math.isfinite(a):
+        a = 0.0
+    else:
+        n = int(a / math.pi)                        # C's truncating conversion
+        a -= (n + int(math.fmod(n, 2))) * math.pi    # C's remainder keeps the sign
+    return [x, y, z, 0.0 if abs(a) < 1e-30 else a]
+
+
+def _qmul(a, b):
+    """Quaternion product (include/Quaternion.h:117-124); (s, x, y, z)."""
+    s1, x1, y1, z1 = a
+    s2, x2, y2, z2 = b
+    return (s1 * s2 - (x1 * x2 + y1 * y2 + z1 * z2),
+            s1 * x2 + x1 * s2 + y1 * z2 - z1 * y2,
+            s1 * y2 + y1 * s2 + z1 * x2 - x1 * z2,
+            s1 * z2 + z1 * s2 + x1 * y2 - y1 * x2)
+
+
+def _qaxis(axis, angle):
+    n = math.sqrt(sum(c * c for c in axis))
+    sh = math.sin(angle / 2)
+    q = (math.cos(angle / 2), axis[0] / n * sh, axis[1] / n * sh, axis[2] / n * sh)
+    return tuple(0.0 if abs(c) < 1e-30 else c for c in q)
+
+
+def views_for_refinement(view, step: float) -> np.ndarray:
+    """views_for_refinement (src/util/View.cpp:371-436): 27 x (x, y, z, angle)."""
+    v = _view_normalize(view)
+    ax1 = [v[1], -v[0], 0.0]
+    if math.sqrt(sum(c * c for c in ax1)) < 0.001:
+        ax1[0] = 1.0
+    n1 = math.sqrt(sum(c * c for c in ax1))
+    ax1 = [c / n1 for c in ax1]
+    ax2 = [v[1] * ax1[2] - v[2] * ax1[1], v[2] * ax1[0] - v[0] * ax1[2],
+           v[0] * ax1[1] - v[1] * ax1[0]]
+    if math.sqrt(sum(c * c for c in ax2)) < 0.001:
+        ax2[1] = 1.0
+    n2 = math.sqrt(sum(c * c for c in ax2))
+    ax2 = [c / n2 for c in ax2]
+    qv = (0.0, v[0], v[1], v[2])
+    out = []
+    steps = []
+    a = -step
+    while a <= step:                                # the reference's floating loop
+        steps.append(a)
+        a += step
+    for a1 in steps:
+        q1 = _qaxis(ax1, a1)
+        for a2 in steps:
+            q = _qmul(_qaxis(ax2, a2), q1)
+            n2q = sum(c * c for c in q)
+            qi = (q[0] / n2q, -q[1] / n2q, -q[2] / n2q, -q[3] / n2q)
+            r = _qmul(_qmul(q, qv), qi)
+            for a3 in steps:
+                out.append(_view_normalize([r[1], r[2], r[3], v[3] + a3]))
+    return np.asarray(out)
+
+
+def refine_views(score, view, step: float, accuracy: float, max_iter: int = 10000):
+    """The view loop of img_refine_grid (src/mg/mg_refine.cpp:1116-1140) with
+    a scoring function score(views (27, 4)) -> (27,) FOMs: returns (best view,
+    best FOM, passes)."""
+    best_view = _view_normalize(view)
+    bestfom = -1.0
+    it = 0
+    while step >= accuracy and it < max_iter:
+        prefom = bestfom
+        bestfom = -1.0
+        grid = views_for_refinement(best_view, step)
+        f = score(grid)
+        ib = 0
+        for i in range(len(grid)):
+            if bestfom < f[i]:
+                bestfom = float(f[i])
+                ib = i
+                best_view = list(grid[i])
+        if ib == len(grid) // 2 or abs(prefom - bestfom) < 1e-6:
+            step /= 2
+        it += 1
+    return np.asarray(best_view), bestfom, it

@@ -371,3 +371,61 @@ def test_fit_origin_edge_cases():
     ori, _, _ = s.fit_origin(mats, 1.0, 1.0, ft=ft)
     torch.cuda.synchronize()
     assert np.all(np.abs(ori.cpu().numpy() - n // 2) <= 1.0)
+
+
+def test_refine_views_matches_host_loop():
+    """bsx_search_refine_views (img_refine_grid's view loop over the
+    views_for_refinement grid) against the same loop driven from the host:
+    grids from oracle/search_ref.views_for_refinement, scores from the
+    library's FRC at those rotations, the reference's pick and contraction
+    rules applied per particle. Both must take the same passes and end at
+    the same views; on noise-free particles started from the best 4-degree
+    grid view they end within a fraction of a degree of the truth."""
+    n, R, P = 64, 16, 48
+    rng = np.random.default_rng(61)
+    bm = synth.make_blob_map(n, 40, rng)
+    ft = map_transform(bm.volume().cuda())
+    lm = search.view_matrices(search.views_c1(4.0, angle_step_deg=4.0))
+    s = search.Search(n, R, 1.5, kernel_width=3, topk=8)
+    s.set_reference(ft, lm)
+    ps = synth.make_particles(bm, P, n, rng, snr=None, seed=61)
+    s.stage(ps.images.cuda().contiguous(), ps.parts)
+    start = s.view_mats(search.exhaustive_search(s, steps_deg=()).grid_view)
+    step0, acc = math.radians(2.0), math.radians(0.25)
+    say(f"refine_views: {P} particles, step 2 deg, accuracy 0.25 deg")
+    mats, frc, passes = s.refine_views(start, 2.0, 0.25, max_iter=64)
+    # the host-driven loop, all particles in lockstep like the library
+    views = search.matrix_views(start.cpu().numpy())
+    step = np.full(P, step0)
+    fom = np.full(P, -1.0)
+    active = step >= acc
+    it = 0
+    while active.any() and it < 64:
+        grids = np.stack([sr.views_for_refinement(v, st) for v, st in zip(views, step)])
+        M = search.view_matrices(grids.reshape(-1, 4)).reshape(P, 27, 3, 3)
+        f = s.score_mats(torch.as_tensor(M)).cpu().numpy().astype(np.float64)
+        for p in np.nonzero(active)[0]:
+            prefom, best, ib = fom[p], -1.0, 0
+            for c in range(27):
+                if best < f[p, c]:
+                    best, ib = f[p, c], c
+            views[p] = grids[p, ib]
+            fom[p] = best
+            if ib == 13 or abs(prefom - best) < 1e-6:
+                step[p] /= 2
+            active[p] = step[p] >= acc
+        it += 1
+    host = synth.view_to_matrix(views)
+    d = search.rotation_angle_deg(mats.cpu().numpy(), host)
+    truth = synth.view_to_matrix(ps.parts["view"])
+    err = search.rotation_angle_deg(mats.cpu().numpy(), truth)
+    gerr = search.rotation_angle_deg(start.cpu().numpy(), truth)
+    say(f"  {passes} passes (host {it}); max difference to the host loop {d.max():.2e} deg; "
+        f"median error vs truth {np.median(gerr):.2f} -> {np.median(err):.2f} deg")
+    assert passes == it
+    # same views: angles below 1e-4 deg are the arccos floor of the angle
+    # between equal rotations (about 2e-6 deg)
+    same = d < 1e-4
+    assert np.sum(~same) <= 1
+    np.testing.assert_allclose(frc.cpu().numpy()[same], fom[same], atol=1e-6)
+    assert np.median(err) < 0.5 and np.median(err) < np.median(gerr)

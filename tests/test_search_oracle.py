@@ -6,6 +6,9 @@ import numpy as np
 import pytest
 
 from bsoft_amd import search, synth
+import ctypes as C
+
+from bsoft_amd import abi
 from bsoft_amd.abi import Particle
 from oracle import search_ref as sr
 
@@ -200,3 +203,89 @@ def test_fit_grid_defocus_scan_without_ctf(oracle64):
     part = _part(def_avg=dt, ori=(cx, cy))
     _, d, _, num = sr.fit_grid(v0, part, sv, hk, rs, n, oracle64, 1.0, 1.0, sd, ctf_flag=False)
     assert d == dt - 5 * sd and num == 11 + 9
+
+
+# ------------------------------------------------- views_for_refinement grid
+def _grid_closed_form(view, step):
+    """The 27 views from the geometry alone: with v the unit view vector,
+    ax1 = unit(v x z) (x when v is along z) and ax2 = v x ax1, the turned
+    vector is cos a1 cos a2 v + cos a1 sin a2 ax1 - sin a1 ax2 (a turn by a1
+    about ax1, then by a2 about ax2) and the in-plane angle is angle + a3."""
+    v = np.asarray(view[:3], dtype=float)
+    v /= np.linalg.norm(v)
+    ax1 = np.cross(v, [0.0, 0.0, 1.0])
+    if np.linalg.norm(ax1) < 1e-3:
+        ax1 = np.array([1.0, ax1[1], ax1[2]])
+    ax1 /= np.linalg.norm(ax1)
+    ax2 = np.cross(v, ax1)
+    out = []
+    for a1 in (-step, 0.0, step):
+        for a2 in (-step, 0.0, step):
+            vec = (math.cos(a1) * math.cos(a2) * v + math.cos(a1) * math.sin(a2) * ax1
+                   - math.sin(a1) * ax2)
+            for a3 in (-step, 0.0, step):
+                out.append(np.concatenate([vec, [view[3] + a3]]))
+    return np.asarray(out)
+
+
+@pytest.mark.parametrize("view", [(0, 0, 1, 0.3), (0.3, -0.5, 0.8, 2.9), (0, 0, -1, -3.0),
+                                  (0.6, 0.8, 0, -1.2), (-0.2, 0.1, 0.97, 3.1)])
+@pytest.mark.parametrize("deg", [0.5, 2.0, 7.5])
+def test_views_for_refinement_closed_form(view, deg):
+    step = math.radians(deg)
+    g = sr.views_for_refinement(view, step)
+    assert g.shape == (27, 4)
+    cf = _grid_closed_form(view, step)
+    np.testing.assert_allclose(g[:, :3], cf[:, :3], atol=1e-12)
+    # in-plane angles: angle + a3 brought to (-pi, pi]
+    d = np.remainder(g[:, 3] - cf[:, 3] + math.pi, 2 * math.pi) - math.pi
+    np.testing.assert_allclose(d, 0, atol=1e-12)
+    assert np.all(np.abs(g[:, 3]) <= math.pi + 1e-12)
+    # the centre is the normalised view; its vector has unit length
+    v = np.asarray(view[:3], float) / np.linalg.norm(view[:3])
+    np.testing.assert_allclose(g[13, :3], v, atol=1e-15)
+    # the library's grid (bsx_views_for_refinement) equals the restatement
+    out = np.zeros(108)
+    assert abi.lib().bsx_views_for_refinement((C.c_double * 4)(*view), step,
+                                              out.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    np.testing.assert_allclose(out.reshape(27, 4), g, atol=1e-15)
+
+
+def test_views_for_refinement_along_z():
+    """View (0, 0, 1): ax1 = x, ax2 = y, so entry (a1, a2) carries the vector
+    (cos a1 sin a2, -sin a1, cos a1 cos a2)."""
+    s = math.radians(3)
+    g = sr.views_for_refinement((0, 0, 1, 0), s)
+    assert np.allclose(g[0 * 9 + 2 * 3 + 1, :3], (math.cos(-s) * math.sin(s), math.sin(s),
+                                                  math.cos(s) * math.cos(s)), atol=1e-15)
+    assert np.allclose(g[2 * 9 + 1 * 3 + 0, :3], (0, -math.sin(s), math.cos(s)), atol=1e-15)
+    assert g[2 * 9 + 1 * 3 + 0, 3] == pytest.approx(-s)
+
+
+def _cos_score(target):
+    """Score of views: cosine of the rotation angle to the target rotation."""
+    T = synth.view_to_matrix(np.asarray(target, float)[None])[0]
+    def f(views):
+        M = synth.view_to_matrix(views)
+        return (np.einsum("nij,ij->n", M, T) - 1) / 2
+    return f
+
+
+def test_refine_views_loop_rules():
+    """img_refine_grid's view loop: a score peaked at the start view only
+    contracts (2, 1, 0.5 degrees: three passes above a 0.4-degree accuracy);
+    a smooth score with its peak 3 degrees away is climbed to within the
+    accuracy."""
+    start = (0.2, -0.3, 0.93, 0.7)
+    v, f, it = sr.refine_views(_cos_score(start), start, math.radians(2), math.radians(0.4))
+    assert it == 3 and f == pytest.approx(1.0)
+    target = synth.view_to_matrix(np.asarray(start)[None])[0]
+    # the peak: the start rotated by 3 degrees about a skew axis
+    ax = np.array([0.3, 0.8, -0.5]) / np.linalg.norm([0.3, 0.8, -0.5])
+    a = math.radians(3)
+    K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+    Rt = (np.eye(3) + math.sin(a) * K + (1 - math.cos(a)) * K @ K) @ target
+    tv = search.matrix_views(Rt[None])[0]
+    v, f, it = sr.refine_views(_cos_score(tv), start, math.radians(2), math.radians(0.1))
+    err = search.rotation_angle_deg(synth.view_to_matrix(v[None]), Rt[None])[0]
+    assert err < 0.2 and it > 3
