@@ -841,7 +841,10 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         for (int i = tid; i < BSX_STAGE_CAP; i += NT) st_r[i] = -1;
         __syncthreads();
     }
-    const PartDev& P = recs[valid ? p : 0];
+    // At FT 512 a wavefront is one column (T == 64), so its particle is
+    // uniform: the record is then read with scalar loads, kept in SGPRs.
+    const int prec = valid ? p : 0;
+    const PartDev& P = recs[T == 64 ? __builtin_amdgcn_readfirstlane(prec) : prec];
     const unsigned keybase = (unsigned)P.half * (unsigned)(K.NT * K.NT * K.NT) * 4096u;
     const float inv = 1.0f / FT;
     const float b = valid ? bkg[p] : 0.f;
