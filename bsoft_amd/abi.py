@@ -181,6 +181,12 @@ SIGNATURES = {
     "bsx_cnn_forward": ([_vp, _vp, C.c_int32, _vp, _vp], C.c_int),
     "bsx_cnn_gradients": ([_vp, _vp, _vp, C.c_int32, _vp, _vp, _vp], C.c_int),
     "bsx_cnn_train_step": ([_vp, _vp, _vp, C.c_int32, C.c_float, C.c_float, _vp, _vp], C.c_int),
+    "bsx_cnn_embed": ([_vp, _vp, C.c_int64, _vp, _vp], C.c_int),
+    "bsx_cnn_train_pairs": ([_vp, _vp, C.c_int32, C.c_float, C.c_float, C.c_float, _vp, _vp],
+                            C.c_int),
+    "bsx_cnn_pair_gradients": ([_vp, _vp, C.c_int32, C.c_float, _vp, _vp, _vp], C.c_int),
+    "bsx_cnn_match_topk": ([_vp, _vp, C.c_int64, _vp, C.c_int32, C.c_int32, C.c_float, _vp, _vp,
+                            _vp], C.c_int),
     "bsx_cnn_infer_topk": ([_vp, _vp, C.c_int64, C.c_int32, _vp, _vp, _vp], C.c_int),
     "bsx_matrix_views": ([_dp, C.c_int64, _dp], C.c_int),
     "bsx_view_matrices": ([_dp, C.c_int64, _dp], C.c_int),

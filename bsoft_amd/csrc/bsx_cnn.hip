@@ -351,6 +351,84 @@ __global__ __launch_bounds__(256) void k_row_topk(const float* __restrict__ logi
     }
 }
 
+// --------------------------------------------- contrastive (embedding) head
+// Unit rows u = z / |z| of an n x d matrix (|z| floored at 1e-12); one
+// thread per row.
+__global__ void k_unit_rows(const float* __restrict__ z, long n, int d, float* __restrict__ u,
+                            float* __restrict__ nrm) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = z + i * d;
+    float ss = 0.f;
+    for (int c = 0; c < d; ++c) ss = fmaf(r[c], r[c], ss);
+    const float nz = fmaxf(sqrtf(ss), 1e-12f), inv = 1.f / nz;
+    for (int c = 0; c < d; ++c) u[i * d + c] = r[c] * inv;
+    if (nrm) nrm[i] = nz;
+}
+
+// Softmax of the pair scores S (B x B, cosines) over temperature tau, by row
+// (dim 0: p_ij over j) or by column (dim 1: q_ij over i), one workgroup per
+// row / column; loss[i] = -log of the matched pair's probability.
+__global__ __launch_bounds__(256) void k_pair_softmax(const float* __restrict__ S, int B,
+                                                      float inv_tau, int dim, float* __restrict__ P,
+                                                      float* __restrict__ loss) {
+    __shared__ float red[256];
+    const int i = blockIdx.x;
+    const long st = dim == 0 ? 1 : B, base = dim == 0 ? (long)i * B : i;
+    float mx = -INFINITY;
+    for (int j = threadIdx.x; j < B; j += blockDim.x) mx = fmaxf(mx, S[base + j * st] * inv_tau);
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    mx = red[0];
+    __syncthreads();
+    float se = 0.f;
+    for (int j = threadIdx.x; j < B; j += blockDim.x) se += expf(S[base + j * st] * inv_tau - mx);
+    red[threadIdx.x] = se;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    se = red[0];
+    for (int j = threadIdx.x; j < B; j += blockDim.x)
+        P[base + j * st] = expf(S[base + j * st] * inv_tau - mx) / se;
+    if (threadIdx.x == 0) loss[i] = logf(se) + mx - S[base + (long)i * st] * inv_tau;
+}
+
+// dL/dS of the symmetric loss mean_i (row_i + col_i) / 2 over the B pairs:
+// (p + q - 2 I) / (2 B tau); loss_out[i] = (row_i + col_i) / 2.
+__global__ void k_pair_grad(const float* __restrict__ P, const float* __restrict__ Q, int B,
+                            float inv_tau, const float* __restrict__ lrow,
+                            const float* __restrict__ lcol, float* __restrict__ dS,
+                            float* __restrict__ loss_out) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)B * B) return;
+    const int i = (int)(t / B), j = (int)(t % B);
+    dS[t] = (P[t] + Q[t] - (i == j ? 2.f : 0.f)) * inv_tau / (2.f * B);
+    if (j == 0 && loss_out) loss_out[i] = 0.5f * (lrow[i] + lcol[i]);
+}
+
+// Backward of u = z / |z|: dz = (du - (du . u) u) / |z|, one thread per row.
+__global__ void k_unit_rows_bwd(const float* __restrict__ du, const float* __restrict__ u,
+                                const float* __restrict__ nrm, long n, int d,
+                                float* __restrict__ dz) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float dot = 0.f;
+    for (int c = 0; c < d; ++c) dot = fmaf(du[i * d + c], u[i * d + c], dot);
+    const float inv = 1.f / nrm[i];
+    for (int c = 0; c < d; ++c) dz[i * d + c] = (du[i * d + c] - dot * u[i * d + c]) * inv;
+}
+
+__global__ void k_scale(float* __restrict__ x, long n, float a) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] *= a;
+}
+
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, long n, float lr, float b1, float b2, float eps,
                        float c1, float c2, float wd) {
@@ -559,6 +637,12 @@ struct bsx_cnn {
     int icap = 0;
     uint16_t* d_ib[5] = {};
     float* d_ilog = nullptr;
+    // contrastive training scratch (pairs of up to maxb / 2): unit rows,
+    // norms, pair scores and their softmaxes, gradients
+    float* d_nce = nullptr;
+    long nce_cap = 0;
+    float* d_match = nullptr;    // embedding top-k: scores of a particle chunk
+    long match_cap = 0;
 };
 
 namespace {
@@ -574,6 +658,8 @@ void free_cnn(bsx_cnn* h) {
         if (h->d_ib[i]) (void)hipFree(h->d_ib[i]);
     }
     if (h->d_ilog) (void)hipFree(h->d_ilog);
+    if (h->d_nce) (void)hipFree(h->d_nce);
+    if (h->d_match) (void)hipFree(h->d_match);
 }
 
 long act_size(const bsx_cnn* h, int l, int B) {
@@ -614,11 +700,10 @@ int forward(bsx_cnn* h, const float* x, int B, const TW* P, hipStream_t s) {
     return BSX_OK;
 }
 
-int backward(bsx_cnn* h, const float* x, const int* labels, int B, hipStream_t s) {
+// Backpropagation from dL/dlogits in d_gact[0] (B rows) to the parameter
+// gradients in d_g.
+int backward_body(bsx_cnn* h, const float* x, int B, hipStream_t s) {
     const int F = (int)h->feat, C = h->nclass;
-    hipLaunchKernelGGL(k_softmax_ce, dim3(B), dim3(256), 0, s, h->d_act[0], labels, B, C,
-                       h->d_gact[0], h->d_loss);
-    BSX_HIP(hipGetLastError());
     float* P = h->d_p;
     float* G = h->d_g;
     // FC: dW = dl^T x, db = colsum(dl), dx = dl W
@@ -651,6 +736,99 @@ int backward(bsx_cnn* h, const float* x, const int* labels, int B, hipStream_t s
                                P + h->woff[l - 1], h->d_gact[l - 1], B, Cin, H, H, Cout, Ho, Ho, stride);
             BSX_HIP(hipGetLastError());
         }
+    }
+    return BSX_OK;
+}
+
+
+int backward(bsx_cnn* h, const float* x, const int* labels, int B, hipStream_t s) {
+    hipLaunchKernelGGL(k_softmax_ce, dim3(B), dim3(256), 0, s, h->d_act[0], labels, B, h->nclass,
+                       h->d_gact[0], h->d_loss);
+    BSX_HIP(hipGetLastError());
+    return backward_body(h, x, B, s);
+}
+
+// Network outputs of n inputs in chunks (BF16 MFMA layers when the channel
+// counts allow, else the FP32-accumulating BF16 path): calls
+// use(logits, first, count) for each chunk, on stream s.
+template <class Use>
+int infer_chunks(bsx_cnn* h, const float* x_dev, int64_t n, hipStream_t s, const char* who, Use use) {
+    // the MFMA layers need input channel counts (c1..c3 and the FC input) in multiples of 32
+    const bool mfma = h->ch[1] % 32 == 0 && h->ch[2] % 32 == 0 && h->ch[3] % 32 == 0 &&
+                      h->feat % 32 == 0;
+    if (!h->pb_valid) {
+        hipLaunchKernelGGL(k_to_bf16, dim3(nblk(h->nparam)), dim3(256), 0, s, h->d_p, h->d_pb, h->nparam);
+        BSX_HIP(hipGetLastError());
+        if (mfma) {
+            for (int l = 1; l < 4; ++l) {
+                const long nw = (long)h->ch[l + 1] * h->ch[l] * 9;
+                hipLaunchKernelGGL(k_perm_conv, dim3(nblk(nw)), dim3(256), 0, s, h->d_p + h->woff[l],
+                                   h->ch[l + 1], h->ch[l], h->d_wi[l]);
+                BSX_HIP(hipGetLastError());
+            }
+            const long nf = h->feat * h->nclass;
+            hipLaunchKernelGGL(k_perm_fc, dim3(nblk(nf)), dim3(256), 0, s, h->d_p + h->woff[4],
+                               h->nclass, h->ch[4], h->hw[4] * h->hw[4], h->d_wi[4]);
+            BSX_HIP(hipGetLastError());
+        }
+        h->pb_valid = true;
+    }
+    const long in = (long)h->S * h->S;
+    // MFMA path: chunks of up to 4096 particles in the inference buffers
+    int chunk = h->maxb;
+    if (mfma) {
+        chunk = (int)std::min<int64_t>(std::max<int64_t>(n, 1), 4096);
+        chunk = std::max(chunk, std::min(h->maxb, 4096));
+        if (chunk > h->icap) {
+            for (int l = 1; l < 5; ++l) {
+                if (h->d_ib[l]) (void)hipFree(h->d_ib[l]);
+                h->d_ib[l] = nullptr;
+            }
+            if (h->d_ilog) (void)hipFree(h->d_ilog);
+            h->d_ilog = nullptr;
+            h->icap = 0;
+            for (int l = 1; l < 5; ++l)
+                if (hipMalloc(&h->d_ib[l], sizeof(uint16_t) * act_size(h, l, chunk)) != hipSuccess) {
+                    set_error("%s: inference activations (%d particles)", who, chunk);
+                    return BSX_ERR_NOMEM;
+                }
+            if (hipMalloc(&h->d_ilog, sizeof(float) * (size_t)chunk * h->nclass) != hipSuccess) {
+                set_error("%s: logits (%d particles)", who, chunk);
+                return BSX_ERR_NOMEM;
+            }
+            h->icap = chunk;
+        }
+        chunk = h->icap;
+    }
+    for (int64_t b0 = 0; b0 < n; b0 += chunk) {
+        const int nb = (int)std::min<int64_t>(chunk, n - b0);
+        if (!mfma) {
+            int rc = forward<uint16_t>(h, x_dev + b0 * in, nb, h->d_pb, s);
+            if (rc) return rc;
+        } else {
+            const uint16_t* P = h->d_pb;
+            const long pos1 = (long)nb * in * (h->ch[1] / 8);
+            hipLaunchKernelGGL(k_conv1_nhwc, dim3(nblk(pos1)), dim3(256),
+                               sizeof(float) * 10 * h->ch[1], s, x_dev + b0 * in, P + h->woff[0],
+                               P + h->boff[0], nb, h->S, h->ch[1], h->d_ib[1]);
+            BSX_HIP(hipGetLastError());
+            for (int l = 2; l <= 4; ++l) {
+                const long M = (long)nb * h->hw[l] * h->hw[l];
+                dim3 g((unsigned)((M + IM - 1) / IM), (unsigned)((h->ch[l] + IN - 1) / IN));
+                hipLaunchKernelGGL((k_conv_mfma<false>), g, dim3(256), 0, s, h->d_ib[l - 1], nb,
+                                   h->hw[l - 1], h->hw[l - 1], h->ch[l - 1], 3, 2, h->hw[l], h->hw[l],
+                                   h->d_wi[l - 1], P + h->boff[l - 1], h->ch[l], (long)h->ch[l],
+                                   (void*)h->d_ib[l]);
+                BSX_HIP(hipGetLastError());
+            }
+            dim3 g((unsigned)((nb + IM - 1) / IM), (unsigned)((h->nclass + IN - 1) / IN));
+            hipLaunchKernelGGL((k_conv_mfma<true>), g, dim3(256), 0, s, h->d_ib[4], nb, 1, 1,
+                               (int)h->feat, 1, 1, 1, 1, h->d_wi[4], P + h->boff[4], h->nclass,
+                               (long)h->nclass, (void*)h->d_ilog);
+            BSX_HIP(hipGetLastError());
+        }
+        const int rc = use(mfma ? h->d_ilog : h->d_act[0], b0, nb);
+        if (rc) return rc;
     }
     return BSX_OK;
 }
@@ -815,90 +993,145 @@ int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int
                 "bsx_cnn_infer_topk: null argument");
     BSX_REQUIRE(k >= 1 && k <= 32 && k <= h->nclass, BSX_ERR_ARG, "bsx_cnn_infer_topk: k %d out of range", k);
     hipStream_t s = (hipStream_t)stream;
-    // the MFMA layers need input channel counts (c1..c3 and the FC input) in multiples of 32
-    const bool mfma = h->ch[1] % 32 == 0 && h->ch[2] % 32 == 0 && h->ch[3] % 32 == 0 &&
-                      h->feat % 32 == 0;
-    if (!h->pb_valid) {
-        hipLaunchKernelGGL(k_to_bf16, dim3(nblk(h->nparam)), dim3(256), 0, s, h->d_p, h->d_pb, h->nparam);
+    return infer_chunks(h, x_dev, n, s, "bsx_cnn_infer_topk", [&](const float* lg, int64_t b0, int nb) {
+        if (k <= 16)
+            hipLaunchKernelGGL(k_row_topk<16>, dim3(nb), dim3(256), 0, s, lg, h->nclass, k,
+                               cls_dev + b0 * k, prob_dev + b0 * k);
+        else
+            hipLaunchKernelGGL(k_row_topk<32>, dim3(nb), dim3(256), 0, s, lg, h->nclass, k,
+                               cls_dev + b0 * k, prob_dev + b0 * k);
         BSX_HIP(hipGetLastError());
-        if (mfma) {
-            for (int l = 1; l < 4; ++l) {
-                const long nw = (long)h->ch[l + 1] * h->ch[l] * 9;
-                hipLaunchKernelGGL(k_perm_conv, dim3(nblk(nw)), dim3(256), 0, s, h->d_p + h->woff[l],
-                                   h->ch[l + 1], h->ch[l], h->d_wi[l]);
-                BSX_HIP(hipGetLastError());
-            }
-            const long nf = h->feat * h->nclass;
-            hipLaunchKernelGGL(k_perm_fc, dim3(nblk(nf)), dim3(256), 0, s, h->d_p + h->woff[4],
-                               h->nclass, h->ch[4], h->hw[4] * h->hw[4], h->d_wi[4]);
-            BSX_HIP(hipGetLastError());
-        }
-        h->pb_valid = true;
-    }
-    const long in = (long)h->S * h->S;
-    // MFMA path: chunks of up to 4096 particles in the inference buffers
-    int chunk = h->maxb;
-    if (mfma) {
-        chunk = (int)std::min<int64_t>(std::max<int64_t>(n, 1), 4096);
-        chunk = std::max(chunk, std::min(h->maxb, 4096));
-        if (chunk > h->icap) {
-            for (int l = 1; l < 5; ++l) {
-                if (h->d_ib[l]) (void)hipFree(h->d_ib[l]);
-                h->d_ib[l] = nullptr;
-            }
-            if (h->d_ilog) (void)hipFree(h->d_ilog);
-            h->d_ilog = nullptr;
-            h->icap = 0;
-            for (int l = 1; l < 5; ++l)
-                if (hipMalloc(&h->d_ib[l], sizeof(uint16_t) * act_size(h, l, chunk)) != hipSuccess) {
-                    set_error("bsx_cnn_infer_topk: inference activations (%d particles)", chunk);
-                    return BSX_ERR_NOMEM;
-                }
-            if (hipMalloc(&h->d_ilog, sizeof(float) * (size_t)chunk * h->nclass) != hipSuccess) {
-                set_error("bsx_cnn_infer_topk: logits (%d particles)", chunk);
-                return BSX_ERR_NOMEM;
-            }
-            h->icap = chunk;
-        }
-        chunk = h->icap;
+        return BSX_OK;
+    });
+}
+
+int bsx_cnn_embed(bsx_cnn* h, const float* x_dev, int64_t n, float* emb_dev, void* stream) {
+    BSX_REQUIRE(h && (n == 0 || (x_dev && emb_dev)), BSX_ERR_ARG, "bsx_cnn_embed: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int d = h->nclass;
+    return infer_chunks(h, x_dev, n, s, "bsx_cnn_embed", [&](const float* lg, int64_t b0, int nb) {
+        hipLaunchKernelGGL(k_unit_rows, dim3(nblk(nb)), dim3(256), 0, s, lg, (long)nb, d,
+                           emb_dev + b0 * d, (float*)nullptr);
+        BSX_HIP(hipGetLastError());
+        return BSX_OK;
+    });
+}
+
+int bsx_cnn_match_topk(bsx_cnn* h, const float* p_dev, int64_t n, const float* r_dev, int32_t m,
+                       int32_t k, float tau, int32_t* cls_dev, float* prob_dev, void* stream) {
+    BSX_REQUIRE(h && (n == 0 || (p_dev && r_dev && cls_dev && prob_dev)), BSX_ERR_ARG,
+                "bsx_cnn_match_topk: null argument");
+    BSX_REQUIRE(m >= 1 && k >= 1 && k <= 32 && k <= m, BSX_ERR_ARG,
+                "bsx_cnn_match_topk: k %d / references %d out of range", k, m);
+    BSX_REQUIRE(tau > 0 && std::isfinite(tau), BSX_ERR_ARG, "bsx_cnn_match_topk: bad temperature");
+    if (n == 0) return BSX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int d = h->nclass;
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(n, (256ll << 20) / (4ll * m)));
+    if (chunk * m > h->match_cap) {
+        if (h->d_match) (void)hipFree(h->d_match);
+        h->d_match = nullptr;
+        h->match_cap = 0;
+        BSX_HIP(hipMalloc(&h->d_match, sizeof(float) * chunk * m));
+        h->match_cap = chunk * m;
     }
     for (int64_t b0 = 0; b0 < n; b0 += chunk) {
         const int nb = (int)std::min<int64_t>(chunk, n - b0);
-        if (!mfma) {
-            int rc = forward<uint16_t>(h, x_dev + b0 * in, nb, h->d_pb, s);
-            if (rc) return rc;
-        } else {
-            const uint16_t* P = h->d_pb;
-            const long pos1 = (long)nb * in * (h->ch[1] / 8);
-            hipLaunchKernelGGL(k_conv1_nhwc, dim3(nblk(pos1)), dim3(256),
-                               sizeof(float) * 10 * h->ch[1], s, x_dev + b0 * in, P + h->woff[0],
-                               P + h->boff[0], nb, h->S, h->ch[1], h->d_ib[1]);
-            BSX_HIP(hipGetLastError());
-            for (int l = 2; l <= 4; ++l) {
-                const long M = (long)nb * h->hw[l] * h->hw[l];
-                dim3 g((unsigned)((M + IM - 1) / IM), (unsigned)((h->ch[l] + IN - 1) / IN));
-                hipLaunchKernelGGL((k_conv_mfma<false>), g, dim3(256), 0, s, h->d_ib[l - 1], nb,
-                                   h->hw[l - 1], h->hw[l - 1], h->ch[l - 1], 3, 2, h->hw[l], h->hw[l],
-                                   h->d_wi[l - 1], P + h->boff[l - 1], h->ch[l], (long)h->ch[l],
-                                   (void*)h->d_ib[l]);
-                BSX_HIP(hipGetLastError());
-            }
-            dim3 g((unsigned)((nb + IM - 1) / IM), (unsigned)((h->nclass + IN - 1) / IN));
-            hipLaunchKernelGGL((k_conv_mfma<true>), g, dim3(256), 0, s, h->d_ib[4], nb, 1, 1,
-                               (int)h->feat, 1, 1, 1, 1, h->d_wi[4], P + h->boff[4], h->nclass,
-                               (long)h->nclass, (void*)h->d_ilog);
-            BSX_HIP(hipGetLastError());
-        }
+        // scores = P R^T / tau: the rows' softmax over the references
+        dim3 g((m + 63) / 64, (nb + 63) / 64);
+        hipLaunchKernelGGL((k_sgemm<float>), g, dim3(256), 0, s, nb, m, d, p_dev + b0 * d, (long)d,
+                           1l, r_dev, 1l, (long)d, (const float*)nullptr, h->d_match, (long)m);
+        BSX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_scale, dim3(nblk((long)nb * m)), dim3(256), 0, s, h->d_match,
+                           (long)nb * m, 1.f / tau);
+        BSX_HIP(hipGetLastError());
         if (k <= 16)
-            hipLaunchKernelGGL(k_row_topk<16>, dim3(nb), dim3(256), 0, s,
-                               mfma ? h->d_ilog : h->d_act[0], h->nclass, k, cls_dev + b0 * k,
-                               prob_dev + b0 * k);
+            hipLaunchKernelGGL(k_row_topk<16>, dim3(nb), dim3(256), 0, s, h->d_match, m, k,
+                               cls_dev + b0 * k, prob_dev + b0 * k);
         else
-            hipLaunchKernelGGL(k_row_topk<32>, dim3(nb), dim3(256), 0, s,
-                               mfma ? h->d_ilog : h->d_act[0], h->nclass, k, cls_dev + b0 * k,
-                               prob_dev + b0 * k);
+            hipLaunchKernelGGL(k_row_topk<32>, dim3(nb), dim3(256), 0, s, h->d_match, m, k,
+                               cls_dev + b0 * k, prob_dev + b0 * k);
         BSX_HIP(hipGetLastError());
     }
+    return BSX_OK;
+}
+
+int bsx_cnn_pair_gradients(bsx_cnn* h, const float* x_dev, int32_t batch, float tau,
+                           float* grad_dev, float* loss_dev, void* stream) {
+    BSX_REQUIRE(h && x_dev, BSX_ERR_ARG, "bsx_cnn_pair_gradients: null argument");
+    BSX_REQUIRE(batch >= 2 && 2 * batch <= h->maxb, BSX_ERR_ARG,
+                "bsx_cnn_pair_gradients: batch %d out of 2..%d (pairs of the network's batch)",
+                batch, h->maxb / 2);
+    BSX_REQUIRE(tau > 0 && std::isfinite(tau), BSX_ERR_ARG,
+                "bsx_cnn_pair_gradients: bad temperature");
+    hipStream_t s = (hipStream_t)stream;
+    const int B = batch, B2 = 2 * batch, d = h->nclass;
+    // scratch: U (2B d), dU (2B d), norms (2B), S, P, Q, dS (B^2 each), losses (2B)
+    const long need = 2l * B2 * d + 2l * B2 + 4l * B * B;
+    if (need > h->nce_cap) {
+        if (h->d_nce) (void)hipFree(h->d_nce);
+        h->d_nce = nullptr;
+        h->nce_cap = 0;
+        BSX_HIP(hipMalloc(&h->d_nce, sizeof(float) * need));
+        h->nce_cap = need;
+    }
+    float* U = h->d_nce;
+    float* dU = U + (long)B2 * d;
+    float* nrm = dU + (long)B2 * d;
+    float* lrl = nrm + B2;                // row losses (B), column losses (B)
+    float* S = lrl + B2;
+    float* P = S + (long)B * B;
+    float* Q = P + (long)B * B;
+    float* dS = Q + (long)B * B;
+    int rc = forward<float>(h, x_dev, B2, h->d_p, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_unit_rows, dim3(nblk(B2)), dim3(256), 0, s, h->d_act[0], (long)B2, d, U, nrm);
+    BSX_HIP(hipGetLastError());
+    const float* Up = U;
+    const float* Ur = U + (long)B * d;
+    // S = Up Ur^T (cosines)
+    hipLaunchKernelGGL((k_sgemm<float>), dim3((B + 63) / 64, (B + 63) / 64), dim3(256), 0, s, B, B, d,
+                       Up, (long)d, 1l, Ur, 1l, (long)d, (const float*)nullptr, S, (long)B);
+    BSX_HIP(hipGetLastError());
+    const float it = 1.f / tau;
+    hipLaunchKernelGGL(k_pair_softmax, dim3(B), dim3(256), 0, s, S, B, it, 0, P, lrl);
+    hipLaunchKernelGGL(k_pair_softmax, dim3(B), dim3(256), 0, s, S, B, it, 1, Q, lrl + B);
+    hipLaunchKernelGGL(k_pair_grad, dim3(nblk((long)B * B)), dim3(256), 0, s, P, Q, B, it, lrl,
+                       lrl + B, dS, h->d_loss);
+    BSX_HIP(hipGetLastError());
+    // dUp = dS Ur, dUr = dS^T Up
+    dim3 gd((d + 63) / 64, (B + 63) / 64);
+    hipLaunchKernelGGL((k_sgemm<float>), gd, dim3(256), 0, s, B, d, B, dS, (long)B, 1l, Ur, (long)d,
+                       1l, (const float*)nullptr, dU, (long)d);
+    hipLaunchKernelGGL((k_sgemm<float>), gd, dim3(256), 0, s, B, d, B, dS, 1l, (long)B, Up, (long)d,
+                       1l, (const float*)nullptr, dU + (long)B * d, (long)d);
+    BSX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_unit_rows_bwd, dim3(nblk(B2)), dim3(256), 0, s, dU, U, nrm, (long)B2, d,
+                       h->d_gact[0]);
+    BSX_HIP(hipGetLastError());
+    rc = backward_body(h, x_dev, B2, s);
+    if (rc) return rc;
+    if (grad_dev)
+        BSX_HIP(hipMemcpyAsync(grad_dev, h->d_g, sizeof(float) * h->nparam, hipMemcpyDeviceToDevice, s));
+    if (loss_dev)
+        BSX_HIP(hipMemcpyAsync(loss_dev, h->d_loss, sizeof(float) * B, hipMemcpyDeviceToDevice, s));
+    return BSX_OK;
+}
+
+int bsx_cnn_train_pairs(bsx_cnn* h, const float* x_dev, int32_t batch, float tau, float lr,
+                        float weight_decay, float* loss_dev, void* stream) {
+    BSX_REQUIRE(lr > 0 && std::isfinite(lr) && weight_decay >= 0, BSX_ERR_ARG,
+                "bsx_cnn_train_pairs: bad learning rate / weight decay");
+    int rc = bsx_cnn_pair_gradients(h, x_dev, batch, tau, nullptr, loss_dev, stream);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    h->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)h->step), c2 = 1.f - powf(b2, (float)h->step);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(h->nparam)), dim3(256), 0, s, h->d_p, h->d_g, h->d_m, h->d_v,
+                       h->nparam, lr, b1, b2, 1e-8f, c1, c2, weight_decay);
+    BSX_HIP(hipGetLastError());
+    h->pb_valid = false;
     return BSX_OK;
 }
 

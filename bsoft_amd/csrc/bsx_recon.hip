@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_rows(const float* __restrict__ imgs,
     constexpr int G = 256 / T;
     constexpr int NT = G * T;
     constexpr int HX = FT / 2 + 1;
-    __shared__ float2 lds[LINES * FT + FT];
+    __shared__ __attribute__((aligned(16))) float2 lds[LINES * FT + FT];
     float2* tw = lds + LINES * FT;
     const int tid = threadIdx.x;
     const int p = blockIdx.x / tiles_per_part;
@@ -203,6 +203,31 @@ __global__ __launch_bounds__(256) void k_rows(const float* __restrict__ imgs,
             }
             lds[j * FT + x] = make_float2(a, c);
         }
+    } else if ((box & 3) == 0) {
+        // rows y0 + 2j and y0 + 2j + 1 as float4 loads, written as four
+        // (a, c) pairs; x in [box, FT) zero
+        const int q4 = box >> 2;
+        for (int idx = tid; idx < LINES * q4; idx += NT) {
+            const int j = idx / q4, x4 = idx - j * q4;
+            const int ya = y0 + 2 * j, yb = ya + 1;
+            float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
+            if (ya < box) {
+                a = *(const float4*)(img + (size_t)ya * box + 4 * x4);
+                a.x -= b; a.y -= b; a.z -= b; a.w -= b;
+            }
+            if (yb < box) {
+                c = *(const float4*)(img + (size_t)yb * box + 4 * x4);
+                c.x -= b; c.y -= b; c.z -= b; c.w -= b;
+            }
+            float4* d = (float4*)(lds + j * FT + 4 * x4);
+            d[0] = make_float4(a.x, c.x, a.y, c.y);
+            d[1] = make_float4(a.z, c.z, a.w, c.w);
+        }
+        const int zq = (FT - box) >> 1;   // float4 (two pairs) per line beyond the image
+        for (int idx = tid; idx < LINES * zq; idx += NT) {
+            const int j = idx / zq, x2 = idx - j * zq;
+            *(float4*)(lds + j * FT + box + 2 * x2) = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     } else {
         for (int idx = tid; idx < LINES * FT; idx += NT) {
             const int j = idx / FT, x = idx - j * FT;
@@ -222,6 +247,20 @@ __global__ __launch_bounds__(256) void k_rows(const float* __restrict__ imgs,
     // store: for each x in [0, FT/2], the 2*LINES consecutive rows
     constexpr int ROWS = 2 * LINES;
     float2* out = inter + (size_t)p * HX * box;
+    if ((box & 1) == 0) {
+        // rows y0 + 2j (A) and y0 + 2j + 1 (B) of column x as one 16-byte store
+        for (int idx = tid; idx < HX * LINES; idx += NT) {
+            const int x = idx / LINES, j = idx - x * LINES;
+            const int y = y0 + 2 * j;
+            if (y >= box) continue;
+            const float2 z = lds[j * FT + x];
+            const float2 zm = lds[j * FT + ((FT - x) & (FT - 1))];
+            *(float4*)(out + (size_t)x * box + y) =
+                make_float4(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y),
+                            0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+        }
+        return;
+    }
     for (int idx = tid; idx < HX * ROWS; idx += NT) {
         const int x = idx / ROWS, r = idx - x * ROWS;
         const int y = y0 + r;

@@ -173,14 +173,84 @@ class ProposalNet:
         return hist
 
 
+class EmbeddingNet(ProposalNet):
+    """The same network read as an embedding of `dim` values (bsx_cnn_embed):
+    trained contrastively on (particle, clean projection) pairs of many maps
+    (bsx_cnn_train_pairs), it proposes orientations for a map it has not seen
+    by comparing each particle with that map's own class projections."""
+
+    def __init__(self, size: int, dim: int = 64, channels=(32, 64, 128, 128),
+                 max_batch: int = 512, seed: int = 1, device="cuda"):
+        super().__init__(size, dim, channels, max_batch, seed, device)
+        self.dim = dim
+
+    def embed(self, x: torch.Tensor) -> torch.Tensor:
+        x = x.contiguous()
+        out = torch.empty((len(x), self.dim), dtype=torch.float32, device=self.device)
+        if len(x):
+            call("bsx_cnn_embed", self.h, _ptr(x), len(x), _ptr(out), _stream())
+        return out
+
+    def train_pairs(self, xp: torch.Tensor, xr: torch.Tensor, tau: float, lr: float,
+                    weight_decay: float = 0.0) -> torch.Tensor:
+        x = torch.cat([xp, xr]).contiguous()
+        loss = torch.empty(len(xp), dtype=torch.float32, device=self.device)
+        call("bsx_cnn_train_pairs", self.h, _ptr(x), len(xp), float(tau), float(lr),
+             float(weight_decay), _ptr(loss), _stream())
+        return loss
+
+    def pair_gradients(self, xp: torch.Tensor, xr: torch.Tensor, tau: float):
+        """(flat gradient, per-pair losses) of the contrastive loss, no update."""
+        x = torch.cat([xp, xr]).contiguous()
+        g = torch.empty(self.nparams, dtype=torch.float32, device=self.device)
+        loss = torch.empty(len(xp), dtype=torch.float32, device=self.device)
+        call("bsx_cnn_pair_gradients", self.h, _ptr(x), len(xp), float(tau), _ptr(g), _ptr(loss),
+             _stream())
+        return g, loss
+
+    def match_topk(self, p: torch.Tensor, r: torch.Tensor, k: int, tau: float):
+        """The k best references (rows of r) of each particle embedding (rows of p)."""
+        p, r = p.contiguous(), r.contiguous()
+        cls = torch.empty((len(p), k), dtype=torch.int32, device=self.device)
+        prob = torch.empty((len(p), k), dtype=torch.float32, device=self.device)
+        if len(p):
+            call("bsx_cnn_match_topk", self.h, _ptr(p), len(p), _ptr(r), len(r), int(k), float(tau),
+                 _ptr(cls), _ptr(prob), _stream())
+        return cls, prob
+
+    def fit_pairs(self, xp: torch.Tensor, xr: torch.Tensor, epochs: int = 4, batch: int = 128,
+                  lr: float = 2e-3, tau: float = 0.1, weight_decay: float = 0.0, seed: int = 0,
+                  log=None):
+        """Adam over shuffled pair minibatches, cosine-decayed learning rate."""
+        n = len(xp)
+        g = torch.Generator(device="cpu")
+        g.manual_seed(seed)
+        steps = epochs * (n // batch)
+        it = 0
+        hist = []
+        for ep in range(epochs):
+            perm = torch.randperm(n, generator=g).to(self.device)
+            tot = torch.zeros((), device=self.device)
+            for s in range(0, n - batch + 1, batch):
+                idx = perm[s:s + batch]
+                cur = lr * 0.5 * (1 + math.cos(math.pi * it / max(1, steps)))
+                tot += self.train_pairs(xp[idx], xr[idx], tau, cur, weight_decay).sum()
+                it += 1
+            hist.append(tot.item() / max(1, (n // batch) * batch))
+            if log:
+                log(f"epoch {ep + 1}/{epochs}: mean pair loss {hist[-1]:.4f}")
+        return hist
+
+
 # ------------------------------------------------------------ training data
 def extracted_projections(ft: torch.Tensor, box: int, mats: np.ndarray, rng: np.random.Generator,
                           *, pix=1.5, volt=300e3, cs=2.7e7, amp_contrast=0.07,
-                          defocus=(1e4, 3e4), snr=0.1, kernel_width=4, chunk=2048):
+                          defocus=(1e4, 3e4), snr=0.1, kernel_width=4, chunk=2048, ctf=True):
     """Particle images made from central sections that the library extracts
     from the map transform (bsx_extract, FSI kernel), with a per-particle CTF
-    (sin chi) and white noise of the given SNR. Returns (images (n, box, box)
-    float32 on the device, PARTICLE_DTYPE records)."""
+    (sin chi, unless ctf is False) and white noise of the given SNR (none when
+    snr is None). Returns (images (n, box, box) float32 on the device,
+    PARTICLE_DTYPE records)."""
     n = len(mats)
     parts = np.zeros(n, dtype=synth.PARTICLE_DTYPE)
     parts["ori"] = box // 2
@@ -201,12 +271,17 @@ def extracted_projections(ft: torch.Tensor, box: int, mats: np.ndarray, rng: np.
     for s in range(0, n, chunk):
         e = min(n, s + chunk)
         sec = central_sections(ft, mats[s:e].reshape(-1, 9), pix, 2 * pix, kernel_width, 2, True)
-        dfa = torch.as_tensor(parts["def_avg"][s:e], device=dev, dtype=torch.float64)[:, None, None]
-        zero = torch.zeros_like(dfa)
-        ctf = synth.ctf_sin_chi(hh, kk, box, pix, volt, cs, math.asin(amp_contrast), dfa, zero, zero)
-        sec = (sec * ctf.to(torch.float32)).contiguous()
+        if ctf:
+            dfa = torch.as_tensor(parts["def_avg"][s:e], device=dev, dtype=torch.float64)[:, None, None]
+            zero = torch.zeros_like(dfa)
+            c = synth.ctf_sin_chi(hh, kk, box, pix, volt, cs, math.asin(amp_contrast), dfa, zero, zero)
+            sec = sec * c.to(torch.float32)
+        sec = sec.contiguous()
         fft(sec, 2, +1, 0)
         img = sec.real.contiguous()
+        if snr is None:
+            imgs[s:e] = img
+            continue
         sd = img.reshape(e - s, -1).std(dim=1)
         noise = torch.randn(img.shape, generator=gen, device=dev, dtype=torch.float32)
         imgs[s:e] = img + noise * (sd / math.sqrt(snr))[:, None, None]
@@ -229,6 +304,45 @@ def training_set(ft: torch.Tensor, box: int, n: int, classes: OrientationClasses
     return xs, torch.as_tensor(labels, dtype=torch.int32, device=ft.device), mats
 
 
+def pair_set(ft: torch.Tensor, box: int, n: int, s_part: Search, s_ref: Search, size: int,
+             rng: np.random.Generator, chunk: int = 8192, **kw):
+    """n (particle, reference) network-input pairs of one map at random
+    orientations: the particle with CTF and noise staged through s_part (a
+    handle with the CTF flag: its input is CTF-flipped), the reference the
+    clean projection at the same orientation staged through s_ref (no CTF)."""
+    views = synth.uniform_views(n, rng)
+    mats = view_matrices(views)
+    xp = torch.empty((n, size, size), dtype=torch.float32, device=ft.device)
+    xr = torch.empty_like(xp)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        imgs, parts = extracted_projections(ft, box, mats[a:b], rng, **kw)
+        s_part.stage(imgs, parts)
+        xp[a:b] = s_part.cnn_input(size)
+        kw_ref = dict(kw)
+        kw_ref.update(snr=None, ctf=False)
+        imgs, parts = extracted_projections(ft, box, mats[a:b], rng, **kw_ref)
+        s_ref.stage(imgs, parts)
+        xr[a:b] = s_ref.cnn_input(size)
+    return xp, xr, mats
+
+
+def class_references(net: "EmbeddingNet", ft: torch.Tensor, box: int, classes: OrientationClasses,
+                     s_ref: Search, size: int, chunk: int = 8192, **kw) -> torch.Tensor:
+    """Embeddings of a map's clean projections at every class rotation
+    (classes.n x dim): what a particle of that map is compared with."""
+    rng = np.random.default_rng(0)
+    out = []
+    kw = dict(kw)
+    kw.update(snr=None, ctf=False)
+    for a in range(0, classes.n, chunk):
+        b = min(classes.n, a + chunk)
+        imgs, parts = extracted_projections(ft, box, classes.mats[a:b], rng, **kw)
+        s_ref.stage(imgs, parts)
+        out.append(net.embed(s_ref.cnn_input(size)))
+    return torch.cat(out)
+
+
 # ------------------------------------------------------------ pipeline
 @dataclass
 class LearnedStats:
@@ -238,17 +352,24 @@ class LearnedStats:
 
 
 def learned_search(s: Search, net: ProposalNet, *, k: int = 16, min_prob: float = 0.0,
-                   steps_deg=(2.0, 1.0), timer=None):
-    """Learned pipeline on the staged particles: CNN top-k classes -> BF16
-    GEMM against the member views of those classes only (routed, grouped) ->
-    FP32 FRC of the best candidates -> local refinement. Particles whose top
-    class probability is below min_prob also get the exhaustive candidates.
-    The classes' member table must be registered with s.set_classes."""
+                   steps_deg=(2.0, 1.0), timer=None, refs: torch.Tensor | None = None,
+                   tau: float = 0.1):
+    """Learned pipeline on the staged particles: top-k classes from the
+    network -> BF16 GEMM against the member views of those classes only
+    (routed, grouped) -> FP32 FRC of the best candidates -> local refinement.
+    The classes come from the classifier's softmax, or, with `refs` (the
+    class embeddings of the map, class_references), from the embedding
+    network's cosine matches. Particles whose top probability is below
+    min_prob also get the exhaustive candidates. The classes' member table
+    must be registered with s.set_classes."""
     st = LearnedStats()
     t = timer or (lambda name: None)
     x = s.cnn_input(net.size)
     t("cnn_input")
-    cls, prob = net.topk(x, k)
+    if refs is None:
+        cls, prob = net.topk(x, k)
+    else:
+        cls, prob = net.match_topk(net.embed(x), refs, k, tau)
     t("cnn")
     cand, _ = s.proposed(cls)
     t("proposed_gemm")

@@ -487,6 +487,32 @@ int bsx_cnn_train_step(bsx_cnn* h, const float* x_dev, const int32_t* labels_dev
  * probable classes per input (cls_dev n x k) and their softmax probabilities. */
 int bsx_cnn_infer_topk(bsx_cnn* h, const float* x_dev, int64_t n, int32_t k, int32_t* cls_dev,
                        float* prob_dev, void* stream);
+/* The same network as an embedding (no counterpart in the reference): its
+ * nclasses outputs are read as a d = nclasses vector, normalised to unit
+ * length. A network trained with bsx_cnn_train_pairs maps a noisy,
+ * CTF-modulated particle image and the clean projection of the same map at
+ * the same orientation to nearby unit vectors, for maps it was not trained
+ * on, so that a map's orientation classes are found by comparing a particle
+ * with the map's own projections.
+ * bsx_cnn_embed: BF16-weight inference of n inputs -> emb_dev (n x d, FP32). */
+int bsx_cnn_embed(bsx_cnn* h, const float* x_dev, int64_t n, float* emb_dev, void* stream);
+/* One Adam step of contrastive training: x_dev holds 2 * batch inputs, the
+ * particles 0..batch-1 followed by their matching references. The loss is the
+ * symmetric cross-entropy of the cosine similarities over temperature tau
+ * (pair i matches particle i with reference i, the other references of the
+ * batch are the negatives). 2 * batch <= max_batch. loss_dev (batch) may be
+ * NULL. */
+int bsx_cnn_train_pairs(bsx_cnn* h, const float* x_dev, int32_t batch, float tau, float lr,
+                        float weight_decay, float* loss_dev, void* stream);
+/* The gradients of that loss (grad_dev: the flat parameter vector's size,
+ * may be NULL) and the per-pair losses (batch, may be NULL), no update. */
+int bsx_cnn_pair_gradients(bsx_cnn* h, const float* x_dev, int32_t batch, float tau,
+                           float* grad_dev, float* loss_dev, void* stream);
+/* The k <= 32 best of m reference embeddings for each of n particle
+ * embeddings (both unit rows of d = nclasses values, device FP32), by cosine;
+ * prob_dev receives the softmax of cosine / tau over all m references. */
+int bsx_cnn_match_topk(bsx_cnn* h, const float* p_dev, int64_t n, const float* r_dev, int32_t m,
+                       int32_t k, float tau, int32_t* cls_dev, float* prob_dev, void* stream);
 
 /* asymmetric_unit_views for C1 (src/util/symmetry.cpp:383-470; flag bit 0
  * full sphere, bit 1 no in-plane alignment) then view_list_expand_angles

@@ -193,15 +193,22 @@ static void fft1_exec(const fft1* p, orr* x, orr* work, int sign)
     orr* dst = work;
     long Ns = 1;
     double vr[5], vi[5];
+    double dr[25], di[25];   /* the R-point DFT matrix of a stage */
     for (int s = 0; s < p->nf; s++) {
         int R = p->f[s];
         long L = n / R;
         long tws = n / (Ns * R);
+        for (int q = 0; q < R; q++)
+            for (int r = 0; r < R; r++) {
+                long a = (long)((q * r) % R) * (n / R);
+                dr[q * R + r] = p->tw[2 * a];
+                di[q * R + r] = sign * p->tw[2 * a + 1];
+            }
         for (long j = 0; j < L; j++) {
             long k = j % Ns;
             for (int r = 0; r < R; r++) {
                 double ar = src[2 * (j + r * L)], ai = src[2 * (j + r * L) + 1];
-                long a = (k * r * tws) % n;
+                long a = k * r * tws;            /* below n: k < Ns, r < R */
                 double cr = p->tw[2 * a], ci = sign * p->tw[2 * a + 1];
                 vr[r] = ar * cr - ai * ci;
                 vi[r] = ar * ci + ai * cr;
@@ -210,8 +217,7 @@ static void fft1_exec(const fft1* p, orr* x, orr* work, int sign)
             for (int q = 0; q < R; q++) {
                 double sr = 0, si = 0;
                 for (int r = 0; r < R; r++) {
-                    long a = ((long)q * r * (n / R)) % n;
-                    double cr = p->tw[2 * a], ci = sign * p->tw[2 * a + 1];
+                    double cr = dr[q * R + r], ci = di[q * R + r];
                     sr += vr[r] * cr - vi[r] * ci;
                     si += vr[r] * ci + vi[r] * cr;
                 }

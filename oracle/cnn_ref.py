@@ -78,18 +78,9 @@ def forward(p: np.ndarray, x: np.ndarray, S: int, ch, ncls: int, round_act=None)
     return logits, (cache, f)
 
 
-def loss_and_grad(p: np.ndarray, x: np.ndarray, labels: np.ndarray, S: int, ch, ncls: int):
-    """Mean cross-entropy, its gradient (flat, layout above) and per-sample losses."""
+def backward(p: np.ndarray, cache, f, dl: np.ndarray, S: int, ch, ncls: int) -> np.ndarray:
+    """Flat parameter gradient given dL/dlogits (B, ncls) and the forward cache."""
     off, n, feat, chans = layout(S, ch, ncls)
-    logits, (cache, f) = forward(p, x, S, ch, ncls)
-    B = len(x)
-    m = logits.max(axis=1, keepdims=True)
-    e = np.exp(logits - m)
-    se = e.sum(axis=1, keepdims=True)
-    losses = (np.log(se) + m)[:, 0] - logits[np.arange(B), labels]
-    dl = e / se
-    dl[np.arange(B), labels] -= 1
-    dl /= B
     g = np.zeros(n)
     o, ob, _ = off[4]
     Wf = p[o:ob].reshape(ncls, feat)
@@ -106,4 +97,48 @@ def loss_and_grad(p: np.ndarray, x: np.ndarray, labels: np.ndarray, S: int, ch, 
         if l > 0:
             dcols = np.einsum("oikl,bouv->bikluv", W, dz)
             da = _col2im(dcols, a_in.shape, stride)
-    return losses.mean(), g, losses
+    return g
+
+
+def loss_and_grad(p: np.ndarray, x: np.ndarray, labels: np.ndarray, S: int, ch, ncls: int):
+    """Mean cross-entropy, its gradient (flat, layout above) and per-sample losses."""
+    logits, (cache, f) = forward(p, x, S, ch, ncls)
+    B = len(x)
+    m = logits.max(axis=1, keepdims=True)
+    e = np.exp(logits - m)
+    se = e.sum(axis=1, keepdims=True)
+    losses = (np.log(se) + m)[:, 0] - logits[np.arange(B), labels]
+    dl = e / se
+    dl[np.arange(B), labels] -= 1
+    dl /= B
+    return losses.mean(), backward(p, cache, f, dl, S, ch, ncls), losses
+
+
+def pair_loss(z: np.ndarray, B: int, tau: float):
+    """Symmetric contrastive loss of 2B network outputs (particles 0..B-1,
+    their references B..2B-1): unit rows u, cosines C = u_p u_r^T, per pair
+    (row + column cross-entropy of C / tau at the matching entry) / 2.
+    Returns (per-pair losses, dL/dz for L = their mean)."""
+    nz = np.maximum(np.linalg.norm(z, axis=1, keepdims=True), 1e-12)
+    u = z / nz
+    up, ur = u[:B], u[B:]
+    Cm = up @ ur.T / tau
+    def softmax(a, axis):
+        e = np.exp(a - a.max(axis=axis, keepdims=True))
+        return e / e.sum(axis=axis, keepdims=True)
+    P = softmax(Cm, 1)
+    Q = softmax(Cm, 0)
+    idx = np.arange(B)
+    losses = 0.5 * (-np.log(P[idx, idx]) - np.log(Q[idx, idx]))
+    dC = (P + Q - 2 * np.eye(B)) / (2 * B) / tau
+    du = np.concatenate([dC @ ur, dC.T @ up])
+    dz = (du - np.sum(du * u, axis=1, keepdims=True) * u) / nz
+    return losses, dz
+
+
+def pair_loss_and_grad(p: np.ndarray, x: np.ndarray, B: int, tau: float, S: int, ch, d: int):
+    """Mean contrastive loss of the pairs (x: 2B inputs), its flat gradient and
+    the per-pair losses."""
+    z, (cache, f) = forward(p, x, S, ch, d)
+    losses, dz = pair_loss(z, B, tau)
+    return losses.mean(), backward(p, cache, f, dz, S, ch, d), losses

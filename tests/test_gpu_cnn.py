@@ -13,7 +13,8 @@ import pytest
 import torch
 
 from bsoft_amd import search, synth
-from bsoft_amd.proposal import OrientationClasses, ProposalNet, learned_search, training_set
+from bsoft_amd.proposal import (EmbeddingNet, OrientationClasses, ProposalNet, class_references,
+                                learned_search, pair_set, training_set)
 from bsoft_amd.recon import map_transform
 from oracle import cnn_ref
 
@@ -54,6 +55,46 @@ def test_forward_and_gradients_match_oracle(S, ch, ncls, B):
             err = np.abs(g[a:b] - ref).max()
             say(f"  block [{a}:{b}] max|g| {np.abs(ref).max():.3e} err {err:.3e}")
             assert err <= 1e-4 * np.abs(ref).max() + 1e-7
+
+
+@pytest.mark.parametrize("S,ch,d,B", [(16, (8, 8, 16, 8), 12, 6), (32, (16, 32, 64, 64), 64, 24)])
+def test_pair_gradients_match_oracle(S, ch, d, B):
+    """bsx_cnn_pair_gradients (contrastive loss of B particle / reference
+    pairs) against the FP64 restatement, and the embedding's unit rows."""
+    say(f"pairs S={S} ch={ch} dim={d} batch={B}")
+    net = EmbeddingNet(S, d, ch, max_batch=64, seed=4)
+    rng = np.random.default_rng(S + 1)
+    x = rng.normal(size=(2 * B, S, S)).astype(np.float32)
+    xd = torch.as_tensor(x).cuda()
+    tau = 0.2
+    g, loss = net.pair_gradients(xd[:B], xd[B:], tau)
+    p = net.params().cpu().numpy().astype(np.float64)
+    L, og, olosses = cnn_ref.pair_loss_and_grad(p, x.astype(np.float64), B, tau, S, ch, d)
+    np.testing.assert_allclose(loss.cpu().numpy(), olosses, rtol=1e-5, atol=1e-5)
+    g = g.cpu().numpy()
+    off, n, feat, _ = cnn_ref.layout(S, ch, d)
+    for o, ob, c in off:
+        for a, b in ((o, ob), (ob, ob + c)):
+            ref = og[a:b]
+            err = np.abs(g[a:b] - ref).max()
+            say(f"  block [{a}:{b}] max|g| {np.abs(ref).max():.3e} err {err:.3e}")
+            assert err <= 1e-4 * np.abs(ref).max() + 1e-7
+    # embeddings: unit rows of the (BF16-weight) outputs
+    e = net.embed(xd).cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(e, axis=1), 1.0, atol=1e-5)
+    z, _ = cnn_ref.forward(bf16_round(p), x.astype(np.float64), S, ch, d)
+    u = z / np.linalg.norm(z, axis=1, keepdims=True)
+    assert np.abs(e - u).max() <= 2e-2
+    # match top-k: the best references by cosine, probabilities a softmax over all
+    cls, prob = net.match_topk(torch.as_tensor(e[:B], dtype=torch.float32).cuda(),
+                               torch.as_tensor(e[B:], dtype=torch.float32).cuda(), 3, tau)
+    cos = e[:B] @ e[B:].T
+    for i in range(B):
+        best = np.argsort(-cos[i], kind="stable")[:3]
+        assert np.allclose(cos[i][cls[i].cpu().numpy()], cos[i][best], atol=1e-5)
+        sm = np.exp(cos[i] / tau - (cos[i] / tau).max())
+        sm /= sm.sum()
+        np.testing.assert_allclose(prob[i].cpu().numpy(), sm[cls[i].cpu().numpy()], rtol=1e-4)
 
 
 @pytest.mark.parametrize("ch", [(8, 8, 16, 8), (32, 64, 32, 64)])
@@ -134,3 +175,4 @@ def test_trained_proposal_beats_untrained():
     say(f"  learned vs exhaustive median {np.median(d):.2f} deg (exhaustive vs truth "
         f"{np.median(et):.2f}); {st.candidates} candidates/particle")
     assert np.median(d) <= 1.0
+

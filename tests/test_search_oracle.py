@@ -144,6 +144,33 @@ def test_cnn_oracle_gradient_finite_differences():
         assert g[i] == pytest.approx((lp - lm) / (2 * h), rel=1e-5, abs=1e-8)
 
 
+def test_cnn_oracle_pair_gradient_finite_differences():
+    """The contrastive loss of oracle/cnn_ref.py: its gradient equals central
+    finite differences, and on embeddings where every particle equals its
+    reference and the pairs are orthogonal the loss is the closed form
+    -log(e^(1/tau) / (e^(1/tau) + B - 1))."""
+    from oracle import cnn_ref
+    S, ch, d, B, tau = 8, (4, 4, 8, 4), 6, 3, 0.5
+    off, n, feat, chans = cnn_ref.layout(S, ch, d)
+    rng = np.random.default_rng(1)
+    p = rng.normal(size=n) * 0.4
+    x = rng.normal(size=(2 * B, S, S))
+    L, g, _ = cnn_ref.pair_loss_and_grad(p, x, B, tau, S, ch, d)
+    idx = np.concatenate([rng.choice(np.arange(o, ob + c), size=6, replace=False)
+                          for o, ob, c in off])
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(n)
+        e[i] = h
+        lp = cnn_ref.pair_loss_and_grad(p + e, x, B, tau, S, ch, d)[0]
+        lm = cnn_ref.pair_loss_and_grad(p - e, x, B, tau, S, ch, d)[0]
+        assert g[i] == pytest.approx((lp - lm) / (2 * h), rel=1e-5, abs=1e-8)
+    z = np.concatenate([np.eye(B, 5) * 3.0, np.eye(B, 5) * 0.5])   # same directions, other norms
+    losses, dz = cnn_ref.pair_loss(z, B, tau)
+    exact = -math.log(math.exp(1 / tau) / (math.exp(1 / tau) + B - 1))
+    np.testing.assert_allclose(losses, exact, rtol=1e-12)
+
+
 def test_matrix_view_round_trip():
     rng = np.random.default_rng(9)
     v = synth.uniform_views(500, rng)
