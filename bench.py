@@ -78,9 +78,14 @@ def main() -> int:
     ap.add_argument("--kernel-width", type=int, default=3, help="FSI kernel width of the search")
     ap.add_argument("--grid-step", type=float, default=4.0)
     ap.add_argument("--class-step", type=float, default=15.0)
-    ap.add_argument("--train", type=int, default=60000, help="training projections")
-    ap.add_argument("--epochs", type=int, default=6)
-    ap.add_argument("--min-prob", type=float, default=0.05)
+    ap.add_argument("--train", type=int, default=60000, help="training projections (--mixed)")
+    ap.add_argument("--train-maps", type=int, default=10,
+                    help="blob maps (other than the one searched) the proposal network learns from")
+    ap.add_argument("--train-pairs", type=int, default=6000, help="training pairs per map")
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--tau", type=float, default=0.1, help="temperature of the cosine matches")
+    ap.add_argument("--min-prob", type=float, default=0.01,
+                    help="top-class probability below which a particle also gets the exhaustive search")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -98,7 +103,7 @@ def main() -> int:
 
     import torch
     from bsoft_amd import abi, search, synth
-    from bsoft_amd.proposal import OrientationClasses, ProposalNet, learned_search, training_set
+    from bsoft_amd.proposal import OrientationClasses, class_references, learned_search
     from bsoft_amd.recon import Reconstruction, check_resolution, fsc, map_transform, part_ft_size
 
     if args.mixed:
@@ -158,12 +163,27 @@ def main() -> int:
         finish_result(args, result, ps, rec, ft, dev)
         return 0
 
-    # ---------------------------------------------- one-time set-up per map
+    # ------------------------------- one-time, map-independent: the proposal
+    # network, trained on (particle, clean projection) pairs of other maps
     setup = {}
+    S = 32
     t0 = time.perf_counter()
-    mapft = map_transform(bm.volume(device=dev))
+    net, train_info = train_embedding(args, box, pix, S, dev)
+    setup["map_independent"] = train_info
+    log(f"[bench] proposal network: {train_info}")
+
+    # the orientation classes and their member views in the library grid
+    # depend on the grid only, not on the map
+    t0 = time.perf_counter()
     views = search.views_c1(args.grid_step, angle_step_deg=args.grid_step)
     lib_mats = search.view_matrices(views)
+    classes = OrientationClasses(args.class_step)
+    members = classes.members(lib_mats)
+    setup["map_independent"]["classes_s"] = round(time.perf_counter() - t0, 2)
+
+    # ---------------------------------------------- set-up per map
+    t0 = time.perf_counter()
+    mapft = map_transform(bm.volume(device=dev))
     s = search.Search(box, args.radius, pix, kernel_width=args.kernel_width, topk=8)
     s.set_reference(mapft, lib_mats)
     torch.cuda.synchronize()
@@ -171,21 +191,13 @@ def main() -> int:
     log(f"[bench] library: {len(lib_mats)} views, {s.npoints} disk points, GEMM K {s.kdim}, "
         f"{setup['library_s']} s")
     t0 = time.perf_counter()
-    classes = OrientationClasses(args.class_step)
-    s.set_classes(classes.members(lib_mats))
-    S = 32
-    x, y, _ = training_set(mapft, box, args.train, classes, s, S, np.random.default_rng(1),
-                           snr=0.05, defocus=(8e3, 3.5e4))
+    s.set_classes(members)
+    s_ref = search.Search(box, args.radius, pix, kernel_width=args.kernel_width, topk=8, ctf=False)
+    refs = class_references(net, mapft, box, classes, s_ref, S)
     torch.cuda.synchronize()
-    setup["training_set_s"] = round(time.perf_counter() - t0, 2)
-    t0 = time.perf_counter()
-    net = ProposalNet(S, classes.n, (32, 64, 128, 128), max_batch=512, seed=1)
-    hist = net.fit(x, y, epochs=args.epochs, batch=256, lr=2e-3, log=log)
-    torch.cuda.synchronize()
-    setup["training_s"] = round(time.perf_counter() - t0, 2)
-    setup["train_loss"] = [round(h, 4) for h in hist]
+    setup["class_references_s"] = round(time.perf_counter() - t0, 2)
     setup["classes"] = classes.n
-    del x, y
+    del s_ref
     log(f"[bench] set-up: {setup}")
 
     # ---------------------------------------------- the two pipelines
@@ -211,7 +223,8 @@ def main() -> int:
             timer("refine")
         else:
             res, stats, _, _ = learned_search(s, net, k=16, min_prob=args.min_prob,
-                                              steps_deg=(2.0, 1.0), timer=timer)
+                                              steps_deg=(2.0, 1.0), timer=timer, refs=refs,
+                                              tau=args.tau)
             mats = res.mats
         found = mats.cpu().numpy()
         parts = ps.parts.copy()
@@ -306,10 +319,11 @@ def fsc_summary(f: np.ndarray, N: int, pix: float) -> dict:
 
 
 def amortised(setup: dict, P: int, T_plain: float, T_learn: float) -> dict:
-    """The learned pipeline's one-time set-up per map (library, training set,
-    network training, autotuning) spread over one or more passes."""
-    learn_setup = sum(setup.get(k, 0.0) for k in ("library_s", "training_set_s", "training_s",
-                                                  "autotune_s"))
+    """Set-up per map spread over one or more passes: library sections and
+    autotuning for both pipelines, plus the class projections' embeddings for
+    the learned one. The proposal network itself is trained once, on other
+    maps, and is not repeated per map (setup_s.map_independent)."""
+    learn_setup = sum(setup.get(k, 0.0) for k in ("library_s", "class_references_s", "autotune_s"))
     plain_setup = sum(setup.get(k, 0.0) for k in ("library_s", "autotune_s"))
     gain = T_plain - T_learn
     return {"learned_setup_s": round(learn_setup, 2), "plain_setup_s": round(plain_setup, 2),
@@ -317,6 +331,43 @@ def amortised(setup: dict, P: int, T_plain: float, T_learn: float) -> dict:
             "plain_one_pass_particles_per_s": round(P / (T_plain + plain_setup), 1),
             "passes_to_break_even": (round((learn_setup - plain_setup) / gain, 1)
                                      if gain > 0 else None)}
+
+
+def train_embedding(args, box: int, pix: float, S: int, dev, radius: int | None = None):
+    """The proposal network of the learned pipeline: trained once on (noisy
+    CTF particle, clean projection) pairs extracted by the library from
+    args.train_maps blob maps other than the one searched (seeds 1000 +),
+    SNR 0.05, defocus U[0.8, 3.5] um."""
+    import torch
+    from bsoft_amd import search, synth
+    from bsoft_amd.proposal import EmbeddingNet, pair_set
+    from bsoft_amd.recon import map_transform
+    t0 = time.perf_counter()
+    R = radius or args.radius
+    sp = search.Search(box, R, pix, kernel_width=args.kernel_width, topk=8)
+    sr = search.Search(box, R, pix, kernel_width=args.kernel_width, topk=8, ctf=False)
+    xps, xrs = [], []
+    for m in range(args.train_maps):
+        rng = np.random.default_rng(1000 + m)
+        bm = synth.make_blob_map(box, args.blobs, rng)
+        ft = map_transform(bm.volume(device=dev))
+        xp, xr, _ = pair_set(ft, box, args.train_pairs, sp, sr, S, rng, pix=pix, snr=0.05,
+                             defocus=(8e3, 3.5e4))
+        xps.append(xp)
+        xrs.append(xr)
+        del ft
+    xp, xr = torch.cat(xps), torch.cat(xrs)
+    del xps, xrs, sp, sr
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net = EmbeddingNet(S, 128, (32, 64, 128, 128), max_batch=512, seed=5)
+    hist = net.fit_pairs(xp, xr, epochs=args.epochs, batch=256, lr=2e-3, tau=args.tau, log=log)
+    torch.cuda.synchronize()
+    info = {"maps": args.train_maps, "pairs": len(xp), "pairs_s": round(t_pairs, 2),
+            "training_s": round(time.perf_counter() - t0, 2), "epochs": args.epochs,
+            "pair_loss": [round(h, 4) for h in hist], "embedding_dim": 128, "tau": args.tau}
+    return net, info
 
 
 def band_samples(ft: int, pix: float, hi_res: float, lo_res: float = 0.0) -> int:
@@ -335,10 +386,11 @@ def run_mixed(args) -> int:
     """Boxes {128, 192, 256} in a 50/30/20 ratio (default 60k particles) of one
     object (a 384-A blob model sampled at 384/box A/px), each size with its own
     map, view library and reconstruction volume; one proposal network (its
-    32x32 input is the same 24-A low-pass disk at every size) serves all."""
+    32x32 input is the same 24-A low-pass disk at every size), trained once
+    on other maps at box 128, serves all."""
     import torch
     from bsoft_amd import abi, search, synth
-    from bsoft_amd.proposal import OrientationClasses, ProposalNet, learned_search, training_set
+    from bsoft_amd.proposal import OrientationClasses, class_references, learned_search
     from bsoft_amd.recon import Reconstruction, map_transform, part_ft_size
 
     dev = torch.device("cuda:0")
@@ -368,25 +420,22 @@ def run_mixed(args) -> int:
         groups[box] = dict(ps=ps, s=s, rec=rec, mapft=mapft, pix=pix,
                            truth=synth.view_to_matrix(ps.parts["view"]))
         log(f"[bench] box {box}: {n} particles, pix {pix:.2f}, disk radius {R}, batch {rec.batch()}")
-    # one network, trained on extracted projections of every size in proportion
-    xs, ys = [], []
-    for box, frac in boxes.items():
-        g = groups[box]
-        x, y, _ = training_set(g["mapft"], box, int(args.train * frac), classes, g["s"], S,
-                               np.random.default_rng(box), pix=g["pix"], snr=0.05,
-                               defocus=(8e3, 3.5e4))
-        xs.append(x)
-        ys.append(y)
-    net = ProposalNet(S, classes.n, (32, 64, 128, 128), max_batch=512, seed=1)
-    net.fit(torch.cat(xs), torch.cat(ys), epochs=args.epochs, batch=256, lr=2e-3, log=log)
-    del xs, ys
+    # one network for every size, trained on other maps (box 128, 3 A/px)
+    net, train_info = train_embedding(args, 128, 3.0, S, dev, radius=16)
+    log(f"[bench] proposal network: {train_info}")
+    for box, g in groups.items():
+        s_ref = search.Search(box, g["s"].radius, g["pix"], kernel_width=args.kernel_width, topk=8,
+                              ctf=False)
+        g["refs"] = class_references(net, g["mapft"], box, classes, s_ref, S, pix=g["pix"])
+        del s_ref
 
     def step():
         found = {}
         for box, g in groups.items():
             s, ps = g["s"], g["ps"]
             s.stage(ps.images, ps.parts)
-            res, st, _, _ = learned_search(s, net, k=16, min_prob=args.min_prob)
+            res, st, _, _ = learned_search(s, net, k=16, min_prob=args.min_prob, refs=g["refs"],
+                                           tau=args.tau)
             m = res.mats.cpu().numpy()
             parts = ps.parts.copy()
             parts["view"] = search.matrix_views(m)
