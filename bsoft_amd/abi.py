@@ -125,6 +125,11 @@ SIGNATURES = {
                             _vp], C.c_int),
     "bsx_recon_graph_launches": ([_vp, C.POINTER(C.c_int64)], C.c_int),
     "bsx_recon_fallbacks": ([_vp, C.POINTER(C.c_int32)], C.c_int),
+    "bsx_recon_group_create": ([C.POINTER(_vp), C.c_int, C.POINTER(_vp)], C.c_int),
+    "bsx_recon_group_run": ([_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(C.c_int64),
+                             C.POINTER(_vp), _vp, _vp], C.c_int),
+    "bsx_recon_group_graph_launches": ([_vp, C.POINTER(C.c_int64)], C.c_int),
+    "bsx_recon_group_destroy": ([_vp], C.c_int),
     "bsx_rescale_to_avg_std": ([_vp, C.c_int64, C.c_double, C.c_double, _vp], C.c_int),
     "bsx_filter_sinc": ([_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _vp], C.c_int),
     "bsx_recon_set_symmetry": ([_vp, C.c_char_p], C.c_int),

@@ -977,6 +977,17 @@ __global__ __launch_bounds__(1024) void k_piece_scan(const int* __restrict__ fil
     if (tid == 0) *total += epart[0];
 }
 
+// Fills n 32-bit words with v; a kernel rather than a memset so that the
+// clears of a group run sit in its captured graph. vec: p is 16-byte aligned.
+__global__ void k_set32(unsigned* __restrict__ p, long long n, unsigned v, int vec) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long n4 = vec ? n >> 2 : 0;
+    uint4* q = reinterpret_cast<uint4*>(p);
+    for (long long i = t; i < n4; i += stride) q[i] = make_uint4(v, v, v, v);
+    for (long long i = 4 * n4 + t; i < n; i += stride) p[i] = v;
+}
+
 // Clears the bin counters and the used chunk-table entries for the next
 // batch (kernels, not memsets, so that the batch can be captured in a graph).
 __global__ void k_bin_reset(int* __restrict__ fill, int* __restrict__ chunks, int maxc, int nbins,
@@ -1454,6 +1465,11 @@ struct bsx_recon {
                                         // the legacy default stream, which cannot capture)
     std::vector<int> sym_order;         // point-group operator orders (empty: C1)
     std::vector<double> sym_axis;       // their unit axes, 3 per operator
+    // group runs (bsx_recon_group_run): weighed transform and the finish
+    // sums (coverage count, 4 FP64 sums), kept so that the finish needs no
+    // allocation inside a captured graph
+    float2* gvol = nullptr;
+    unsigned long long* gfin = nullptr;
 };
 
 // KT_FALLBACK holds the atomic-fallback flags of the insert, not a time
@@ -2064,6 +2080,8 @@ int bsx_recon_destroy(bsx_recon* h) {
         (void)hipEventDestroy(h->copied);
     }
     if (h->hrecs) (void)hipHostFree(h->hrecs);
+    if (h->gvol) (void)hipFree(h->gvol);
+    if (h->gfin) (void)hipFree(h->gfin);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     delete h;
@@ -2092,13 +2110,12 @@ int bsx_recon_device_bytes(const bsx_recon* h, int64_t* bytes) {
     return BSX_OK;
 }
 
-int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* parts, int64_t n,
-                     void* stream) {
-    BSX_REQUIRE(h, BSX_ERR_ARG, "null handle");
-    if (n == 0) return BSX_OK;
-    BSX_REQUIRE(n > 0 && imgs_dev && parts, BSX_ERR_ARG, "bsx_recon_insert: bad arguments");
-    hipStream_t s = (hipStream_t)stream;
-    BSX_HIP(hipSetDevice(h->dev));
+// Host side of an insert: validation, symmetry expansion, and the particle
+// records (insertion order, run-merge decision) uploaded to h->recs on
+// stream s. *nrec is the number of records; add[q] counts the particles of
+// half q, for the caller to add once the kernels are launched.
+static int insert_prepare(bsx_recon* h, const bsx_particle* parts, int64_t n, hipStream_t s,
+                          int64_t* nrec, int64_t add[2]) {
     {
         // validation on host threads; the lowest failing particle is reported
         std::vector<int64_t> bad(16, INT64_MAX);
@@ -2234,6 +2251,27 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
     }
     BSX_HIP(hipMemcpyAsync(h->recs, h->hrecs, sizeof(PartDev) * n, hipMemcpyHostToDevice, s));
     BSX_HIP(hipEventRecord(h->copied, s));
+    add[0] = add[1] = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (image_of.empty() || i == 0 || image_of[i] != image_of[i - 1])
+            add[h->p.nhalves > 1 ? (parts[i].half & 1) : 0]++;
+    *nrec = n;
+    return BSX_OK;
+}
+
+int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* parts, int64_t n,
+                     void* stream) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "null handle");
+    if (n == 0) return BSX_OK;
+    BSX_REQUIRE(n > 0 && imgs_dev && parts, BSX_ERR_ARG, "bsx_recon_insert: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    BSX_HIP(hipSetDevice(h->dev));
+    int64_t add[2] = {0, 0};
+    {
+        const int rp = insert_prepare(h, parts, n, s, &n, add);
+        if (rp) return rp;
+    }
+    const bool rescale = h->p.flags & BSX_FLAG_RESCALE;
     const bool check_nan = h->p.flags & BSX_FLAG_CHECK_NAN;
     if (check_nan) BSX_HIP(hipMemsetAsync(h->nonfinite, 0, sizeof(int), s));
     h->timing = h->p.flags & BSX_FLAG_TIMING;
@@ -2303,9 +2341,8 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
             h->g_piece = h->piece;
         }
     }
-    for (int64_t i = 0; i < n; ++i)
-        if (image_of.empty() || i == 0 || image_of[i] != image_of[i - 1])
-            h->inserted[h->p.nhalves > 1 ? (parts[i].half & 1) : 0]++;
+    h->inserted[0] += add[0];
+    h->inserted[1] += add[1];
     h->last_launches = nbatches;
     if (h->timing) {
         BSX_HIP(hipStreamSynchronize(s));
@@ -2854,6 +2891,57 @@ int bsx_recon_stats(bsx_recon* h, int imap, int nmaps, double resolution, double
     return BSX_OK;
 }
 
+// Weighing of the summed halves in hmask into vol (and the FOM map) with the
+// coverage count and the FP64 weight and Friedel sums accumulated into fin
+// (a coverage counter followed by 4 doubles), which the caller has cleared.
+static int launch_weigh(bsx_recon* h, int hmask, float2* vol, float* fom_dev,
+                        unsigned long long* fin, hipStream_t s) {
+    const int N = h->K.N, nz = h->K.nz;
+    const long long n3 = (long long)N * N * nz;
+    const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
+    double* dsum = (double*)(fin + 1);
+    hipLaunchKernelGGL(k_weigh, dim3(2048), dim3(256), 0, s, h->acc, n3, N, h->K.NT, nrec,
+                       h->K.stride, hmask, h->p.nhalves, h->K.fom, vol, fom_dev, fin, dsum);
+    BSX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_friedel, dim3(1024), dim3(256), 0, s, vol, N, nz, dsum + 1);
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+// Inverse transform of the weighed (centre-shifted) transform in work, which
+// it overwrites; the real part, scaled by 1/sqrt(voxels), goes to map_dev.
+static int launch_inverse(bsx_recon* h, float2* work, float* map_dev, hipStream_t s) {
+    const int N = h->K.N, nz = h->K.nz;
+    const float sc = (float)(1.0 / sqrt((double)N * N * nz));
+    int r = fft_axis_ex(work, nullptr, N, N, nz, 1, 0, +1, 1.0f, s);
+    if (nz == 1) {
+        if (r == BSX_OK) r = fft_axis_ex(work, map_dev, N, N, 1, 1, 1, +1, sc, s);
+    } else {
+        if (r == BSX_OK) r = fft_axis_ex(work, nullptr, N, N, N, 1, 1, +1, 1.0f, s);
+        if (r == BSX_OK) r = fft_axis_ex(work, map_dev, N, N, N, 1, 2, +1, sc, s);
+    }
+    return r;
+}
+
+// The stats of a finish from the coverage count and the sums of launch_weigh.
+static void fill_stats(const bsx_recon* h, int hmask, unsigned long long cov, const double* sum,
+                       bsx_stats* stats) {
+    stats->coverage = (int64_t)cov;
+    stats->weight_sum = sum[0];
+    double sw = sum[1], rr = sum[2], ri = sum[3];
+    double fr = 0;
+    if (sw > 0) {
+        rr = sqrt(rr / sw);
+        ri = sqrt(ri / sw);
+        fr = sqrt(rr * rr + ri * ri);
+    }
+    stats->friedel = fr;
+    int64_t np = 0;
+    for (int q = 0; q < h->p.nhalves; ++q)
+        if ((hmask >> q) & 1) np += h->inserted[q];
+    stats->particles = np;
+}
+
 int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* fom_dev,
                      float* ft_dev, bsx_stats* stats, void* stream) {
     BSX_REQUIRE(h, BSX_ERR_ARG, "null handle");
@@ -2884,12 +2972,10 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
     BSX_HIP(hipMalloc((void**)&dcov, sizeof(unsigned long long) + 4 * sizeof(double)));
     dsum = (double*)(dcov + 1);
     BSX_HIP(hipMemsetAsync(dcov, 0, sizeof(unsigned long long) + 4 * sizeof(double), s));
-    const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
-    hipLaunchKernelGGL(k_weigh, dim3(2048), dim3(256), 0, s, h->acc, n3, N, h->K.NT, nrec,
-                       h->K.stride, hmask, h->p.nhalves, h->K.fom, vol, fom_dev, dcov, dsum);
-    BSX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_friedel, dim3(1024), dim3(256), 0, s, vol, N, nz, dsum + 1);
-    BSX_HIP(hipGetLastError());
+    {
+        const int rw = launch_weigh(h, hmask, vol, fom_dev, dcov, s);
+        if (rw) return rw;
+    }
     unsigned long long hcov = 0;
     double hsum[4] = {0, 0, 0, 0};
     BSX_HIP(hipMemcpyAsync(&hcov, dcov, sizeof(hcov), hipMemcpyDeviceToHost, s));
@@ -2911,14 +2997,7 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
             BSX_HIP(hipMemcpyAsync(wtmp, vol, sizeof(float2) * n3, hipMemcpyDeviceToDevice, s));
             work = wtmp;
         }
-        const float sc = (float)(1.0 / sqrt((double)n3));
-        r = fft_axis_ex(work, nullptr, N, N, nz, 1, 0, +1, 1.0f, s);
-        if (nz == 1) {
-            if (r == BSX_OK) r = fft_axis_ex(work, map_dev, N, N, 1, 1, 1, +1, sc, s);
-        } else {
-            if (r == BSX_OK) r = fft_axis_ex(work, nullptr, N, N, N, 1, 1, +1, 1.0f, s);
-            if (r == BSX_OK) r = fft_axis_ex(work, map_dev, N, N, N, 1, 2, +1, sc, s);
-        }
+        r = launch_inverse(h, work, map_dev, s);
         BSX_HIP(hipStreamSynchronize(s));
         if (wtmp) BSX_HIP(hipFree(wtmp));
     }
@@ -2927,21 +3006,203 @@ int bsx_recon_finish(bsx_recon* h, int imap, int nmaps, float* map_dev, float* f
     if (tmp) BSX_HIP(hipFree(tmp));
     BSX_HIP(hipFree(dcov));
     if (r != BSX_OK) return r;
-    if (stats) {
-        stats->coverage = (int64_t)hcov;
-        stats->weight_sum = hsum[0];
-        double sw = hsum[1], rr = hsum[2], ri = hsum[3];
-        double fr = 0;
-        if (sw > 0) {
-            rr = sqrt(rr / sw);
-            ri = sqrt(ri / sw);
-            fr = sqrt(rr * rr + ri * ri);
+    if (stats) fill_stats(h, hmask, hcov, hsum, stats);
+    return BSX_OK;
+}
+
+
+// ---- group runs: several reconstructions (e.g. one per particle box) as one graph ----
+
+static void set32(void* p, long long n, unsigned v, hipStream_t s) {
+    const long long blocks = std::max(1LL, std::min(2048LL, (n / 4 + 255) / 256));
+    hipLaunchKernelGGL(k_set32, dim3((unsigned)blocks), dim3(256), 0, s, (unsigned*)p, n, v,
+                       ((uintptr_t)p & 15) == 0 ? 1 : 0);
+}
+
+// One member's part of a group run, kernels only: accumulator and bin-state
+// clears, staging of the n records uploaded by insert_prepare, the batch
+// loop, then the fold, weighing and inverse transform into map_dev.
+static int group_member_launch(bsx_recon* h, const float* imgs, int64_t n, float* map_dev,
+                               hipStream_t s) {
+    set32(h->acc, (long long)h->acc_elems, 0u, s);
+    if (h->binned) {
+        set32(h->xacc, (long long)h->acc_elems, 0u, s);
+        set32(h->fill, h->nbins, 0u, s);
+        set32(h->chunks, (long long)h->nbins * h->maxc, 0xFFFFFFFFu, s);
+        set32(h->pool, 2, 0u, s);
+        set32(h->entries, 2, 0u, s);
+    }
+    set32(h->gfin, 10, 0u, s);
+    hipLaunchKernelGGL(k_background, dim3((unsigned)n), dim3(256), 0, s, imgs, h->recs, h->p.box,
+                       h->bkg, nullptr, (h->p.flags & BSX_FLAG_RESCALE) ? h->resc : nullptr);
+    BSX_HIP(hipGetLastError());
+    const int64_t nbatches = (n + h->batch - 1) / h->batch;
+    for (int64_t b = 0; b < nbatches; ++b) {
+        const int64_t p0 = b * h->batch;
+        const int rr = run_batch(h, imgs, p0, (int)std::min<int64_t>(h->batch, n - p0), s);
+        if (rr) return rr;
+    }
+    int r = fold(h, s);
+    if (r) return r;
+    if ((r = launch_weigh(h, h->p.nhalves > 1 ? 3 : 1, h->gvol, nullptr, h->gfin, s))) return r;
+    return launch_inverse(h, h->gvol, map_dev, s);
+}
+
+struct bsx_recon_group {
+    std::vector<bsx_recon*> hs;
+    int dev = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipStream_t cap_stream = nullptr;
+    std::vector<uintptr_t> key;    // everything a captured launch depends on
+    int seen = 0;                  // identical runs seen (capture on the second)
+    int64_t graph_launches = 0;
+};
+
+int bsx_recon_group_create(bsx_recon* const* recons, int count, bsx_recon_group** out) {
+    BSX_REQUIRE(out && recons && count >= 1 && count <= 64, BSX_ERR_ARG,
+                "bsx_recon_group_create: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < count; ++i) {
+        BSX_REQUIRE(recons[i], BSX_ERR_ARG, "bsx_recon_group_create: null handle %d", i);
+        BSX_REQUIRE(recons[i]->dev == recons[0]->dev, BSX_ERR_ARG,
+                    "bsx_recon_group_create: handles on different devices");
+        for (int j = 0; j < i; ++j)
+            BSX_REQUIRE(recons[j] != recons[i], BSX_ERR_ARG,
+                        "bsx_recon_group_create: handle %d repeated", i);
+    }
+    bsx_recon_group* g = new bsx_recon_group();
+    g->hs.assign(recons, recons + count);
+    g->dev = recons[0]->dev;
+    *out = g;
+    return BSX_OK;
+}
+
+int bsx_recon_group_destroy(bsx_recon_group* g) {
+    if (!g) return BSX_OK;
+    (void)hipSetDevice(g->dev);
+    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+    if (g->cap_stream) (void)hipStreamDestroy(g->cap_stream);
+    delete g;
+    return BSX_OK;
+}
+
+int bsx_recon_group_graph_launches(const bsx_recon_group* g, int64_t* n) {
+    BSX_REQUIRE(g && n, BSX_ERR_ARG, "bsx_recon_group_graph_launches: null argument");
+    *n = g->graph_launches;
+    return BSX_OK;
+}
+
+int bsx_recon_group_run(bsx_recon_group* g, const float* const* imgs_dev,
+                        const bsx_particle* const* parts, const int64_t* counts,
+                        float* const* maps_dev, bsx_stats* stats, void* stream) {
+    BSX_REQUIRE(g && imgs_dev && parts && counts && maps_dev, BSX_ERR_ARG,
+                "bsx_recon_group_run: null argument");
+    hipStream_t s = (hipStream_t)stream;
+    BSX_HIP(hipSetDevice(g->dev));
+    const int m = (int)g->hs.size();
+    for (int i = 0; i < m; ++i) {
+        const bsx_recon* h = g->hs[i];
+        BSX_REQUIRE(counts[i] > 0 && imgs_dev[i] && parts[i] && maps_dev[i], BSX_ERR_ARG,
+                    "bsx_recon_group_run: member %d: bad arguments", i);
+        BSX_REQUIRE(!(h->p.flags & BSX_FLAG_CHECK_NAN), BSX_ERR_STATE,
+                    "bsx_recon_group_run: member %d: BSX_FLAG_CHECK_NAN needs a host check "
+                    "between staging and insertion, which a single graph cannot hold", i);
+    }
+    // finish buffers (allocated outside any capture)
+    for (bsx_recon* h : g->hs) {
+        const size_t n3 = (size_t)h->K.N * h->K.N * h->K.nz;
+        if (!h->gvol) BSX_HIP(hipMalloc((void**)&h->gvol, sizeof(float2) * n3));
+        if (!h->gfin) BSX_HIP(hipMalloc((void**)&h->gfin, 64));
+    }
+    // host side of every member's insert: records uploaded on s
+    std::vector<int64_t> nrec(m), add(2 * m);
+    for (int i = 0; i < m; ++i) {
+        bsx_recon* h = g->hs[i];
+        h->timing = false;
+        const int r = insert_prepare(h, parts[i], counts[i], s, &nrec[i], &add[2 * i]);
+        if (r) return r;
+    }
+    std::vector<uintptr_t> key;
+    key.reserve(12 * m);
+    for (int i = 0; i < m; ++i) {
+        const bsx_recon* h = g->hs[i];
+        const uintptr_t k[] = {(uintptr_t)h, (uintptr_t)imgs_dev[i], (uintptr_t)nrec[i],
+                               (uintptr_t)h->batch, (uintptr_t)h->K.merge, (uintptr_t)h->piece,
+                               (uintptr_t)maps_dev[i], (uintptr_t)h->recs, (uintptr_t)h->bkg,
+                               (uintptr_t)h->resc, (uintptr_t)h->gvol, (uintptr_t)h->gfin};
+        key.insert(key.end(), k, k + 12);
+    }
+    auto launch_all = [&](hipStream_t st) -> int {
+        for (int i = 0; i < m; ++i) {
+            const int r = group_member_launch(g->hs[i], imgs_dev[i], nrec[i], maps_dev[i], st);
+            if (r) return r;
         }
-        stats->friedel = fr;
-        int64_t np = 0;
-        for (int q = 0; q < h->p.nhalves; ++q)
-            if ((hmask >> q) & 1) np += h->inserted[q];
-        stats->particles = np;
+        return BSX_OK;
+    };
+    const bool same = key == g->key;
+    if (same && g->gexec) {
+        BSX_HIP(hipGraphLaunch(g->gexec, s));
+        g->graph_launches++;
+    } else if (same && g->seen >= 1) {
+        // second identical run: the lazily created resources (twiddle tables)
+        // exist, so the whole run is captured once and replayed from now on
+        hipGraph_t gr = nullptr;
+        if (!g->cap_stream)
+            BSX_HIP(hipStreamCreateWithFlags(&g->cap_stream, hipStreamNonBlocking));
+        // the capture stream starts after the record uploads queued on s
+        hipEvent_t up = nullptr;
+        BSX_HIP(hipEventCreateWithFlags(&up, hipEventDisableTiming));
+        BSX_HIP(hipEventRecord(up, s));
+        BSX_HIP(hipStreamWaitEvent(g->cap_stream, up, 0));
+        (void)hipEventDestroy(up);
+        BSX_HIP(hipStreamBeginCapture(g->cap_stream, hipStreamCaptureModeThreadLocal));
+        const int r = launch_all(g->cap_stream);
+        const hipError_t ce = hipStreamEndCapture(g->cap_stream, &gr);
+        if (r) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return r;
+        }
+        BSX_HIP(ce);
+        const hipError_t ie = hipGraphInstantiate(&g->gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ie != hipSuccess) g->gexec = nullptr;
+        BSX_HIP(ie);
+        BSX_HIP(hipGraphLaunch(g->gexec, s));
+        g->graph_launches++;
+    } else {
+        if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+        g->gexec = nullptr;
+        g->key = key;
+        g->seen = 1;
+        const int r = launch_all(s);
+        if (r) {
+            g->key.clear();
+            return r;
+        }
+    }
+    if (same) g->seen++;
+    // finish sums and capacity-fallback flags of every member
+    std::vector<unsigned long long> fin(5 * m);
+    std::vector<int> flags(m, 0);
+    for (int i = 0; i < m; ++i) {
+        bsx_recon* h = g->hs[i];
+        BSX_HIP(hipMemcpyAsync(&fin[5 * i], h->gfin, 5 * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s));
+        if (h->binned)
+            BSX_HIP(hipMemcpyAsync(&flags[i], h->pool + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    BSX_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < m; ++i) {
+        bsx_recon* h = g->hs[i];
+        h->inserted[0] = add[2 * i];
+        h->inserted[1] = add[2 * i + 1];
+        h->fallback_flags |= flags[i];
+        h->last_launches = (nrec[i] + h->batch - 1) / h->batch;
+        if (stats) {
+            double sum[4];
+            memcpy(sum, &fin[5 * i + 1], sizeof(sum));
+            fill_stats(h, h->p.nhalves > 1 ? 3 : 1, fin[5 * i], sum, &stats[i]);
+        }
     }
     return BSX_OK;
 }

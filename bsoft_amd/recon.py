@@ -214,6 +214,69 @@ class Reconstruction:
         return rows[: nrow.value], (float(est[0]), float(est[1]))
 
 
+class ReconGroup:
+    """Several reconstructions (one per particle box of a mixed-size data
+    set) run as one graph: clears, insertion and finish of every member
+    (bsx_recon_group_run). The reference runs breconstruct once per box
+    (breconstruct.cpp:204-349); a run here gives each member's full map."""
+
+    def __init__(self, recons: list[Reconstruction]):
+        self.recons = list(recons)
+        hs = (C.c_void_p * len(self.recons))(*[r.h for r in self.recons])
+        self.g = C.c_void_p()
+        call("bsx_recon_group_create", hs, len(self.recons), C.byref(self.g))
+
+    def close(self):
+        if self.g:
+            abi.lib().bsx_recon_group_destroy(self.g)
+            self.g = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, imgs: list[torch.Tensor], parts: list[np.ndarray],
+            maps: list[torch.Tensor] | None = None, stream=None):
+        """Reconstruct every member from its particles. Returns (maps, stats),
+        stats a list of FinishResult without map copies. Pass the same `maps`
+        (and image tensors) again for the run to replay the captured graph."""
+        m = len(self.recons)
+        if len(imgs) != m or len(parts) != m:
+            raise ValueError(f"{m} image and particle sets expected")
+        parts = [np.ascontiguousarray(p, dtype=PARTICLE_DTYPE) for p in parts]
+        if maps is None:
+            maps = []
+            for r in self.recons:
+                n = r.size
+                nz = 1 if r.params.flags & abi.BSX_FLAG_2D else n
+                maps.append(torch.empty((nz, n, n), device="cuda", dtype=torch.float32))
+        for r, im, p, mp in zip(self.recons, imgs, parts, maps):
+            b = r.params.box
+            if (len(p) == 0 or im.dtype != torch.float32 or tuple(im.shape[1:]) != (b, b)
+                    or im.shape[0] < len(p)):
+                raise ValueError(f"images must be float32 (n, {b}, {b}) with n >= 1 particles")
+            nz = 1 if r.params.flags & abi.BSX_FLAG_2D else r.size
+            if tuple(mp.shape) != (nz, r.size, r.size) or mp.dtype != torch.float32:
+                raise ValueError("map buffer of the member's size expected")
+        ip = (C.c_void_p * m)(*[_ptr(t).value for t in imgs])
+        pp = (C.c_void_p * m)(*[p.ctypes.data for p in parts])
+        cnt = (C.c_int64 * m)(*[len(p) for p in parts])
+        mp = (C.c_void_p * m)(*[_ptr(t).value for t in maps])
+        st = (Stats * m)()
+        call("bsx_recon_group_run", self.g, ip, pp, cnt, mp, C.cast(st, C.c_void_p),
+             _stream(stream))
+        res = [FinishResult(mm, None, None, s.coverage, s.friedel, s.particles, s.weight_sum)
+               for mm, s in zip(maps, st)]
+        return maps, res
+
+    def graph_launches(self) -> int:
+        n = C.c_int64()
+        call("bsx_recon_group_graph_launches", self.g, C.byref(n))
+        return n.value
+
+
 def particle_reconstruct(imgs: torch.Tensor, parts: np.ndarray, size: int, sampling: float,
                          hi_res: float = 0.0, interp_type: int = 0, ctf_action: int = 0,
                          wiener: float = 0.0, flags: int = 0, pad_factor: int = 2,

@@ -190,6 +190,31 @@ int bsx_recon_graph_launches(const bsx_recon* h, int64_t* n);
  * added with float atomics: the sums are complete, the insert was slower. */
 int bsx_recon_fallbacks(const bsx_recon* h, int32_t* flags);
 
+/* ---- group runs (no counterpart in the reference) ----
+ * Several reconstructions run as ONE hipGraph, e.g. one per particle box of a
+ * mixed-size data set, where the reference runs breconstruct once per box
+ * (breconstruct.cpp:204-349). A run of member i clears its accumulators,
+ * inserts counts[i] particles (as bsx_recon_insert), then sums all half-sets,
+ * weighs and inverse-transforms them into maps_dev[i] (size^3 FP32, size^2
+ * with BSX_FLAG_2D), as bsx_recon_finish(h, 0, 1, ...) does. The first run
+ * launches the kernels directly; the second identical run (same images,
+ * counts, outputs and tuning; the particle parameters may change) captures
+ * every member's clears, staging, batches and finish into one graph, and
+ * later identical runs replay it. Only the particle records are uploaded
+ * per run. Afterwards each member holds this run's sums, so bsx_recon_finish
+ * can still give half maps. stats (array of count, may be NULL) get each
+ * member's finishing statistics. BSX_FLAG_CHECK_NAN is refused (its host
+ * check cannot sit inside one graph); BSX_FLAG_TIMING is ignored. Destroy a
+ * group before its members. Synchronises `stream`. */
+typedef struct bsx_recon_group bsx_recon_group;
+int bsx_recon_group_create(bsx_recon* const* recons, int count, bsx_recon_group** out);
+int bsx_recon_group_run(bsx_recon_group* g, const float* const* imgs_dev,
+                        const bsx_particle* const* parts, const int64_t* counts,
+                        float* const* maps_dev, bsx_stats* stats, void* stream);
+/* Number of runs served by the group's graph so far. */
+int bsx_recon_group_graph_launches(const bsx_recon_group* g, int64_t* n);
+int bsx_recon_group_destroy(bsx_recon_group* g);
+
 /* Fourier shell statistics of a weighed reconstruction, the table that
  * fspace_reconstruction_stats (src/img/Bimage_reconstruct.cpp:526-599) prints
  * from img_reconstruction_sum_weigh (src/mg/mg_reconstruct.cpp:346-347):

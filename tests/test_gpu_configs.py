@@ -118,3 +118,74 @@ def test_projection_matching_128_full_grid(gpu):
         f"within 5 deg: {np.mean(err < 5):.3f}")
     assert np.median(gerr) <= 3.0
     assert np.median(err) <= 1.5
+
+
+def test_mixed_boxes_one_graph_parity(gpu):
+    """Mixed particle boxes (32, 48 and 64 in a 50/30/20 ratio) of one object,
+    each box reconstructed into its own volume by one group run
+    (bsx_recon_group_run): the first run launches directly, the second
+    captures clears, staging, batches and finish of all three members into
+    one graph, the third replays it, and a fourth replays it with new particle
+    parameters. Every run's map equals the FP64 oracle's on the same
+    particles (max relative voxel error 1e-4), and the sums each member keeps
+    afterwards give transforms with FSC >= 0.9999 against the oracle's."""
+    from bsoft_amd import abi
+    from bsoft_amd.abi import ReconParams
+    from bsoft_amd.recon import Reconstruction, ReconGroup, part_ft_size
+    from oracle.oracle import OracleRecon
+    rng = np.random.default_rng(71)
+    base = synth.make_blob_map(64, 20, rng)
+    total = 240
+    sets = []
+    for box, frac in ((32, 0.5), (48, 0.3), (64, 0.2)):
+        f = box / 64
+        bm = synth.BlobMap(centers=base.centers * f, sigmas=base.sigmas * f, amps=base.amps, n=box)
+        pix = 96.0 / box
+        n = int(round(total * frac))
+        ps = synth.make_particles(bm, n, box, rng, pix=pix, snr=0.2, seed=box,
+                                  astig=(0.0, 1500.0))
+        sets.append((box, pix, ps))
+    recs = [Reconstruction(box, box, part_ft_size(box, 1, 2), pix, hi_res=2 * pix,
+                           ctf_action=abi.BSX_CTF_FLIP, batch=16)
+            for box, pix, _ in sets]
+    grp = ReconGroup(recs)
+    imgs = [ps.images.cuda().contiguous() for _, _, ps in sets]
+    maps = [torch.empty((box, box, box), device="cuda") for box, _, _ in sets]
+
+    def oracle(box, pix, parts, images):
+        rp = ReconParams(size=box, box=box, ft_size=part_ft_size(box, 1, 2), interp=0,
+                         sampling=pix, hi_res=2 * pix, lo_res=0.0, ctf_action=abi.BSX_CTF_FLIP,
+                         flags=0, wiener=0.0, nhalves=1, batch=0, mem_cap=0)
+        o = OracleRecon(rp, 2, 64)
+        o.insert(synth.particles_list(parts), images)
+        return o.finish(want_ft=True)
+
+    refs = [oracle(box, pix, ps.parts, ps.images.numpy()) for box, pix, ps in sets]
+    new_parts = []
+    for box, pix, ps in sets:
+        q = ps.parts.copy()
+        q["view"] = synth.uniform_views(len(q), np.random.default_rng(box + 5))
+        new_parts.append(q)
+    new_refs = [oracle(box, pix, q, ps.images.numpy())
+                for (box, pix, ps), q in zip(sets, new_parts)]
+    for run in range(4):
+        parts = [ps.parts for _, _, ps in sets] if run < 3 else new_parts
+        want = refs if run < 3 else new_refs
+        say(f"group run {run}: boxes 32/48/64, {[len(p) for p in parts]} particles")
+        _, stats = grp.run(imgs, parts, maps)
+        torch.cuda.synchronize()
+        for (box, pix, ps), mp, st, (m, _, ftc, ost) in zip(sets, maps, stats, want):
+            gm = mp.cpu().numpy().astype(np.float64)
+            err = np.abs(gm - m).max() / np.abs(m).max()
+            say(f"  box {box}: max rel err {err:.2e}, coverage {st.coverage}/{ost.coverage}, "
+                f"particles {st.particles}")
+            assert err <= 1e-4
+            assert st.coverage == ost.coverage and st.particles == ost.particles
+            assert abs(st.weight_sum - ost.weight_sum) <= 1e-5 * ost.weight_sum
+        assert grp.graph_launches() == max(0, run)
+    for rec, (box, pix, ps), (m, _, ftc, ost) in zip(recs, sets, new_refs):
+        g = rec.finish(want_ft=True)
+        fsc, sup = Oracle(64).fsc_with_support(g.ft.cpu().numpy(), ftc, pix, 2 * pix)
+        say(f"  box {box}: member sums after the runs, min FSC {fsc[sup].min():.6f}")
+        assert fsc[sup].min() >= 0.9999
+    grp.close()
