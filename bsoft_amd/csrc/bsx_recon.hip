@@ -47,6 +47,8 @@ struct alignas(16) PartDev {
     double c0, c20, c22, c2m2, c40;   // aberration weights   ctf.h:321-335
     double fscale;      // 1 / (ft * pixel_size)              img_ctf_calculate freq_scale
     double ew;          // Ewald offset lambda/2, 0 if off
+    double nyq[2];      // cos, sin of -ft phy: the phase factor of a Nyquist-row
+                        // mate is conj(sample's) times this
     double hox, hoy;    // background-ellipse origin (image header)
     long long src;      // index of the image in the caller's stack
     float wp;           // particle weight incl. pad ratio
@@ -335,26 +337,25 @@ struct Emit {
 #define BSX_SPIN_LIMIT 4096    // polls of a chunk-table entry before the atomic fallback
 
 // Fill-pass staging in LDS (nearest / weighted-nearest insertion without
-// Ewald): a workgroup appends its merged entries to an LDS list and counts
-// them per bin in a small LDS hash table; at each flush one integer atomic
-// per (workgroup, bin) reserves the bin range (at most two chunks) and the
-// entries are written out.
-#define BSX_STAGE_CAP 1024     // entries per flush: 4 rounds x 1 binned emission x 256 threads
+// Ewald): a workgroup appends its merged entries to a dense LDS list (one
+// 16-B record each; a wavefront takes its slots with one LDS atomic) and
+// counts them per bin in a small LDS hash table; at each flush one integer
+// atomic per (workgroup, bin) reserves the bin range (at most two chunks)
+// and the entries are written out.
+#define BSX_STAGE_CAP 512      // entries per flush: 2 rounds x 1 binned emission x 256 threads
 #define BSX_STAGE_HASH 128     // bins per flush (more fall back to direct reservations)
 static_assert(BSX_STAGE_CAP <= BSX_CHUNK, "a flushed bin range must span at most two chunks");
 static_assert(BSX_STAGE_CAP <= 1024 && BSX_STAGE_HASH <= 256, "list ranks take 10 bits, slots 8");
 
 struct Stage {
-    float* re;       // entry sums, structure of arrays (12 B per entry)
-    float* im;
-    float* w;
-    int* r;          // (hash slot << 22) | (rank in the bin << 12) | record in tile; -1 = empty
+    float4* ent;     // entries: (sum w re, sum w im, sum w, r) with r =
+                     // (hash slot << 22) | (rank in the bin << 12) | record in tile,
+                     // r = -1: slot taken by an entry reserved on its own
+    int* nent;       // entries in the list
     int* hkey;       // bin per hash slot, -1 = free
     int* hcnt;       // entries per hash slot
-    int* hbase;      // reserved base per hash slot
-    int* hc0;        // pool chunk of the range start
-    int* hc1;        // pool chunk of the range end
-    int slot0;       // list slot of thread 0 in this round (one entry per thread and round)
+    int4* hinfo;     // at a flush, per hash slot: reserved base, chunks of the
+                     // range start and end, bin
     int on;
 };
 
@@ -440,13 +441,12 @@ __device__ __forceinline__ void range_chunks(const BinCtx& bc, int b, int base, 
     }
 }
 
-// One entry into the workgroup's list at the thread's slot of this round
-// (per-lane LDS atomics for the bin table: measured faster here than
-// grouping the lanes of a wavefront by bin first).
-__device__ __forceinline__ void stage_entry(const BinCtx& bc, int key, float re, float im,
+// One entry into slot i of the workgroup's list (per-lane LDS atomics for
+// the bin table: measured faster here than grouping the lanes of a
+// wavefront by bin first).
+__device__ __forceinline__ void stage_entry(const BinCtx& bc, int i, int key, float re, float im,
                                            float w) {
     const int bin = (int)((unsigned)key >> 12);
-    const int i = bc.st.slot0 + (int)threadIdx.x;
     int slot = -1;
     int hs = (int)(((unsigned)bin * 2654435761u) >> 25) & (BSX_STAGE_HASH - 1);
     for (int t = 0; t < BSX_STAGE_HASH; ++t) {
@@ -460,13 +460,25 @@ __device__ __forceinline__ void stage_entry(const BinCtx& bc, int key, float re,
     range_chunks(bc, bin, pos, 1, alone, c0, c1);
     if (alone) {
         put_entry(bc, c0, pos, re, im, w, key);
+        bc.st.ent[i] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
         return;
     }
     const int rank = atomicAdd(&bc.st.hcnt[slot], 1);
-    bc.st.re[i] = re;
-    bc.st.im[i] = im;
-    bc.st.w[i] = w;
-    bc.st.r[i] = (slot << 22) | (rank << 12) | (key & 4095);
+    bc.st.ent[i] = make_float4(re, im, w, __int_as_float((slot << 22) | (rank << 12) | (key & 4095)));
+}
+
+// The lanes with `out` set append their entries: dense slots from one LDS
+// atomic per wavefront (all lanes call this together).
+__device__ __forceinline__ void stage_append(const BinCtx& bc, bool out, int key, float re,
+                                            float im, float w) {
+    const unsigned long long m = __ballot(out);
+    if (m == 0) return;
+    const int lane = __lane_id();
+    const int leader = __builtin_ctzll(m);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(bc.st.nent, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (out) stage_entry(bc, base + __popcll(m & ((1ull << lane) - 1ull)), key, re, im, w);
 }
 
 // Workgroup-uniform: reserve every staged bin range, write the staged
@@ -483,29 +495,26 @@ __device__ __forceinline__ void stage_flush(const BinCtx& bc) {
         }
     }
     range_chunks(bc, b, base, cnt, b >= 0, c0, c1);
-    if (b >= 0) {
-        bc.st.hbase[tid] = base;
-        bc.st.hc0[tid] = c0;
-        bc.st.hc1[tid] = c1;
-    }
+    if (b >= 0) bc.st.hinfo[tid] = make_int4(base, c0, c1, b);
     __syncthreads();
-    for (int i = tid; i < BSX_STAGE_CAP; i += blockDim.x) {
-        const int pr = bc.st.r[i];
+    const int n = *bc.st.nent;
+    for (int i = tid; i < n; i += blockDim.x) {
+        const float4 e = bc.st.ent[i];
+        const int pr = __float_as_int(e.w);
         if (pr < 0) continue;
-        bc.st.r[i] = -1;
-        const int hs = pr >> 22;
-        const int hb = bc.st.hbase[hs];
-        const int pos = hb + ((pr >> 12) & 1023);
-        const int c = pos / BSX_CHUNK == hb / BSX_CHUNK ? bc.st.hc0[hs] : bc.st.hc1[hs];
+        const int4 hi = bc.st.hinfo[pr >> 22];
+        const int pos = hi.x + ((pr >> 12) & 1023);
+        const int c = pos / BSX_CHUNK == hi.x / BSX_CHUNK ? hi.y : hi.z;
         // full record key (bin from the hash slot) for the atomic fallback
-        const int key = (int)(((unsigned)bc.st.hkey[hs] << 12) | (unsigned)(pr & 4095));
-        put_entry(bc, c, pos, bc.st.re[i], bc.st.im[i], bc.st.w[i], key);
+        const int key = (int)(((unsigned)hi.w << 12) | (unsigned)(pr & 4095));
+        put_entry(bc, c, pos, e.x, e.y, e.z, key);
     }
     __syncthreads();
     if (tid < BSX_STAGE_HASH) {
         bc.st.hkey[tid] = -1;
         bc.st.hcnt[tid] = 0;
     }
+    if (tid == 0) *bc.st.nent = 0;
     __syncthreads();
 }
 
@@ -560,7 +569,7 @@ __device__ __forceinline__ void merge_step(Emit& e, int& pk, int lg) {
 // (MODE_DIRECT) or reserves a bin slot: lanes of a wavefront are grouped by
 // bin with ballots and one integer atomic per (wavefront, bin).
 template <int MG, bool FOM, int MODE, bool FAST = false>
-__device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool merge, Emit e,
+__device__ __forceinline__ void emit(float* __restrict__ acc, int stride, int merge, Emit e,
                                      const BinCtx& bc) {
     bool head = true;
     if (MG > 1 && merge) {
@@ -569,8 +578,10 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
         int pk = (e.kk * 4) | (e.bin << 1) | e.incl;
         merge_step<1, MG, FOM>(e, pk, lg);
         merge_step<2, MG, FOM>(e, pk, lg);
-        merge_step<4, MG, FOM>(e, pk, lg);
-        merge_step<8, MG, FOM>(e, pk, lg);
+        if (merge == 1) {   // runs may exceed 4 samples (merge 2: the host showed they do not)
+            merge_step<4, MG, FOM>(e, pk, lg);
+            merge_step<8, MG, FOM>(e, pk, lg);
+        }
         e.incl = pk & 1;
         const int prev_key = dpp_up1(e.key);
         const int prev_pk = dpp_up1(pk);
@@ -592,7 +603,7 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, bool m
     }
     const bool out = out0;   // lanes still here hold binned entries
     if (FAST || bc.st.on) {
-        if (out) stage_entry(bc, e.key, e.re, e.im, e.w);
+        stage_append(bc, out, e.key, e.re, e.im, e.w);
         return;
     }
     // per wavefront: one reservation per bin, all issued before any chunk
@@ -721,7 +732,7 @@ __device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev&
 template <int MG, bool FOM, int MODE, bool FAST = false>
 __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const ReconConst& K,
                                             const double m[3], bool inband, double w, float fr,
-                                            float fi, int kk, bool merge, unsigned keybase,
+                                            float fi, int kk, int merge, unsigned keybase,
                                             const BinCtx& bc, int bin = 1) {
     if (FAST || K.interp < 2) {
         const double fx = floor(m[0] + 0.5), fy = floor(m[1] + 0.5), fz = floor(m[2] + 0.5);
@@ -776,15 +787,23 @@ __device__ __forceinline__ void coords(const ReconConst& K, const PartDev& P, do
 template <int MG, bool FOM, int MODE, bool FAST = false>
 __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const ReconConst& K,
                                             const PartDev& P, bool active, bool mirror, long h,
-                                            long k, long km, float2 F, bool merge,
+                                            long k, long km, float2 F, int merge,
                                             unsigned keybase, const BinCtx& bc, float2 ph) {
     double s0, s1, s2, w;
     float fr, fi;
     const bool inband =
         sample_value<MODE>(K, P, active, h, k, F, &s0, &s1, &s2, &w, &fr, &fi, &ph);
     double m[3];
-    const double mz0 = P.ew * s2;
-    coords(K, P, s0, s1, mz0, m);
+    const double mz0 = FAST ? 0.0 : P.ew * s2;
+    if (FAST) {
+        // no Ewald offset: the z term of coords is a signed zero, which
+        // changes neither a non-zero sum nor the voxel that a zero rounds to
+        m[0] = (s0 * P.R[0] + s1 * P.R[1]) * K.mscale;
+        m[1] = (s0 * P.R[3] + s1 * P.R[4]) * K.mscale;
+        m[2] = (s0 * P.R[6] + s1 * P.R[7]) * K.mscz;
+    } else {
+        coords(K, P, s0, s1, mz0, m);
+    }
     if (!FAST && K.ewald) {   // all lanes: two emissions per frequency, mates computed in full
         interp_emit<MG, FOM, MODE, FAST>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
         coords(K, P, s0, s1, -mz0, m);
@@ -810,6 +829,18 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
         mm[0] = -m[0];
         mm[1] = -m[1];
         mm[2] = -m[2];
+    } else if (FAST) {
+        // the Nyquist row (k = km = -FT/2): the mate's phase angle is
+        // -(h phx + k phy) - FT phy, so its factor follows from the sample's
+        double t0, t1, t2;
+        const float2 pm = make_float2(
+            (float)((double)ph.x * P.nyq[0] + (double)ph.y * P.nyq[1]),
+            (float)((double)ph.x * P.nyq[1] - (double)ph.y * P.nyq[0]));
+        ib = sample_value<MODE>(K, P, mirror, -h, km, make_float2(F.x, -F.y), &t0, &t1, &t2,
+                                &tw, &gr, &gi, &pm);
+        mm[0] = (t0 * P.R[0] + t1 * P.R[1]) * K.mscale;
+        mm[1] = (t0 * P.R[3] + t1 * P.R[4]) * K.mscale;
+        mm[2] = (t0 * P.R[6] + t1 * P.R[7]) * K.mscz;
     } else {
         double t0, t1, t2;
         ib = sample_value<MODE>(K, P, mirror, -h, km, make_float2(F.x, -F.y), &t0, &t1, &t2,
@@ -833,8 +864,12 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
 }
 
 // Column tasks: task = (particle, x) with x in [0, FT/2]; G tasks per WG.
+// The nearest-neighbour specialisation (FAST) is held to 5 waves per SIMD
+// (at most 96 VGPRs): its 31.7 KB of LDS allows 5 workgroups per CU, and one
+// register more would round the allocation up to 104 and allow only 4.
 template <int FT, bool FOM, int MODE, bool FAST = false>
-__global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAST ? 5 : 1)))
+void k_cols(const float2* __restrict__ inter,
                                               const float* __restrict__ bkg,
                                               const PartDev* __restrict__ recs, int nb, int box,
                                               ReconConst K, float* __restrict__ acc,
@@ -848,9 +883,9 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     constexpr int SCAP = MODE == MODE_FILL && !FOM ? BSX_STAGE_CAP : 1;
     constexpr int SHASH = MODE == MODE_FILL && !FOM ? BSX_STAGE_HASH : 1;
     __shared__ float2 lds[LDSN];
-    __shared__ float st_re[SCAP], st_im[SCAP], st_w[SCAP];
-    __shared__ int st_r[SCAP];
-    __shared__ int st_hkey[SHASH], st_hcnt[SHASH], st_hbase[SHASH], st_hc0[SHASH], st_hc1[SHASH];
+    __shared__ float4 st_ent[SCAP];
+    __shared__ int4 st_hinfo[SHASH];
+    __shared__ int st_hkey[SHASH], st_hcnt[SHASH], st_nent;
     const int tid = threadIdx.x;
     const int g = tid / T, t = tid - g * T;
     const long task = (long)blockIdx.x * G + g;
@@ -867,17 +902,17 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         FftLine<FT, -1, T>::run(line, tw, t);
     }
     // staging (pair entries: at most one binned emission per thread and
-    // round, so 4 rounds fit the list)
+    // round, so 2 rounds fit the list; the 512-entry list keeps the LDS of a
+    // workgroup under 32 KB, 5 workgroups per CU)
     const bool staged = FAST || (MODE == MODE_FILL && !FOM && K.pairs && NT == 256 &&
-                                 4 * NT <= BSX_STAGE_CAP);
-    bc.st = Stage{st_re, st_im, st_w, st_r, st_hkey, st_hcnt, st_hbase, st_hc0, st_hc1, 0,
-                  staged ? 1 : 0};
+                                 2 * NT <= BSX_STAGE_CAP);
+    bc.st = Stage{st_ent, &st_nent, st_hkey, st_hcnt, st_hinfo, staged ? 1 : 0};
     if (staged) {
         if (tid < BSX_STAGE_HASH) {
             st_hkey[tid] = -1;
             st_hcnt[tid] = 0;
         }
-        for (int i = tid; i < BSX_STAGE_CAP; i += NT) st_r[i] = -1;
+        if (tid == 0) st_nent = 0;
         __syncthreads();
     }
     // At FT 512 a wavefront is one column (T == 64), so its particle is
@@ -889,7 +924,7 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
     const float b = valid ? bkg[p] : 0.f;
     const long h = (x <= (FT - 1) / 2) ? x : x - FT;
     const bool mirror = valid && x >= 1 && x <= FT / 2 - 1;
-    const bool merge = K.merge;
+    const int merge = K.merge;
     // phase factor of this thread's sample, advanced by exact FP64 rotations
     // (k grows by T per round and wraps by -FT after round 3, as y passes FT/2)
     double er, ei;
@@ -910,7 +945,6 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
         // Friedel mate of the column: array position (FT-x, FT-y), value conj
         const int ym = (FT - y) & (FT - 1);
         const long km = (ym <= (FT - 1) / 2) ? ym : ym - FT;
-        bc.st.slot0 = (r & 3) * NT;
         insert_pair<MG, FOM, MODE, FAST>(acc, K, P, valid, mirror, h, k, km, F, merge, keybase, bc,
                                    make_float2((float)er, (float)ei));
         {
@@ -919,7 +953,7 @@ __global__ __launch_bounds__(256) void k_cols(const float2* __restrict__ inter,
             ei = er * sd + ei * c;
             er = nr;
         }
-        if (MODE == MODE_FILL && staged && (r & 3) == 3) stage_flush(bc);
+        if (MODE == MODE_FILL && staged && (r & 1) == 1) stage_flush(bc);
     }
 }
 
@@ -1510,6 +1544,8 @@ static void make_rec(const bsx_recon* h, const bsx_particle& q, PartDev& r) {
     r.rot[1] = sin(T * r.phy);
     r.rot[2] = cos((T - ft) * r.phy);
     r.rot[3] = sin((T - ft) * r.phy);
+    r.nyq[0] = cos(-ft * r.phy);
+    r.nyq[1] = sin(-ft * r.phy);
     double wl = lambda_of(q.volt);
     double t1 = M_PI_2 * wl * wl * wl, t2 = -M_PI * wl, dev = fabs(q.def_dev);
     r.c0 = -q.amp_shift;
@@ -2229,25 +2265,39 @@ static int insert_prepare(bsx_recon* h, const bsx_particle* parts, int64_t n, hi
                          [&](int64_t a, int64_t b) { return key[a] < key[b]; });
     }
     double max_step = 0;
+    int max_run = 0;
     {
         std::vector<double> tmax(16, 0.0);
+        std::vector<int> trun(16, 0);
         host_parallel(n, 8192, [&](int64_t b, int64_t e, unsigned t) {
             double m = 0;
+            int run = 0;
             for (int64_t i = b; i < e; ++i) {
-                make_rec(h, parts[perm[i]], h->hrecs[i]);
-                h->hrecs[i].src = image_of.empty() ? perm[i] : image_of[perm[i]];
-                m = std::max(m, h->hrecs[i].invscale * h->K.mscale);
+                PartDev& q = h->hrecs[i];
+                make_rec(h, parts[perm[i]], q);
+                q.src = image_of.empty() ? perm[i] : image_of[perm[i]];
+                m = std::max(m, q.invscale * h->K.mscale);
+                // a run of one voxel along a column: one coordinate moves by
+                // a >= the largest step per sample row, so it spans at most
+                // floor(1 / a) + 1 rows
+                const double a = q.invscale * std::max(std::max(fabs(q.R[1]), fabs(q.R[4])) *
+                                                           h->K.mscale,
+                                                       fabs(q.R[7]) * h->K.mscz);
+                run = std::max(run, a > 1e-3 ? (int)std::floor(1.0 / a + 1e-6) + 1 : 1 << 20);
             }
             tmax[t] = m;
+            trun[t] = run;
         });
         for (double m : tmax) max_step = std::max(max_step, m);
+        for (int r : trun) max_run = std::max(max_run, r);
     }
     // run merge: equal voxels are contiguous within a merge group of up to 16
-    // consecutive samples when 15 steps stay shorter than N - 1 voxels
+    // consecutive samples when 15 steps stay shorter than N - 1 voxels; runs
+    // of at most 4 samples need only the first two merge steps
     {
         const int T = h->p.ft_size / 8, MG = T < 16 ? T : 16;
         const bool safe = !h->K.ewald && (MG - 1) * max_step < h->K.N - 1;
-        h->K.merge = (h->p.merge != 2) && safe ? 1 : 0;
+        h->K.merge = (h->p.merge != 2) && safe ? (max_run <= 4 ? 2 : 1) : 0;
     }
     BSX_HIP(hipMemcpyAsync(h->recs, h->hrecs, sizeof(PartDev) * n, hipMemcpyHostToDevice, s));
     BSX_HIP(hipEventRecord(h->copied, s));
