@@ -6,7 +6,10 @@
 * batched projection matching at 128^3: 20,000 particles of 128^2 (SNR 0.1)
   against the 4-degree grid with 90 in-plane angles (230,940 views), BF16
   GEMM top-8 then FP32 FRC rescoring, checked against the FP64 scoring on a
-  subset, against ground truth, and by properties over every particle.
+  subset, against ground truth, and by properties over every particle;
+* mixed particle boxes reconstructed as one graph (bsx_recon_group_run)
+  against the FP64 oracle, and group members with half-sets or a 2D map
+  against the handles' own insert + finish.
 """
 import sys
 
@@ -188,4 +191,54 @@ def test_mixed_boxes_one_graph_parity(gpu):
         fsc, sup = Oracle(64).fsc_with_support(g.ft.cpu().numpy(), ftc, pix, 2 * pix)
         say(f"  box {box}: member sums after the runs, min FSC {fsc[sup].min():.6f}")
         assert fsc[sup].min() >= 0.9999
+    grp.close()
+
+
+def test_group_run_half_sets_and_2d_member(gpu):
+    """A group of a 3D member with two half-sets and a 2D member
+    (BSX_FLAG_2D): every group run (direct, captured, replayed) gives the
+    maps of each handle's own insert + finish on the same particles, and the
+    half maps read from the sums a run leaves (bsx_recon_finish, nmaps 2)
+    equal the direct path's half maps."""
+    from bsoft_amd import abi
+    from bsoft_amd.recon import Reconstruction, ReconGroup, part_ft_size
+    rng = np.random.default_rng(83)
+    bm = synth.make_blob_map(32, 12, rng)
+    ps = synth.make_particles(bm, 60, 32, rng, pix=3.0, snr=0.3, seed=5)
+    ps.parts["half"] = np.arange(len(ps.parts)) % 2
+    ft = part_ft_size(32, 1, 2)
+    kinds = [dict(nhalves=2, flags=0), dict(nhalves=1, flags=abi.BSX_FLAG_2D)]
+
+    def make(kind):
+        return Reconstruction(32, 32, ft, 3.0, hi_res=6.0, ctf_action=abi.BSX_CTF_FLIP, batch=16,
+                              **kind)
+
+    imgs = ps.images.cuda().contiguous()
+    direct = []
+    for kind in kinds:
+        r = make(kind)
+        r.insert(imgs, ps.parts)
+        full = r.finish(0, 1).map.cpu().numpy()
+        halves = [r.finish(i, 2).map.cpu().numpy() for i in range(2)] if kind["nhalves"] == 2 else []
+        direct.append((full, halves))
+        r.close()
+    recs = [make(kind) for kind in kinds]
+    grp = ReconGroup(recs)
+    maps = [torch.empty((32, 32, 32), device="cuda"), torch.empty((1, 32, 32), device="cuda")]
+    for run in range(3):
+        say(f"group run {run}: 3D half sets + 2D member, 60 particles each")
+        _, stats = grp.run([imgs, imgs], [ps.parts, ps.parts], maps)
+        torch.cuda.synchronize()
+        for m, st, (full, _) in zip(maps, stats, direct):
+            g = m.cpu().numpy()
+            err = np.abs(g - full).max() / np.abs(full).max()
+            say(f"  shape {tuple(g.shape)}: max rel err vs direct {err:.2e}, particles {st.particles}")
+            assert err <= 1e-5 and st.particles == len(ps.parts)
+        assert grp.graph_launches() == run
+    for i in range(2):
+        h = recs[0].finish(i, 2).map.cpu().numpy()
+        ref = direct[0][1][i]
+        err = np.abs(h - ref).max() / np.abs(ref).max()
+        say(f"  half {i} after the group runs: max rel err {err:.2e}")
+        assert err <= 1e-5
     grp.close()
