@@ -547,7 +547,7 @@ def pipeline_roofline(out, s, P, args, recon_roof):
     refine_bytes = 2 * 27 * P * s.npoints * args.kernel_width ** 3 * 8
     roof = dict(recon_roof)
     roof["other_kernels"] = {
-        "k_gemm_topk<8> (plain search)": {
+        "k_gemm_topk4<8> (plain search)": {
             "bound": "mfma", "achieved_TFLOPs": round(gemm_tf, 1), "peak_TFLOPs": BF16_PEAK_TFLOPS,
             "frac": round(gemm_tf / BF16_PEAK_TFLOPS, 4), "ms": round(gemm_ms, 3),
             "algorithmic_flops": flops},

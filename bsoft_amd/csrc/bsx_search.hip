@@ -15,9 +15,9 @@
 //                    CTF values, and the BF16 row of the coarse score
 //   k_lib_build<W> : library sections at the disk points by FSI-kernel
 //                    interpolation (FP32 values) and their BF16 rows
-//   k_gemm_topk<T> : BF16 MFMA (v_mfma_f32_32x32x16_bf16) GEMM of particle rows
+//   k_gemm_topk4<T>: BF16 MFMA (v_mfma_f32_32x32x16_bf16) GEMM of particle rows
 //                    against every library row with FP32 accumulation and a
-//                    fused per-particle top-T selection
+//                    fused per-particle top-T selection (256 x 256 tiles)
 //   k_topk_merge<T>: merge of the per-segment top-T lists
 //   k_rescore      : FP32 FRC of the reference formula for candidate library
 //                    views, best candidate per particle
@@ -252,146 +252,171 @@ __global__ __launch_bounds__(256) void k_lib_build(
 }
 
 // ------------------------------------------------------------ GEMM + top-k
-// Workgroup (256 threads, 4 waves) = one 128-row block of particles x one
-// segment of library column tiles. Per 128x128 tile: K loop in steps of 64
-// staged through double-buffered LDS, each wave a 64x64 quadrant as 2x2
-// v_mfma_f32_32x32x16_bf16 blocks; then the tile goes to LDS and thread t
-// folds row t%128, columns 64*(t/128)..+63 into its running top-T list.
-//
-// MFMA operand layout (32x32x16): lane l supplies row (A) / column (B) l%32 at
-// k = 8*(l/32)..+7; the result holds column l%32, rows 8*(r/4)+4*(l/32)+r%4 in
-// element r.
+// Workgroup (512 threads, 8 waves as 4 x 2) = one 256-row block of particles
+// x one segment of 256-column library tiles; each wave computes a 64 x 128
+// block as 2 x 4 v_mfma_f32_32x32x16_bf16 blocks. At this tile size the
+// operands cost 1 byte per 128 FLOP from L2 (128 x 128 tiles: 1 per 64, as
+// much as the L2 -> CU path delivers at the MFMA rate). Operands go from
+// global memory straight to LDS (global_load_lds_dwordx4) into unpadded,
+// XOR-swizzled tiles: chunk c (16 B) of row r sits in slot c ^ (r & 7) of its
+// 128-B row, so a wave's load instruction (1 KB, 8 rows in lane order) takes
+// each lane's chunk from the slot it fills, and the b128 fragment reads of 8
+// consecutive rows hit 8 distinct slots (all 32 banks); two stages, the
+// loads of step k + 1 in flight while step k multiplies. The epilogue goes
+// through LDS in two 128-row halves (rows of 260 floats, so b128 row reads
+// stay conflict-free); thread t folds row t % 128 of the half, columns
+// 64 (t / 128) .. + 63, into its running top-T list, 4 columns at a time
+// against the list's T-th value. The top-T lists come out per (row,
+// segment, column quarter).
+constexpr int GM4 = 256, GN4 = 256;
+constexpr int GSTAGE4 = (GM4 + GN4) * 128;        // bytes per pipeline stage
+constexpr int CROW4 = GN4 + 4;
+constexpr int GEMM4_LDS = (2 * GSTAGE4 > 128 * CROW4 * 4) ? 2 * GSTAGE4 : 128 * CROW4 * 4;
+
 template <int T>
-__global__ __launch_bounds__(256) void k_gemm_topk(const uint16_t* __restrict__ A,
-                                                   const uint16_t* __restrict__ B, int Kp,
-                                                   int ncols, int tiles_per_seg, int ntiles,
-                                                   float* __restrict__ pscore,
-                                                   int* __restrict__ pidx) {
-    __shared__ __attribute__((aligned(16))) char lds[GEMM_LDS];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const size_t row0 = (size_t)blockIdx.x * GM;
-    const int seg = blockIdx.y, nseg = gridDim.y;
+__global__ __launch_bounds__(512) void k_gemm_topk4(const uint16_t* __restrict__ A,
+                                                    const uint16_t* __restrict__ B, int Kp,
+                                                    int ncols, int tiles_per_seg, int ntiles,
+                                                    int rblocks, int nseg,
+                                                    float* __restrict__ pscore,
+                                                    int* __restrict__ pidx) {
+    __shared__ __attribute__((aligned(16))) char lds[GEMM4_LDS];
+    typedef __attribute__((address_space(3))) void* lds_ptr;
+    // XCD-aware order: hardware block b runs on XCD b % 8; logical block
+    // L = (b % 8) * per + b / 8 gives each XCD a contiguous range, row blocks
+    // fastest, so the workgroups resident on one XCD share library tiles
+    // through its L2.
+    const int nblk = rblocks * nseg;
+    const int per = (nblk + 7) / 8;
+    const int L = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (L >= nblk) return;
+    const size_t row0 = (size_t)(L % rblocks) * GM4;
+    const int seg = L / rblocks;
     const int t0 = seg * tiles_per_seg;
     const int t1 = min(ntiles, t0 + tiles_per_seg);
-    const int my_row = tid & (GM - 1), my_half = tid >> 7;
-
-    float best[T];
-    int bidx[T];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int my_row = tid & 127, my_q = tid >> 7;   // epilogue: row of a half, column quarter
+    float best[2][T];
+    int bidx[2][T];
 #pragma unroll
-    for (int q = 0; q < T; ++q) { best[q] = -INFINITY; bidx[q] = -1; }
-
+    for (int hh = 0; hh < 2; ++hh)
+#pragma unroll
+        for (int q = 0; q < T; ++q) { best[hh][q] = -INFINITY; bidx[hh][q] = -1; }
     const int nk = Kp / GK;
-    // global -> LDS staging: 1024 16-byte chunks per operand tile, 4 per thread
-    int ld_row[4], ld_part[4];
+    // this wave's load instructions: A rows 32 w .. + 31, B rows 32 w .. + 31
+    const uint16_t* asrc[4];
+    const uint16_t* bsrc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int c = tid + 256 * q;
-        ld_row[q] = c >> 3;
-        ld_part[q] = c & 7;
+        const int r = 8 * (wave * 4 + q) + (lane >> 3);
+        asrc[q] = A + (row0 + r) * Kp + ((lane & 7) ^ (r & 7)) * 8;
+        bsrc[q] = B + (size_t)r * Kp + ((lane & 7) ^ (r & 7)) * 8;
     }
-
     for (int t = t0; t < t1; ++t) {
-        const size_t col0 = (size_t)t * GN;
-        f32x16 acc[2][2];
+        const size_t cofs = (size_t)t * GN4 * Kp;
+        f32x16 acc[2][4];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+            for (int j = 0; j < 4; ++j)
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-
-        // the staged chunks live in named registers (an array here ends up in
-        // scratch memory)
-        uint4 ra0, ra1, ra2, ra3, rb0, rb1, rb2, rb3;
-#define BSX_GLOAD(K0)                                                               \
-    ra0 = *(const uint4*)(A + (row0 + ld_row[0]) * Kp + (K0) + ld_part[0] * 8);     \
-    ra1 = *(const uint4*)(A + (row0 + ld_row[1]) * Kp + (K0) + ld_part[1] * 8);     \
-    ra2 = *(const uint4*)(A + (row0 + ld_row[2]) * Kp + (K0) + ld_part[2] * 8);     \
-    ra3 = *(const uint4*)(A + (row0 + ld_row[3]) * Kp + (K0) + ld_part[3] * 8);     \
-    rb0 = *(const uint4*)(B + (col0 + ld_row[0]) * Kp + (K0) + ld_part[0] * 8);     \
-    rb1 = *(const uint4*)(B + (col0 + ld_row[1]) * Kp + (K0) + ld_part[1] * 8);     \
-    rb2 = *(const uint4*)(B + (col0 + ld_row[2]) * Kp + (K0) + ld_part[2] * 8);     \
-    rb3 = *(const uint4*)(B + (col0 + ld_row[3]) * Kp + (K0) + ld_part[3] * 8);
-#define BSX_LSTORE(BUF)                                                             \
-    *(uint4*)((BUF) + ld_row[0] * LROW + ld_part[0] * 16) = ra0;                    \
-    *(uint4*)((BUF) + ld_row[1] * LROW + ld_part[1] * 16) = ra1;                    \
-    *(uint4*)((BUF) + ld_row[2] * LROW + ld_part[2] * 16) = ra2;                    \
-    *(uint4*)((BUF) + ld_row[3] * LROW + ld_part[3] * 16) = ra3;                    \
-    *(uint4*)((BUF) + (GM + ld_row[0]) * LROW + ld_part[0] * 16) = rb0;             \
-    *(uint4*)((BUF) + (GM + ld_row[1]) * LROW + ld_part[1] * 16) = rb1;             \
-    *(uint4*)((BUF) + (GM + ld_row[2]) * LROW + ld_part[2] * 16) = rb2;             \
-    *(uint4*)((BUF) + (GM + ld_row[3]) * LROW + ld_part[3] * 16) = rb3;
-        BSX_GLOAD(0)
-        BSX_LSTORE(lds)
+        auto stage = [&](int buf, size_t k0) {
+            char* base = lds + buf * GSTAGE4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                __builtin_amdgcn_global_load_lds(asrc[q] + k0, (lds_ptr)(base + (wave * 4 + q) * 1024),
+                                                 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(bsrc[q] + cofs + k0,
+                                                 (lds_ptr)(base + GM4 * 128 + (wave * 4 + q) * 1024),
+                                                 16, 0, 0);
+            }
+        };
+        stage(0, 0);
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         for (int kt = 0; kt < nk; ++kt) {
-            const bool more = kt + 1 < nk;
-            if (more) {
-                const size_t k0 = (size_t)(kt + 1) * GK;
-                BSX_GLOAD(k0)
-            }
-            const char* buf = lds + (kt & 1) * (GM + GN) * LROW;
+            // buffer (kt + 1) & 1 was read in step kt - 1, which every wave
+            // has left (barrier at its end)
+            if (kt + 1 < nk) stage((kt + 1) & 1, (size_t)(kt + 1) * GK);
+            const char* abuf = lds + (kt & 1) * GSTAGE4;
+            const char* bbuf = abuf + GM4 * 128;
 #pragma unroll
             for (int ks = 0; ks < GK / 16; ++ks) {
-                bf16x8 fa[2], fb[2];
-                const int off = ks * 32 + (lane >> 5) * 16;
+                const int c = ks * 2 + (lane >> 5);
+                bf16x8 fa[2], fb[4];
 #pragma unroll
                 for (int i = 0; i < 2; ++i) {
-                    fa[i] = *(const bf16x8*)(buf + (wm * 64 + i * 32 + (lane & 31)) * LROW + off);
-                    fb[i] = *(const bf16x8*)(buf + (GM + wn * 64 + i * 32 + (lane & 31)) * LROW + off);
+                    const int ra = wm * 64 + i * 32 + (lane & 31);
+                    fa[i] = *(const bf16x8*)(abuf + ra * 128 + ((c ^ (ra & 7)) << 4));
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int rb = wn * 128 + j * 32 + (lane & 31);
+                    fb[j] = *(const bf16x8*)(bbuf + rb * 128 + ((c ^ (rb & 7)) << 4));
                 }
 #pragma unroll
                 for (int i = 0; i < 2; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j)
+                    for (int j = 0; j < 4; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
             }
-            if (more) {
-                char* nb = lds + ((kt + 1) & 1) * (GM + GN) * LROW;
-                BSX_LSTORE(nb)
-            }
+            __builtin_amdgcn_s_waitcnt(0);   // step kt + 1 has landed (and this step's reads)
             __syncthreads();
         }
-#undef BSX_GLOAD
-#undef BSX_LSTORE
-        // epilogue: tile to LDS, then per-row top-T
+        // epilogue, one 128-row half at a time
         float* C = (float*)lds;
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int hh = 0; hh < 2; ++hh) {
+            if ((wm >> 1) == hh) {
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
+                for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int row = wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                    const int col = wn * 64 + j * 32 + (lane & 31);
-                    C[row * CROW + col] = acc[i][j][r];
-                }
-        __syncthreads();
-        const float* cr = C + my_row * CROW + my_half * 64;
-        const int cbase = (int)col0 + my_half * 64;
-        const int jmax = min(64, ncols - cbase);
-        for (int j = 0; j < jmax; ++j) {
-            const float v = cr[j];
-            if (v > best[T - 1]) {
-                best[T - 1] = v;
-                bidx[T - 1] = cbase + j;
+                    for (int j = 0; j < 4; ++j)
 #pragma unroll
-                for (int q = T - 1; q > 0; --q) {
-                    if (best[q] > best[q - 1]) {
-                        const float tv = best[q]; best[q] = best[q - 1]; best[q - 1] = tv;
-                        const int ti = bidx[q]; bidx[q] = bidx[q - 1]; bidx[q - 1] = ti;
+                        for (int r = 0; r < 16; ++r) {
+                            const int row = (wm & 1) * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                            const int col = wn * 128 + j * 32 + (lane & 31);
+                            C[row * CROW4 + col] = acc[i][j][r];
+                        }
+            }
+            __syncthreads();
+            const float* cr = C + my_row * CROW4 + my_q * 64;
+            const int cbase = t * GN4 + my_q * 64;
+            const int jmax = min(64, ncols - cbase);
+            for (int j = 0; j < jmax; j += 4) {
+                const float4 v4 = *(const float4*)(cr + j);
+                if (fmaxf(fmaxf(v4.x, v4.y), fmaxf(v4.z, v4.w)) > best[hh][T - 1]) {
+                    const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float v = vv[e];
+                        if (j + e < jmax && v > best[hh][T - 1]) {
+                            best[hh][T - 1] = v;
+                            bidx[hh][T - 1] = cbase + j + e;
+#pragma unroll
+                            for (int q = T - 1; q > 0; --q) {
+                                if (best[hh][q] > best[hh][q - 1]) {
+                                    const float tv = best[hh][q]; best[hh][q] = best[hh][q - 1]; best[hh][q - 1] = tv;
+                                    const int ti = bidx[hh][q]; bidx[hh][q] = bidx[hh][q - 1]; bidx[hh][q - 1] = ti;
+                                }
+                            }
+                        }
                     }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
-    const size_t o = (((row0 + my_row) * nseg + seg) * 2 + my_half) * T;
 #pragma unroll
-    for (int q = 0; q < T; ++q) {
-        pscore[o + q] = best[q];
-        pidx[o + q] = bidx[q];
+    for (int hh = 0; hh < 2; ++hh) {
+        const size_t o = (((row0 + hh * 128 + my_row) * nseg + seg) * 4 + my_q) * T;
+#pragma unroll
+        for (int q = 0; q < T; ++q) {
+            pscore[o + q] = best[hh][q];
+            pidx[o + q] = bidx[hh][q];
+        }
     }
 }
 
@@ -1447,7 +1472,7 @@ int bsx_search_set_reference(bsx_search* h, const float* ft_dev, const double* m
                     "bsx_search_set_reference: matrix %lld is not a rotation", (long long)(i / 9));
     hipStream_t s = (hipStream_t)stream;
     const int N = h->p.box, Nd = h->Nd, Kp = h->Kp;
-    const int64_t vpad = round_up(std::max<int64_t>(nviews, 1), GN);
+    const int64_t vpad = round_up(std::max<int64_t>(nviews, 1), GN4);
     int rc = grow(h, &h->d_lib, &h->lib_cap, vpad * Nd, "library values");
     if (!rc) rc = grow(h, &h->d_lbf, &h->lbf_cap, vpad * Kp, "library rows");
     if (rc) return rc;
@@ -1510,7 +1535,7 @@ int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* p
         }
     });
     hipStream_t s = (hipStream_t)stream;
-    const int64_t ppad = round_up(std::max<int64_t>(n, 1), GM);
+    const int64_t ppad = round_up(std::max<int64_t>(n, 1), GM4);
     int rc = grow(h, &h->d_fp, &h->fp_cap, ppad * Nd, "particle values");
     if (!rc) rc = grow(h, &h->d_ctf, &h->ctf_cap, ppad * Nd, "particle CTF");
     if (!rc) rc = grow(h, &h->d_pbf, &h->pbf_cap, ppad * Kp, "particle rows");
@@ -1559,39 +1584,40 @@ int bsx_search_stage(bsx_search* h, const float* imgs_dev, const bsx_particle* p
 
 namespace {
 
-// GEMM + top-k of `nrows` BF16 rows at A (allocated to a multiple of GM rows)
+// GEMM + top-k of `nrows` BF16 rows at A (allocated to a multiple of GM4 rows)
 // against the whole library.
 int run_exhaustive(bsx_search* h, const uint16_t* A, int64_t nrows, int32_t* cand_dev,
                    float* score_dev, hipStream_t s) {
     const int T = h->p.topk <= 8 ? 8 : 16;
-    const int64_t rblocks = round_up(nrows, GM) / GM;
-    const int ntiles = (int)(round_up(h->nviews, GN) / GN);
-    // enough segments to give the chip >= 2048 workgroups, at least 4 tiles each
+    const int64_t rblocks = round_up(nrows, GM4) / GM4;
+    const int ntiles = (int)(round_up(h->nviews, GN4) / GN4);
+    // enough segments to give the chip >= 1024 workgroups (one per CU at a
+    // time), at least 4 tiles each
     int nseg = (int)std::max<int64_t>(1, std::min<int64_t>(ntiles / 4 > 0 ? ntiles / 4 : 1,
-                                                           (2048 + rblocks - 1) / rblocks));
+                                                           (1024 + rblocks - 1) / rblocks));
     nseg = std::min(nseg, 64);
     const int tps = (ntiles + nseg - 1) / nseg;
     nseg = (ntiles + tps - 1) / tps;
-    const int64_t need = rblocks * GM * nseg * 2 * T;
+    const int64_t need = rblocks * GM4 * nseg * 4 * T;
     int rc = grow(h, &h->d_pscore, &h->pscore_cap, need, "top-k partials");
     if (!rc) rc = grow(h, &h->d_pidx, &h->pidx_cap, need, "top-k partials");
     if (rc) return rc;
-    BSX_REQUIRE(rblocks < (1ll << 31), BSX_ERR_ARG, "bsx_search_exhaustive: too many particles");
-    dim3 grid((unsigned)rblocks, (unsigned)nseg);
+    BSX_REQUIRE(rblocks * nseg < (1ll << 31) - 8, BSX_ERR_ARG, "bsx_search_exhaustive: too many particles");
+    const dim3 grid((unsigned)(8 * ((rblocks * nseg + 7) / 8)));
     if (T == 8)
-        hipLaunchKernelGGL((k_gemm_topk<8>), grid, dim3(256), 0, s, A, h->d_lbf, h->Kp,
-                           (int)h->nviews, tps, ntiles, h->d_pscore, h->d_pidx);
+        hipLaunchKernelGGL((k_gemm_topk4<8>), grid, dim3(512), 0, s, A, h->d_lbf, h->Kp,
+                           (int)h->nviews, tps, ntiles, (int)rblocks, nseg, h->d_pscore, h->d_pidx);
     else
-        hipLaunchKernelGGL((k_gemm_topk<16>), grid, dim3(256), 0, s, A, h->d_lbf, h->Kp,
-                           (int)h->nviews, tps, ntiles, h->d_pscore, h->d_pidx);
+        hipLaunchKernelGGL((k_gemm_topk4<16>), grid, dim3(512), 0, s, A, h->d_lbf, h->Kp,
+                           (int)h->nviews, tps, ntiles, (int)rblocks, nseg, h->d_pscore, h->d_pidx);
     BSX_HIP(hipGetLastError());
     const unsigned mg = (unsigned)((nrows + 255) / 256);
     if (T == 8)
         hipLaunchKernelGGL((k_topk_merge<8>), dim3(mg), dim3(256), 0, s, h->d_pscore, h->d_pidx,
-                           2 * nseg, nrows, h->p.topk, cand_dev, score_dev);
+                           4 * nseg, nrows, h->p.topk, cand_dev, score_dev);
     else
         hipLaunchKernelGGL((k_topk_merge<16>), dim3(mg), dim3(256), 0, s, h->d_pscore, h->d_pidx,
-                           2 * nseg, nrows, h->p.topk, cand_dev, score_dev);
+                           4 * nseg, nrows, h->p.topk, cand_dev, score_dev);
     BSX_HIP(hipGetLastError());
     return BSX_OK;
 }
@@ -1627,7 +1653,7 @@ int bsx_search_exhaustive_rows(bsx_search* h, const int32_t* rows_dev, int64_t m
                 (long long)h->nparts);
     if (m == 0) return BSX_OK;
     hipStream_t s = (hipStream_t)stream;
-    int rc = grow(h, &h->d_sub, &h->sub_cap, round_up(m, GM) * h->Kp, "row subset");
+    int rc = grow(h, &h->d_sub, &h->sub_cap, round_up(m, GM4) * h->Kp, "row subset");
     if (rc) return rc;
     const int64_t tot = m * (h->Kp / 8);
     hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, h->d_pbf,
