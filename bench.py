@@ -90,6 +90,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--recon-only", action="store_true")
+    ap.add_argument("--gemm-only", action="store_true",
+                    help="only the exhaustive search's BF16 GEMM + top-8 over the 4-degree library "
+                         "(for kernel profiles)")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed-size run: boxes 128/192/256 in a 50/30/20 ratio")
     ap.add_argument("--calib-copy", action="store_true",
@@ -123,6 +126,8 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"[bench] data ready in {time.perf_counter() - t0:.1f} s")
 
+    if args.gemm_only:
+        return run_gemm_only(args, bm, ps, box, pix, dev)
     rec = Reconstruction(N, box, ft, pix, ctf_action=abi.BSX_CTF_FLIP, flags=abi.BSX_FLAG_TIMING)
     result = {
         "metric": "particles/s (search+recon, 256^3) learned-proposal vs plain GPU; FSC vs CPU",
@@ -486,6 +491,42 @@ def run_mixed(args) -> int:
               "group_graph_launches": group.graph_launches(),
               "per_box": per_box, "roofline": None, "cpu_baseline": None}
     print(json.dumps(result), flush=True)
+    return 0
+
+
+def run_gemm_only(args, bm, ps, box, pix, dev) -> int:
+    """The exhaustive search's GEMM alone: the staged particles' BF16 disk
+    rows against every view of the 4-degree library (K = the disk's
+    2(Nd - 1) real values), fused per-row top-8 and the merge of the
+    per-segment lists; one step is one exhaustive() call."""
+    import torch
+    from bsoft_amd import search
+    from bsoft_amd.recon import map_transform
+    views = search.views_c1(args.grid_step, angle_step_deg=args.grid_step)
+    s = search.Search(box, args.radius, pix, kernel_width=args.kernel_width, topk=8)
+    s.set_reference(map_transform(bm.volume(device=dev)), search.view_matrices(views))
+    s.stage(ps.images, ps.parts)
+    for _ in range(args.warmup):
+        s.exhaustive()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.exhaustive()
+    torch.cuda.synchronize()
+    T = (time.perf_counter() - t0) / args.steps
+    P = len(ps.parts)
+    flops = 2.0 * P * s.nviews * 2 * (s.npoints - 1)
+    tf = flops / T / 1e12
+    print(json.dumps({
+        "metric": "exhaustive-search GEMM + top-8 (particles x library views)", "value": round(P / T, 1),
+        "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * T, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": f"{P} particles {box}^2 x {s.nviews} views, disk radius "
+                               f"{args.radius}, GEMM K {s.kdim}"},
+        "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 4), "traffic": None},
+        "cpu_baseline": None}), flush=True)
     return 0
 
 
