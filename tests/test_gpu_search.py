@@ -115,6 +115,39 @@ def test_gemm_topk_exact_on_bf16_rows(setup64, topk):
         assert np.all(np.diff(score[i]) <= 0)
 
 
+def test_gemm_topk_row_blocks_and_k_steps():
+    """The GEMM over more than one 256-row block (300 particles) and disk
+    radii whose K = 2(Nd - 1), rounded up to 64, gives both odd and even
+    numbers of 64-deep K steps: every row's top-8 are the top-8 of the exact
+    BF16 products."""
+    n = 64
+    rng = np.random.default_rng(29)
+    bm = synth.make_blob_map(n, 30, rng)
+    ft = map_transform(bm.volume().cuda())
+    ps = synth.make_particles(bm, 300, n, rng, snr=0.5, seed=29)
+    mats = search.view_matrices(search.views_c1(8.0, angle_step_deg=45.0))
+    parities = set()
+    for R in (10, 11, 14, 16):   # 5, 6, 10 and 13 K steps
+        s = search.Search(n, R, 1.5, topk=8)
+        s.set_reference(ft, mats)
+        s.stage(ps.images.cuda().contiguous(), ps.parts)
+        cand, score = s.exhaustive()
+        torch.cuda.synchronize()
+        nk = s.kdim // 64
+        parities.add(nk % 2)
+        say(f"gemm R={R} K={s.kdim} ({nk} steps) rows={len(ps.parts)} views={len(mats)}")
+        A = bf16_to_f32(s.particle_data()[2])
+        B = bf16_to_f32(s.library_data()[1])
+        full = A @ B.T
+        cand, score = cand.cpu().numpy(), score.cpu().numpy()
+        for i in range(len(A)):
+            np.testing.assert_allclose(score[i], full[i, cand[i]], atol=1e-4)
+            kth = np.sort(full[i])[::-1][7]
+            assert np.all(full[i, cand[i]] >= kth - 2e-4)
+            assert len(set(cand[i])) == 8
+    assert parities == {0, 1}
+
+
 def test_rescore_and_score_mats_match_reference_frc(setup64):
     n, R, bm, vol, ft, ps = setup64
     views = search.views_c1(10.0, angle_step_deg=45.0)
