@@ -165,14 +165,30 @@ __global__ __launch_bounds__(256) void k_stage_fused(
     const float* img = imgs + (size_t)b * N * N;
     for (int i = tid; i < N; i += blockDim.x) tw[i] = twg[i];
     float2* line = lines + g * N;
+    // the rows of the next G lines are loaded into registers while the
+    // current lines are transformed
+    constexpr int PT = N / T;
+    static_assert(N % T == 0, "whole values per thread");
+    float2 pre[PT];
+    auto load = [&](int l0) {
+        const int l = l0 + g;
+        const float* ra = img + (size_t)(2 * l) * N;
+#pragma unroll
+        for (int q = 0; q < PT; ++q) {
+            const int x = t + q * T;
+            pre[q] = (g < G && l < N / 2) ? make_float2(ra[x], ra[x + N]) : make_float2(0.f, 0.f);
+        }
+    };
+    load(0);
     for (int l0 = 0; l0 < N / 2; l0 += G) {
         const int l = l0 + g;
         const bool ok = g < G && l < N / 2;
         if (ok) {
-            const float* ra = img + (size_t)(2 * l) * N;
-            for (int x = t; x < N; x += T) line[x] = make_float2(ra[x], ra[x + N]);
+#pragma unroll
+            for (int q = 0; q < PT; ++q) line[t + q * T] = pre[q];
         }
         __syncthreads();
+        if (l0 + G < N / 2) load(l0 + G);
         FftLine<N, -1, T>::run(line, tw, t);
         if (ok) {
             for (int h = t; h < R; h += T) {
