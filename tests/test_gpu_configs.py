@@ -242,3 +242,40 @@ def test_group_run_half_sets_and_2d_member(gpu):
         say(f"  half {i} after the group runs: max rel err {err:.2e}")
         assert err <= 1e-5
     grp.close()
+
+
+def test_group_run_rejects_bad_input(gpu):
+    """bsx_recon_group_create / _run refuse a repeated handle, a member with
+    BSX_FLAG_CHECK_NAN, an empty particle set and a map buffer of the wrong
+    size with an error (no launch), and the group still runs afterwards."""
+    from bsoft_amd import abi
+    from bsoft_amd.abi import BsxError
+    from bsoft_amd.recon import Reconstruction, ReconGroup, part_ft_size
+    rng = np.random.default_rng(5)
+    bm = synth.make_blob_map(32, 8, rng)
+    ps = synth.make_particles(bm, 20, 32, rng, pix=3.0, snr=0.5, seed=3)
+    imgs = ps.images.cuda().contiguous()
+    ft = part_ft_size(32, 1, 2)
+    a = Reconstruction(32, 32, ft, 3.0, hi_res=6.0, ctf_action=abi.BSX_CTF_FLIP)
+    with pytest.raises(BsxError):
+        ReconGroup([a, a])
+    nan = Reconstruction(32, 32, ft, 3.0, hi_res=6.0, ctf_action=abi.BSX_CTF_FLIP,
+                         flags=abi.BSX_FLAG_CHECK_NAN)
+    g = ReconGroup([a, nan])
+    with pytest.raises(BsxError):
+        g.run([imgs, imgs], [ps.parts, ps.parts])
+    g.close()
+    g = ReconGroup([a])
+    with pytest.raises(ValueError):
+        g.run([imgs], [ps.parts[:0]])
+    with pytest.raises(ValueError):
+        g.run([imgs], [ps.parts], [torch.empty((16, 32, 32), device="cuda")])
+    maps, st = g.run([imgs], [ps.parts])
+    torch.cuda.synchronize()
+    ref = Reconstruction(32, 32, ft, 3.0, hi_res=6.0, ctf_action=abi.BSX_CTF_FLIP)
+    ref.insert(imgs, ps.parts)
+    r = ref.finish()
+    err = float((maps[0] - r.map).abs().max() / r.map.abs().max())
+    say(f"  group run after refused calls: max rel err {err:.2e}, particles {st[0].particles}")
+    assert err <= 1e-5 and st[0].particles == 20
+    g.close()
