@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+INSERT_CHUNKS = 4     # pipeline reconstruction: host preparation overlapped with the insertion
 
 
 def log(*a):
@@ -231,12 +232,19 @@ def main() -> int:
                                               steps_deg=(2.0, 1.0), timer=timer, refs=refs,
                                               tau=args.tau)
             mats = res.mats
+        # the found matrices come back once; the host then converts views
+        # and builds the particle records of one chunk while the GPU inserts
+        # the previous one
         found = mats.cpu().numpy()
-        parts = ps.parts.copy()
-        parts["view"] = search.matrix_views(found)
-        timer("views")
         recp.reset()
-        recp.insert(ps.images, parts)
+        bounds = np.linspace(0, len(found), INSERT_CHUNKS + 1).astype(int)
+        for c in range(INSERT_CHUNKS):
+            lo, hi_ = bounds[c], bounds[c + 1]
+            parts = ps.parts[lo:hi_].copy()
+            parts["view"] = search.matrix_views(found[lo:hi_])
+            if c == 0:
+                timer("views")
+            recp.insert(ps.images[lo:hi_], parts)
         timer("insert")
         fin = recp.finish()
         timer("finish")

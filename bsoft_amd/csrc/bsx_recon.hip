@@ -1485,15 +1485,20 @@ struct bsx_recon {
     int64_t hrecs_cap = 0;
     hipEvent_t copied = nullptr;   // staging buffer reusable once this completes
     bool timing = false;
-    // BSX_FLAG_GRAPH: the batch loop of an insert, captured once per
-    // (images, count, batch) and replayed while the same set is inserted again
-    hipGraphExec_t gexec = nullptr;
-    const float* g_imgs = nullptr;
-    int64_t g_n = -1;
-    int g_batch = 0;
-    int g_merge = -1;              // kernel constants baked into the graph
-    int g_piece = 0;
-    int g_seen = 0;                // identical calls seen (capture on the second)
+    // BSX_FLAG_GRAPH: the batch loop of an insert, captured on the second
+    // call with the same (images, record count, batch, run merge, piece) and
+    // replayed while that set is inserted again; up to 8 such sets (e.g. the
+    // chunks of a pipelined insert) are kept, the least recently used evicted
+    struct GraphEntry {
+        const float* imgs;
+        int64_t n;
+        int batch, merge, piece;   // kernel constants baked into the graph
+        int seen;                  // calls seen (capture on the second)
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_clock = 0;
     int64_t graph_launches = 0;
     hipStream_t cap_stream = nullptr;   // capture happens here (the caller's stream may be
                                         // the legacy default stream, which cannot capture)
@@ -1505,6 +1510,14 @@ struct bsx_recon {
     float2* gvol = nullptr;
     unsigned long long* gfin = nullptr;
 };
+
+// Drops every captured insert graph (their kernels hold the record buffers'
+// addresses and the record count).
+static void clear_graphs(bsx_recon* h) {
+    for (auto& e : h->graphs)
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    h->graphs.clear();
+}
 
 // KT_FALLBACK holds the atomic-fallback flags of the insert, not a time
 enum { KT_BG = 0, KT_ROWS, KT_FALLBACK, KT_SCAN, KT_FILL, KT_TILE, KT_DIRECT, KT_END };
@@ -2118,7 +2131,7 @@ int bsx_recon_destroy(bsx_recon* h) {
     if (h->hrecs) (void)hipHostFree(h->hrecs);
     if (h->gvol) (void)hipFree(h->gvol);
     if (h->gfin) (void)hipFree(h->gfin);
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    clear_graphs(h);
     if (h->cap_stream) (void)hipStreamDestroy(h->cap_stream);
     delete h;
     return BSX_OK;
@@ -2207,6 +2220,9 @@ static int insert_prepare(bsx_recon* h, const bsx_particle* parts, int64_t n, hi
     }
     const bool rescale = h->p.flags & BSX_FLAG_RESCALE;
     if (n > h->recs_cap) {
+        // captured inserts read the old record buffers
+        BSX_HIP(hipStreamSynchronize(s));
+        clear_graphs(h);
         if (h->recs) (void)hipFree(h->recs);
         if (h->bkg) (void)hipFree(h->bkg);
         if (h->resc) (void)hipFree(h->resc);
@@ -2351,12 +2367,19 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
         return BSX_OK;
     };
     const bool graph = (h->p.flags & BSX_FLAG_GRAPH) && !h->timing;
-    const bool same = imgs_dev == h->g_imgs && n == h->g_n && h->batch == h->g_batch &&
-                      h->K.merge == h->g_merge && h->piece == h->g_piece;
-    if (graph && same && h->gexec) {
-        BSX_HIP(hipGraphLaunch(h->gexec, s));
+    bsx_recon::GraphEntry* ge = nullptr;
+    if (graph)
+        for (auto& e : h->graphs)
+            if (e.imgs == imgs_dev && e.n == n && e.batch == h->batch && e.merge == h->K.merge &&
+                e.piece == h->piece) {
+                ge = &e;
+                ge->used = ++h->graph_clock;
+                break;
+            }
+    if (ge && ge->exec) {
+        BSX_HIP(hipGraphLaunch(ge->exec, s));
         h->graph_launches++;
-    } else if (graph && same && h->g_seen >= 1) {
+    } else if (ge && ge->seen >= 1) {
         // second identical call: every lazily created resource (twiddles)
         // exists, so the loop can be captured and replayed from now on
         hipGraph_t g = nullptr;
@@ -2369,26 +2392,26 @@ int bsx_recon_insert(bsx_recon* h, const float* imgs_dev, const bsx_particle* pa
             return r;
         }
         BSX_HIP(ce);
-        if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-        h->gexec = nullptr;
-        hipError_t ie = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+        hipError_t ie = hipGraphInstantiate(&ge->exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) ge->exec = nullptr;
         BSX_HIP(ie);
-        BSX_HIP(hipGraphLaunch(h->gexec, s));
+        BSX_HIP(hipGraphLaunch(ge->exec, s));
         h->graph_launches++;
     } else {
         if ((r = batch_loop(s))) return r;
-        if (graph) {
-            h->g_seen = same ? h->g_seen + 1 : 1;
-            if (!same && h->gexec) {
-                (void)hipGraphExecDestroy(h->gexec);
-                h->gexec = nullptr;
+        if (ge) {
+            ge->seen++;
+        } else if (graph) {
+            if (h->graphs.size() >= 8) {
+                auto lru = h->graphs.begin();
+                for (auto it = h->graphs.begin(); it != h->graphs.end(); ++it)
+                    if (it->used < lru->used) lru = it;
+                if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+                h->graphs.erase(lru);
             }
-            h->g_imgs = imgs_dev;
-            h->g_n = n;
-            h->g_batch = h->batch;
-            h->g_merge = h->K.merge;
-            h->g_piece = h->piece;
+            h->graphs.push_back(bsx_recon::GraphEntry{imgs_dev, n, h->batch, h->K.merge, h->piece, 1,
+                                                      nullptr, ++h->graph_clock});
         }
     }
     h->inserted[0] += add[0];
@@ -2522,11 +2545,8 @@ int bsx_recon_set_symmetry(bsx_recon* h, const char* sym) {
         h->sym_order.push_back(o.order);
         h->sym_axis.insert(h->sym_axis.end(), o.ax, o.ax + 3);
     }
-    // the graph of an earlier insert holds the record count of that insert
-    h->g_seen = 0;
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-    h->gexec = nullptr;
-    h->g_n = -1;
+    // the graphs of earlier inserts hold the record counts of those inserts
+    clear_graphs(h);
     return BSX_OK;
 }
 

@@ -255,6 +255,52 @@ def test_variants_agree(interp, ft, n):
             assert e < 2e-5, (method, order, merge, q, e)
 
 
+def test_graph_cache_of_chunked_inserts():
+    """BSX_FLAG_GRAPH keeps one graph per (images, count, tuning): three
+    chunks inserted after each other are each captured on their second
+    appearance and replayed on the third (6 graph launches in all), an
+    insert that reallocates the record buffers in between drops the cache
+    instead of replaying graphs that hold the old buffers, and every pass
+    gives the single-insert accumulators."""
+    import ctypes as C
+    from bsoft_amd.abi import call
+    bm, ps = dataset(32, 10, 300, 32, seed=12)
+    d = ps.images.cuda()
+    ref = Reconstruction(32, 32, 64, 1.5, ctf_action=1, batch=64)
+    ref.insert(d, ps.parts)
+    a0 = ref.accumulators(0)
+    g = Reconstruction(32, 32, 64, 1.5, ctf_action=1, batch=64, flags=abi.BSX_FLAG_GRAPH)
+    bounds = [0, 90, 200, 300]
+
+    def chunked():
+        g.reset()
+        for lo, hi in zip(bounds, bounds[1:]):
+            g.insert(d[lo:hi], ps.parts[lo:hi])
+        torch.cuda.synchronize()
+        a = g.accumulators(0)
+        return float((a[:3] - a0[:3]).abs().max() / a0[:3].abs().max())
+
+    nl = C.c_int64()
+    g.insert(d, ps.parts)   # sizes the record buffers for every chunk
+    for rep in range(3):
+        err = chunked()
+        call("bsx_recon_graph_launches", g.h, C.byref(nl))
+        say(f"  chunked pass {rep}: max rel err {err:.2e}, graph launches {nl.value}")
+        assert err <= 2e-5
+        assert nl.value == 3 * max(0, rep)
+    # a larger insert reallocates the records: the cached graphs are dropped
+    big = torch.cat([d, d])
+    bp = np.concatenate([ps.parts, ps.parts])
+    g.reset()
+    g.insert(big, bp)
+    for rep in range(3):
+        err = chunked()
+        call("bsx_recon_graph_launches", g.h, C.byref(nl))
+        say(f"  after reallocation, pass {rep}: max rel err {err:.2e}, graph launches {nl.value}")
+        assert err <= 2e-5
+        assert nl.value == 6 + 3 * max(0, rep)
+
+
 def test_graph_replay_and_autotune():
     """Inserts replayed from a captured hipGraph give the eager result (the
     replay counter proves the graph ran); the autotuner leaves the handle
