@@ -55,7 +55,9 @@ extern "C" {
                                     (bsx_recon_insert then synchronises)   */
 #define BSX_FLAG_GRAPH     128   /* replay the insertion kernels of a repeated
                                     insert (same images, count and batch) from
-                                    a captured hipGraph                      */
+                                    a captured hipGraph; up to 8 such sets per
+                                    handle (e.g. the chunks of a pipelined
+                                    insert) are kept                         */
 
 /* ---- interpolation: Bimage_reconstruct.cpp:25 (0 NN, 1 weighted NN, 2 trilinear) ---- */
 #define BSX_INTERP_NEAREST   0
