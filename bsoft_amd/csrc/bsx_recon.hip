@@ -130,14 +130,34 @@ __global__ __launch_bounds__(256) void k_background(const float* __restrict__ im
     }
     double2 r = make_double2(1.0, 0.0);
     int bad = 0;
+    // boxes of a multiple of 4 pixels: each thread reads 4 fixed columns
+    // of every R-th row with one 16-byte load
+    const int q = box / 4;
+    const bool vec = (box & 3) == 0 && q <= (int)blockDim.x && ((uintptr_t)img & 15) == 0;
+    const int R = vec ? (int)blockDim.x / q : 0;
+    const int xq = vec ? (int)threadIdx.x % q : 0, yr = vec ? (int)threadIdx.x / q : 0;
     if (resc) {
         double s1 = 0, s2 = 0;
         int unused = 0;
-        for (int i = threadIdx.x; i < npix; i += blockDim.x) {
-            const double v = img[i];
-            if (!isfinite(v)) bad = 1;
-            s1 += v;
-            s2 += v * v;
+        if (vec) {
+            for (int y = yr; yr < R && y < box; y += R) {
+                const float4 v4 = *reinterpret_cast<const float4*>(img + (size_t)y * box + 4 * xq);
+                const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const double v = vv[e];
+                    if (!isfinite(v)) bad = 1;
+                    s1 += v;
+                    s2 += v * v;
+                }
+            }
+        } else {
+            for (int i = threadIdx.x; i < npix; i += blockDim.x) {
+                const double v = img[i];
+                if (!isfinite(v)) bad = 1;
+                s1 += v;
+                s2 += v * v;
+            }
         }
         block_sum256(s1, s2, unused, bad);
         const double avg = s1 / npix;
@@ -150,16 +170,32 @@ __global__ __launch_bounds__(256) void k_background(const float* __restrict__ im
     __syncthreads();
     double sum = 0, dummy = 0;
     int cnt = 0;
-    int yy = (int)threadIdx.x / box, xx = (int)threadIdx.x - yy * box;
-    const int step_y = (int)blockDim.x / box, step_x = (int)blockDim.x - step_y * box;
-    for (int i = threadIdx.x; i < npix; i += blockDim.x) {
-        float v = img[i];
-        if (!isfinite(v)) bad = 1;
-        if (resc) v = rescaled(v, r);
-        if (cx2[xx] + ry2[yy] - 1 > 0) { sum += v; ++cnt; }
-        xx += step_x;
-        yy += step_y;
-        if (xx >= box) { xx -= box; ++yy; }
+    if (vec) {
+        const double c[4] = {cx2[4 * xq], cx2[4 * xq + 1], cx2[4 * xq + 2], cx2[4 * xq + 3]};
+        for (int y = yr; yr < R && y < box; y += R) {
+            const float4 v4 = *reinterpret_cast<const float4*>(img + (size_t)y * box + 4 * xq);
+            const float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+            const double ry = ry2[y];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float v = vv[e];
+                if (!isfinite(v)) bad = 1;
+                if (resc) v = rescaled(v, r);
+                if (c[e] + ry - 1 > 0) { sum += v; ++cnt; }
+            }
+        }
+    } else {
+        int yy = (int)threadIdx.x / box, xx = (int)threadIdx.x - yy * box;
+        const int step_y = (int)blockDim.x / box, step_x = (int)blockDim.x - step_y * box;
+        for (int i = threadIdx.x; i < npix; i += blockDim.x) {
+            float v = img[i];
+            if (!isfinite(v)) bad = 1;
+            if (resc) v = rescaled(v, r);
+            if (cx2[xx] + ry2[yy] - 1 > 0) { sum += v; ++cnt; }
+            xx += step_x;
+            yy += step_y;
+            if (xx >= box) { xx -= box; ++yy; }
+        }
     }
     block_sum256(sum, dummy, cnt, bad);
     if (threadIdx.x == 0) {
