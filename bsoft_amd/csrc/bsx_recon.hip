@@ -722,8 +722,12 @@ __device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev&
     *fi = 0.f;
     if (!inband) return false;
     double ww = P.wp;
+    // taper of the outermost shell: most wavefronts hold no such sample and
+    // skip it; FP32 square root (the weight is stored in FP32)
     const double d = (K.maxs2 - *s2) * K.mscz;
-    if (d < 1) ww *= sqrt(d);
+    if (__ballot(d < 1) != 0) {
+        if (d < 1) ww *= (double)__builtin_amdgcn_sqrtf((float)d);
+    }
     *w = ww;
     float sn, cs;
     if (ph) {
