@@ -94,6 +94,9 @@ def main() -> int:
     ap.add_argument("--gemm-only", action="store_true",
                     help="only the exhaustive search's BF16 GEMM + top-8 over the 4-degree library "
                          "(for kernel profiles)")
+    ap.add_argument("--refine-only", action="store_true",
+                    help="only the local refinement (2 then 1 degree, 27 rotations per step) "
+                         "from orientations 2 degrees off the truth (for kernel profiles)")
     ap.add_argument("--mixed", action="store_true",
                     help="mixed-size run: boxes 128/192/256 in a 50/30/20 ratio")
     ap.add_argument("--calib-copy", action="store_true",
@@ -129,6 +132,8 @@ def main() -> int:
 
     if args.gemm_only:
         return run_gemm_only(args, bm, ps, box, pix, dev)
+    if args.refine_only:
+        return run_refine_only(args, bm, ps, truth, box, pix, dev)
     rec = Reconstruction(N, box, ft, pix, ctf_action=abi.BSX_CTF_FLIP, flags=abi.BSX_FLAG_TIMING)
     result = {
         "metric": "particles/s (search+recon, 256^3) learned-proposal vs plain GPU; FSC vs CPU",
@@ -535,6 +540,49 @@ def run_gemm_only(args, bm, ps, box, pix, dev) -> int:
         "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 4), "traffic": None},
         "cpu_baseline": None}), flush=True)
+    return 0
+
+
+def run_refine_only(args, bm, ps, truth, box, pix, dev) -> int:
+    """The local refinement alone: the staged particles start 2 degrees off
+    their true orientations (about a random axis) and are refined with the
+    2- then 1-degree 3x3x3 rotation grids (k_score_mats, FSI width
+    --kernel-width); one step is one refine() call."""
+    import torch
+    from bsoft_amd import search
+    from bsoft_amd.recon import map_transform
+    s = search.Search(box, args.radius, pix, kernel_width=args.kernel_width, topk=8)
+    s.set_reference(map_transform(bm.volume(device=dev)),
+                    search.view_matrices(search.views_c1(15.0, angle_step_deg=15.0)))
+    s.stage(ps.images, ps.parts)
+    P = len(ps.parts)
+    rng = np.random.default_rng(1)
+    ax = rng.standard_normal((P, 3))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    K = np.zeros((P, 3, 3))
+    K[:, 0, 1], K[:, 0, 2], K[:, 1, 2] = -ax[:, 2], ax[:, 1], -ax[:, 0]
+    K[:, 1, 0], K[:, 2, 0], K[:, 2, 1] = ax[:, 2], -ax[:, 1], ax[:, 0]
+    a = np.radians(2.0)
+    start = torch.tensor((np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * (K @ K)) @ truth,
+                         device=dev)
+    for _ in range(args.warmup):
+        s.refine(start, (2.0, 1.0))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mats, _ = s.refine(start, (2.0, 1.0))
+    torch.cuda.synchronize()
+    T = (time.perf_counter() - t0) / args.steps
+    err = search.rotation_angle_deg(mats.cpu().numpy(), truth)
+    print(json.dumps({
+        "metric": "local refinement (2 + 1 degree, 27 rotations per step)", "value": round(P / T, 1),
+        "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * T, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{P} particles {box}^2, disk radius {args.radius}, "
+                               f"{s.npoints} points, FSI width {args.kernel_width}"},
+        "median_err_vs_truth_deg": round(float(np.median(err)), 3),
+        "roofline": None, "cpu_baseline": None}), flush=True)
     return 0
 
 
