@@ -404,6 +404,8 @@ struct BinCtx {
     int npool;       // chunks in the pool
     int maxc;        // chunks per bin
     Stage st;        // fill-pass staging (st.on = 0: reserve per wavefront)
+    float2* bins2;   // FOM: (sum w|F|^2, sum w^2) of each pool entry, else null
+    int stride;      // floats per volume record (3, or 5 with the FOM sums)
 };
 
 // Takes a pool chunk (-2 when the pool is exhausted) and installs it as chunk
@@ -448,15 +450,20 @@ __device__ __forceinline__ int chunk_wait(const BinCtx& bc, int b, int ci) {
 }
 
 __device__ __forceinline__ void put_entry(const BinCtx& bc, int c, int pos, float re, float im,
-                                          float w, int key) {
+                                          float w, int key, float p = 0.f, float w2 = 0.f) {
     if (c >= 0) {
-        bc.bins[(size_t)c * BSX_CHUNK + (pos & (BSX_CHUNK - 1))] =
-            make_float4(re, im, w, __int_as_float(key & 4095));
+        const size_t e = (size_t)c * BSX_CHUNK + (pos & (BSX_CHUNK - 1));
+        bc.bins[e] = make_float4(re, im, w, __int_as_float(key & 4095));
+        if (bc.bins2) bc.bins2[e] = make_float2(p, w2);
     } else {
-        float* rec = bc.acc + (size_t)(unsigned)key * 3;
+        float* rec = bc.acc + (size_t)(unsigned)key * bc.stride;
         unsafeAtomicAdd(rec + 0, re);
         unsafeAtomicAdd(rec + 1, im);
         unsafeAtomicAdd(rec + 2, w);
+        if (bc.bins2) {
+            unsafeAtomicAdd(rec + 3, p);
+            unsafeAtomicAdd(rec + 4, w2);
+        }
     }
 }
 
@@ -670,7 +677,8 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, int me
     const int c1 = __shfl(myc1, myleader, 64);
     if (out) {
         const int pos = base + myrank;
-        put_entry(bc, pos / BSX_CHUNK == base / BSX_CHUNK ? c0 : c1, pos, e.re, e.im, e.w, e.key);
+        put_entry(bc, pos / BSX_CHUNK == base / BSX_CHUNK ? c0 : c1, pos, e.re, e.im, e.w, e.key,
+                  e.p, e.w2);
     }
 }
 
@@ -1108,13 +1116,17 @@ __device__ __forceinline__ void lds_fadd2(float2* p, float x, float y) {
 // One workgroup per piece of a (half, tile) bin: sum its entries in LDS, then
 // add the tile partial to the volume — a plain read-modify-write when the bin
 // is a single piece, float atomics per touched record for multi-piece bins.
-// Each thread keeps 8 entry loads in flight before its LDS adds.
+// Each thread keeps 8 entry loads in flight before its LDS adds. FOM: the
+// entries' (sum w|F|^2, sum w^2) come from bins2 and the records hold 5 floats.
+template <bool FOM>
 __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill,
                                                     const int* __restrict__ chunks, int maxc,
                                                     int piece_len, const int* __restrict__ poff,
                                                     const int* __restrict__ pbin, int nbins,
                                                     const float4* __restrict__ bins,
+                                                    const float2* __restrict__ bins2,
                                                     float* __restrict__ acc) {
+    constexpr int RS = FOM ? 5 : 3;                 // floats per record
     constexpr int PC = BSX_PIECE_MAX / BSX_CHUNK;   // chunks per piece, at most
     const int pcn = piece_len / BSX_CHUNK;
     const int piece = blockIdx.x;
@@ -1127,27 +1139,31 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
     if (beg >= end) return;
     __shared__ float2 sri[4096];   // (sum re, sum im) per record
     __shared__ float sw[4096];
+    __shared__ float2 spw[FOM ? 4096 : 1];   // (sum w|F|^2, sum w^2) per record
     __shared__ int pc[PC];
     if (threadIdx.x < PC && (int)threadIdx.x < pcn) {
         const int ci = beg / BSX_CHUNK + threadIdx.x;
         pc[threadIdx.x] = (ci < maxc && ci * BSX_CHUNK < end) ? chunks[(size_t)bin * maxc + ci] : -1;
     }
-    float* g = acc + (size_t)bin * 4096 * 3;
+    float* g = acc + (size_t)bin * 4096 * RS;
     for (int i = threadIdx.x; i < 4096; i += blockDim.x) {
         sri[i] = make_float2(0.f, 0.f);
         sw[i] = 0.f;
+        if (FOM) spw[i] = make_float2(0.f, 0.f);
     }
     __syncthreads();
     constexpr int U = 8;
     const int stride = blockDim.x;
     for (int e0 = beg + threadIdx.x; e0 < end; e0 += U * stride) {
         float4 v[U];
+        float2 v2[FOM ? U : 1];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int e = e0 + u * stride;
             const int c = e < end ? pc[(e - beg) / BSX_CHUNK] : -1;
-            v[u] = c >= 0 ? bins[(size_t)c * BSX_CHUNK + (e & (BSX_CHUNK - 1))]
-                          : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+            const size_t at = (size_t)c * BSX_CHUNK + (e & (BSX_CHUNK - 1));
+            v[u] = c >= 0 ? bins[at] : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+            if (FOM) v2[u] = c >= 0 ? bins2[at] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1155,14 +1171,15 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
             if (li >= 0 && li < 4096) {
                 lds_fadd2(&sri[li], v[u].x, v[u].y);
                 lds_fadd(&sw[li], v[u].z);
+                if (FOM) lds_fadd2(&spw[li], v2[FOM ? u : 0].x, v2[FOM ? u : 0].y);
             }
         }
     }
     __syncthreads();
     if (single) {
-        // 12288 floats of the tile as 3072 float4: all loads, then all stores
+        // the tile's 4096 RS floats as float4: all loads, then all stores
         float4* g4 = reinterpret_cast<float4*>(g);
-        constexpr int NV = 3072 / 512;
+        constexpr int NV = 1024 * RS / 512;
         float4 cur[NV];
 #pragma unroll
         for (int u = 0; u < NV; ++u) cur[u] = g4[threadIdx.x + u * 512];
@@ -1172,8 +1189,9 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
             float add[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int li = (f + q) / 3, c = (f + q) - li * 3;
-                add[q] = c == 0 ? sri[li].x : c == 1 ? sri[li].y : sw[li];
+                const int li = (f + q) / RS, c = (f + q) - li * RS;
+                add[q] = c == 0 ? sri[li].x : c == 1 ? sri[li].y : c == 2 ? sw[li]
+                         : c == 3 ? spw[FOM ? li : 0].x : spw[FOM ? li : 0].y;
             }
             cur[u].x += add[0];
             cur[u].y += add[1];
@@ -1184,10 +1202,15 @@ __global__ __launch_bounds__(512) void k_tile_accum(const int* __restrict__ fill
     } else {
         for (int li = threadIdx.x; li < 4096; li += blockDim.x) {
             const float2 ri = sri[li];
-            if (sw[li] != 0.f || ri.x != 0.f || ri.y != 0.f) {
-                unsafeAtomicAdd(g + 3 * li + 0, ri.x);
-                unsafeAtomicAdd(g + 3 * li + 1, ri.y);
-                unsafeAtomicAdd(g + 3 * li + 2, sw[li]);
+            const float2 pw = FOM ? spw[FOM ? li : 0] : make_float2(0.f, 0.f);
+            if (sw[li] != 0.f || ri.x != 0.f || ri.y != 0.f || pw.x != 0.f || pw.y != 0.f) {
+                unsafeAtomicAdd(g + RS * li + 0, ri.x);
+                unsafeAtomicAdd(g + RS * li + 1, ri.y);
+                unsafeAtomicAdd(g + RS * li + 2, sw[li]);
+                if (FOM) {
+                    unsafeAtomicAdd(g + RS * li + 3, pw.x);
+                    unsafeAtomicAdd(g + RS * li + 4, pw.y);
+                }
             }
         }
     }
@@ -1422,31 +1445,33 @@ __global__ __launch_bounds__(256) void k_friedel(const float2* __restrict__ v, i
 // thread per voxel pair {v, -v}; linear order decides which voxel of the
 // pair handles it.
 __global__ void k_fold(float* __restrict__ S, float* __restrict__ X, long long n3, int N, int NT,
-                       long long nrec, int nhalves, int pairs) {
+                       long long nrec, int nhalves, int pairs, int rs) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n3 * nhalves;
          i += (long long)gridDim.x * blockDim.x) {
         const int half = (int)(i / n3);
         const long long j = i - (long long)half * n3;
         const long x = j % N, y = (j / N) % N, z = j / ((long long)N * N);
-        float* s = S + (size_t)half * nrec * 3;
-        float* xa = X + (size_t)half * nrec * 3;
-        const size_t rv = (size_t)rec_index(x, y, z, NT) * 3;
+        float* s = S + (size_t)half * nrec * rs;
+        float* xa = X + (size_t)half * nrec * rs;
+        const size_t rv = (size_t)rec_index(x, y, z, NT) * rs;
         if (!pairs) {
-            s[rv] += xa[rv];
-            s[rv + 1] += xa[rv + 1];
-            s[rv + 2] += xa[rv + 2];
-            xa[rv] = xa[rv + 1] = xa[rv + 2] = 0.f;
+            for (int c = 0; c < rs; ++c) {
+                s[rv + c] += xa[rv + c];
+                xa[rv + c] = 0.f;
+            }
             continue;
         }
         const long mx = x ? N - x : 0, my = y ? N - y : 0, mz = z ? N - z : 0;
         const long long jm = mx + (long long)N * (my + (long long)N * mz);
         if (jm < j) continue;
-        const size_t ru = (size_t)rec_index(mx, my, mz, NT) * 3;
+        const size_t ru = (size_t)rec_index(mx, my, mz, NT) * rs;
         const float vr = xa[rv], vi = xa[rv + 1], vw = xa[rv + 2];
+        // the FOM sums (w|F|^2, w^2) of a pair entry are those of both mates
         if (jm == j) {
             s[rv] += 2.f * vr;
             s[rv + 2] += 2.f * vw;
-            xa[rv] = xa[rv + 1] = xa[rv + 2] = 0.f;
+            for (int c = 3; c < rs; ++c) s[rv + c] += 2.f * xa[rv + c];
+            for (int c = 0; c < rs; ++c) xa[rv + c] = 0.f;
             continue;
         }
         const float ur = xa[ru], ui = xa[ru + 1], uw = xa[ru + 2];
@@ -1456,8 +1481,12 @@ __global__ void k_fold(float* __restrict__ S, float* __restrict__ X, long long n
         s[ru] += ur + vr;
         s[ru + 1] += ui - vi;
         s[ru + 2] += uw + vw;
-        xa[rv] = xa[rv + 1] = xa[rv + 2] = 0.f;
-        xa[ru] = xa[ru + 1] = xa[ru + 2] = 0.f;
+        for (int c = 3; c < rs; ++c) {
+            const float a = xa[rv + c], b = xa[ru + c];
+            s[rv + c] += a + b;
+            s[ru + c] += b + a;
+        }
+        for (int c = 0; c < rs; ++c) xa[rv + c] = xa[ru + c] = 0.f;
     }
 }
 
@@ -1507,6 +1536,7 @@ struct bsx_recon {
     int* pbin = nullptr;           // bin of each piece
     unsigned long long* entries = nullptr;   // bin entries of the current insert
     float4* bins = nullptr;
+    float2* bins2 = nullptr;       // FOM sums of the pool entries (FOM only)
     size_t bins_cap = 0;
     int nbins = 0;
     int batch = 0;
@@ -1937,9 +1967,14 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
     const PartDev* rec = h->recs + p0;
     const float* bk = h->bkg + p0;
     if (h->binned) {
-        BinCtx bf{h->fill, h->chunks, h->pool, h->bins, h->xacc, h->npool, h->maxc, Stage{}};
-        // nearest-neighbour pair entries with LDS staging: the specialised kernel
-        if (h->K.pairs && h->K.interp == 0 && NT == 256)
+        BinCtx bf{h->fill,  h->chunks, h->pool,  h->bins,       h->xacc,
+                  h->npool, h->maxc,   Stage{}, h->bins2,      h->K.stride};
+        // nearest-neighbour pair entries with LDS staging: the specialised
+        // kernel; with the FOM sums the generic kernel reserves per wavefront
+        if (h->K.fom)
+            hipLaunchKernelGGL((k_cols<FT, true, MODE_FILL>), grid, dim3(NT), 0, s, h->inter, bk,
+                               rec, nb, box, h->K, h->acc, tw, bf);
+        else if (h->K.pairs && h->K.interp == 0 && NT == 256)
             hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL, true>), grid, dim3(NT), 0, s, h->inter,
                                bk, rec, nb, box, h->K, h->acc, tw, bf);
         else
@@ -1952,15 +1987,21 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
                            h->piece, h->poff, h->pbin, (int)maxpieces, h->entries);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_SCAN))) return r;
-        hipLaunchKernelGGL(k_tile_accum, dim3(maxpieces), dim3(512), 0, s, h->fill, h->chunks,
-                           h->maxc, h->piece, h->poff, h->pbin, h->nbins, h->bins, h->xacc);
+        if (h->K.fom)
+            hipLaunchKernelGGL(k_tile_accum<true>, dim3(maxpieces), dim3(512), 0, s, h->fill,
+                               h->chunks, h->maxc, h->piece, h->poff, h->pbin, h->nbins, h->bins,
+                               h->bins2, h->xacc);
+        else
+            hipLaunchKernelGGL(k_tile_accum<false>, dim3(maxpieces), dim3(512), 0, s, h->fill,
+                               h->chunks, h->maxc, h->piece, h->poff, h->pbin, h->nbins, h->bins,
+                               nullptr, h->xacc);
         BSX_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_bin_reset, dim3((h->nbins + 255) / 256), dim3(256), 0, s, h->fill,
                            h->chunks, h->maxc, h->nbins, h->pool);
         BSX_HIP(hipGetLastError());
         if ((r = tmark(h, s, KT_TILE))) return r;
     } else {
-        BinCtx bc{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, Stage{}};
+        BinCtx bc{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, Stage{}, nullptr, h->K.stride};
         if (h->K.fom)
             hipLaunchKernelGGL((k_cols<FT, true, MODE_DIRECT>), grid, dim3(NT), 0, s, h->inter,
                                bk, rec, nb, box, h->K, h->acc, tw, bc);
@@ -2054,9 +2095,9 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     K.fom = (p->flags & BSX_FLAG_FOM) ? 1 : 0;
     K.stride = K.fom ? 5 : 3;
     K.NT = (p->size + 15) / 16;
-    // tile-binned insertion for the lean accumulators; the FOM sums use the
-    // direct atomic path
-    h->binned = p->method == 0 && !K.fom;
+    // tile-binned insertion (method 0; the FOM sums ride in a second pool
+    // array) or direct atomics (method 1)
+    h->binned = p->method == 0;
     K.interp = p->interp;
     K.nhalves = p->nhalves;
     // img_ctf_calculate (mg_ctf.cpp:100-107) as particle_reconstruct calls it
@@ -2105,12 +2146,13 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
                                     std::max<size_t>(256, ((size_t)batch * 2048 * mult +
                                                            BSX_CHUNK - 1) / BSX_CHUNK + 1));
     const size_t bins_bytes = h->binned ? (size_t)h->npool * BSX_CHUNK * sizeof(float4) : 0;
+    const size_t bins2_bytes = h->binned && K.fom ? (size_t)h->npool * BSX_CHUNK * sizeof(float2) : 0;
     const size_t bin_idx_bytes =
         h->binned ? ((size_t)h->nbins * (h->maxc + 3) + (size_t)h->npool * BSX_CHUNK / BSX_PIECE_MIN +
                      4) * sizeof(int) + 8
                   : 0;
     h->bytes = h->acc_elems * sizeof(float) * (h->binned ? 2 : 1) + inter_bytes + bins_bytes +
-               bin_idx_bytes;
+               bins2_bytes + bin_idx_bytes;
     if (p->mem_cap > 0 && (int64_t)h->bytes > p->mem_cap) {
         set_error("device memory %zu bytes exceeds the cap %lld", h->bytes, (long long)p->mem_cap);
         delete h;
@@ -2119,6 +2161,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     bool ok = hipMalloc(&h->acc, h->acc_elems * sizeof(float)) == hipSuccess &&
               hipMalloc(&h->inter, inter_bytes) == hipSuccess &&
               hipMalloc(&h->nonfinite, sizeof(int)) == hipSuccess;
+    if (ok && bins2_bytes) ok = hipMalloc(&h->bins2, bins2_bytes) == hipSuccess;
     if (ok && h->binned)
         ok = hipMalloc(&h->bins, bins_bytes) == hipSuccess &&
              hipMalloc(&h->xacc, h->acc_elems * sizeof(float)) == hipSuccess &&
@@ -2157,6 +2200,7 @@ int bsx_recon_destroy(bsx_recon* h) {
     if (h->recs) (void)hipFree(h->recs);
     if (h->nonfinite) (void)hipFree(h->nonfinite);
     if (h->bins) (void)hipFree(h->bins);
+    if (h->bins2) (void)hipFree(h->bins2);
     if (h->fill) (void)hipFree(h->fill);
     if (h->chunks) (void)hipFree(h->chunks);
     if (h->pool) (void)hipFree(h->pool);
@@ -2863,7 +2907,7 @@ static int fold(bsx_recon* h, hipStream_t s) {
     const long long n3 = (long long)h->K.N * h->K.N * h->K.N;
     const long long nrec = (long long)h->K.NT * h->K.NT * h->K.NT * 4096;
     hipLaunchKernelGGL(k_fold, dim3(2048), dim3(256), 0, s, h->acc, h->xacc, n3, h->K.N, h->K.NT,
-                       nrec, h->p.nhalves, h->K.pairs);
+                       nrec, h->p.nhalves, h->K.pairs, h->K.stride);
     BSX_HIP(hipGetLastError());
     return BSX_OK;
 }

@@ -79,7 +79,7 @@ typedef struct bsx_recon_params {
     int32_t nhalves;       /* 1: one accumulator; 2: half-set accumulators        */
     int32_t batch;         /* particles per internal batch (0 = autotuned)        */
     int64_t mem_cap;       /* device-memory cap for this handle (bytes, 0 = none) */
-    int32_t method;        /* 0: tile-binned insertion (FOM sums use 1); 1: direct atomics */
+    int32_t method;        /* 0: tile-binned insertion; 1: direct atomics */
     int32_t order;         /* 0 auto, 1 input order, 2 plane-orientation order     */
     int32_t merge;         /* 0 auto, 1 merge equal-voxel runs before atomics, 2 off */
     int32_t reserved;

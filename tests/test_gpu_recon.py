@@ -255,6 +255,36 @@ def test_variants_agree(interp, ft, n):
             assert e < 2e-5, (method, order, merge, q, e)
 
 
+@pytest.mark.parametrize("interp", [0, 1, 2])
+def test_fom_sums_binned_match_direct(interp):
+    """With BSX_FLAG_FOM the binned insertion carries sum w|F|^2 and sum w^2
+    of every entry in a second pool array and folds them with the Friedel-pair
+    entries: all five accumulator planes of both half sets match the
+    direct-atomic path (method 1) to FP32 summation order, and so do the FOM
+    maps of the finish."""
+    say(f"FOM binned vs direct interp={interp}")
+    n, ft = 32, 64
+    bm, ps = dataset(n, 10, 90, n, seed=13)
+    d = ps.images.cuda()
+    recs = {}
+    for method in (1, 0):
+        r = Reconstruction(n, n, ft, 1.5, ctf_action=1, interp=interp, method=method,
+                           flags=abi.BSX_FLAG_FOM, nhalves=2)
+        r.insert(d, ps.parts)
+        recs[method] = r
+    for half in (0, 1):
+        a0, a = recs[1].accumulators(half), recs[0].accumulators(half)
+        for q in range(5):
+            e = ((a[q] - a0[q]).abs().max() / a0[q].abs().max()).item()
+            assert e < 2e-5, (half, q, e)
+    f0 = recs[1].finish(want_fom=True)
+    f1 = recs[0].finish(want_fom=True)
+    assert ((f1.map - f0.map).abs().max() / f0.map.abs().max()).item() < 1e-5
+    assert ((f1.fom - f0.fom).abs().max() / f0.fom.abs().max()).item() < 1e-4
+    for r in recs.values():
+        r.close()
+
+
 def test_graph_cache_of_chunked_inserts():
     """BSX_FLAG_GRAPH keeps one graph per (images, count, tuning): three
     chunks inserted after each other are each captured on their second
