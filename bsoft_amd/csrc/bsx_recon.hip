@@ -781,7 +781,7 @@ template <int MG, bool FOM, int MODE, bool FAST = false>
 __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const ReconConst& K,
                                             const double m[3], bool inband, double w, float fr,
                                             float fi, int kk, int merge, unsigned keybase,
-                                            const BinCtx& bc, int bin = 1) {
+                                            const BinCtx& bc, int bin = 1, bool flush = false) {
     if (FAST || K.interp < 2) {
         const double fx = floor(m[0] + 0.5), fy = floor(m[1] + 0.5), fz = floor(m[2] + 0.5);
         double ww = w;
@@ -807,6 +807,10 @@ __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const Recon
                                 make_emit<FOM>(keybase + rec_index32(jx, jy, jz, K.NT), kk,
                                                inband, fr, fi, ww, bin),
                                 bc);
+            // staged pair entries: at most one per thread and corner, so the
+            // list is flushed after every second corner (workgroup-uniform:
+            // only the sample's own emission, which every thread makes, asks)
+            if (MODE == MODE_FILL && flush && bc.st.on && (corner & 1)) stage_flush(bc);
         }
     }
 }
@@ -826,10 +830,10 @@ __device__ __forceinline__ void coords(const ReconConst& K, const PartDev& P, do
 // same emits (excluded samples carry incl = 0), so shuffles and ballots stay
 // uniform.
 //
-// With K.pairs (binned nearest-neighbour insertion without Ewald) a sample
-// whose mate rounds to exactly the opposite voxel (-v for v; all but exact
-// .5 ties) is emitted once, as a pair entry at v that k_fold later also adds,
-// conjugated, at -v. Samples without such a mate (the x = 0 and x = FT/2
+// With K.pairs (binned insertion without Ewald) a sample whose mate lands on
+// exactly the opposite voxels (-v for v: nearest neighbour all but exact .5
+// ties, trilinear always) is emitted once, as pair entries at v that k_fold
+// later also adds, conjugated, at -v. Samples without such a mate (the x = 0 and x = FT/2
 // columns, the Nyquist row, ties) add to the volume directly, and the mate
 // of a tie is emitted on its own, as before.
 template <int MG, bool FOM, int MODE, bool FAST = false>
@@ -896,12 +900,17 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
         coords(K, P, t0, t1, 0.0, mm);
     }
     if (FAST || (MODE == MODE_FILL && K.pairs)) {
+        // trilinear: the mate's corners are the sample's negated, with the
+        // same weights (floor(-m) = -floor(m) - 1 and 1 - dx; a coordinate
+        // that is an integer only swaps a zero-weight corner)
         bool sym = km == -k;
+        if (FAST || K.interp < 2) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) sym = sym && floor(-m[q] + 0.5) == -floor(m[q] + 0.5);
+            for (int q = 0; q < 3; ++q) sym = sym && floor(-m[q] + 0.5) == -floor(m[q] + 0.5);
+        }
         const bool pair = mirror && sym;
         interp_emit<MG, FOM, MODE, FAST>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc,
-                                   pair ? 1 : 0);
+                                   pair ? 1 : 0, true);
         const bool mate = ib && !sym;
         if (__ballot(mate) == 0) return;   // wavefront-uniform
         interp_emit<MG, FOM, MODE, FAST>(acc, K, mm, mate, tw, gr, gi, (int)km, merge, keybase, bc, 0);
@@ -2114,7 +2123,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     K.shi2 = 1.0 / (h->hi_res * h->hi_res);          // img_ctf_calculate hires
     K.ewald = (p->flags & BSX_FLAG_EWALD) ? 1 : 0;
     K.merge = 0;
-    K.pairs = h->binned && !K.ewald && p->interp < 2 ? 1 : 0;
+    K.pairs = h->binned && !K.ewald ? 1 : 0;
     const size_t ntile = (size_t)K.NT * K.NT * K.NT;
     const size_t nrec = ntile * 4096;   // records incl. tile padding
     h->acc_elems = nrec * K.stride * p->nhalves;
