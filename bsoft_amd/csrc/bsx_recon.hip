@@ -830,7 +830,7 @@ __device__ __forceinline__ void coords(const ReconConst& K, const PartDev& P, do
 // same emits (excluded samples carry incl = 0), so shuffles and ballots stay
 // uniform.
 //
-// With K.pairs (binned insertion without Ewald) a sample whose mate lands on
+// With K.pairs (binned insertion; Ewald offsets in their own branch) a sample whose mate lands on
 // exactly the opposite voxels (-v for v: nearest neighbour all but exact .5
 // ties, trilinear always) is emitted once, as pair entries at v that k_fold
 // later also adds, conjugated, at -v. Samples without such a mate (the x = 0 and x = FT/2
@@ -856,19 +856,42 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
     } else {
         coords(K, P, s0, s1, mz0, m);
     }
-    if (!FAST && K.ewald) {   // all lanes: two emissions per frequency, mates computed in full
-        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
-        coords(K, P, s0, s1, -mz0, m);
-        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc);
+    if (!FAST && K.ewald) {
+        // two emissions per frequency, at +mz0 (A) and -mz0 (B). Off the
+        // Nyquist row the mate's emissions at +mz0 and -mz0 lie at exactly
+        // -B and -A with the conjugate value, so with pair entries A stands
+        // for the mate's -mz0 emission and B for its +mz0 one (where the
+        // nearest voxels mirror too); the others are computed in full.
+        const bool pr = MODE == MODE_FILL && K.pairs;
+        double mb[3];
+        coords(K, P, s0, s1, -mz0, mb);
+        bool sa = pr && km == -k, sb = sa;
+        if (K.interp < 2) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                sa = sa && floor(-m[q] + 0.5) == -floor(m[q] + 0.5);
+                sb = sb && floor(-mb[q] + 0.5) == -floor(mb[q] + 0.5);
+            }
+        }
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, inband, w, fr, fi, (int)k, merge, keybase, bc,
+                                         mirror && sa ? 1 : 0, pr);
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, mb, inband, w, fr, fi, (int)k, merge, keybase, bc,
+                                         mirror && sb ? 1 : 0, pr);
+        // staged nearest-neighbour entries: at most two per thread here
+        if (MODE == MODE_FILL && pr && bc.st.on && K.interp < 2) stage_flush(bc);
         double t0, t1, t2, tw;
         float gr, gi;
         const bool ib = sample_value<MODE>(K, P, mirror, -h, km, make_float2(F.x, -F.y), &t0,
                                            &t1, &t2, &tw, &gr, &gi);
+        const bool needc = ib && !sb, needd = ib && !sa;   // C = -B, D = -A when paired
+        if (__ballot(needc || needd) == 0) return;   // wavefront-uniform
         const double mz = P.ew * t2;
         coords(K, P, t0, t1, mz, m);
-        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, needc, tw, gr, gi, (int)km, merge, keybase, bc,
+                                         0);
         coords(K, P, t0, t1, -mz, m);
-        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, ib, tw, gr, gi, (int)km, merge, keybase, bc);
+        interp_emit<MG, FOM, MODE, FAST>(acc, K, m, needd, tw, gr, gi, (int)km, merge, keybase, bc,
+                                         0);
         return;
     }
     // the mate, per lane: -m and the conjugate away from the Nyquist row,
@@ -1983,7 +2006,7 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
         if (h->K.fom)
             hipLaunchKernelGGL((k_cols<FT, true, MODE_FILL>), grid, dim3(NT), 0, s, h->inter, bk,
                                rec, nb, box, h->K, h->acc, tw, bf);
-        else if (h->K.pairs && h->K.interp == 0 && NT == 256)
+        else if (h->K.pairs && h->K.interp == 0 && !h->K.ewald && NT == 256)
             hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL, true>), grid, dim3(NT), 0, s, h->inter,
                                bk, rec, nb, box, h->K, h->acc, tw, bf);
         else
@@ -2123,7 +2146,7 @@ int bsx_recon_create(bsx_recon** out, const bsx_recon_params* p) {
     K.shi2 = 1.0 / (h->hi_res * h->hi_res);          // img_ctf_calculate hires
     K.ewald = (p->flags & BSX_FLAG_EWALD) ? 1 : 0;
     K.merge = 0;
-    K.pairs = h->binned && !K.ewald ? 1 : 0;
+    K.pairs = h->binned ? 1 : 0;
     const size_t ntile = (size_t)K.NT * K.NT * K.NT;
     const size_t nrec = ntile * 4096;   // records incl. tile padding
     h->acc_elems = nrec * K.stride * p->nhalves;
