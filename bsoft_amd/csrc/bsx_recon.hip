@@ -611,7 +611,7 @@ __device__ __forceinline__ void merge_step(Emit& e, int& pk, int lg) {
 // first lane of each run then either adds to the volume with atomics
 // (MODE_DIRECT) or reserves a bin slot: lanes of a wavefront are grouped by
 // bin with ballots and one integer atomic per (wavefront, bin).
-template <int MG, bool FOM, int MODE, bool FAST = false>
+template <int MG, bool FOM, int MODE, int FAST = 0>
 __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, int merge, Emit e,
                                      const BinCtx& bc) {
     bool head = true;
@@ -777,7 +777,7 @@ __device__ __forceinline__ bool sample_value(const ReconConst& K, const PartDev&
 
 // Interpolation of one sample at voxel coordinates m (fspace_2D_interpolate,
 // Bimage_reconstruct.cpp:25-88): nearest / weighted nearest / trilinear.
-template <int MG, bool FOM, int MODE, bool FAST = false>
+template <int MG, bool FOM, int MODE, int FAST = 0>
 __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const ReconConst& K,
                                             const double m[3], bool inband, double w, float fr,
                                             float fi, int kk, int merge, unsigned keybase,
@@ -785,7 +785,7 @@ __device__ __forceinline__ void interp_emit(float* __restrict__ acc, const Recon
     if (FAST || K.interp < 2) {
         const double fx = floor(m[0] + 0.5), fy = floor(m[1] + 0.5), fz = floor(m[2] + 0.5);
         double ww = w;
-        if (!FAST && K.interp == 1) {
+        if (FAST == 2 || (!FAST && K.interp == 1)) {   // weighted nearest neighbour
             const double dx = m[0] - fx, dy = m[1] - fy, dz = m[2] - fz;
             ww *= 1 - sqrt(dx * dx + dy * dy + dz * dz);
         }
@@ -836,7 +836,7 @@ __device__ __forceinline__ void coords(const ReconConst& K, const PartDev& P, do
 // later also adds, conjugated, at -v. Samples without such a mate (the x = 0 and x = FT/2
 // columns, the Nyquist row, ties) add to the volume directly, and the mate
 // of a tie is emitted on its own, as before.
-template <int MG, bool FOM, int MODE, bool FAST = false>
+template <int MG, bool FOM, int MODE, int FAST = 0>
 __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const ReconConst& K,
                                             const PartDev& P, bool active, bool mirror, long h,
                                             long k, long km, float2 F, int merge,
@@ -944,10 +944,12 @@ __device__ __forceinline__ void insert_pair(float* __restrict__ acc, const Recon
 }
 
 // Column tasks: task = (particle, x) with x in [0, FT/2]; G tasks per WG.
-// The nearest-neighbour specialisation (FAST) is held to 5 waves per SIMD
-// (at most 96 VGPRs): its 31.7 KB of LDS allows 5 workgroups per CU, and one
-// register more would round the allocation up to 104 and allow only 4.
-template <int FT, bool FOM, int MODE, bool FAST = false>
+// The nearest-neighbour specialisations (FAST 1: nearest, 2: weighted
+// nearest, both with pair entries and staging, no Ewald) are held to 5 waves
+// per SIMD (at most 96 VGPRs): their 31.7 KB of LDS allows 5 workgroups per
+// CU, and one register more would round the allocation up to 104 and allow
+// only 4 (the weighted one spills 12 B for it).
+template <int FT, bool FOM, int MODE, int FAST = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAST ? 5 : 1)))
 void k_cols(const float2* __restrict__ inter,
                                               const float* __restrict__ bkg,
@@ -2007,7 +2009,10 @@ static int launch_batch(bsx_recon* h, const float* imgs, int64_t p0, int nb, hip
             hipLaunchKernelGGL((k_cols<FT, true, MODE_FILL>), grid, dim3(NT), 0, s, h->inter, bk,
                                rec, nb, box, h->K, h->acc, tw, bf);
         else if (h->K.pairs && h->K.interp == 0 && !h->K.ewald && NT == 256)
-            hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL, true>), grid, dim3(NT), 0, s, h->inter,
+            hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL, 1>), grid, dim3(NT), 0, s, h->inter,
+                               bk, rec, nb, box, h->K, h->acc, tw, bf);
+        else if (h->K.pairs && h->K.interp == 1 && !h->K.ewald && NT == 256)
+            hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL, 2>), grid, dim3(NT), 0, s, h->inter,
                                bk, rec, nb, box, h->K, h->acc, tw, bf);
         else
             hipLaunchKernelGGL((k_cols<FT, false, MODE_FILL>), grid, dim3(NT), 0, s, h->inter, bk,
