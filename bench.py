@@ -607,7 +607,7 @@ def insertion_roofline(kt_sum, launches, nparticles, box, ft, N, n_s):
     stage_keys = ("background", "rows", "fill", "scan", "tile", "direct", "fallback")
     ins_ms = max(sum(kt_sum.get(k, 0.0) for k in stage_keys), 1e-9)
     stage_bytes = (4 * box * box + 16 * ft * ft + k3) * nparticles
-    kname = f"k_cols<{ft}, false, 2, true>"
+    kname = f"k_cols<{ft}, false, 2, 1>"
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(kname, ppl) if ft == 512 else None,
