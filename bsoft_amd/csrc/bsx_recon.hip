@@ -393,6 +393,7 @@ struct Stage {
     int4* hinfo;     // at a flush, per hash slot: reserved base, chunks of the
                      // range start and end, bin
     int on;
+    float2* ent2;    // FOM: (sum w|F|^2, sum w^2) of each entry, else null
 };
 
 struct BinCtx {
@@ -488,7 +489,7 @@ __device__ __forceinline__ void range_chunks(const BinCtx& bc, int b, int base, 
 // the bin table: measured faster here than grouping the lanes of a
 // wavefront by bin first).
 __device__ __forceinline__ void stage_entry(const BinCtx& bc, int i, int key, float re, float im,
-                                           float w) {
+                                           float w, float p = 0.f, float w2 = 0.f) {
     const int bin = (int)((unsigned)key >> 12);
     int slot = -1;
     int hs = (int)(((unsigned)bin * 2654435761u) >> 25) & (BSX_STAGE_HASH - 1);
@@ -502,18 +503,19 @@ __device__ __forceinline__ void stage_entry(const BinCtx& bc, int i, int key, fl
     if (alone) pos = atomicAdd(bc.fill + bin, 1);
     range_chunks(bc, bin, pos, 1, alone, c0, c1);
     if (alone) {
-        put_entry(bc, c0, pos, re, im, w, key);
+        put_entry(bc, c0, pos, re, im, w, key, p, w2);
         bc.st.ent[i] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
         return;
     }
     const int rank = atomicAdd(&bc.st.hcnt[slot], 1);
     bc.st.ent[i] = make_float4(re, im, w, __int_as_float((slot << 22) | (rank << 12) | (key & 4095)));
+    if (bc.st.ent2) bc.st.ent2[i] = make_float2(p, w2);
 }
 
 // The lanes with `out` set append their entries: dense slots from one LDS
 // atomic per wavefront (all lanes call this together).
 __device__ __forceinline__ void stage_append(const BinCtx& bc, bool out, int key, float re,
-                                            float im, float w) {
+                                            float im, float w, float p = 0.f, float w2 = 0.f) {
     const unsigned long long m = __ballot(out);
     if (m == 0) return;
     const int lane = __lane_id();
@@ -521,7 +523,7 @@ __device__ __forceinline__ void stage_append(const BinCtx& bc, bool out, int key
     int base = 0;
     if (lane == leader) base = atomicAdd(bc.st.nent, __popcll(m));
     base = __shfl(base, leader, 64);
-    if (out) stage_entry(bc, base + __popcll(m & ((1ull << lane) - 1ull)), key, re, im, w);
+    if (out) stage_entry(bc, base + __popcll(m & ((1ull << lane) - 1ull)), key, re, im, w, p, w2);
 }
 
 // Workgroup-uniform: reserve every staged bin range, write the staged
@@ -550,7 +552,8 @@ __device__ __forceinline__ void stage_flush(const BinCtx& bc) {
         const int c = pos / BSX_CHUNK == hi.x / BSX_CHUNK ? hi.y : hi.z;
         // full record key (bin from the hash slot) for the atomic fallback
         const int key = (int)(((unsigned)hi.w << 12) | (unsigned)(pr & 4095));
-        put_entry(bc, c, pos, e.x, e.y, e.z, key);
+        const float2 e2 = bc.st.ent2 ? bc.st.ent2[i] : make_float2(0.f, 0.f);
+        put_entry(bc, c, pos, e.x, e.y, e.z, key, e2.x, e2.y);
     }
     __syncthreads();
     if (tid < BSX_STAGE_HASH) {
@@ -646,7 +649,7 @@ __device__ __forceinline__ void emit(float* __restrict__ acc, int stride, int me
     }
     const bool out = out0;   // lanes still here hold binned entries
     if (FAST || bc.st.on) {
-        stage_append(bc, out, e.key, e.re, e.im, e.w);
+        stage_append(bc, out, e.key, e.re, e.im, e.w, e.p, e.w2);
         return;
     }
     // per wavefront: one reservation per bin, all issued before any chunk
@@ -962,10 +965,11 @@ void k_cols(const float2* __restrict__ inter,
     constexpr int HX = FT / 2 + 1;
     constexpr int MG = T < 16 ? T : 16;   // run-merge group: lanes of one column
     constexpr int LDSN = G * FT + FT;
-    constexpr int SCAP = MODE == MODE_FILL && !FOM ? BSX_STAGE_CAP : 1;
-    constexpr int SHASH = MODE == MODE_FILL && !FOM ? BSX_STAGE_HASH : 1;
+    constexpr int SCAP = MODE == MODE_FILL ? BSX_STAGE_CAP : 1;
+    constexpr int SHASH = MODE == MODE_FILL ? BSX_STAGE_HASH : 1;
     __shared__ float2 lds[LDSN];
     __shared__ float4 st_ent[SCAP];
+    __shared__ float2 st_ent2[FOM ? SCAP : 1];
     __shared__ int4 st_hinfo[SHASH];
     __shared__ int st_hkey[SHASH], st_hcnt[SHASH], st_nent;
     const int tid = threadIdx.x;
@@ -986,9 +990,10 @@ void k_cols(const float2* __restrict__ inter,
     // staging (pair entries: at most one binned emission per thread and
     // round, so 2 rounds fit the list; the 512-entry list keeps the LDS of a
     // workgroup under 32 KB, 5 workgroups per CU)
-    const bool staged = FAST || (MODE == MODE_FILL && !FOM && K.pairs && NT == 256 &&
+    const bool staged = FAST || (MODE == MODE_FILL && K.pairs && NT == 256 &&
                                  2 * NT <= BSX_STAGE_CAP);
-    bc.st = Stage{st_ent, &st_nent, st_hkey, st_hcnt, st_hinfo, staged ? 1 : 0};
+    bc.st = Stage{st_ent, &st_nent, st_hkey, st_hcnt, st_hinfo, staged ? 1 : 0,
+                  FOM ? st_ent2 : nullptr};
     if (staged) {
         if (tid < BSX_STAGE_HASH) {
             st_hkey[tid] = -1;
