@@ -833,10 +833,11 @@ __device__ __forceinline__ void coords(const ReconConst& K, const PartDev& P, do
 // same emits (excluded samples carry incl = 0), so shuffles and ballots stay
 // uniform.
 //
-// With K.pairs (binned insertion; Ewald offsets in their own branch) a sample whose mate lands on
-// exactly the opposite voxels (-v for v: nearest neighbour all but exact .5
-// ties, trilinear always) is emitted once, as pair entries at v that k_fold
-// later also adds, conjugated, at -v. Samples without such a mate (the x = 0 and x = FT/2
+// With K.pairs (binned insertion; the Ewald offsets have their own branch)
+// a sample whose mate lands on exactly the opposite voxels (-v for v:
+// nearest neighbour all but exact .5 ties, trilinear always) is emitted
+// once, as pair entries at v that k_fold later also adds, conjugated, at
+// -v. Samples without such a mate (the x = 0 and x = FT/2
 // columns, the Nyquist row, ties) add to the volume directly, and the mate
 // of a tie is emitted on its own, as before.
 template <int MG, bool FOM, int MODE, int FAST = 0>
