@@ -195,11 +195,12 @@ def test_mixed_boxes_one_graph_parity(gpu):
 
 
 def test_group_run_half_sets_and_2d_member(gpu):
-    """A group of a 3D member with two half-sets and a 2D member
-    (BSX_FLAG_2D): every group run (direct, captured, replayed) gives the
-    maps of each handle's own insert + finish on the same particles, and the
-    half maps read from the sums a run leaves (bsx_recon_finish, nmaps 2)
-    equal the direct path's half maps."""
+    """A group of a 3D member with two half-sets, a 2D member (BSX_FLAG_2D)
+    and a member with the FOM sums (BSX_FLAG_FOM): every group run (direct,
+    captured, replayed) gives the maps of each handle's own insert + finish on
+    the same particles, the half maps read from the sums a run leaves
+    (bsx_recon_finish, nmaps 2) equal the direct path's half maps, and so
+    does the FOM member's FOM map."""
     from bsoft_amd import abi
     from bsoft_amd.recon import Reconstruction, ReconGroup, part_ft_size
     rng = np.random.default_rng(83)
@@ -207,7 +208,8 @@ def test_group_run_half_sets_and_2d_member(gpu):
     ps = synth.make_particles(bm, 60, 32, rng, pix=3.0, snr=0.3, seed=5)
     ps.parts["half"] = np.arange(len(ps.parts)) % 2
     ft = part_ft_size(32, 1, 2)
-    kinds = [dict(nhalves=2, flags=0), dict(nhalves=1, flags=abi.BSX_FLAG_2D)]
+    kinds = [dict(nhalves=2, flags=0), dict(nhalves=1, flags=abi.BSX_FLAG_2D),
+             dict(nhalves=1, flags=abi.BSX_FLAG_FOM)]
 
     def make(kind):
         return Reconstruction(32, 32, ft, 3.0, hi_res=6.0, ctf_action=abi.BSX_CTF_FLIP, batch=16,
@@ -215,19 +217,23 @@ def test_group_run_half_sets_and_2d_member(gpu):
 
     imgs = ps.images.cuda().contiguous()
     direct = []
+    fom_direct = None
     for kind in kinds:
         r = make(kind)
         r.insert(imgs, ps.parts)
         full = r.finish(0, 1).map.cpu().numpy()
+        if kind["flags"] & abi.BSX_FLAG_FOM:
+            fom_direct = r.finish(0, 1, want_fom=True).fom.cpu().numpy()
         halves = [r.finish(i, 2).map.cpu().numpy() for i in range(2)] if kind["nhalves"] == 2 else []
         direct.append((full, halves))
         r.close()
     recs = [make(kind) for kind in kinds]
     grp = ReconGroup(recs)
-    maps = [torch.empty((32, 32, 32), device="cuda"), torch.empty((1, 32, 32), device="cuda")]
+    maps = [torch.empty((32, 32, 32), device="cuda"), torch.empty((1, 32, 32), device="cuda"),
+            torch.empty((32, 32, 32), device="cuda")]
     for run in range(3):
-        say(f"group run {run}: 3D half sets + 2D member, 60 particles each")
-        _, stats = grp.run([imgs, imgs], [ps.parts, ps.parts], maps)
+        say(f"group run {run}: 3D half sets + 2D member + FOM member, 60 particles each")
+        _, stats = grp.run([imgs] * 3, [ps.parts] * 3, maps)
         torch.cuda.synchronize()
         for m, st, (full, _) in zip(maps, stats, direct):
             g = m.cpu().numpy()
@@ -241,6 +247,10 @@ def test_group_run_half_sets_and_2d_member(gpu):
         err = np.abs(h - ref).max() / np.abs(ref).max()
         say(f"  half {i} after the group runs: max rel err {err:.2e}")
         assert err <= 1e-5
+    fom = recs[2].finish(0, 1, want_fom=True).fom.cpu().numpy()
+    err = np.abs(fom - fom_direct).max() / np.abs(fom_direct).max()
+    say(f"  FOM map after the group runs: max rel err {err:.2e}")
+    assert err <= 1e-4
     grp.close()
 
 
