@@ -1058,6 +1058,88 @@ __global__ __launch_bounds__(256) void k_fit_grid(
     }
 }
 
+// img_fit_in_recip_space_monte (src/mg/mg_refine.cpp:783-860), one workgroup
+// per particle at its current orientation: the section at the disk points is
+// interpolated into LDS as in k_fit_grid, then one wavefront walks the chain.
+// Iteration i > 0 moves the defocus by def_std g (when the CTF applies) and
+// the origin by shift_std a u (u a unit direction), scores the FOM, keeps the
+// best, and accepts the move when fom / prefom exceeds a uniform value, else
+// returns to the last accepted state. prefom starts at -1, so a positive FOM
+// is never accepted and every move starts from the staged origin, as in the
+// reference. The random values come from the caller: draws holds max_iter x 5
+// FP64 per particle, (g, a, ux, uy, uniform); iteration 0 reads only the
+// uniform value. When the defocus is not stepped but the CTF flag is set, the
+// first rejection resets the defocus to 0 (the reference's predef), as there.
+template <int WD>
+__global__ __launch_bounds__(256) void k_fit_monte(
+    const float2* __restrict__ ft, int N, const double* __restrict__ mats, double inv, double rs,
+    const float* __restrict__ ktab, const int2* __restrict__ hk, const int* __restrict__ ring_start,
+    int R, int Nd, const float2* __restrict__ fp, const float* __restrict__ ctf,
+    const SPart* __restrict__ sps, int max_iter, double def_std, double shift_std,
+    const double* __restrict__ draws, double* __restrict__ origin, double* __restrict__ defocus,
+    float* __restrict__ fom) {
+    __shared__ float K[(FSI_NDIV + 1) * WD];
+    extern __shared__ float2 sec[];   // Nd section values
+    for (int i = threadIdx.x; i < (FSI_NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
+    __syncthreads();
+    const int64_t p = blockIdx.x;
+    const SPart sp = sps[p];
+    const double* M = mats + 9 * p;
+    for (int i = threadIdx.x; i < Nd; i += blockDim.x) {
+        const double ivx = (double)hk[i].x * inv, ivy = (double)hk[i].y * inv;
+        double m[3];
+        for (int q = 0; q < 3; ++q) m[q] = (ivx * M[3 * q] + ivy * M[3 * q + 1]) * rs;
+        double vr, vi;
+        fsi_sample<WD>(ft, N, m, K, vr, vi);
+        sec[i] = make_float2((float)vr, (float)vi);
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) return;   // the chain is sequential: one wavefront
+    const int lane = threadIdx.x;
+    const float2* P = fp + p * Nd;
+    const float* Cf = ctf + p * Nd;
+    const double tpn = 2 * M_PI / N;
+    const bool dstep = def_std > 0 && sp.def_avg != 0;
+    const bool apply = sp.flag || dstep;
+    double def = sp.def_avg, predef = dstep ? def : 0, bestdef = predef;
+    double ox = sp.ox, oy = sp.oy, px = ox, py = oy, bx = ox, by = oy;
+    double prefom = -1, bestfom = -1;
+    const double* D = draws + (size_t)p * max_iter * 5;
+    for (int it = 0; it < max_iter; ++it) {
+        const double* d = D + 5 * it;
+        if (it > 0) {
+            if (apply) def = def + def_std * d[0];
+            const double amp = shift_std * d[1];
+            ox = ox + d[2] * amp;
+            oy = oy + d[3] * amp;
+        }
+        const double f = frc_fit(P, Cf, sec, hk, ring_start, R, lane, (ox - sp.ox) * tpn,
+                                 (oy - sp.oy) * tpn, sp, true, def);
+        if (bestfom < f) {
+            bestfom = f;
+            if (apply) bestdef = def;
+            bx = ox;
+            by = oy;
+        }
+        if (f / prefom > d[4]) {
+            if (apply) predef = def;
+            px = ox;
+            py = oy;
+            prefom = f;
+        } else {
+            if (apply) def = predef;
+            ox = px;
+            oy = py;
+        }
+    }
+    if (lane == 0) {
+        origin[2 * p] = bx;
+        origin[2 * p + 1] = by;
+        if (defocus) defocus[p] = apply ? bestdef : sp.def_avg;
+        fom[p] = (float)bestfom;
+    }
+}
+
 // Local refinement grid: candidate c of particle p is R0 * Rot(delta) with
 // delta = step * ((c%3)-1, (c/3%3)-1, (c/9)-1) a rotation vector (Rodrigues).
 __global__ void k_local_mats(const double* __restrict__ r0, int64_t n, double step,
@@ -1854,6 +1936,40 @@ int bsx_search_fit_origin(bsx_search* h, const float* ft_dev, const double* mats
         BSX_FG(1) BSX_FG(2) BSX_FG(3) BSX_FG(4) BSX_FG(5)
         BSX_FG(6) BSX_FG(7) BSX_FG(8) BSX_FG(9) BSX_FG(10)
 #undef BSX_FG
+    }
+    BSX_HIP(hipGetLastError());
+    return BSX_OK;
+}
+
+int bsx_search_fit_monte(bsx_search* h, const float* ft_dev, const double* mats_dev,
+                         int32_t max_iter, double def_std, double shift_std,
+                         const double* draws_dev, double* origin_dev, double* defocus_dev,
+                         float* fom_dev, void* stream) {
+    BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_search_fit_monte: null handle");
+    BSX_REQUIRE(h->nparts == 0 || (ft_dev && mats_dev && draws_dev && origin_dev && fom_dev),
+                BSX_ERR_ARG, "bsx_search_fit_monte: null argument");
+    BSX_REQUIRE(max_iter >= 1 && max_iter <= 1000000 && def_std >= 0 && std::isfinite(def_std) &&
+                    shift_std >= 0 && std::isfinite(shift_std),
+                BSX_ERR_ARG, "bsx_search_fit_monte: bad max_iter %d / def_std %g / shift_std %g",
+                max_iter, def_std, shift_std);
+    if (h->nparts == 0) return BSX_OK;
+    BSX_REQUIRE(h->sp_cap >= h->nparts, BSX_ERR_STATE,
+                "bsx_search_fit_monte: particle records missing (stage with the fused path)");
+    hipStream_t s = (hipStream_t)stream;
+    const int N = h->p.box;
+    const double inv = 1.0 / (N * h->p.sampling), rs = N * h->p.sampling;
+    const size_t shm = sizeof(float2) * h->Nd;
+    switch (h->p.kernel_width) {
+#define BSX_FM(W) case W: \
+        BSX_HIP(hipFuncSetAttribute((const void*)k_fit_monte<W>, \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)); \
+        hipLaunchKernelGGL((k_fit_monte<W>), dim3((unsigned)h->nparts), dim3(256), \
+        shm, s, (const float2*)ft_dev, N, mats_dev, inv, rs, h->d_ktab, h->d_hk, h->d_ring, h->R, \
+        h->Nd, h->d_fp, h->d_ctf, h->d_sp, max_iter, def_std, shift_std, draws_dev, origin_dev, \
+        defocus_dev, fom_dev); break;
+        BSX_FM(1) BSX_FM(2) BSX_FM(3) BSX_FM(4) BSX_FM(5)
+        BSX_FM(6) BSX_FM(7) BSX_FM(8) BSX_FM(9) BSX_FM(10)
+#undef BSX_FM
     }
     BSX_HIP(hipGetLastError());
     return BSX_OK;

@@ -63,6 +63,25 @@ def matrix_views(mats) -> np.ndarray:
     return v
 
 
+def monte_draws(n: int, max_iter: int, seed: int = 0) -> np.ndarray:
+    """Random values for Search.fit_monte, (n, max_iter, 5) float64 per
+    iteration: random_gaussian(0, 1) for the defocus step and for the shift
+    amplitude, the normalised (x, y) of two more (a unit direction,
+    vector3_xy_random_gaussian, src/util/random_numbers.cpp:522-533), and a
+    uniform value in [0, 1) for the acceptance test."""
+    rng = np.random.default_rng(seed)
+    d = np.empty((n, max_iter, 5), dtype=np.float64)
+    d[..., 0] = rng.standard_normal((n, max_iter))
+    d[..., 1] = rng.standard_normal((n, max_iter))
+    xy = rng.standard_normal((n, max_iter, 2))
+    r = np.hypot(xy[..., 0], xy[..., 1])
+    r[r == 0] = 1.0
+    d[..., 2] = xy[..., 0] / r
+    d[..., 3] = xy[..., 1] / r
+    d[..., 4] = rng.random((n, max_iter))
+    return d
+
+
 def rotation_angle_deg(a, b) -> np.ndarray:
     """Geodesic angle (degrees) between rotation matrices a and b (..., 3, 3)."""
     a = np.asarray(a, dtype=np.float64)
@@ -237,6 +256,33 @@ class Search:
         call("bsx_search_fit_origin", self.h, _dev_ptr(ft), _dev_ptr(mats), float(step),
              float(accuracy), float(def_std), _dev_ptr(ori), _dev_ptr(dfc), _dev_ptr(fom),
              _stream())
+        return ori, dfc, fom
+
+    def fit_monte(self, mats: torch.Tensor, max_iter: int = 100, def_std: float = 0.0,
+                  shift_std: float = 1.0, draws: torch.Tensor | None = None, seed: int = 0,
+                  ft: torch.Tensor | None = None):
+        """img_fit_in_recip_space_monte at rotations mats (n, 3, 3): the best
+        origin (n, 2) float64 pixels, defocus (n,) float64 and FOM (n,).
+        draws (n, max_iter, 5) float64: per iteration a standard normal
+        defocus step, a standard normal shift amplitude, a unit shift
+        direction (x, y) and a uniform acceptance value in [0, 1); made from
+        a generator seeded with `seed` when not given (the reference draws
+        from random(), seeded by the process id)."""
+        ft = self._ft if ft is None else ft
+        mats = mats.to(device=self.device, dtype=torch.float64).contiguous()
+        assert mats.shape[0] == self.nparts
+        if draws is None:
+            draws = monte_draws(self.nparts, max_iter, seed)
+        draws = torch.as_tensor(draws, dtype=torch.float64).to(self.device).contiguous()
+        if tuple(draws.shape) != (self.nparts, max_iter, 5):
+            raise ValueError(f"draws: shape {tuple(draws.shape)}, expected "
+                             f"({self.nparts}, {max_iter}, 5)")
+        ori = torch.empty((self.nparts, 2), dtype=torch.float64, device=self.device)
+        dfc = torch.empty(self.nparts, dtype=torch.float64, device=self.device)
+        fom = torch.empty(self.nparts, dtype=torch.float32, device=self.device)
+        call("bsx_search_fit_monte", self.h, _dev_ptr(ft), _dev_ptr(mats), int(max_iter),
+             float(def_std), float(shift_std), _dev_ptr(draws), _dev_ptr(ori), _dev_ptr(dfc),
+             _dev_ptr(fom), _stream())
         return ori, dfc, fom
 
     def particle_data(self):

@@ -463,6 +463,26 @@ int bsx_search_fit_origin(bsx_search* h, const float* ft_dev, const double* mats
                           double accuracy, double def_std, double* origin_dev, double* defocus_dev,
                           float* fom_dev, void* stream);
 
+/* img_fit_in_recip_space_monte (src/mg/mg_refine.cpp:783-860) for the staged
+ * particles at rotations mats_dev: a Metropolis walk of max_iter FOM
+ * evaluations over origin and defocus, with the FOM of
+ * bsx_search_fit_origin. Iteration i > 0 adds def_std * g to the defocus
+ * (when the CTF flag is set or def_std > 0 and the defocus is nonzero) and
+ * shift_std * a * (ux, uy) to the origin; the move is kept when
+ * fom / previous fom > the uniform value, else the walk returns to the last
+ * kept state (the previous FOM starts at -1, as in the reference). The
+ * reference draws from random(); here the caller supplies the draws:
+ * draws_dev is n x max_iter x 5 FP64, (g, a, ux, uy, uniform) with g and a
+ * standard normal, (ux, uy) a unit vector (vector3_xy_random_gaussian,
+ * src/util/random_numbers.cpp:522-533) and uniform in [0, 1); iteration 0
+ * reads only the uniform value. Outputs as bsx_search_fit_origin: the best
+ * origin, its defocus (the staged one when the defocus does not move) and
+ * FOM. Asynchronous. */
+int bsx_search_fit_monte(bsx_search* h, const float* ft_dev, const double* mats_dev,
+                         int32_t max_iter, double def_std, double shift_std,
+                         const double* draws_dev, double* origin_dev, double* defocus_dev,
+                         float* fom_dev, void* stream);
+
 /* Rotation matrices back to views: Matrix3::quaternion (include/Matrix3.h:
  * 129-166) then View(Quaternion&) (include/View.h:52-63). Host only. */
 int bsx_matrix_views(const double* mats, int64_t n, double* views);

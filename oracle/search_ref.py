@@ -168,6 +168,58 @@ def fit_grid(v0: np.ndarray, part: Particle, svals: np.ndarray, hk: np.ndarray,
     return best, bestdef, bestfom, number
 
 
+def fit_monte(v0: np.ndarray, part: Particle, svals: np.ndarray, hk: np.ndarray,
+              ring_start: np.ndarray, n: int, oracle, max_iter: int, def_std: float,
+              shift_std: float, draws: np.ndarray, ctf_flag: bool = True):
+    """img_fit_in_recip_space_monte (mg_refine.cpp:783-860) on the disk
+    points with caller-supplied random values, draws (max_iter, 5):
+    (g, a, ux, uy, uniform) per iteration for random_gaussian(0, def_std),
+    vector3_xy_random_gaussian(0, shift_std) = (ux, uy) * shift_std a and
+    random() * irm. The defocus moves when the CTF flag is set or def_std > 0
+    and the defocus is nonzero (apply_ctf, :787-792); predef and bestdef
+    start at the defocus only in the latter case, else at 0. A move is kept
+    when fom / prefom > uniform (prefom starts at -1), else the walk returns
+    to the last kept origin and defocus. Returns (origin (x, y), defocus,
+    fom) of the best FOM seen."""
+    h, k = hk[:, 0].astype(np.float64), hk[:, 1].astype(np.float64)
+    ox0, oy0 = float(part.ori[0]), float(part.ori[1])
+
+    def fom(ox, oy, d):
+        v = v0 * np.exp(2j * np.pi * (h * ox + k * oy) / n)
+        return frc_half(v, _ctf_at(part, hk, n, d, ctf_flag, oracle), svals, ring_start)
+
+    dstep = def_std > 0 and part.def_avg != 0
+    apply = ctf_flag or dstep
+    d = float(part.def_avg)
+    predef = bestdef = d if dstep else 0.0
+    ox, oy = ox0, oy0
+    pre, best = (ox, oy), (ox, oy)
+    prefom = bestfom = -1.0
+    for it in range(max_iter):
+        g, a, ux, uy, u = (float(x) for x in draws[it])
+        if it > 0:
+            if apply:
+                d = d + def_std * g
+            amp = shift_std * a
+            ox, oy = ox + ux * amp, oy + uy * amp
+        f = fom(ox, oy, d)
+        if bestfom < f:
+            bestfom, best = f, (ox, oy)
+            if apply:
+                bestdef = d
+        with np.errstate(divide="ignore", invalid="ignore"):
+            keep = np.float64(f) / np.float64(prefom) > u
+        if keep:
+            if apply:
+                predef = d
+            pre, prefom = (ox, oy), f
+        else:
+            if apply:
+                d = predef
+            ox, oy = pre
+    return best, (bestdef if apply else float(part.def_avg)), bestfom
+
+
 # ------------------------------------------------------------ local view grid
 def _view_normalize(v):
     """View::normalize (include/View.h:110-116, check :34-38) and

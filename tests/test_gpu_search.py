@@ -383,6 +383,70 @@ def test_chunked_staging_keeps_every_record():
         assert fom[i].item() == pytest.approx(f, abs=2e-5)
 
 
+@pytest.mark.parametrize("ctf", [True, False])
+def test_fit_monte_matches_oracle(ctf):
+    """img_fit_in_recip_space_monte with the same draws on both sides: noisy
+    particles (SNR 1) staged up to 1.2 pixels and two defocus steps off, 24
+    iterations of shift_std 0.6 pixels and def_std 300 A: the best origin and
+    defocus equal the FP64 restatement's (every FOM comparison decides the
+    same way), the FOM agrees to 2e-5. A second particle set with the section
+    sign flipped would walk; here positive FOMs restart every move."""
+    n, R, rng, ft, ps, s, truth = _fit_setup(1.0, 45, ctf)
+    sd, sh, it = 300.0, 0.6, 24
+    parts = ps.parts.copy()
+    parts["ori"] += rng.uniform(-1.2, 1.2, size=(len(parts), 2))
+    parts["def_avg"] += rng.uniform(-2, 2, len(parts)) * sd
+    say(f"fit_monte vs oracle ctf={ctf} particles={len(parts)} iterations={it}")
+    s.stage(ps.images.cuda().contiguous(), parts)
+    draws = search.monte_draws(len(parts), it, seed=9)
+    ori, dfc, fom = s.fit_monte(torch.as_tensor(truth), it, sd, sh, draws=draws, ft=ft)
+    torch.cuda.synchronize()
+    ori, dfc, fom = ori.cpu().numpy(), dfc.cpu().numpy(), fom.cpu().numpy()
+    o = Oracle(64)
+    hk, rs = s.points()
+    ftc = ft.cpu().numpy()
+    plist = synth.particles_list(parts)
+    moved = 0
+    for i in range(0, len(parts), 3):
+        F = np.fft.fft2(ps.images[i].numpy().astype(np.float64)) / n
+        v0 = F[hk[:, 1] % n, hk[:, 0] % n]
+        osec = o.central_section(ftc, 1.5, truth[i], 1.5, 8, 2, 0)
+        sv = osec[hk[:, 1] % n, hk[:, 0] % n]
+        (ox, oy), d, f = sr.fit_monte(v0, plist[i], sv, hk, rs, n, o, it, sd, sh, draws[i], ctf)
+        assert ori[i, 0] == pytest.approx(ox, abs=1e-9) and ori[i, 1] == pytest.approx(oy, abs=1e-9)
+        assert dfc[i] == pytest.approx(d, rel=1e-12)
+        assert fom[i] == pytest.approx(f, abs=2e-5)
+        moved += (ox, oy) != (parts["ori"][i][0], parts["ori"][i][1])
+    assert moved > 0
+
+
+def test_fit_monte_edge_cases():
+    """No staged particles: nothing to do. max_iter < 1, a negative standard
+    deviation or draws of the wrong shape are argument errors. One iteration
+    returns the staged origin and defocus."""
+    from bsoft_amd.abi import BsxError
+    n, R = 32, 8
+    rng = np.random.default_rng(5)
+    bm = synth.make_blob_map(n, 6, rng)
+    ft = map_transform(bm.volume().cuda())
+    s = search.Search(n, R, 1.5)
+    s.stage(torch.empty((0, n, n), device="cuda"), synth.make_particles(bm, 1, n, rng).parts[:0])
+    ori, dfc, fom = s.fit_monte(torch.empty((0, 3, 3), dtype=torch.float64), 4, ft=ft)
+    assert ori.shape == (0, 2) and fom.shape == (0,)
+    ps = synth.make_particles(bm, 4, n, rng, snr=None)
+    s.stage(ps.images.cuda().contiguous(), ps.parts)
+    mats = torch.as_tensor(synth.view_to_matrix(ps.parts["view"]))
+    for it, dstd, sstd in ((0, 0.0, 1.0), (4, -1.0, 1.0), (4, 0.0, -1.0)):
+        with pytest.raises(BsxError):
+            s.fit_monte(mats, it, dstd, sstd, ft=ft)
+    with pytest.raises(ValueError):
+        s.fit_monte(mats, 4, draws=np.zeros((4, 3, 5)), ft=ft)
+    ori, dfc, _ = s.fit_monte(mats, 1, 500.0, 1.0, ft=ft)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(ori.cpu().numpy(), ps.parts["ori"], atol=1e-9)
+    np.testing.assert_array_equal(dfc.cpu().numpy(), ps.parts["def_avg"])
+
+
 def test_fit_origin_edge_cases():
     """No staged particles: nothing to do. A step or accuracy that is not
     positive, or a step / accuracy ratio of 1e6 or more, is an argument error."""

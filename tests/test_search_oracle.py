@@ -232,6 +232,55 @@ def test_fit_grid_defocus_scan_without_ctf(oracle64):
     assert d == dt - 5 * sd and num == 11 + 9
 
 
+def _monte_draws(rows):
+    """draws (max_iter, 5) from (g, a, ux, uy, uniform) rows."""
+    return np.array(rows, dtype=np.float64)
+
+
+def test_fit_monte_moves_start_from_the_kept_state(oracle64):
+    """Positive FOMs are never kept (prefom starts at -1), so every move
+    starts again from the staged origin and defocus: from (cx + 0.5, cy) at
+    dt + sd, a first move by (+0.5, 0) and +sd, then one by (-0.5, 0) and
+    -sd lands exactly on (cx, cy, dt), where the FOM is 1. A walk that kept
+    the first move would end at (cx + 0.5, cy, dt + sd) instead."""
+    n, cx, cy, dt, sd = 32, 16.0, 15.0, 1.5e4, 700.0
+    hk, rs, sv, v0 = _fit_case(oracle64, n, 12, (cx, cy), dt)
+    part = _part(def_avg=dt + sd, ori=(cx + 0.5, cy))
+    dr = _monte_draws([(0, 0, 1, 0, 0.5), (1, 1, 1, 0, 0.5), (-1, 1, -1, 0, 0.5)])
+    (ox, oy), d, f = sr.fit_monte(v0, part, sv, hk, rs, n, oracle64, 3, sd, 0.5, dr)
+    assert (ox, oy) == (cx, cy) and d == dt
+    assert f == pytest.approx(1.0, abs=1e-9)
+
+
+def test_fit_monte_negative_fom_walk(oracle64):
+    """With the section negated the FOM is negative, fom / prefom > 0 = the
+    uniform value keeps every move, so the walk accumulates: the result is
+    the point of the cumulative path with the highest (least negative) FOM,
+    found here by scoring every point of the path directly."""
+    n, cx, cy, dt = 32, 16.0, 15.0, 1.5e4
+    hk, rs, sv, v0 = _fit_case(oracle64, n, 12, (cx, cy), dt, ctf=False)
+    part = _part(def_avg=dt, ori=(cx, cy))
+    rng = np.random.default_rng(3)
+    m = 12
+    dr = np.zeros((m, 5))
+    dr[:, 1] = rng.standard_normal(m)
+    ang = rng.uniform(0, 2 * np.pi, m)
+    dr[:, 2], dr[:, 3] = np.cos(ang), np.sin(ang)
+    (ox, oy), d, f = sr.fit_monte(v0, part, -sv, hk, rs, n, oracle64, m, 0.0, 0.3, dr,
+                                  ctf_flag=False)
+    h, k = hk[:, 0], hk[:, 1]
+    path, x, y = [], cx, cy
+    for i in range(m):
+        if i > 0:
+            amp = 0.3 * dr[i, 1]
+            x, y = x + dr[i, 2] * amp, y + dr[i, 3] * amp
+        v = v0 * np.exp(2j * np.pi * (h * x + k * y) / n)
+        path.append((sr.frc_half(v, np.ones(len(hk)), -sv, rs), x, y))
+    assert all(p[0] < 0 for p in path)
+    best = max(path, key=lambda p: p[0])
+    assert (ox, oy) == (best[1], best[2]) and f == best[0] and d == dt
+
+
 # ------------------------------------------------- views_for_refinement grid
 def _grid_closed_form(view, step):
     """The 27 views from the geometry alone: with v the unit view vector,
