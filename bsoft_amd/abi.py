@@ -178,7 +178,7 @@ SIGNATURES = {
     "bsx_search_fit_origin": ([_vp, _vp, _vp, C.c_double, C.c_double, C.c_double, _vp, _vp, _vp,
                                _vp], C.c_int),
     "bsx_search_fit_monte": ([_vp, _vp, _vp, C.c_int32, C.c_double, C.c_double, _vp, _vp, _vp,
-                              _vp, _vp], C.c_int),
+                              _vp, _vp, _vp], C.c_int),
     "bsx_search_cnn_input": ([_vp, C.c_int32, _vp, _vp], C.c_int),
     "bsx_cnn_create": ([C.POINTER(_vp), C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
                         C.c_uint64], C.c_int),

@@ -1076,8 +1076,8 @@ __global__ __launch_bounds__(256) void k_fit_monte(
     const float* __restrict__ ktab, const int2* __restrict__ hk, const int* __restrict__ ring_start,
     int R, int Nd, const float2* __restrict__ fp, const float* __restrict__ ctf,
     const SPart* __restrict__ sps, int max_iter, double def_std, double shift_std,
-    const double* __restrict__ draws, double* __restrict__ origin, double* __restrict__ defocus,
-    float* __restrict__ fom) {
+    const double* __restrict__ draws, const double* __restrict__ def0,
+    double* __restrict__ origin, double* __restrict__ defocus, float* __restrict__ fom) {
     __shared__ float K[(FSI_NDIV + 1) * WD];
     extern __shared__ float2 sec[];   // Nd section values
     for (int i = threadIdx.x; i < (FSI_NDIV + 1) * WD; i += blockDim.x) K[i] = ktab[i];
@@ -1099,9 +1099,10 @@ __global__ __launch_bounds__(256) void k_fit_monte(
     const float2* P = fp + p * Nd;
     const float* Cf = ctf + p * Nd;
     const double tpn = 2 * M_PI / N;
-    const bool dstep = def_std > 0 && sp.def_avg != 0;
+    const double def_in = def0 ? def0[p] : sp.def_avg;   // the defocus the walk starts from
+    const bool dstep = def_std > 0 && def_in != 0;
     const bool apply = sp.flag || dstep;
-    double def = sp.def_avg, predef = dstep ? def : 0, bestdef = predef;
+    double def = def_in, predef = dstep ? def : 0, bestdef = predef;
     double ox = sp.ox, oy = sp.oy, px = ox, py = oy, bx = ox, by = oy;
     double prefom = -1, bestfom = -1;
     const double* D = draws + (size_t)p * max_iter * 5;
@@ -1135,7 +1136,7 @@ __global__ __launch_bounds__(256) void k_fit_monte(
     if (lane == 0) {
         origin[2 * p] = bx;
         origin[2 * p + 1] = by;
-        if (defocus) defocus[p] = apply ? bestdef : sp.def_avg;
+        if (defocus) defocus[p] = apply ? bestdef : def_in;
         fom[p] = (float)bestfom;
     }
 }
@@ -1943,8 +1944,8 @@ int bsx_search_fit_origin(bsx_search* h, const float* ft_dev, const double* mats
 
 int bsx_search_fit_monte(bsx_search* h, const float* ft_dev, const double* mats_dev,
                          int32_t max_iter, double def_std, double shift_std,
-                         const double* draws_dev, double* origin_dev, double* defocus_dev,
-                         float* fom_dev, void* stream) {
+                         const double* draws_dev, const double* defocus_start_dev,
+                         double* origin_dev, double* defocus_dev, float* fom_dev, void* stream) {
     BSX_REQUIRE(h, BSX_ERR_ARG, "bsx_search_fit_monte: null handle");
     BSX_REQUIRE(h->nparts == 0 || (ft_dev && mats_dev && draws_dev && origin_dev && fom_dev),
                 BSX_ERR_ARG, "bsx_search_fit_monte: null argument");
@@ -1965,8 +1966,8 @@ int bsx_search_fit_monte(bsx_search* h, const float* ft_dev, const double* mats_
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm)); \
         hipLaunchKernelGGL((k_fit_monte<W>), dim3((unsigned)h->nparts), dim3(256), \
         shm, s, (const float2*)ft_dev, N, mats_dev, inv, rs, h->d_ktab, h->d_hk, h->d_ring, h->R, \
-        h->Nd, h->d_fp, h->d_ctf, h->d_sp, max_iter, def_std, shift_std, draws_dev, origin_dev, \
-        defocus_dev, fom_dev); break;
+        h->Nd, h->d_fp, h->d_ctf, h->d_sp, max_iter, def_std, shift_std, draws_dev, \
+        defocus_start_dev, origin_dev, defocus_dev, fom_dev); break;
         BSX_FM(1) BSX_FM(2) BSX_FM(3) BSX_FM(4) BSX_FM(5)
         BSX_FM(6) BSX_FM(7) BSX_FM(8) BSX_FM(9) BSX_FM(10)
 #undef BSX_FM

@@ -82,6 +82,45 @@ def monte_draws(n: int, max_iter: int, seed: int = 0) -> np.ndarray:
     return d
 
 
+def monte_view_draws(n: int, max_iter: int, seed: int = 0) -> np.ndarray:
+    """Random values for Search.refine_monte, (n, max_iter, 6) float64 per
+    iteration: random_gaussian(0, 1) for the view step amplitude, a unit
+    vector (x, y, z) (vector3_random_gaussian, src/util/random_numbers.cpp:
+    502-510), a uniform value for the angle step and one for the acceptance
+    test, both in [0, 1)."""
+    rng = np.random.default_rng(seed)
+    d = np.empty((n, max_iter, 6), dtype=np.float64)
+    d[..., 0] = rng.standard_normal((n, max_iter))
+    xyz = rng.standard_normal((n, max_iter, 3))
+    r = np.sqrt((xyz * xyz).sum(-1))
+    r[r == 0] = 1.0
+    d[..., 1:4] = xyz / r[..., None]
+    d[..., 4] = rng.random((n, max_iter))
+    d[..., 5] = rng.random((n, max_iter))
+    return d
+
+
+def views_normalize(v: np.ndarray) -> np.ndarray:
+    """View::normalize (include/View.h:110-116) of (n, 4) rows: the view
+    vector to unit length ((0, 0, 1) when below 1e-30, components below 1e-30
+    set to 0) and the angle into [-pi, pi] by angle_set_negPI_to_PI
+    (src/util/utilities.cpp:606-615: n = trunc(a / pi), a -= (n + n % 2) pi)."""
+    v = np.array(v, dtype=np.float64).reshape(-1, 4)
+    s = np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2])
+    small = s < 1e-30
+    v[~small, :3] = v[~small, :3] / s[~small, None]
+    v[small, :3] = (0.0, 0.0, 1.0)
+    xyz = v[:, :3]
+    xyz[np.abs(xyz) < 1e-30] = 0.0
+    a = v[:, 3]
+    fin = np.isfinite(a)
+    q = np.trunc(a[fin] / np.pi)
+    a[fin] = a[fin] - (q + np.fmod(q, 2)) * np.pi
+    a[~fin] = 0.0
+    a[np.abs(a) < 1e-30] = 0.0
+    return v
+
+
 def rotation_angle_deg(a, b) -> np.ndarray:
     """Geodesic angle (degrees) between rotation matrices a and b (..., 3, 3)."""
     a = np.asarray(a, dtype=np.float64)
@@ -148,6 +187,7 @@ class Search:
         call("bsx_search_stage", self.h, _dev_ptr(images) if n else None,
              particles_ctypes(arr) if n else None, n, _stream())
         self.nparts = n
+        self._defocus = arr["def_avg"].astype(np.float64) if n else np.zeros(0)
 
     def exhaustive(self):
         """BF16 GEMM against every view: (cand, score), each (n, topk)."""
@@ -260,14 +300,15 @@ class Search:
 
     def fit_monte(self, mats: torch.Tensor, max_iter: int = 100, def_std: float = 0.0,
                   shift_std: float = 1.0, draws: torch.Tensor | None = None, seed: int = 0,
-                  ft: torch.Tensor | None = None):
+                  ft: torch.Tensor | None = None, def_start=None):
         """img_fit_in_recip_space_monte at rotations mats (n, 3, 3): the best
         origin (n, 2) float64 pixels, defocus (n,) float64 and FOM (n,).
         draws (n, max_iter, 5) float64: per iteration a standard normal
         defocus step, a standard normal shift amplitude, a unit shift
         direction (x, y) and a uniform acceptance value in [0, 1); made from
         a generator seeded with `seed` when not given (the reference draws
-        from random(), seeded by the process id)."""
+        from random(), seeded by the process id). def_start (n,): the defocus
+        each walk starts from, the staged one when None."""
         ft = self._ft if ft is None else ft
         mats = mats.to(device=self.device, dtype=torch.float64).contiguous()
         assert mats.shape[0] == self.nparts
@@ -277,13 +318,74 @@ class Search:
         if tuple(draws.shape) != (self.nparts, max_iter, 5):
             raise ValueError(f"draws: shape {tuple(draws.shape)}, expected "
                              f"({self.nparts}, {max_iter}, 5)")
+        d0 = None
+        if def_start is not None:
+            d0 = torch.as_tensor(def_start, dtype=torch.float64).to(self.device).contiguous()
+            if tuple(d0.shape) != (self.nparts,):
+                raise ValueError(f"def_start: shape {tuple(d0.shape)}, expected ({self.nparts},)")
         ori = torch.empty((self.nparts, 2), dtype=torch.float64, device=self.device)
         dfc = torch.empty(self.nparts, dtype=torch.float64, device=self.device)
         fom = torch.empty(self.nparts, dtype=torch.float32, device=self.device)
         call("bsx_search_fit_monte", self.h, _dev_ptr(ft), _dev_ptr(mats), int(max_iter),
-             float(def_std), float(shift_std), _dev_ptr(draws), _dev_ptr(ori), _dev_ptr(dfc),
-             _dev_ptr(fom), _stream())
+             float(def_std), float(shift_std), _dev_ptr(draws), _dev_ptr(d0), _dev_ptr(ori),
+             _dev_ptr(dfc), _dev_ptr(fom), _stream())
         return ori, dfc, fom
+
+    def refine_monte(self, views, max_iter: int = 10, def_std: float = 0.0,
+                     shift_std: float = 1.0, view_std: float = 0.01, max_angle: float = 0.01,
+                     draws=None, inner_draws=None, seed: int = 0,
+                     ft: torch.Tensor | None = None):
+        """img_refine_monte (src/mg/mg_refine.cpp:979-1060) for the staged
+        particles from views (n, 4), without the magnification steps:
+        iteration i > 0 adds view_std g (ux, uy, uz) to the view vector and
+        max_angle (2 u - 1) to the angle, normalises, then runs the Monte
+        Carlo origin/defocus fit (fit_monte, max_iter steps) at that view
+        from the staged origin (and the staged defocus when the CTF flag is
+        set and def_std > 0 with a nonzero defocus; otherwise the defocus the
+        previous fit left). The best view, its fit's origin and defocus and
+        the FOM are kept; a view is kept for the next step when
+        fom / prefom > uniform (prefom starts at -1), else the walk returns to
+        the last kept view. draws (n, max_iter, 6) as monte_view_draws,
+        inner_draws (n, max_iter, max_iter, 5) as monte_draws per iteration.
+        Returns views (n, 4), origins (n, 2), defocus (n,), FOM (n,)."""
+        n = self.nparts
+        view = views_normalize(np.asarray(views, dtype=np.float64).reshape(n, 4))
+        if draws is None:
+            draws = monte_view_draws(n, max_iter, seed)
+        if inner_draws is None:
+            inner_draws = np.stack([monte_draws(n, max_iter, seed + 1 + i)
+                                    for i in range(max_iter)], axis=1)
+        draws, inner_draws = np.asarray(draws, np.float64), np.asarray(inner_draws, np.float64)
+        if draws.shape != (n, max_iter, 6) or inner_draws.shape != (n, max_iter, max_iter, 5):
+            raise ValueError("draws: (n, max_iter, 6) and inner_draws: (n, max_iter, max_iter, 5)")
+        d0 = self._defocus.copy()
+        apply = (def_std > 0) & (d0 != 0) & bool(self.params.flags & abi.BSX_SEARCH_CTF)
+        cur = d0.copy()
+        pre = view.copy()
+        prefom, bestfom = np.full(n, -1.0), np.full(n, -1.0)
+        bestview, bestori, bestdef = view.copy(), np.zeros((n, 2)), d0.copy()
+        for it in range(max_iter):
+            if it > 0:
+                g = view_std * draws[:, it, 0]
+                step = view.copy()
+                step[:, :3] = view[:, :3] + draws[:, it, 1:4] * g[:, None]
+                step[:, 3] = view[:, 3] + max_angle * (draws[:, it, 4] * 2 - 1)
+                view = views_normalize(step)
+            start = np.where(apply, d0, cur)
+            mats = torch.as_tensor(view_matrices(view))
+            ori, dfc, fom = self.fit_monte(mats, max_iter, def_std, shift_std,
+                                           draws=inner_draws[:, it], ft=ft, def_start=start)
+            ori, cur = ori.cpu().numpy(), dfc.cpu().numpy()
+            fom = fom.cpu().numpy().astype(np.float64)
+            better = bestfom < fom
+            bestfom[better] = fom[better]
+            bestview[better], bestori[better] = view[better], ori[better]
+            bestdef[better & apply] = cur[better & apply]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                keep = fom / prefom > draws[:, it, 5]
+            pre[keep], prefom[keep] = view[keep], fom[keep]
+            view[~keep] = pre[~keep]
+        return bestview, bestori, np.where(apply, bestdef, cur), bestfom
 
     def particle_data(self):
         v = torch.empty((self.nparts, self.npoints), dtype=torch.complex64, device=self.device)

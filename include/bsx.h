@@ -475,13 +475,15 @@ int bsx_search_fit_origin(bsx_search* h, const float* ft_dev, const double* mats
  * draws_dev is n x max_iter x 5 FP64, (g, a, ux, uy, uniform) with g and a
  * standard normal, (ux, uy) a unit vector (vector3_xy_random_gaussian,
  * src/util/random_numbers.cpp:522-533) and uniform in [0, 1); iteration 0
- * reads only the uniform value. Outputs as bsx_search_fit_origin: the best
- * origin, its defocus (the staged one when the defocus does not move) and
- * FOM. Asynchronous. */
+ * reads only the uniform value. defocus_start_dev (n FP64, optional): the
+ * defocus each walk starts from instead of the staged one (img_refine_monte
+ * leaves the previous fit's defocus in place when it does not reset it).
+ * Outputs as bsx_search_fit_origin: the best origin, its defocus (the
+ * starting one when the defocus does not move) and FOM. Asynchronous. */
 int bsx_search_fit_monte(bsx_search* h, const float* ft_dev, const double* mats_dev,
                          int32_t max_iter, double def_std, double shift_std,
-                         const double* draws_dev, double* origin_dev, double* defocus_dev,
-                         float* fom_dev, void* stream);
+                         const double* draws_dev, const double* defocus_start_dev,
+                         double* origin_dev, double* defocus_dev, float* fom_dev, void* stream);
 
 /* Rotation matrices back to views: Matrix3::quaternion (include/Matrix3.h:
  * 129-166) then View(Quaternion&) (include/View.h:52-63). Host only. */

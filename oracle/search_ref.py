@@ -170,7 +170,8 @@ def fit_grid(v0: np.ndarray, part: Particle, svals: np.ndarray, hk: np.ndarray,
 
 def fit_monte(v0: np.ndarray, part: Particle, svals: np.ndarray, hk: np.ndarray,
               ring_start: np.ndarray, n: int, oracle, max_iter: int, def_std: float,
-              shift_std: float, draws: np.ndarray, ctf_flag: bool = True):
+              shift_std: float, draws: np.ndarray, ctf_flag: bool = True,
+              def_start: float | None = None):
     """img_fit_in_recip_space_monte (mg_refine.cpp:783-860) on the disk
     points with caller-supplied random values, draws (max_iter, 5):
     (g, a, ux, uy, uniform) per iteration for random_gaussian(0, def_std),
@@ -180,7 +181,8 @@ def fit_monte(v0: np.ndarray, part: Particle, svals: np.ndarray, hk: np.ndarray,
     start at the defocus only in the latter case, else at 0. A move is kept
     when fom / prefom > uniform (prefom starts at -1), else the walk returns
     to the last kept origin and defocus. Returns (origin (x, y), defocus,
-    fom) of the best FOM seen."""
+    fom) of the best FOM seen. def_start: the defocus the walk starts from
+    (the particle's when None)."""
     h, k = hk[:, 0].astype(np.float64), hk[:, 1].astype(np.float64)
     ox0, oy0 = float(part.ori[0]), float(part.ori[1])
 
@@ -188,9 +190,10 @@ def fit_monte(v0: np.ndarray, part: Particle, svals: np.ndarray, hk: np.ndarray,
         v = v0 * np.exp(2j * np.pi * (h * ox + k * oy) / n)
         return frc_half(v, _ctf_at(part, hk, n, d, ctf_flag, oracle), svals, ring_start)
 
-    dstep = def_std > 0 and part.def_avg != 0
+    d = float(part.def_avg if def_start is None else def_start)
+    d_in = d
+    dstep = def_std > 0 and d != 0
     apply = ctf_flag or dstep
-    d = float(part.def_avg)
     predef = bestdef = d if dstep else 0.0
     ox, oy = ox0, oy0
     pre, best = (ox, oy), (ox, oy)
@@ -217,7 +220,43 @@ def fit_monte(v0: np.ndarray, part: Particle, svals: np.ndarray, hk: np.ndarray,
             if apply:
                 d = predef
             ox, oy = pre
-    return best, (bestdef if apply else float(part.def_avg)), bestfom
+    return best, (bestdef if apply else d_in), bestfom
+
+
+def refine_monte(v0, part, section, hk, ring_start, n, oracle, view, max_iter, def_std,
+                 shift_std, view_std, max_angle, draws, inner_draws, ctf_flag=True):
+    """img_refine_monte (mg_refine.cpp:979-1060) without magnification
+    steps for one particle. section(view) gives the section values at hk;
+    draws (max_iter, 6): (g, ux, uy, uz, u_angle, u_accept); inner_draws
+    (max_iter, max_iter, 5) for fit_monte. The defocus restarts from the
+    particle's only when def_std, the defocus and the CTF flag are all
+    nonzero (apply_ctf, :984-988); otherwise each fit starts from the
+    defocus the previous one left. Returns (view, origin, defocus, fom)."""
+    d0 = float(part.def_avg)
+    apply = bool(def_std and d0 and ctf_flag)
+    cur = d0
+    view = _view_normalize(view)
+    pre, prefom, bestfom = list(view), -1.0, -1.0
+    best = (list(view), (0.0, 0.0), d0)
+    for it in range(max_iter):
+        g, ux, uy, uz, ua, uk = (float(x) for x in draws[it])
+        if it > 0:
+            amp = view_std * g
+            view = _view_normalize([view[0] + ux * amp, view[1] + uy * amp,
+                                    view[2] + uz * amp, view[3] + max_angle * (ua * 2 - 1)])
+        start = d0 if apply else cur
+        ori, cur, f = fit_monte(v0, part, section(view), hk, ring_start, n, oracle, max_iter,
+                                def_std, shift_std, inner_draws[it], ctf_flag, def_start=start)
+        if bestfom < f:
+            bestfom = f
+            best = (list(view), ori, cur if apply else best[2])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            keep = np.float64(f) / np.float64(prefom) > uk
+        if keep:
+            pre, prefom = list(view), f
+        else:
+            view = list(pre)
+    return best[0], best[1], (best[2] if apply else cur), bestfom
 
 
 # ------------------------------------------------------------ local view grid

@@ -71,5 +71,5 @@ def test_argument_errors_of_map_and_fit_entry_points():
     assert L.bsx_recon_stats(None, 0, 1, 2.0, None, 0, C.byref(n), None, None) == abi.BSX_ERR_ARG
     assert L.bsx_search_fit_origin(None, buf, buf, 1.0, 0.25, 0.0, buf, None, buf,
                                    None) == abi.BSX_ERR_ARG
-    assert L.bsx_search_fit_monte(None, buf, buf, 10, 0.0, 1.0, buf, buf, None, buf,
+    assert L.bsx_search_fit_monte(None, buf, buf, 10, 0.0, 1.0, buf, None, buf, None, buf,
                                   None) == abi.BSX_ERR_ARG

@@ -281,6 +281,29 @@ def test_fit_monte_negative_fom_walk(oracle64):
     assert (ox, oy) == (best[1], best[2]) and f == best[0] and d == dt
 
 
+def test_refine_monte_view_steps_start_from_the_kept_view(oracle64):
+    """A section whose FOM falls as cos(5 (a - a_true)) with the view angle
+    a: from a_true + 0.01, a first angle step of +0.01 (worse, not kept, as
+    no positive FOM is) and a second of -0.01 from the staged view land on
+    a_true exactly, FOM 1. A walk that kept the first step would end at
+    a_true + 0.01."""
+    n, cx, cy, dt, at = 32, 16.0, 15.0, 1.5e4, 0.25
+    hk, rs, sv, v0 = _fit_case(oracle64, n, 12, (cx, cy), dt, ctf=False)
+    part = _part(def_avg=dt, ori=(cx, cy))
+
+    def section(view):
+        return sv * np.exp(1j * 5 * (view[3] - at))
+
+    dr = np.array([(0, 0, 0, 1, 0.5, 0.5), (0, 0, 0, 1, 1.0, 0.5), (0, 0, 0, 1, 0.0, 0.5)])
+    inner = np.zeros((3, 3, 5))
+    inner[..., 2] = 1.0
+    view, (ox, oy), d, f = sr.refine_monte(v0, part, section, hk, rs, n, oracle64,
+                                           [0, 0, 1, at + 0.01], 3, 0.0, 0.0, 0.0, 0.01, dr,
+                                           inner, ctf_flag=False)
+    assert view[3] == pytest.approx(at, abs=1e-15) and (ox, oy) == (cx, cy) and d == dt
+    assert f == pytest.approx(1.0, abs=1e-9)
+
+
 # ------------------------------------------------- views_for_refinement grid
 def _grid_closed_form(view, step):
     """The 27 views from the geometry alone: with v the unit view vector,
