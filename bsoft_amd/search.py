@@ -148,6 +148,7 @@ class Search:
         self.box, self.radius, self.topk = box, radius, topk
         self.nviews = 0
         self.nparts = 0
+        self._defocus = np.zeros(0)   # staged defocus per particle (refine_monte)
         self._ft = None
 
     def __del__(self):
